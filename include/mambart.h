@@ -1,0 +1,320 @@
+/*
+ * mambart — MI355X (gfx950) wavefront path-tracing core, C ABI.
+ *
+ * Drop-in for the OpenCL kernel launches of mambastudio/MambaTracer's
+ * wavefront path tracer.  The Java host today launches each kernel through
+ * the un-vendored wrapper.core/JOCL layer
+ * (OpenCLConfiguration.createKernel / execute1DKernel, e.g.
+ * src/cl/device/CDeviceGI.java:230-276, src/cl/algorithms/CCompact.java:79-89,
+ * src/cl/algorithms/PrefixSumInteger.java:80-88,
+ * src/cl/algorithms/CAdaptiveEnvironment.java:69-145); each entry point below
+ * replaces one such launch sequence and cites it.
+ *
+ * Conventions (kept from the reference, src/cl/kernel/*.cl):
+ *  - The caller owns every buffer; kernels never allocate caller-visible
+ *    memory.  Buffers are device pointers (mt_buf_alloc, or any HIP
+ *    allocation on the context's device).
+ *  - Counts that flow between launches are passed BY DEVICE POINTER
+ *    (`const int32_t* d_count`), as the reference passes `global int* count`
+ *    (NormalBVH.cl:97,120), so no host synchronisation is needed between
+ *    stages.  `capacity` is the number of slots the buffers hold (the
+ *    reference's global work size); *d_count is clamped to it.
+ *  - All calls are asynchronous on the context's stream; mt_ctx_sync waits.
+ *  - Every function returns 0 (MT_OK) or a negative MT_ERR_* code;
+ *    mt_last_error copies a thread-local message.  A Java/Panama binding
+ *    turns a non-zero status into an exception, as JOCL's
+ *    CL.setExceptionsEnabled(true) does (src/cl/ui/fx/main/TracerAPI.java:79).
+ *  - Scene data stays in the reference's flat layout
+ *    (src/cl/kernel/Util.cl:95-121, NormalBVH.cl:1-11):
+ *      points  float4[V]            (x, y, z, w = 0)
+ *      faces   int32[10*N]          {v0,v1,v2, uv0,uv1,uv2, n0,n1,n2, mat}
+ *      nodes   int32[7*(2N-1)]      {bound, sibling, left, right, parent, isLeaf, child}
+ *      bounds  float[8*(2N-1)]      {min.xyzw, max.xyzw}
+ *      materials 36 x 4-byte words  (4 int flags + 8 float4, Bsdf.cl:24-42)
+ *  - Rays are two float4 streams (SoA of 16-byte records):
+ *      o_tmin[i] = (o.x, o.y, o.z, tMin),  d_tmax[i] = (d.x, d.y, d.z, tMax)
+ *    replacing the 80-byte AoS Ray (Util.cl:129-138); inv_d and sign are
+ *    recomputed in-kernel exactly as initGlobalRay does (Util.cl:577-600).
+ *
+ * Thread safety: calls on one context are issued in order from one thread
+ * (the reference's one render thread per device, CDeviceGI.java:55);
+ * distinct contexts may be used from distinct threads.
+ */
+#ifndef MAMBART_H
+#define MAMBART_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MT_OK 0
+#define MT_ERR_ARG (-1)      /* invalid argument / shape */
+#define MT_ERR_HIP (-2)      /* HIP runtime error */
+#define MT_ERR_ALLOC (-3)    /* device or host allocation failed */
+#define MT_ERR_DEPTH (-4)    /* BVH deeper than the traversal supports */
+#define MT_ERR_CAPACITY (-5) /* request exceeds a documented maximum */
+#define MT_ERR_STATE (-6)    /* object used in the wrong state */
+
+typedef struct mt_context mt_context;
+typedef struct mt_scene mt_scene;
+typedef struct mt_lightgrid mt_lightgrid;
+typedef struct mt_envmap mt_envmap;
+
+/* ------------------------------------------------------------------ */
+/* Errors, context, buffers (replace wrapper.core.OpenCLConfiguration / */
+/* CMemory: createBufferB/I/F, transferToDevice, finish —               */
+/* usage e.g. src/cl/device/CDeviceGI.java:150-179, 428)               */
+/* ------------------------------------------------------------------ */
+int32_t mt_last_error(char* buf, int32_t len);
+const char* mt_version(void);
+
+/* flags: MT_CTX_OWN_STREAM creates a private non-blocking stream and ignores
+ * `stream`; otherwise work is issued on `stream` (a hipStream_t; NULL is the
+ * device's null stream). */
+#define MT_CTX_OWN_STREAM 1
+int32_t mt_ctx_create(int32_t device, void* stream, int32_t flags, mt_context** out);
+int32_t mt_ctx_destroy(mt_context* ctx);
+int32_t mt_ctx_sync(mt_context* ctx); /* OpenCLConfiguration.finish() */
+
+int32_t mt_buf_alloc(mt_context* ctx, uint64_t bytes, void** dptr);
+int32_t mt_buf_free(mt_context* ctx, void* dptr);
+int32_t mt_buf_upload(mt_context* ctx, void* dst, const void* src, uint64_t bytes);
+int32_t mt_buf_download(mt_context* ctx, void* dst, const void* src, uint64_t bytes);
+int32_t mt_buf_fill_i32(mt_context* ctx, int32_t* dst, int32_t value, uint64_t count);
+
+/* ------------------------------------------------------------------ */
+/* Host-side scene preparation                                        */
+/* ------------------------------------------------------------------ */
+
+/* CNormalBVH.build (src/cl/scene/CNormalBVH.java:45-148): longest-axis
+ * centroid-midpoint split, one primitive per leaf, 2N-1 nodes allocated
+ * depth-first with sibling pairs adjacent.  Host arrays; nthreads <= 0
+ * uses all cores.  max_depth receives the deepest leaf (root = 0). */
+int32_t mt_bvh_build(int32_t ntris, const float* points, const int32_t* faces,
+                     int32_t* nodes, float* bounds, int32_t* max_depth,
+                     int32_t nthreads);
+
+/* Seeded synthetic inputs (no counterpart in the reference; they make the
+ * benchmark workloads).  A "soup" is random triangles whose centroids are
+ * uniform in [c_lo, c_hi]^3 and whose two edges have uniformly random
+ * directions and lengths uniform in [e_lo, e_hi]; vertex 0 is
+ * c - (e1 + e2)/3.  Outputs are host arrays sized by the caller:
+ * points float4[3*n], faces int32[10*n] (uv/normal indices -1, material
+ * `mat`), with vertex and face numbering starting at `first_vertex`. */
+int32_t mt_gen_soup(uint64_t seed, int32_t n, float c_lo, float c_hi,
+                    float e_lo, float e_hi, int32_t first_vertex, int32_t mat,
+                    float* points, int32_t* faces);
+/* The 12-triangle room box [-1,1]^3 with inward-facing triangles:
+ * points float4[8], faces int32[10*12]. */
+int32_t mt_gen_room(int32_t mat, float* points, int32_t* faces);
+/* Pinhole primary rays at pixel centres: origin pos, looking at `look`,
+ * up vector `up`, vertical field of view vfov_deg, w x h, tMin 0.001,
+ * tMax +inf (initGlobalRay, Util.cl:577-587). */
+int32_t mt_gen_camera_rays(const float* pos, const float* look, const float* up,
+                           float vfov_deg, int32_t w, int32_t h,
+                           float* o_tmin, float* d_tmax);
+/* Rays with origins uniform in the box [lo, hi] and directions uniform on
+ * the sphere (tMin 0.001, tMax +inf); index i depends only on (seed, i). */
+int32_t mt_gen_random_rays(uint64_t seed, int64_t first, int64_t n,
+                           const float* lo, const float* hi,
+                           float* o_tmin, float* d_tmax, int32_t nthreads);
+
+/* UV spheres: `count` spheres of 64 slices x 33 rings (4096 triangles,
+ * 2112 vertices with per-vertex normals each), radii lognormal(mu, sigma),
+ * centres uniform in [c_lo, c_hi]^3, material index = sphere index, plus a
+ * 2-triangle ground quad at y = ground_y spanning [-g, g]^2 with material
+ * `count`.  Sizes: points/normals float4[2112*count + 4],
+ * faces int32[10*(4096*count + 2)]. */
+int32_t mt_gen_spheres(uint64_t seed, int32_t count, float mu, float sigma,
+                       float c_lo, float c_hi, float ground_y, float g,
+                       float* points, float* normals, int32_t* faces);
+
+/* ------------------------------------------------------------------ */
+/* Device scene                                                       */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const float* points;   /* device float4[num_points] */
+    const float* normals;  /* device float4[], or NULL */
+    const float* uvs;      /* device float2[], or NULL */
+    const int32_t* faces;  /* device int32[10*num_tris] */
+    const int32_t* nodes;  /* device int32[7*(2*num_tris-1)] */
+    const float* bounds;   /* device float[8*(2*num_tris-1)] */
+    int32_t num_tris;
+    int32_t num_points;
+    int32_t max_depth;     /* from mt_bvh_build; > 128 is rejected */
+} mt_scene_desc;
+
+/* Validates the description and derives the packed traversal layout on the
+ * device (the reference-layout arrays are left untouched and stay
+ * readable).  CDeviceGI.set(mesh, bvh) + IntersectPrimitives' mesh/bvh
+ * arguments (CDeviceGI.java:172, 405-411). */
+int32_t mt_scene_create(mt_context* ctx, const mt_scene_desc* desc, mt_scene** out);
+int32_t mt_scene_destroy(mt_scene* scene);
+
+/* ------------------------------------------------------------------ */
+/* A. Traversal                                                       */
+/* ------------------------------------------------------------------ */
+#define MT_TRACE_PACKED 0    /* packed 64-B node / 48-B triangle layout (default) */
+#define MT_TRACE_REFERENCE 1 /* reads the reference's flat arrays directly */
+
+/* Closest hit (intersectMesh with bailout = false, NormalBVH.cl:14-92, as
+ * run by IntersectPrimitives :94-123; launched at CDeviceGI.java:230 and
+ * CDeviceRT.java:176).  prim[i] = hit primitive or -1; t[i] = the ray's
+ * final tMax (the hit distance on a hit).  Slots i >= *d_count are not
+ * written.  stats, if non-NULL, receives per-slot {interior steps, leaf
+ * visits, backtrack pops} as int32[3*capacity]. */
+int32_t mt_trace_closest(mt_context* ctx, const mt_scene* scene,
+                         const float* o_tmin, const float* d_tmax,
+                         const int32_t* d_count, int32_t capacity,
+                         int32_t* prim, float* t, int32_t* stats, int32_t mode);
+
+/* Any hit (intersectOcclusion, NormalBVH.cl:319-348; testOcclusion
+ * :235-317 inside DirectLight, Render.cl:456).  occluded[i] = 0/1. */
+int32_t mt_trace_any(mt_context* ctx, const mt_scene* scene,
+                     const float* o_tmin, const float* d_tmax,
+                     const int32_t* d_count, int32_t capacity,
+                     int32_t* occluded, int32_t* stats, int32_t mode);
+
+/* IntersectPrimitives' hit shading (NormalBVH.cl:124-232) for slots with
+ * prim >= 0: p (after the shadow-terminator offset), shading normal n,
+ * geometric normal ng (float4 each), uv (float2), material id (-1 on
+ * miss). */
+int32_t mt_shade_hits(mt_context* ctx, const mt_scene* scene,
+                      const float* o_tmin, const float* d_tmax,
+                      const int32_t* prim, const float* t,
+                      const int32_t* d_count, int32_t capacity,
+                      float* p, float* n, float* ng, float* uv, int32_t* mat);
+
+/* Shadow rays toward a point light from each hit (Render.cl:453 with
+ * Light.cl:93-97): origin getPoint(ray, t), direction (light - p)/dist,
+ * tMin 0.0001, tMax dist - 2*EPS_RAY.  valid[i] = prim[i] >= 0. */
+int32_t mt_shadow_rays(mt_context* ctx, const float* o_tmin, const float* d_tmax,
+                       const int32_t* prim, const float* t, const float* light_xyz,
+                       const int32_t* d_count, int32_t capacity,
+                       float* s_o_tmin, float* s_d_tmax, int32_t* valid);
+
+/* ------------------------------------------------------------------ */
+/* D. Scan and compaction                                             */
+/* ------------------------------------------------------------------ */
+
+/* PrefixSumInteger.execute (PrefixSumInteger.java:80-88; kernels
+ * Util.cl:948-980): out = exclusive scan of in[0..*d_len), d_total =
+ * in[n-1] + out[n-1].  Single pass (decoupled look-back). */
+int32_t mt_scan_excl_i32(mt_context* ctx, const int32_t* in, int32_t* out,
+                         const int32_t* d_len, int32_t capacity, int32_t* d_total);
+
+/* One payload stream moved by a compaction: element i of `src` (elem_bytes
+ * 4, 8 or 16) goes to slot rank(i) of `dst`. */
+typedef struct {
+    const void* src;
+    void* dst;
+    int32_t elem_bytes;
+} mt_payload;
+
+/* Stable stream compaction of slots [0, *d_len) with flag != 0:
+ * out_index[rank] = i (prepareLightInfo semantics, LightConfiguration.cl:
+ * 31-55) and each payload moved to its rank; *d_count = survivors.
+ * Replaces CCompact.execute's seven launches (CCompact.java:79-89) whose
+ * CompactAtomic (Compact.cl:21-40) yields the same set in atomic order.
+ * flag_mode 0: flags is int32 != 0; 1: flags is int32 >= 0 (a hit's
+ * primitive id). out_index may be NULL.  npayload <= 8. */
+int32_t mt_compact(mt_context* ctx, const int32_t* flags, int32_t flag_mode,
+                   const int32_t* d_len, int32_t capacity, int32_t* out_index,
+                   int32_t* d_count, const mt_payload* payloads, int32_t npayload);
+
+/* Synchronises the context and reports MT_ERR_STATE if a scan or compaction
+ * since the last check gave up waiting on a predecessor tile (a bounded spin
+ * that should never trigger); clears the flag.  No counterpart in the
+ * reference, whose scans have no error channel. */
+int32_t mt_scan_check(mt_context* ctx);
+
+/* identifyMeshLights + PrefixSumInteger + prepareLightInfo
+ * (CLightConfiguration.initLights, CLightConfiguration.java:43-87):
+ * light_faces[k] = k-th face (in index order) whose material is an emitter
+ * (Luminance(emission_color * emission_param.x) != 0, Material.cl:27-31). */
+int32_t mt_mesh_lights(mt_context* ctx, const mt_scene* scene, const float* materials,
+                       int32_t* light_faces, int32_t* d_count);
+
+/* ------------------------------------------------------------------ */
+/* C. Adaptive light grid and environment SATs                        */
+/* ------------------------------------------------------------------ */
+
+/* LightGrid (LightGrid.cl:9-37): nu = nv = 1600, tiles 16 x 32, 100 x 50
+ * cells; accum/func/sat are float[2 560 000] each, row-major 1600 wide. */
+int32_t mt_lightgrid_create(mt_context* ctx, mt_lightgrid** out);
+int32_t mt_lightgrid_destroy(mt_lightgrid* grid);
+int32_t mt_lightgrid_arrays(mt_lightgrid* grid, float** accum, float** func, float** sat);
+/* initLightGrid (LightGrid.cl:404-410; CAdaptiveEnvironment.clearBuffers :123-127) */
+int32_t mt_lightgrid_clear(mt_context* ctx, mt_lightgrid* grid);
+/* setToAllFunc + row/col SAT (CAdaptiveEnvironment.setHDRLuminance :69-88):
+ * copies the 512-entry tile `tile` (host, x fastest) into every cell, then
+ * builds the SAT. */
+int32_t mt_lightgrid_set_tile_func(mt_context* ctx, mt_lightgrid* grid, const float* tile);
+/* CAdaptiveEnvironment.update (:135-145): func <- accum, then the tile SAT
+ * (initFuncAndSat, calculateFunc, prefixSumRowSubgrid, prefixSumColSubgrid
+ * in one kernel). */
+int32_t mt_lightgrid_update(mt_context* ctx, mt_lightgrid* grid);
+/* prefixSumRowSubgrid + prefixSumColSubgrid on the current func. */
+int32_t mt_lightgrid_build_sat(mt_context* ctx, mt_lightgrid* grid);
+/* sampleSubgridContinuous (LightGrid.cl:310-320) for n (cell, u0, u1). */
+int32_t mt_lightgrid_sample(mt_context* ctx, const mt_lightgrid* grid, int32_t n,
+                            const int32_t* cell, const float* u, int32_t* tile_xy,
+                            float* uv, float* pdf);
+/* subgridIndexFromCamera / tileIndexXYFromCamera (LightGrid.cl:68-90). */
+int32_t mt_lightgrid_cells(mt_context* ctx, int32_t n, const float* dirs,
+                           int32_t* cell, int32_t* tile_xy);
+/* Generic tile SAT over a nu x nv grid with rx-wide row and ry-tall column
+ * segments (the kernels above on other grid sizes, e.g. the 16x16 grid of
+ * test/lightgrid/SimpleGrid.cl). rx, ry powers of two <= 64. */
+int32_t mt_sat_tiles(mt_context* ctx, const float* func, float* sat,
+                     int32_t nu, int32_t nv, int32_t rx, int32_t ry);
+
+/* Environment map: rgba float4[w*h]; lum = Luminance(rgb)
+ * (InitEnvironmentLum, Initialize.cl:100-115); sat = calculateSAT order
+ * (SAT.cl:304-310: sequential rows, then sequential columns). */
+int32_t mt_env_lum_sat(mt_context* ctx, const float* rgba, int32_t w, int32_t h,
+                       float* lum, float* sat);
+/* Holds device pointers to rgba/lum/sat (caller-owned) for the light calls. */
+int32_t mt_env_create(mt_context* ctx, const float* rgba, const float* lum,
+                      const float* sat, int32_t w, int32_t h, mt_envmap** out);
+int32_t mt_env_destroy(mt_envmap* env);
+/* sampleContinuous on a sub-region (SAT.cl:244-323). */
+int32_t mt_sat_sample_region(mt_context* ctx, const float* func, const float* sat,
+                             int32_t nu, int32_t nv, int32_t minx, int32_t miny,
+                             int32_t sx, int32_t sy, int32_t n, const float* u,
+                             float* uv, int32_t* offset, float* pdf);
+/* illuminateEnvironmentLight (Light.cl:195-227). */
+int32_t mt_env_illuminate(mt_context* ctx, const mt_envmap* env, int32_t n,
+                          const float* u, float* dir, float* contrib, float* pdfw,
+                          int32_t* texel);
+/* getRadianceEnvironmentLight (Light.cl:270-286). */
+int32_t mt_env_radiance(mt_context* ctx, const mt_envmap* env, int32_t n,
+                        const float* dir, float* contrib, float* pdfw, int32_t* texel);
+
+/* ------------------------------------------------------------------ */
+/* B. BSDF                                                            */
+/* ------------------------------------------------------------------ */
+#define MT_BSDF_WORDS 20 /* frame mX,mY,mZ (9), localDirFix (3), localGeomN (3),
+                            diffProb, glossyProb, 3 reserved */
+/* setupBsdf (Bsdf.cl:130-160) for n hits: bsdf float[20*n], mat_id (-1 if
+ * the incoming direction is grazing). */
+int32_t mt_bsdf_setup(mt_context* ctx, int32_t n, const float* ray_d,
+                      const float* isect_n, const float* isect_ng, const int32_t* mat,
+                      const float* materials, float* bsdf, int32_t* mat_id);
+/* SampleBsdf (Bsdf.cl:555-606): u float[3n]; f, dir float[3n]; pdf and
+ * cos float[n]. */
+int32_t mt_bsdf_sample(mt_context* ctx, int32_t n, const float* bsdf, const int32_t* mat_id,
+                       const float* materials, const float* u, float* f, float* dir,
+                       float* pdf, float* cos_out);
+/* EvaluateBsdf (Bsdf.cl:525-545): dir float[3n] world directions. */
+int32_t mt_bsdf_eval(mt_context* ctx, int32_t n, const float* bsdf, const int32_t* mat_id,
+                     const float* materials, const float* dir, float* f, float* pdf,
+                     float* cos_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,508 @@
+"""Python host mirror of the reference's operator interface for the hot path.
+
+Each class mirrors the Java host driver the C ABI replaces (same names, same
+argument meaning); device memory is torch tensors on the context's device
+and every call goes through libmambart.so.  There is no CPU fallback: a
+missing library or a non-zero status raises.
+
+  Context             wrapper.core.OpenCLConfiguration (one queue per device)
+  CNormalBVH          src/cl/scene/CNormalBVH.java:45-148 (host builder)
+  Scene               CMesh buffers + CNormalBVH nodes/bounds on the device
+  intersect_primitives / intersect_occlusion
+                      IntersectPrimitives / intersectOcclusion
+                      (src/cl/kernel/NormalBVH.cl:94-123,319-348)
+  PrefixSumInteger    src/cl/algorithms/PrefixSumInteger.java:43-117
+  CCompact            src/cl/algorithms/CCompact.java:30-89
+  CAdaptiveEnvironment src/cl/algorithms/CAdaptiveEnvironment.java:37-145
+  CEnvironment        src/cl/algorithms/CEnvironment.java:46-60
+  bsdf_setup / bsdf_sample / bsdf_eval
+                      setupBsdf / SampleBsdf / EvaluateBsdf (Bsdf.cl)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import call
+
+LG_SIZE = 1600 * 1600
+
+
+def _ptr(x) -> Optional[int]:
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        assert x.flags["C_CONTIGUOUS"], "numpy array must be C-contiguous"
+        return x.ctypes.data
+    return x.data_ptr()
+
+
+def _torch():
+    import torch
+    return torch
+
+
+# ---------------------------------------------------------------------- context
+class Context:
+    """One device, one in-order stream (torch's current stream by default)."""
+
+    def __init__(self, device: int = 0, stream=None):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("libmambart needs a HIP device; none is visible")
+        self.device = device
+        self.torch_device = torch.device("cuda", device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.torch_device)
+        self.stream = stream
+        h = C.c_void_p()
+        # the library issues its work on torch's stream (handle 0 = null stream),
+        # so torch copies and library kernels are ordered without extra syncs
+        call("mt_ctx_create", device, C.c_void_p(stream.cuda_stream), 0, C.byref(h))
+        self.handle = h
+
+    def sync(self) -> None:
+        call("mt_ctx_sync", self.handle)
+
+    def check_scans(self) -> None:
+        call("mt_scan_check", self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.load().mt_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # helpers -------------------------------------------------------------
+    def empty(self, shape, dtype):
+        return _torch().empty(shape, dtype=dtype, device=self.torch_device)
+
+    def to_device(self, arr: np.ndarray):
+        return _torch().from_numpy(np.ascontiguousarray(arr)).to(self.torch_device)
+
+    def scalar_i32(self, value: int):
+        return _torch().tensor([value], dtype=_torch().int32, device=self.torch_device)
+
+
+# ---------------------------------------------------------------------- host-side
+class CNormalBVH:
+    """CNormalBVH.build on the host (C++, multi-threaded, identical numbering)."""
+
+    @staticmethod
+    def build(points: np.ndarray, faces: np.ndarray, nthreads: int = 0):
+        points = np.ascontiguousarray(points, dtype=np.float32)
+        faces = np.ascontiguousarray(faces, dtype=np.int32)
+        n = faces.shape[0]
+        nodes = np.zeros((2 * n - 1, 7), dtype=np.int32)
+        bounds = np.zeros((2 * n - 1, 8), dtype=np.float32)
+        depth = C.c_int32(0)
+        call("mt_bvh_build", n, _ptr(points), _ptr(faces), _ptr(nodes), _ptr(bounds),
+             C.byref(depth), nthreads)
+        return nodes, bounds, int(depth.value)
+
+
+def gen_soup(seed: int, n: int, c_lo: float, c_hi: float, e_lo: float, e_hi: float,
+             first_vertex: int = 0, mat: int = 0):
+    points = np.zeros((3 * n, 4), dtype=np.float32)
+    faces = np.zeros((n, 10), dtype=np.int32)
+    call("mt_gen_soup", seed, n, c_lo, c_hi, e_lo, e_hi, first_vertex, mat, _ptr(points), _ptr(faces))
+    return points, faces
+
+
+def gen_room(mat: int = 0):
+    points = np.zeros((8, 4), dtype=np.float32)
+    faces = np.zeros((12, 10), dtype=np.int32)
+    call("mt_gen_room", mat, _ptr(points), _ptr(faces))
+    return points, faces
+
+
+def gen_camera_rays(pos, look, up, vfov_deg: float, w: int, h: int):
+    o = np.zeros((w * h, 4), dtype=np.float32)
+    d = np.zeros((w * h, 4), dtype=np.float32)
+    p3 = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    pos, look, up = p3(pos), p3(look), p3(up)
+    call("mt_gen_camera_rays", _ptr(pos), _ptr(look), _ptr(up), vfov_deg, w, h, _ptr(o), _ptr(d))
+    return o, d
+
+
+def gen_random_rays(seed: int, first: int, n: int, lo, hi, nthreads: int = 0):
+    o = np.zeros((n, 4), dtype=np.float32)
+    d = np.zeros((n, 4), dtype=np.float32)
+    lo = np.ascontiguousarray(lo, dtype=np.float32)
+    hi = np.ascontiguousarray(hi, dtype=np.float32)
+    call("mt_gen_random_rays", seed, first, n, _ptr(lo), _ptr(hi), _ptr(o), _ptr(d), nthreads)
+    return o, d
+
+
+def gen_spheres(seed: int, count: int, mu: float = -2.5, sigma: float = 0.5, c_lo: float = -5.0,
+                c_hi: float = 5.0, ground_y: float = -5.0, g: float = 50.0):
+    nv = 2112 * count + 4
+    nt = 4096 * count + 2
+    points = np.zeros((nv, 4), dtype=np.float32)
+    normals = np.zeros((nv, 4), dtype=np.float32)
+    faces = np.zeros((nt, 10), dtype=np.int32)
+    call("mt_gen_spheres", seed, count, mu, sigma, c_lo, c_hi, ground_y, g, _ptr(points),
+         _ptr(normals), _ptr(faces))
+    return points, normals, faces
+
+
+# ---------------------------------------------------------------------- scene
+class Scene:
+    """Scene arrays (reference flat layout) resident on the device."""
+
+    def __init__(self, ctx: Context, points: np.ndarray, faces: np.ndarray, nodes: np.ndarray,
+                 bounds: np.ndarray, max_depth: int, normals: Optional[np.ndarray] = None,
+                 uvs: Optional[np.ndarray] = None):
+        torch = _torch()
+        self.ctx = ctx
+        self.points = ctx.to_device(np.asarray(points, dtype=np.float32))
+        self.faces = ctx.to_device(np.asarray(faces, dtype=np.int32))
+        self.nodes = ctx.to_device(np.asarray(nodes, dtype=np.int32))
+        self.bounds = ctx.to_device(np.asarray(bounds, dtype=np.float32))
+        self.normals = None if normals is None else ctx.to_device(np.asarray(normals, dtype=np.float32))
+        self.uvs = None if uvs is None else ctx.to_device(np.asarray(uvs, dtype=np.float32))
+        self.num_tris = int(faces.shape[0])
+        desc = _lib.SceneDesc(
+            _ptr(self.points), _ptr(self.normals), _ptr(self.uvs), _ptr(self.faces),
+            _ptr(self.nodes), _ptr(self.bounds), self.num_tris, int(points.shape[0]), int(max_depth))
+        h = C.c_void_p()
+        torch.cuda.synchronize(ctx.torch_device)
+        call("mt_scene_create", ctx.handle, C.byref(desc), C.byref(h))
+        self.handle = h
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            _lib.load().mt_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def intersect_primitives(ctx: Context, scene: Scene, o_tmin, d_tmax, count=None, mode: int = 0,
+                         stats: bool = False):
+    """Closest hit: returns (prim int32[n], t float32[n], stats int32[n,3] or None)."""
+    torch = _torch()
+    n = o_tmin.shape[0]
+    if count is None:
+        count = ctx.scalar_i32(n)
+    prim = ctx.empty((n,), torch.int32)
+    t = ctx.empty((n,), torch.float32)
+    st = ctx.empty((n, 3), torch.int32) if stats else None
+    call("mt_trace_closest", ctx.handle, scene.handle, _ptr(o_tmin), _ptr(d_tmax), _ptr(count), n,
+         _ptr(prim), _ptr(t), _ptr(st), mode)
+    return prim, t, st
+
+
+def intersect_occlusion(ctx: Context, scene: Scene, o_tmin, d_tmax, count=None, mode: int = 0,
+                        stats: bool = False):
+    """Any hit: returns (occluded int32[n], stats or None)."""
+    torch = _torch()
+    n = o_tmin.shape[0]
+    if count is None:
+        count = ctx.scalar_i32(n)
+    occ = ctx.empty((n,), torch.int32)
+    st = ctx.empty((n, 3), torch.int32) if stats else None
+    call("mt_trace_any", ctx.handle, scene.handle, _ptr(o_tmin), _ptr(d_tmax), _ptr(count), n,
+         _ptr(occ), _ptr(st), mode)
+    return occ, st
+
+
+def shade_hits(ctx: Context, scene: Scene, o_tmin, d_tmax, prim, t, count=None):
+    torch = _torch()
+    n = o_tmin.shape[0]
+    if count is None:
+        count = ctx.scalar_i32(n)
+    p = ctx.empty((n, 4), torch.float32)
+    nn = ctx.empty((n, 4), torch.float32)
+    ng = ctx.empty((n, 4), torch.float32)
+    uv = ctx.empty((n, 2), torch.float32)
+    mat = ctx.empty((n,), torch.int32)
+    call("mt_shade_hits", ctx.handle, scene.handle, _ptr(o_tmin), _ptr(d_tmax), _ptr(prim), _ptr(t),
+         _ptr(count), n, _ptr(p), _ptr(nn), _ptr(ng), _ptr(uv), _ptr(mat))
+    return p, nn, ng, uv, mat
+
+
+def shadow_rays(ctx: Context, o_tmin, d_tmax, prim, t, light_xyz, count=None):
+    torch = _torch()
+    n = o_tmin.shape[0]
+    if count is None:
+        count = ctx.scalar_i32(n)
+    so = ctx.empty((n, 4), torch.float32)
+    sd = ctx.empty((n, 4), torch.float32)
+    valid = ctx.empty((n,), torch.int32)
+    light = np.ascontiguousarray(light_xyz, dtype=np.float32)
+    call("mt_shadow_rays", ctx.handle, _ptr(o_tmin), _ptr(d_tmax), _ptr(prim), _ptr(t), _ptr(light),
+         _ptr(count), n, _ptr(so), _ptr(sd), _ptr(valid))
+    return so, sd, valid
+
+
+# ---------------------------------------------------------------------- D
+class PrefixSumInteger:
+    """PrefixSumInteger (PrefixSumInteger.java:43-117): exclusive scan + total."""
+
+    def __init__(self, ctx: Context, cdata):
+        torch = _torch()
+        self.ctx = ctx
+        self.cdata = cdata
+        self.length = int(cdata.shape[0])
+        self.clength = ctx.scalar_i32(self.length)
+        self.cprefixsum = ctx.empty((self.length,), torch.int32)
+        self.ctotal = ctx.scalar_i32(0)
+
+    def execute(self) -> None:
+        call("mt_scan_excl_i32", self.ctx.handle, _ptr(self.cdata), _ptr(self.cprefixsum),
+             _ptr(self.clength), self.length, _ptr(self.ctotal))
+
+    def getPrefixSum(self):
+        return self.cprefixsum
+
+    def getCTotal(self):
+        return self.ctotal
+
+    def getCount(self) -> int:
+        self.ctx.check_scans()
+        return int(self.ctotal.item())
+
+
+def compact(ctx: Context, flags, flag_mode: int = 0, length=None, out_index=True,
+            payloads: Sequence = ()):
+    """Stable compaction; returns (out_index or None, d_count, [dst tensors])."""
+    torch = _torch()
+    n = int(flags.shape[0])
+    if length is None:
+        length = ctx.scalar_i32(n)
+    idx = ctx.empty((n,), torch.int32) if out_index else None
+    count = ctx.scalar_i32(0)
+    arr = (_lib.Payload * max(1, len(payloads)))()
+    dsts = []
+    for j, src in enumerate(payloads):
+        dst = torch.empty_like(src)
+        eb = src.element_size() * (src[0].numel() if src.dim() > 1 else 1)
+        arr[j] = _lib.Payload(_ptr(src), _ptr(dst), eb)
+        dsts.append(dst)
+    call("mt_compact", ctx.handle, _ptr(flags), flag_mode, _ptr(length), n, _ptr(idx), _ptr(count),
+         arr, len(payloads))
+    return idx, count, dsts
+
+
+class CCompact:
+    """CCompact (CCompact.java:30-89): compacts hit slots and their pixel ids.
+
+    init() takes the per-slot hit flags (int32, Intersection.hit), the pixel
+    index queue and the device count; execute() leaves the survivors' pixel
+    ids (and any extra payload streams) in slot order 0..count-1 and the
+    survivor count in totalCount, like the seven reference launches."""
+
+    def init(self, ctx: Context, hit_flags, pixels, totalCount, extra: Sequence = ()):
+        self.ctx = ctx
+        self.hit = hit_flags
+        self.pixels = pixels
+        self.totalCount = totalCount
+        self.extra = list(extra)
+        self.length = ctx.scalar_i32(int(hit_flags.shape[0]))
+
+    def execute(self) -> None:
+        ctx = self.ctx
+        streams = [self.pixels, *self.extra]
+        _, cnt, dsts = compact(ctx, self.hit, 0, length=self.length, out_index=False, payloads=streams)
+        for src, dst in zip(streams, dsts):
+            src.copy_(dst)
+        self.totalCount.copy_(cnt)
+
+
+def mesh_lights(ctx: Context, scene: Scene, materials):
+    torch = _torch()
+    out = ctx.empty((scene.num_tris,), torch.int32)
+    cnt = ctx.scalar_i32(0)
+    call("mt_mesh_lights", ctx.handle, scene.handle, _ptr(materials), _ptr(out), _ptr(cnt))
+    return out, cnt
+
+
+# ---------------------------------------------------------------------- C
+class CAdaptiveEnvironment:
+    """CAdaptiveEnvironment (CAdaptiveEnvironment.java:37-145) over the 1600^2 grid."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = C.c_void_p()
+        call("mt_lightgrid_create", ctx.handle, C.byref(h))
+        self.handle = h
+        torch = _torch()
+        a, f, s = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        call("mt_lightgrid_arrays", h, C.byref(a), C.byref(f), C.byref(s))
+        self._ptrs = (a.value, f.value, s.value)
+
+    def read_arrays(self):
+        """(accum, func, sat) copied to host numpy arrays (1600 x 1600 each)."""
+        out = []
+        for p in self._ptrs:
+            a = np.empty((1600, 1600), dtype=np.float32)
+            call("mt_buf_download", self.ctx.handle, _ptr(a), C.c_void_p(p), a.nbytes)
+            out.append(a)
+        return tuple(out)
+
+    def write_accum(self, accum: np.ndarray) -> None:
+        """Host -> grid accum (what accumLightGrid gathers on the device, Light.cl:35-38)."""
+        a = np.ascontiguousarray(accum, dtype=np.float32).reshape(-1)
+        assert a.size == LG_SIZE
+        call("mt_buf_upload", self.ctx.handle, C.c_void_p(self._ptrs[0]), _ptr(a), a.nbytes)
+
+    def device_arrays(self):
+        """Raw device pointers (accum, func, sat) for other library calls."""
+        return self._ptrs
+
+    def clearBuffers(self) -> None:
+        call("mt_lightgrid_clear", self.ctx.handle, self.handle)
+
+    def setHDRLuminance(self, tile: np.ndarray) -> None:
+        tile = np.ascontiguousarray(tile, dtype=np.float32).reshape(-1)
+        assert tile.size == 512
+        self.clearBuffers()
+        call("mt_lightgrid_set_tile_func", self.ctx.handle, self.handle, _ptr(tile))
+
+    def update(self) -> None:
+        call("mt_lightgrid_update", self.ctx.handle, self.handle)
+
+    def buildSAT(self) -> None:
+        call("mt_lightgrid_build_sat", self.ctx.handle, self.handle)
+
+    def sample(self, cell, u):
+        torch = _torch()
+        n = int(cell.shape[0])
+        txy = self.ctx.empty((n, 2), torch.int32)
+        uv = self.ctx.empty((n, 2), torch.float32)
+        pdf = self.ctx.empty((n,), torch.float32)
+        call("mt_lightgrid_sample", self.ctx.handle, self.handle, n, _ptr(cell), _ptr(u), _ptr(txy),
+             _ptr(uv), _ptr(pdf))
+        return txy, uv, pdf
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().mt_lightgrid_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def lightgrid_cells(ctx: Context, dirs):
+    torch = _torch()
+    n = int(dirs.shape[0])
+    cell = ctx.empty((n,), torch.int32)
+    txy = ctx.empty((n, 2), torch.int32)
+    call("mt_lightgrid_cells", ctx.handle, n, _ptr(dirs), _ptr(cell), _ptr(txy))
+    return cell, txy
+
+
+def sat_tiles(ctx: Context, func, nu: int, nv: int, rx: int, ry: int):
+    sat = _torch().empty_like(func)
+    call("mt_sat_tiles", ctx.handle, _ptr(func), _ptr(sat), nu, nv, rx, ry)
+    return sat
+
+
+class CEnvironment:
+    """CEnvironment.setEnvironmentMap (CEnvironment.java:46-60): env RGB, luminance, SAT."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.handle = None
+
+    def setEnvironmentMap(self, rgba: np.ndarray) -> None:
+        torch = _torch()
+        h, w = rgba.shape[0], rgba.shape[1]
+        self.w, self.h = int(w), int(h)
+        self.rgba = self.ctx.to_device(np.asarray(rgba, dtype=np.float32).reshape(h * w, 4))
+        self.lum = self.ctx.empty((h * w,), torch.float32)
+        self.sat = self.ctx.empty((h * w,), torch.float32)
+        call("mt_env_lum_sat", self.ctx.handle, _ptr(self.rgba), self.w, self.h, _ptr(self.lum),
+             _ptr(self.sat))
+        self.close()
+        hd = C.c_void_p()
+        call("mt_env_create", self.ctx.handle, _ptr(self.rgba), _ptr(self.lum), _ptr(self.sat), self.w,
+             self.h, C.byref(hd))
+        self.handle = hd
+
+    def illuminate(self, u):
+        torch = _torch()
+        n = int(u.shape[0])
+        d = self.ctx.empty((n, 4), torch.float32)
+        c = self.ctx.empty((n, 4), torch.float32)
+        pw = self.ctx.empty((n,), torch.float32)
+        tx = self.ctx.empty((n,), torch.int32)
+        call("mt_env_illuminate", self.ctx.handle, self.handle, n, _ptr(u), _ptr(d), _ptr(c), _ptr(pw),
+             _ptr(tx))
+        return d, c, pw, tx
+
+    def radiance(self, dirs):
+        torch = _torch()
+        n = int(dirs.shape[0])
+        c = self.ctx.empty((n, 4), torch.float32)
+        pw = self.ctx.empty((n,), torch.float32)
+        tx = self.ctx.empty((n,), torch.int32)
+        call("mt_env_radiance", self.ctx.handle, self.handle, n, _ptr(dirs), _ptr(c), _ptr(pw), _ptr(tx))
+        return c, pw, tx
+
+    def close(self):
+        if self.handle:
+            _lib.load().mt_env_destroy(self.handle)
+            self.handle = None
+
+
+def sat_sample_region(ctx: Context, func, sat, nu, nv, minx, miny, sx, sy, u):
+    torch = _torch()
+    n = int(u.shape[0])
+    uv = ctx.empty((n, 2), torch.float32)
+    off = ctx.empty((n, 2), torch.int32)
+    pdf = ctx.empty((n,), torch.float32)
+    call("mt_sat_sample_region", ctx.handle, _ptr(func), _ptr(sat), nu, nv, minx, miny, sx, sy, n, _ptr(u),
+         _ptr(uv), _ptr(off), _ptr(pdf))
+    return uv, off, pdf
+
+
+# ---------------------------------------------------------------------- B
+def bsdf_setup(ctx: Context, ray_d, n, ng, mat, materials):
+    torch = _torch()
+    cnt = int(ray_d.shape[0])
+    rec = ctx.empty((cnt, _lib.MT_BSDF_WORDS), torch.float32)
+    mid = ctx.empty((cnt,), torch.int32)
+    call("mt_bsdf_setup", ctx.handle, cnt, _ptr(ray_d), _ptr(n), _ptr(ng), _ptr(mat), _ptr(materials),
+         _ptr(rec), _ptr(mid))
+    return rec, mid
+
+
+def bsdf_sample(ctx: Context, rec, mat_id, materials, u):
+    torch = _torch()
+    cnt = int(rec.shape[0])
+    f = ctx.empty((cnt, 3), torch.float32)
+    d = ctx.empty((cnt, 3), torch.float32)
+    pdf = ctx.empty((cnt,), torch.float32)
+    cs = ctx.empty((cnt,), torch.float32)
+    call("mt_bsdf_sample", ctx.handle, cnt, _ptr(rec), _ptr(mat_id), _ptr(materials), _ptr(u), _ptr(f),
+         _ptr(d), _ptr(pdf), _ptr(cs))
+    return f, d, pdf, cs
+
+
+def bsdf_eval(ctx: Context, rec, mat_id, materials, dirs):
+    torch = _torch()
+    cnt = int(rec.shape[0])
+    f = ctx.empty((cnt, 3), torch.float32)
+    pdf = ctx.empty((cnt,), torch.float32)
+    cs = ctx.empty((cnt,), torch.float32)
+    call("mt_bsdf_eval", ctx.handle, cnt, _ptr(rec), _ptr(mat_id), _ptr(materials), _ptr(dirs), _ptr(f),
+         _ptr(pdf), _ptr(cs))
+    return f, pdf, cs

@@ -1,0 +1,170 @@
+// Context, device buffers and error reporting.
+//
+// Replaces the un-vendored wrapper.core.OpenCLConfiguration / CMemory layer
+// the Java host uses to create buffers, move data and finish the queue
+// (e.g. src/cl/device/CDeviceGI.java:150-179 and :428).  One context owns one
+// HIP stream on one device, mirroring the reference's one in-order OpenCL
+// queue per device.
+#include "mt_internal.h"
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+__global__ void k_fill_i32(int32_t* dst, int32_t v, uint64_t n)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = v;
+}
+
+static thread_local char g_err[512] = "";
+
+void mt_set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" int32_t mt_last_error(char* buf, int32_t len)
+{
+    if (!buf || len <= 0) return MT_ERR_ARG;
+    snprintf(buf, (size_t)len, "%s", g_err);
+    return MT_OK;
+}
+
+extern "C" const char* mt_version(void) { return "mambart 0.1 gfx950"; }
+
+extern "C" int32_t mt_ctx_create(int32_t device, void* stream, int32_t flags, mt_context** out)
+{
+    MT_ARG_CHECK(out != nullptr, "mt_ctx_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    MT_HIP_CHECK(hipGetDeviceCount(&ndev));
+    MT_ARG_CHECK(device >= 0 && device < ndev, "mt_ctx_create: device %d not in [0,%d)", device, ndev);
+    MT_HIP_CHECK(hipSetDevice(device));
+    mt_context* c = new (std::nothrow) mt_context();
+    if (!c) {
+        mt_set_error("mt_ctx_create: out of host memory");
+        return MT_ERR_ALLOC;
+    }
+    c->device = device;
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+    c->epoch = 0;
+    if (!(flags & MT_CTX_OWN_STREAM)) {
+        c->stream = (hipStream_t)stream;
+        c->own_stream = false;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            mt_set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
+            return MT_ERR_HIP;
+        }
+        c->own_stream = true;
+    }
+    *out = c;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_ctx_destroy(mt_context* ctx)
+{
+    if (!ctx) return MT_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    if (ctx->scratch) hipFree(ctx->scratch);
+    if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_ctx_sync(mt_context* ctx)
+{
+    MT_ARG_CHECK(ctx != nullptr, "mt_ctx_sync: ctx is NULL");
+    MT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MT_OK;
+}
+
+int32_t mt_ctx_scratch(mt_context* ctx, size_t bytes, void** out)
+{
+    if (bytes <= ctx->scratch_bytes) {
+        *out = ctx->scratch;
+        return MT_OK;
+    }
+    size_t want = bytes < (size_t)1 << 20 ? (size_t)1 << 20 : bytes;
+    MT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (ctx->scratch) {
+        MT_HIP_CHECK(hipFree(ctx->scratch));
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        mt_set_error("scratch allocation of %zu bytes failed: %s", want, hipGetErrorString(e));
+        return MT_ERR_ALLOC;
+    }
+    // Status words start at epoch 0; every call uses a fresh non-zero epoch.
+    // Cleared with a fill kernel rather than a memset node.
+    uint64_t words = want / 4;
+    int blocks = (int)((words + 255) / 256 < 65536 ? (words + 255) / 256 : 65536);
+    k_fill_i32<<<blocks, 256, 0, ctx->stream>>>((int32_t*)p, 0, words);
+    MT_LAUNCH_CHECK();
+    ctx->scratch = p;
+    ctx->scratch_bytes = want;
+    *out = p;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_buf_alloc(mt_context* ctx, uint64_t bytes, void** dptr)
+{
+    MT_ARG_CHECK(ctx && dptr, "mt_buf_alloc: NULL argument");
+    *dptr = nullptr;
+    MT_HIP_CHECK(hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 4);
+    if (e != hipSuccess) {
+        mt_set_error("hipMalloc(%llu) failed: %s", (unsigned long long)bytes, hipGetErrorString(e));
+        return MT_ERR_ALLOC;
+    }
+    return MT_OK;
+}
+
+extern "C" int32_t mt_buf_free(mt_context* ctx, void* dptr)
+{
+    MT_ARG_CHECK(ctx != nullptr, "mt_buf_free: ctx is NULL");
+    if (!dptr) return MT_OK;
+    MT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    MT_HIP_CHECK(hipFree(dptr));
+    return MT_OK;
+}
+
+extern "C" int32_t mt_buf_upload(mt_context* ctx, void* dst, const void* src, uint64_t bytes)
+{
+    MT_ARG_CHECK(ctx && dst && src, "mt_buf_upload: NULL argument");
+    MT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    MT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MT_OK;
+}
+
+extern "C" int32_t mt_buf_download(mt_context* ctx, void* dst, const void* src, uint64_t bytes)
+{
+    MT_ARG_CHECK(ctx && dst && src, "mt_buf_download: NULL argument");
+    MT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    MT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MT_OK;
+}
+
+extern "C" int32_t mt_buf_fill_i32(mt_context* ctx, int32_t* dst, int32_t value, uint64_t count)
+{
+    MT_ARG_CHECK(ctx && dst, "mt_buf_fill_i32: NULL argument");
+    if (count == 0) return MT_OK;
+    int blocks = (int)((count + 255) / 256 < 65536 ? (count + 255) / 256 : 65536);
+    k_fill_i32<<<blocks, 256, 0, ctx->stream>>>(dst, value, count);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}

@@ -1,0 +1,93 @@
+// Internal declarations shared by the libmambart translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/mambart.h"
+
+// ---------------------------------------------------------------- errors
+void mt_set_error(const char* fmt, ...);
+
+#define MT_HIP_CHECK(call)                                                   \
+    do {                                                                     \
+        hipError_t mt_e_ = (call);                                           \
+        if (mt_e_ != hipSuccess) {                                           \
+            mt_set_error("%s failed: %s (%s:%d)", #call,                     \
+                         hipGetErrorString(mt_e_), __FILE__, __LINE__);      \
+            return MT_ERR_HIP;                                               \
+        }                                                                    \
+    } while (0)
+
+#define MT_ARG_CHECK(cond, ...)                                              \
+    do {                                                                     \
+        if (!(cond)) {                                                       \
+            mt_set_error(__VA_ARGS__);                                       \
+            return MT_ERR_ARG;                                               \
+        }                                                                    \
+    } while (0)
+
+#define MT_LAUNCH_CHECK() MT_HIP_CHECK(hipGetLastError())
+
+// ---------------------------------------------------------------- objects
+struct mt_context {
+    int device;
+    hipStream_t stream;
+    bool own_stream;
+    // Device workspace reused by the scan/compaction look-back (tile status
+    // words and the tile ticket).  Grown on demand; calls on one context are
+    // serialised on its stream, so one workspace suffices.
+    void* scratch;
+    size_t scratch_bytes;
+    uint32_t epoch;   // tags look-back status words so they never need clearing
+};
+
+// Packed traversal layout derived from the reference arrays.
+//   node (64 B): float4 a = (L.min.xyz, L.max.x), b = (L.max.yz, R.min.xy),
+//                float4 c = (R.min.z, R.max.xyz), int4 d = (left, right, parent, sibling)
+//   refs: >= 0 interior node (packed index), < 0 leaf ~rank
+//   tri  (48 B): float4 (p1.xyz, e1.x), (e1.yz, e2.xy), (e2.z, prim bits, 0, 0)
+struct mt_scene {
+    mt_context* ctx;
+    mt_scene_desc desc;
+    int32_t num_nodes;     // 2N - 1
+    int32_t num_interior;  // N - 1
+    float4* pnodes;        // [max(num_interior,1) * 4]
+    float4* ptris;         // [num_tris * 3]
+    int32_t root_ref;      // 0, or ~0 when the tree is a single leaf
+};
+
+struct mt_lightgrid {
+    mt_context* ctx;
+    float* accum;
+    float* func;
+    float* sat;
+};
+
+struct mt_envmap {
+    mt_context* ctx;
+    const float* rgba;
+    const float* lum;
+    const float* sat;
+    int32_t w, h;
+};
+
+int32_t mt_ctx_scratch(mt_context* ctx, size_t bytes, void** out);
+
+// Light-grid geometry (LightGrid.cl:9-37, CLightGrid.java:47-58)
+constexpr int LG_NU = 1600;
+constexpr int LG_NV = 1600;
+constexpr int LG_RX = 16;
+constexpr int LG_RY = 32;
+constexpr int LG_CX = LG_NU / LG_RX;  // 100
+constexpr int LG_CY = LG_NV / LG_RY;  // 50
+constexpr int LG_CELLS = LG_CX * LG_CY;
+constexpr int LG_SIZE = LG_NU * LG_NV;  // 2 560 000
+
+inline int mt_blocks(int64_t n, int threads)
+{
+    int64_t b = (n + threads - 1) / threads;
+    return b < 1 ? 1 : (int)b;
+}

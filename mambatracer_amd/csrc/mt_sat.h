@@ -1,0 +1,244 @@
+// Stage C device functions: light-grid index math and subgrid sampling
+// (src/cl/kernel/LightGrid.cl:39-343), region SAT sampling (SAT.cl:1-342),
+// spherical mappings (Util.cl:397-439) and environment light
+// illuminate/radiance (Light.cl:129-137,195-227,270-286).
+//
+// All comparisons and divisions follow the reference order so cell, tile and
+// texel picks round identically to the CPU restatement.  Deviations: reads
+// the reference makes outside its arrays (getSubgridValueSAT evaluates
+// sat[index] before select() discards it, LightGrid.cl:145; getFunc with
+// index -1, SAT.cl:95-99; texel y == height, Util.cl:406) return 0 or are
+// clamped to the map here.
+#pragma once
+
+#include "mt_devmath.h"
+#include "mt_internal.h"
+
+// ---------------------------------------------------------------- light grid
+__device__ __forceinline__ int lg_global_index(int s, int lx, int ly)
+{ // globalIndexInSubgrid, LightGrid.cl:104-139
+    int cx = s % LG_CX;
+    int cy = s / LG_CX;
+    int gi = cx * LG_RX + cy * (LG_RX * LG_RY) * LG_CX;
+    int gx = gi % LG_NU;
+    int gy = gi / LG_NV; // subgridGlobalIndexXY divides by nv (:122); nu == nv
+    return (gx + lx) + (gy + ly) * LG_NU;
+}
+
+__device__ __forceinline__ float lg_sat(const float* __restrict__ sat, int s, int sx, int sy)
+{ // getSubgridValueSAT, :142-148
+    bool in = !(sx >= LG_RX || sx < 0) && !(sy >= LG_RY || sy < 0);
+    return in ? sat[lg_global_index(s, sx, sy)] : 0.0f;
+}
+
+__device__ __forceinline__ float lg_range(const float* __restrict__ sat, int s, int minx, int miny,
+                                          int maxx, int maxy)
+{ // getSubgridSATRange, :150-157
+    float a1 = lg_sat(sat, s, maxx, maxy);
+    float b1 = lg_sat(sat, s, minx - 1, maxy);
+    float c1 = lg_sat(sat, s, maxx, miny - 1);
+    float d1 = lg_sat(sat, s, minx - 1, miny - 1);
+    return ((a1 + d1) - b1) - c1;
+}
+
+__device__ __forceinline__ float lg_func_int(const float* sat, int s, int y)
+{
+    return lg_range(sat, s, 0, y, LG_RX - 1, y);
+}
+
+__device__ __forceinline__ float lg_cond(const float* sat, int s, int x, int y)
+{ // getSubgridConditional, :183-187
+    float fi = lg_func_int(sat, s, y);
+    return lg_range(sat, s, 0, y, x - 1, y) / fi;
+}
+
+__device__ __forceinline__ float lg_marg(const float* sat, int s, int y)
+{ // getSubgridMarginal, :189-193
+    float last = lg_sat(sat, s, LG_RX - 1, LG_RY - 1);
+    return lg_sat(sat, s, LG_RX - 1, y - 1) / last;
+}
+
+// sampleSubgridContinuous, LightGrid.cl:310-320 (marginal :279-296, conditional :249-266,
+// branchless upper bounds :212-247)
+__device__ __forceinline__ void lg_sample(const float* __restrict__ func, const float* __restrict__ sat,
+                                          int s, float u0, float u1, int& ox, int& oy, float& uu,
+                                          float& vv, float& pdf)
+{
+    int b = 0, e = LG_RY;
+    while (b != e) {
+        int mid = b + (e - b) / 2;
+        bool right = !(u1 < lg_marg(sat, s, mid));
+        b = right ? mid + 1 : b;
+        e = right ? e : mid;
+    }
+    oy = b - 1 > 0 ? b - 1 : 0;
+    float m0 = lg_marg(sat, s, oy);
+    float du = (u1 - m0) / (lg_marg(sat, s, oy + 1) - m0);
+    float last = lg_sat(sat, s, LG_RX - 1, LG_RY - 1);
+    float pdf_v = (lg_func_int(sat, s, oy) * (float)(LG_RX * LG_RY)) / last;
+    vv = ((float)oy + du) / (float)LG_RY;
+    b = 0;
+    e = LG_RX;
+    while (b != e) {
+        int mid = b + (e - b) / 2;
+        bool right = !(u0 < lg_cond(sat, s, mid, oy));
+        b = right ? mid + 1 : b;
+        e = right ? e : mid;
+    }
+    ox = b - 1 > 0 ? b - 1 : 0;
+    float c0 = lg_cond(sat, s, ox, oy);
+    float du2 = (u0 - c0) / (lg_cond(sat, s, ox + 1, oy) - c0);
+    float pdf_u = func[lg_global_index(s, ox, oy)] / lg_func_int(sat, s, oy);
+    uu = ((float)ox + du2) / (float)LG_RX;
+    pdf = pdf_u * pdf_v;
+}
+
+// ---------------------------------------------------------------- spherical maps
+__device__ __forceinline__ void sph_xy(int w, int h, v3 d, int& x, int& y)
+{ // getSphericalGridXY, Util.cl:397-410
+    float phi = acosf(d.y);
+    float theta = atan2f(d.z, d.x);
+    x = (int)((0.5f - (0.5f * theta) / MT_PI) * (float)w);
+    y = (int)((phi / MT_PI) * (float)h);
+}
+
+__device__ __forceinline__ int sph_index(int w, int h, v3 d)
+{ // getSphericalGridIndex, Util.cl:413-417 (clamped to the map)
+    int x, y;
+    sph_xy(w, h, d, x, y);
+    int idx = (int)((float)x + (float)y * (float)w);
+    idx = idx < 0 ? 0 : idx;
+    return idx > w * h - 1 ? w * h - 1 : idx;
+}
+
+__device__ __forceinline__ v3 sph_direction(float u, float v)
+{ // getSphericalDirection, Util.cl:423-439
+    float phi = MT_PI * v;
+    float theta = (u * 2.0f) * MT_PI;
+    return mk3(-sinf(phi) * cosf(theta), cosf(phi), sinf(phi) * sinf(theta));
+}
+
+// ---------------------------------------------------------------- region SAT
+struct SatRegion {
+    int nu, nv, minx, miny, sx, sy;
+};
+
+__device__ __forceinline__ float r_value(const SatRegion& R, const float* __restrict__ sat, int x, int y)
+{ // getValueSAT, SAT.cl:135-147
+    if (x < 0 || y < 0) return 0.0f;
+    if (x > R.minx + R.sx - 1) return 0.0f;
+    if (y > R.miny + R.sy - 1) return 0.0f;
+    if (x > R.nu - 1 || y > R.nv - 1) return 0.0f;
+    return sat[(size_t)y * R.nu + x];
+}
+
+__device__ __forceinline__ float r_range(const SatRegion& R, const float* sat, int minx, int miny,
+                                         int maxx, int maxy)
+{ // getSATRange, SAT.cl:149-156
+    float d1 = r_value(R, sat, maxx, maxy);
+    float c1 = r_value(R, sat, minx - 1, maxy);
+    float b1 = r_value(R, sat, maxx, miny - 1);
+    float a1 = r_value(R, sat, minx - 1, miny - 1);
+    return ((a1 + d1) - b1) - c1;
+}
+
+__device__ __forceinline__ float r_func_int(const SatRegion& R, const float* sat, int y)
+{
+    return r_range(R, sat, R.minx, y, R.minx + R.sx - 1, y);
+}
+__device__ __forceinline__ float r_cond(const SatRegion& R, const float* sat, int x, int y)
+{
+    float fi = r_func_int(R, sat, y);
+    return r_range(R, sat, R.minx, y, x - 1, y) / fi;
+}
+__device__ __forceinline__ float r_marg(const SatRegion& R, const float* sat, int y)
+{
+    float last = r_range(R, sat, R.minx, R.miny, R.minx + R.sx - 1, R.miny + R.sy - 1);
+    return r_range(R, sat, R.minx, R.miny, R.minx + R.sx - 1, y - 1) / last;
+}
+__device__ __forceinline__ float r_func(const SatRegion& R, const float* __restrict__ func, int x, int y)
+{
+    if (x < 0 || y < 0 || x > R.nu - 1 || y > R.nv - 1) return 0.0f;
+    return func[(size_t)y * R.nu + x];
+}
+__device__ __forceinline__ float r_pdf_cond(const SatRegion& R, const float* sat, const float* func,
+                                            int x, int y)
+{
+    return r_func(R, func, x, y) / r_func_int(R, sat, y);
+}
+__device__ __forceinline__ float r_pdf_marg(const SatRegion& R, const float* sat, int y)
+{
+    float fi = r_func_int(R, sat, y);
+    float last = r_range(R, sat, R.minx, R.miny, R.minx + R.sx - 1, R.miny + R.sy - 1);
+    return (fi * (float)(R.sx * R.sy)) / last;
+}
+
+// sampleContinuous, SAT.cl:244-323
+__device__ __forceinline__ void r_sample(const SatRegion& R, const float* __restrict__ sat,
+                                         const float* __restrict__ func, float u0, float u1,
+                                         float& uu, float& vv, int& ox, int& oy, float& pdf)
+{
+    int b = R.miny, e = R.miny + R.sy;
+    while (b != e) {
+        int mid = b + (e - b) / 2;
+        bool right = !(u1 < r_marg(R, sat, mid));
+        b = right ? mid + 1 : b;
+        e = right ? e : mid;
+    }
+    oy = b - 1 > 0 ? b - 1 : 0;
+    float m0 = r_marg(R, sat, oy);
+    float du = (u1 - m0) / (r_marg(R, sat, oy + 1) - m0);
+    float pdf_v = r_pdf_marg(R, sat, oy);
+    vv = ((float)oy + du) / (float)R.nv;
+    b = R.minx;
+    e = R.minx + R.sx;
+    while (b != e) {
+        int mid = b + (e - b) / 2;
+        bool right = !(u0 < r_cond(R, sat, mid, oy));
+        b = right ? mid + 1 : b;
+        e = right ? e : mid;
+    }
+    ox = b - 1 > 0 ? b - 1 : 0;
+    float c0 = r_cond(R, sat, ox, oy);
+    float du2 = (u0 - c0) / (r_cond(R, sat, ox + 1, oy) - c0);
+    float pdf_u = r_pdf_cond(R, sat, func, ox, oy);
+    uu = ((float)ox + du2) / (float)R.nu;
+    pdf = pdf_u * pdf_v;
+}
+
+// ---------------------------------------------------------------- environment light
+struct EnvView {
+    const float4* rgba;
+    const float* lum;
+    const float* sat;
+    int w, h;
+};
+
+// illuminateEnvironmentLight, Light.cl:195-227: direction, radiance, solid-angle pdf
+__device__ __forceinline__ float4 env_illuminate(const EnvView& E, float u0, float u1, v3& dir,
+                                                 float& pdfw, int& texel)
+{
+    SatRegion R{E.w, E.h, 0, 0, E.w, E.h};
+    float uu, vv, pdf;
+    int ox, oy;
+    r_sample(R, E.sat, E.lum, u0, u1, uu, vv, ox, oy, pdf);
+    dir = sph_direction(uu, vv);
+    texel = sph_index(E.w, E.h, dir);
+    float s = sinf(vv * MT_PI);
+    pdfw = pdf / (((2.0f * MT_PI) * MT_PI) * sinf(vv * MT_PI));
+    if (s == 0.0f) pdfw = 0.0f;
+    return E.rgba[texel];
+}
+
+// getRadianceEnvironmentLight, Light.cl:270-286
+__device__ __forceinline__ float4 env_radiance(const EnvView& E, v3 d, float& pdfw, int& texel)
+{
+    SatRegion R{E.w, E.h, 0, 0, E.w, E.h};
+    int x, y;
+    sph_xy(E.w, E.h, d, x, y);
+    texel = sph_index(E.w, E.h, d);
+    float p = r_pdf_cond(R, E.sat, E.lum, x, y) * r_pdf_marg(R, E.sat, y);
+    float v = (float)y / (float)E.h;
+    pdfw = p / (((2.0f * MT_PI) * MT_PI) * sinf(v * MT_PI));
+    return E.rgba[texel];
+}

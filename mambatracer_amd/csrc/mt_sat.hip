@@ -1,0 +1,464 @@
+// Stage C entry points: adaptive light-grid SAT build and sampling, and the
+// environment-map luminance/SAT and light sampling.
+//
+// Light-grid build.  The reference rebuilds the 1600x1600 grid's tile SATs
+// with four launches over 2.56M work-items (initFuncAndSat, calculateFunc,
+// prefixSumRowSubgrid with 16-wide work-groups, prefixSumColSubgrid with
+// 32-wide work-groups and strided column reads; LightGrid.cl:347-423,
+// CAdaptiveEnvironment.java:135-145).  Here one wave owns one 16x32 tile:
+// it reads the tile's 32 rows (a work-group of 4 waves covers 4 adjacent
+// tiles, so every row access is a contiguous 256 B), runs the 16-wide
+// Kogge-Stone row scans with lane shuffles, transposes through LDS, runs the
+// 32-tall Kogge-Stone column scans with shuffles, and writes the SAT once.
+// Each scan step computes x[l] = x[l-o] + x[l] for l >= o, o = 1, 2, 4, ...,
+// the reference's exact summation tree, so the SAT is bit-identical.
+// HBM traffic: read func (or accum) 4 B + write sat 4 B (+ func 4 B when
+// updating from accum) per cell.
+#include "mt_sat.h"
+
+namespace {
+
+constexpr int LDS_STRIDE = LG_RX + 1; // padded row stride for the column pass
+
+// mode 0: src = func (build_sat); mode 1: src = accum, also func <- accum (update)
+__global__ __launch_bounds__(256) void k_lg_build(const float* __restrict__ src,
+                                                  float* __restrict__ func,
+                                                  float* __restrict__ sat, int mode)
+{
+    __shared__ float lds[4][LG_RY * LDS_STRIDE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int s = blockIdx.x * 4 + wave;
+    if (s >= LG_CELLS) return; // LG_CELLS is a multiple of 4: whole blocks exit together
+    const int x0 = (s % LG_CX) * LG_RX, y0 = (s / LG_CX) * LG_RY;
+    float* L = lds[wave];
+    const int c = lane & 15, r4 = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < LG_RY / 4; j++) {
+        int r = 4 * j + r4;
+        size_t g = (size_t)(y0 + r) * LG_NU + x0 + c;
+        float x = src[g];
+        if (mode == 1) func[g] = x;
+#pragma unroll
+        for (int o = 1; o < LG_RX; o <<= 1) {
+            float y = __shfl_up(x, o, 16);
+            if (c >= o) x = y + x;
+        }
+        L[r * LDS_STRIDE + c] = x;
+    }
+    __syncthreads();
+    const int rr = lane & 31, half = lane >> 5;
+#pragma unroll
+    for (int j = 0; j < LG_RX / 2; j++) {
+        int cc = 2 * j + half;
+        float x = L[rr * LDS_STRIDE + cc];
+#pragma unroll
+        for (int o = 1; o < LG_RY; o <<= 1) {
+            float y = __shfl_up(x, o, 32);
+            if (rr >= o) x = y + x;
+        }
+        L[rr * LDS_STRIDE + cc] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < LG_RY / 4; j++) {
+        int r = 4 * j + r4;
+        sat[(size_t)(y0 + r) * LG_NU + x0 + c] = L[r * LDS_STRIDE + c];
+    }
+}
+
+__global__ void k_lg_clear(float* __restrict__ a, float* __restrict__ b, float* __restrict__ c)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < LG_SIZE) {
+        if (a) a[i] = 0.0f;
+        b[i] = 0.0f;
+        c[i] = 0.0f;
+    }
+}
+
+__global__ void k_lg_set_tile(float* __restrict__ func, const float* __restrict__ tile)
+{ // setToAllFunc, CLightGrid.java:86-100: entry i of the tile at local (i % 16, i / 16)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= LG_SIZE) return;
+    int s = i / (LG_RX * LG_RY), k = i % (LG_RX * LG_RY);
+    func[lg_global_index(s, k % LG_RX, k / LG_RX)] = tile[k];
+}
+
+// Generic tile SAT (test sizes): row segments of rx, then column segments of ry.
+__global__ void k_sat_rows(const float* __restrict__ func, float* __restrict__ sat, int64_t nseg, int rx)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nseg) return;
+    float a[64];
+    for (int l = 0; l < rx; l++) a[l] = func[g * rx + l];
+    for (int o = 1; o < rx; o *= 2)
+        for (int l = rx - 1; l >= o; l--) a[l] = a[l - o] + a[l];
+    for (int l = 0; l < rx; l++) sat[g * rx + l] = a[l];
+}
+
+__global__ void k_sat_cols(float* __restrict__ sat, int64_t nseg, int nu, int nv, int ry)
+{ // prefixSumColSubgrid indexing, LightGrid.cl:379-383
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nseg) return;
+    int64_t i = (int64_t)ry * g;
+    int64_t xi = i / nv, yi0 = i % nv;
+    float a[64];
+    for (int l = 0; l < ry; l++) a[l] = sat[xi + (yi0 + l) * nu];
+    for (int o = 1; o < ry; o *= 2)
+        for (int l = ry - 1; l >= o; l--) a[l] = a[l - o] + a[l];
+    for (int l = 0; l < ry; l++) sat[xi + (yi0 + l) * nu] = a[l];
+}
+
+__global__ void k_lg_sample(const float* __restrict__ func, const float* __restrict__ sat, int n,
+                            const int* __restrict__ cell, const float2* __restrict__ u,
+                            int2* __restrict__ tile_xy, float2* __restrict__ uv, float* __restrict__ pdf)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = cell[i];
+    if (s < 0 || s >= LG_CELLS) {
+        tile_xy[i] = make_int2(-1, -1);
+        uv[i] = make_float2(0, 0);
+        pdf[i] = 0.0f;
+        return;
+    }
+    float2 uu = u[i];
+    int ox, oy;
+    float a, b, p;
+    lg_sample(func, sat, s, uu.x, uu.y, ox, oy, a, b, p);
+    tile_xy[i] = make_int2(ox, oy);
+    uv[i] = make_float2(a, b);
+    pdf[i] = p;
+}
+
+__global__ void k_lg_cells(int n, const float4* __restrict__ dirs, int* __restrict__ cell,
+                           int2* __restrict__ tile_xy)
+{ // subgridIndexFromCamera / tileIndexXYFromCamera, LightGrid.cl:68-90
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    v3 v = xyz(dirs[i]);
+    cell[i] = sph_index(LG_CX, LG_CY, norm3(v));
+    int t = sph_index(LG_RX, LG_RY, v);
+    tile_xy[i] = make_int2(t % LG_RX, t / LG_RX);
+}
+
+// ---------------------------------------------------------------- environment map
+__global__ void k_env_lum(const float4* __restrict__ rgba, int64_t n, float* __restrict__ lum,
+                          float* __restrict__ sat)
+{ // InitEnvironmentLum, Initialize.cl:100-115
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 c = rgba[i];
+    float l = luminance3(c.x, c.y, c.z);
+    lum[i] = l;
+    sat[i] = l;
+}
+
+// calculateSAT rows (SAT.cl:114-122,306-307): one thread per row carries the
+// running sum left to right; 64x64 blocks are staged through LDS so global
+// accesses stay coalesced.
+__global__ __launch_bounds__(64) void k_env_rows(float* __restrict__ sat, int w, int h)
+{
+    __shared__ float t[64][65];
+    const int tid = threadIdx.x;
+    const int y0 = blockIdx.x * 64;
+    float val = 0.0f;
+    for (int x0 = 0; x0 < w; x0 += 64) {
+        for (int r = 0; r < 64; r++) {
+            int y = y0 + r, x = x0 + tid;
+            t[r][tid] = (y < h && x < w) ? sat[(size_t)y * w + x] : 0.0f;
+        }
+        __syncthreads();
+        int lim = w - x0 < 64 ? w - x0 : 64;
+        for (int k = 0; k < lim; k++) {
+            val = val + t[tid][k];
+            t[tid][k] = val;
+        }
+        __syncthreads();
+        for (int r = 0; r < 64; r++) {
+            int y = y0 + r, x = x0 + tid;
+            if (y < h && x < w) sat[(size_t)y * w + x] = t[r][tid];
+        }
+        __syncthreads();
+    }
+}
+
+// calculateSAT columns (SAT.cl:125-133,308-309): one thread per column; lanes
+// of a wave walk adjacent columns, so each step is a coalesced row access.
+__global__ __launch_bounds__(256) void k_env_cols(float* __restrict__ sat, int w, int h)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= w) return;
+    float val = 0.0f;
+    int y = 0;
+    for (; y + 8 <= h; y += 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = sat[(size_t)(y + k) * w + x];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            val = val + v[k];
+            sat[(size_t)(y + k) * w + x] = val;
+        }
+    }
+    for (; y < h; y++) {
+        val = val + sat[(size_t)y * w + x];
+        sat[(size_t)y * w + x] = val;
+    }
+}
+
+__global__ void k_region_sample(const float* __restrict__ func, const float* __restrict__ sat, SatRegion R,
+                                int n, const float2* __restrict__ u, float2* __restrict__ uv,
+                                int2* __restrict__ off, float* __restrict__ pdf)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 q = u[i];
+    float a, b, p;
+    int ox, oy;
+    r_sample(R, sat, func, q.x, q.y, a, b, ox, oy, p);
+    uv[i] = make_float2(a, b);
+    off[i] = make_int2(ox, oy);
+    pdf[i] = p;
+}
+
+__global__ void k_env_illuminate(EnvView E, int n, const float2* __restrict__ u, float4* __restrict__ dir,
+                                 float4* __restrict__ contrib, float* __restrict__ pdfw,
+                                 int* __restrict__ texel)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 q = u[i];
+    v3 d;
+    float pw;
+    int tx;
+    float4 c = env_illuminate(E, q.x, q.y, d, pw, tx);
+    dir[i] = make_float4(d.x, d.y, d.z, 0.0f);
+    contrib[i] = c;
+    pdfw[i] = pw;
+    texel[i] = tx;
+}
+
+__global__ void k_env_radiance(EnvView E, int n, const float4* __restrict__ dir, float4* __restrict__ contrib,
+                               float* __restrict__ pdfw, int* __restrict__ texel)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float pw;
+    int tx;
+    float4 c = env_radiance(E, xyz(dir[i]), pw, tx);
+    contrib[i] = c;
+    pdfw[i] = pw;
+    texel[i] = tx;
+}
+
+} // namespace
+
+extern "C" int32_t mt_lightgrid_create(mt_context* ctx, mt_lightgrid** out)
+{
+    MT_ARG_CHECK(ctx && out, "mt_lightgrid_create: NULL argument");
+    *out = nullptr;
+    mt_lightgrid* g = new mt_lightgrid();
+    g->ctx = ctx;
+    float* p = nullptr;
+    if (hipMalloc(&p, sizeof(float) * 3 * (size_t)LG_SIZE) != hipSuccess) {
+        delete g;
+        mt_set_error("mt_lightgrid_create: device allocation failed");
+        return MT_ERR_ALLOC;
+    }
+    g->accum = p;
+    g->func = p + LG_SIZE;
+    g->sat = p + 2 * (size_t)LG_SIZE;
+    *out = g;
+    return mt_lightgrid_clear(ctx, g);
+}
+
+extern "C" int32_t mt_lightgrid_destroy(mt_lightgrid* g)
+{
+    if (!g) return MT_OK;
+    hipStreamSynchronize(g->ctx->stream);
+    hipFree(g->accum);
+    delete g;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_lightgrid_arrays(mt_lightgrid* g, float** accum, float** func, float** sat)
+{
+    MT_ARG_CHECK(g != nullptr, "mt_lightgrid_arrays: NULL grid");
+    if (accum) *accum = g->accum;
+    if (func) *func = g->func;
+    if (sat) *sat = g->sat;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_lightgrid_clear(mt_context* ctx, mt_lightgrid* g)
+{
+    MT_ARG_CHECK(ctx && g, "mt_lightgrid_clear: NULL argument");
+    k_lg_clear<<<mt_blocks(LG_SIZE, 256), 256, 0, ctx->stream>>>(g->accum, g->func, g->sat);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_lightgrid_set_tile_func(mt_context* ctx, mt_lightgrid* g, const float* tile)
+{
+    MT_ARG_CHECK(ctx && g && tile, "mt_lightgrid_set_tile_func: NULL argument");
+    float* dtile = nullptr;
+    MT_HIP_CHECK(hipMalloc(&dtile, sizeof(float) * LG_RX * LG_RY));
+    hipError_t e = hipMemcpyAsync(dtile, tile, sizeof(float) * LG_RX * LG_RY, hipMemcpyHostToDevice,
+                                  ctx->stream);
+    if (e == hipSuccess) {
+        k_lg_set_tile<<<mt_blocks(LG_SIZE, 256), 256, 0, ctx->stream>>>(g->func, dtile);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        k_lg_build<<<LG_CELLS / 4, 256, 0, ctx->stream>>>(g->func, g->func, g->sat, 0);
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    hipFree(dtile);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        mt_set_error("mt_lightgrid_set_tile_func: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+        return MT_ERR_HIP;
+    }
+    return MT_OK;
+}
+
+extern "C" int32_t mt_lightgrid_update(mt_context* ctx, mt_lightgrid* g)
+{
+    MT_ARG_CHECK(ctx && g, "mt_lightgrid_update: NULL argument");
+    k_lg_build<<<LG_CELLS / 4, 256, 0, ctx->stream>>>(g->accum, g->func, g->sat, 1);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_lightgrid_build_sat(mt_context* ctx, mt_lightgrid* g)
+{
+    MT_ARG_CHECK(ctx && g, "mt_lightgrid_build_sat: NULL argument");
+    k_lg_build<<<LG_CELLS / 4, 256, 0, ctx->stream>>>(g->func, g->func, g->sat, 0);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_lightgrid_sample(mt_context* ctx, const mt_lightgrid* g, int32_t n,
+                                       const int32_t* cell, const float* u, int32_t* tile_xy,
+                                       float* uv, float* pdf)
+{
+    MT_ARG_CHECK(ctx && g && cell && u && tile_xy && uv && pdf, "mt_lightgrid_sample: NULL argument");
+    MT_ARG_CHECK(n >= 0, "mt_lightgrid_sample: negative n");
+    if (n == 0) return MT_OK;
+    k_lg_sample<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(g->func, g->sat, n, cell,
+                                                            reinterpret_cast<const float2*>(u),
+                                                            reinterpret_cast<int2*>(tile_xy),
+                                                            reinterpret_cast<float2*>(uv), pdf);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_lightgrid_cells(mt_context* ctx, int32_t n, const float* dirs, int32_t* cell,
+                                      int32_t* tile_xy)
+{
+    MT_ARG_CHECK(ctx && dirs && cell && tile_xy, "mt_lightgrid_cells: NULL argument");
+    MT_ARG_CHECK(n >= 0, "mt_lightgrid_cells: negative n");
+    if (n == 0) return MT_OK;
+    k_lg_cells<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(n, reinterpret_cast<const float4*>(dirs), cell,
+                                                           reinterpret_cast<int2*>(tile_xy));
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_sat_tiles(mt_context* ctx, const float* func, float* sat, int32_t nu, int32_t nv,
+                                int32_t rx, int32_t ry)
+{
+    MT_ARG_CHECK(ctx && func && sat, "mt_sat_tiles: NULL argument");
+    MT_ARG_CHECK(nu > 0 && nv > 0 && rx > 0 && ry > 0 && rx <= 64 && ry <= 64,
+                 "mt_sat_tiles: bad sizes %d x %d tiles %d x %d", nu, nv, rx, ry);
+    int64_t total = (int64_t)nu * nv;
+    MT_ARG_CHECK(total % rx == 0 && total % ry == 0 && nv % ry == 0,
+                 "mt_sat_tiles: tiles must divide the grid");
+    k_sat_rows<<<mt_blocks(total / rx, 128), 128, 0, ctx->stream>>>(func, sat, total / rx, rx);
+    MT_LAUNCH_CHECK();
+    k_sat_cols<<<mt_blocks(total / ry, 128), 128, 0, ctx->stream>>>(sat, total / ry, nu, nv, ry);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_env_lum_sat(mt_context* ctx, const float* rgba, int32_t w, int32_t h, float* lum,
+                                  float* sat)
+{
+    MT_ARG_CHECK(ctx && rgba && lum && sat, "mt_env_lum_sat: NULL argument");
+    MT_ARG_CHECK(w > 0 && h > 0 && (int64_t)w * h < (1ll << 31), "mt_env_lum_sat: bad size %d x %d", w, h);
+    int64_t n = (int64_t)w * h;
+    k_env_lum<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(reinterpret_cast<const float4*>(rgba), n, lum, sat);
+    MT_LAUNCH_CHECK();
+    k_env_rows<<<mt_blocks(h, 64), 64, 0, ctx->stream>>>(sat, w, h);
+    MT_LAUNCH_CHECK();
+    k_env_cols<<<mt_blocks(w, 256), 256, 0, ctx->stream>>>(sat, w, h);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_env_create(mt_context* ctx, const float* rgba, const float* lum, const float* sat,
+                                 int32_t w, int32_t h, mt_envmap** out)
+{
+    MT_ARG_CHECK(ctx && rgba && lum && sat && out, "mt_env_create: NULL argument");
+    MT_ARG_CHECK(w > 0 && h > 0, "mt_env_create: bad size");
+    mt_envmap* e = new mt_envmap();
+    e->ctx = ctx;
+    e->rgba = rgba;
+    e->lum = lum;
+    e->sat = sat;
+    e->w = w;
+    e->h = h;
+    *out = e;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_env_destroy(mt_envmap* e)
+{
+    delete e;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_sat_sample_region(mt_context* ctx, const float* func, const float* sat, int32_t nu,
+                                        int32_t nv, int32_t minx, int32_t miny, int32_t sx, int32_t sy,
+                                        int32_t n, const float* u, float* uv, int32_t* offset, float* pdf)
+{
+    MT_ARG_CHECK(ctx && func && sat && u && uv && offset && pdf, "mt_sat_sample_region: NULL argument");
+    MT_ARG_CHECK(minx >= 0 && miny >= 0 && sx > 0 && sy > 0 && minx + sx <= nu && miny + sy <= nv,
+                 "mt_sat_sample_region: region outside the map");
+    if (n <= 0) return MT_OK;
+    SatRegion R{nu, nv, minx, miny, sx, sy};
+    k_region_sample<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(func, sat, R, n,
+                                                                reinterpret_cast<const float2*>(u),
+                                                                reinterpret_cast<float2*>(uv),
+                                                                reinterpret_cast<int2*>(offset), pdf);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_env_illuminate(mt_context* ctx, const mt_envmap* env, int32_t n, const float* u,
+                                     float* dir, float* contrib, float* pdfw, int32_t* texel)
+{
+    MT_ARG_CHECK(ctx && env && u && dir && contrib && pdfw && texel, "mt_env_illuminate: NULL argument");
+    if (n <= 0) return MT_OK;
+    EnvView E{reinterpret_cast<const float4*>(env->rgba), env->lum, env->sat, env->w, env->h};
+    k_env_illuminate<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(E, n, reinterpret_cast<const float2*>(u),
+                                                                 reinterpret_cast<float4*>(dir),
+                                                                 reinterpret_cast<float4*>(contrib), pdfw,
+                                                                 texel);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_env_radiance(mt_context* ctx, const mt_envmap* env, int32_t n, const float* dir,
+                                   float* contrib, float* pdfw, int32_t* texel)
+{
+    MT_ARG_CHECK(ctx && env && dir && contrib && pdfw && texel, "mt_env_radiance: NULL argument");
+    if (n <= 0) return MT_OK;
+    EnvView E{reinterpret_cast<const float4*>(env->rgba), env->lum, env->sat, env->w, env->h};
+    k_env_radiance<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(E, n, reinterpret_cast<const float4*>(dir),
+                                                               reinterpret_cast<float4*>(contrib), pdfw,
+                                                               texel);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}

@@ -1,0 +1,389 @@
+// Stage D: single-pass exclusive scan and stable stream compaction.
+//
+// Replaces PrefixSumInteger's four launches + host readback
+// (src/cl/algorithms/PrefixSumInteger.java:80-98; kernels Util.cl:917-980)
+// and CCompact's seven launches (src/cl/algorithms/CCompact.java:79-89;
+// CompactAtomic, Compact.cl:21-40) with one kernel each.
+//
+// Each work-group takes a tile of 2048 elements (256 lanes x 8 rows, element
+// base + row*256 + lane, so every load is a coalesced 1 KB wave access).
+// Within a tile, 0/1 predicates are ranked with wave64 ballots and
+// mbcnt (no LDS scan at all); integer values use a 6-step shuffle scan per
+// wave.  Tiles are chained by decoupled look-back: a tile publishes its
+// aggregate, then wave 0 inspects up to 64 predecessors at once until it
+// meets an inclusive prefix.  Tile numbers come from an atomic ticket, so a
+// tile only ever waits on tiles that started before it; the wait is also
+// bounded by a spin limit that raises an error flag instead of hanging.
+// Status words carry a per-call epoch and never need clearing.
+//
+// Deviation from the reference: localScanInteger's exclusive conversion
+// `aux[l] > 0 ? aux[l-1] : aux[0]` (Util.cl:941-944) reads aux[-1] for lane 0
+// and is exclusive only for non-negative inputs; this scan is the exclusive
+// prefix sum for any int32 input (wrapping), which is what the reference's
+// callers expect (PrefixSumIntegerTest.java:30-36, LightConfiguration.cl:
+// 31-55).  CompactAtomic's order is whatever its atomic_inc produces; this
+// compaction is stable (survivors in index order), matching prepareLightInfo.
+#include "mt_internal.h"
+
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ROWS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ROWS;
+constexpr int SCAN_WAVES = SCAN_THREADS / 64;
+constexpr uint32_t FLAG_AGG = 1u, FLAG_INC = 2u;
+constexpr uint32_t SPIN_LIMIT = 1u << 26;
+
+struct ScanWork {
+    uint32_t* ticket; // [0] ticket, [1] error flag (spin limit hit)
+    uint64_t* status; // [ntiles]
+};
+
+__device__ __forceinline__ uint64_t pack_status(uint32_t epoch, uint32_t flag, uint32_t v)
+{
+    return ((uint64_t)(epoch & 0x3FFFFFFFu) << 34) | ((uint64_t)flag << 32) | (uint64_t)v;
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
+{ // number of set bits of `mask` below this lane (v_mbcnt_lo/hi)
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Decoupled look-back by wave 0 of the tile.  Returns the exclusive prefix of
+// `tile` (all lanes of wave 0 agree); publishes the tile's inclusive prefix.
+__device__ uint32_t look_back(ScanWork W, int tile, uint32_t aggregate, uint32_t epoch)
+{
+    const int lane = lane_id();
+    uint64_t* st = W.status;
+    if (tile == 0) {
+        if (lane == 0)
+            __hip_atomic_store(&st[0], pack_status(epoch, FLAG_INC, aggregate), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        return 0u;
+    }
+    if (lane == 0)
+        __hip_atomic_store(&st[tile], pack_status(epoch, FLAG_AGG, aggregate), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t prefix = 0;
+    int window_end = tile - 1; // predecessor examined by lane 0
+    const uint32_t ep = epoch & 0x3FFFFFFFu;
+    uint32_t spins = 0;
+    for (;;) {
+        int idx = window_end - lane;
+        uint64_t w;
+        uint32_t flag, val;
+        for (;;) {
+            if (idx >= 0) {
+                w = __hip_atomic_load(&st[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                bool cur = (uint32_t)(w >> 34) == ep;
+                flag = cur ? (uint32_t)(w >> 32) & 3u : 0u;
+                val = (uint32_t)w;
+            } else { // before tile 0: an inclusive zero
+                flag = FLAG_INC;
+                val = 0;
+            }
+            if (__all(flag != 0)) break;
+            if (++spins > SPIN_LIMIT) {
+                if (lane == 0) atomicOr(&W.ticket[1], 1u);
+                return prefix; // results invalid; the host reports the error flag
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        uint64_t inc = __ballot(flag == FLAG_INC);
+        // lanes up to and including the nearest inclusive predecessor contribute
+        int first_inc = inc ? __builtin_ctzll(inc) : 64;
+        uint32_t contrib = lane <= first_inc ? val : 0u;
+        for (int o = 32; o > 0; o >>= 1) contrib += __shfl_xor(contrib, o);
+        prefix += contrib;
+        if (inc) break;
+        window_end -= 64;
+    }
+    if (lane == 0)
+        __hip_atomic_store(&st[tile], pack_status(epoch, FLAG_INC, prefix + aggregate),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return prefix;
+}
+
+__global__ void k_scan_reset(ScanWork W, const int* __restrict__ d_len, int capacity,
+                             int* __restrict__ d_total)
+{
+    if (threadIdx.x == 0) {
+        W.ticket[0] = 0u;
+        int len = *d_len;
+        if (len > capacity) len = capacity;
+        if (len <= 0 && d_total) *d_total = 0;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int* __restrict__ in,
+                                                       int* __restrict__ out,
+                                                       const int* __restrict__ d_len,
+                                                       int capacity, int* __restrict__ d_total,
+                                                       ScanWork W, uint32_t epoch)
+{
+    __shared__ int s_tile;
+    __shared__ uint32_t s_wave[SCAN_ROWS][SCAN_WAVES];
+    __shared__ uint32_t s_prefix;
+    const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    if (tid == 0) s_tile = (int)atomicAdd(&W.ticket[0], 1u);
+    __syncthreads();
+    const int tile = s_tile;
+    int len = *d_len;
+    if (len > capacity) len = capacity;
+    const int64_t base = (int64_t)tile * SCAN_TILE;
+    if (base >= len) return;
+    uint32_t v[SCAN_ROWS], ex[SCAN_ROWS];
+#pragma unroll
+    for (int k = 0; k < SCAN_ROWS; k++) {
+        int64_t idx = base + k * SCAN_THREADS + tid;
+        v[k] = idx < len ? (uint32_t)in[idx] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < SCAN_ROWS; k++) {
+        uint32_t x = v[k];
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        ex[k] = x - v[k];
+        if (lane == 63) s_wave[k][wave] = x;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < SCAN_ROWS; k++)
+            for (int w = 0; w < SCAN_WAVES; w++) {
+                uint32_t t = s_wave[k][w];
+                s_wave[k][w] = run;
+                run += t;
+            }
+        s_prefix = run; // tile aggregate, replaced by the exclusive prefix below
+    }
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t agg = s_prefix;
+        uint32_t pre = look_back(W, tile, agg, epoch);
+        if (lane == 0) {
+            s_prefix = pre;
+            if ((int64_t)len - 1 < base + SCAN_TILE && d_total) *d_total = (int)(pre + agg);
+        }
+    }
+    __syncthreads();
+    const uint32_t pre = s_prefix;
+#pragma unroll
+    for (int k = 0; k < SCAN_ROWS; k++) {
+        int64_t idx = base + k * SCAN_THREADS + tid;
+        if (idx < len) out[idx] = (int)(pre + s_wave[k][wave] + ex[k]);
+    }
+}
+
+struct PayloadSet {
+    const void* src[8];
+    void* dst[8];
+    int bytes[8];
+    int n;
+};
+
+__device__ __forceinline__ void move_payload(const PayloadSet& P, int64_t from, int64_t to)
+{
+    for (int j = 0; j < P.n; j++) {
+        int b = P.bytes[j];
+        if (b == 4)
+            reinterpret_cast<uint32_t*>(P.dst[j])[to] = reinterpret_cast<const uint32_t*>(P.src[j])[from];
+        else if (b == 8)
+            reinterpret_cast<uint2*>(P.dst[j])[to] = reinterpret_cast<const uint2*>(P.src[j])[from];
+        else
+            reinterpret_cast<uint4*>(P.dst[j])[to] = reinterpret_cast<const uint4*>(P.src[j])[from];
+    }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_compact(const int* __restrict__ flags,
+                                                          int flag_mode,
+                                                          const int* __restrict__ d_len,
+                                                          int capacity,
+                                                          int* __restrict__ out_index,
+                                                          int* __restrict__ d_count,
+                                                          PayloadSet P, ScanWork W,
+                                                          uint32_t epoch)
+{
+    __shared__ int s_tile;
+    __shared__ uint32_t s_wave[SCAN_ROWS][SCAN_WAVES];
+    __shared__ uint32_t s_prefix;
+    const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    if (tid == 0) s_tile = (int)atomicAdd(&W.ticket[0], 1u);
+    __syncthreads();
+    const int tile = s_tile;
+    int len = *d_len;
+    if (len > capacity) len = capacity;
+    const int64_t base = (int64_t)tile * SCAN_TILE;
+    if (base >= len) return;
+    uint64_t mask[SCAN_ROWS];
+#pragma unroll
+    for (int k = 0; k < SCAN_ROWS; k++) {
+        int64_t idx = base + k * SCAN_THREADS + tid;
+        bool f = false;
+        if (idx < len) {
+            int x = flags[idx];
+            f = flag_mode == 0 ? (x != 0) : (x >= 0);
+        }
+        mask[k] = __ballot(f);
+        if (lane == 0) s_wave[k][wave] = (uint32_t)__popcll(mask[k]);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < SCAN_ROWS; k++)
+            for (int w = 0; w < SCAN_WAVES; w++) {
+                uint32_t t = s_wave[k][w];
+                s_wave[k][w] = run;
+                run += t;
+            }
+        s_prefix = run;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t agg = s_prefix;
+        uint32_t pre = look_back(W, tile, agg, epoch);
+        if (lane == 0) {
+            s_prefix = pre;
+            if ((int64_t)len - 1 < base + SCAN_TILE) *d_count = (int)(pre + agg);
+        }
+    }
+    __syncthreads();
+    const uint32_t pre = s_prefix;
+#pragma unroll
+    for (int k = 0; k < SCAN_ROWS; k++) {
+        if ((mask[k] >> lane) & 1ull) {
+            int64_t idx = base + k * SCAN_THREADS + tid;
+            int64_t r = (int64_t)(pre + s_wave[k][wave] + lane_rank(mask[k]));
+            if (out_index) out_index[r] = (int)idx;
+            move_payload(P, idx, r);
+        }
+    }
+}
+
+int32_t scan_workspace(mt_context* ctx, int capacity, ScanWork& W)
+{
+    int64_t ntiles = ((int64_t)capacity + SCAN_TILE - 1) / SCAN_TILE;
+    size_t bytes = 256 + sizeof(uint64_t) * (size_t)(ntiles > 0 ? ntiles : 1);
+    void* p = nullptr;
+    int32_t rc = mt_ctx_scratch(ctx, bytes, &p);
+    if (rc) return rc;
+    W.ticket = reinterpret_cast<uint32_t*>(p);
+    W.status = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(p) + 256);
+    return MT_OK;
+}
+
+uint32_t next_epoch(mt_context* ctx)
+{
+    ctx->epoch = (ctx->epoch + 1) & 0x3FFFFFFFu;
+    if (ctx->epoch == 0) ctx->epoch = 1;
+    return ctx->epoch;
+}
+
+} // namespace
+
+extern "C" int32_t mt_scan_excl_i32(mt_context* ctx, const int32_t* in, int32_t* out,
+                                    const int32_t* d_len, int32_t capacity, int32_t* d_total)
+{
+    MT_ARG_CHECK(ctx && in && out && d_len, "mt_scan_excl_i32: NULL argument");
+    MT_ARG_CHECK(capacity >= 0, "mt_scan_excl_i32: negative capacity");
+    ScanWork W;
+    int32_t rc = scan_workspace(ctx, capacity, W);
+    if (rc) return rc;
+    uint32_t ep = next_epoch(ctx);
+    k_scan_reset<<<1, 64, 0, ctx->stream>>>(W, d_len, capacity, d_total);
+    MT_LAUNCH_CHECK();
+    if (capacity == 0) return MT_OK;
+    k_scan<<<mt_blocks(capacity, SCAN_TILE), SCAN_THREADS, 0, ctx->stream>>>(in, out, d_len, capacity,
+                                                                          d_total, W, ep);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_compact(mt_context* ctx, const int32_t* flags, int32_t flag_mode,
+                              const int32_t* d_len, int32_t capacity, int32_t* out_index,
+                              int32_t* d_count, const mt_payload* payloads, int32_t npayload)
+{
+    MT_ARG_CHECK(ctx && flags && d_len && d_count, "mt_compact: NULL argument");
+    MT_ARG_CHECK(capacity >= 0, "mt_compact: negative capacity");
+    MT_ARG_CHECK(flag_mode == 0 || flag_mode == 1, "mt_compact: flag_mode must be 0 or 1");
+    MT_ARG_CHECK(npayload >= 0 && npayload <= 8, "mt_compact: npayload %d not in [0,8]", npayload);
+    MT_ARG_CHECK(npayload == 0 || payloads, "mt_compact: payloads is NULL");
+    PayloadSet P;
+    P.n = npayload;
+    for (int j = 0; j < npayload; j++) {
+        MT_ARG_CHECK(payloads[j].src && payloads[j].dst, "mt_compact: payload %d has a NULL pointer", j);
+        int b = payloads[j].elem_bytes;
+        MT_ARG_CHECK(b == 4 || b == 8 || b == 16, "mt_compact: payload %d elem_bytes %d not 4/8/16", j, b);
+        P.src[j] = payloads[j].src;
+        P.dst[j] = payloads[j].dst;
+        P.bytes[j] = b;
+    }
+    ScanWork W;
+    int32_t rc = scan_workspace(ctx, capacity, W);
+    if (rc) return rc;
+    uint32_t ep = next_epoch(ctx);
+    k_scan_reset<<<1, 64, 0, ctx->stream>>>(W, d_len, capacity, d_count);
+    MT_LAUNCH_CHECK();
+    if (capacity == 0) return MT_OK;
+    k_compact<<<mt_blocks(capacity, SCAN_TILE), SCAN_THREADS, 0, ctx->stream>>>(
+        flags, flag_mode, d_len, capacity, out_index, d_count, P, W, ep);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+// Reports (and clears) the look-back spin-limit flag of the context.  A set
+// flag means a scan/compaction gave up waiting and its output is invalid.
+extern "C" int32_t mt_scan_check(mt_context* ctx)
+{
+    MT_ARG_CHECK(ctx != nullptr, "mt_scan_check: ctx is NULL");
+    if (!ctx->scratch) return MT_OK;
+    uint32_t flag = 0;
+    uint32_t* tk = reinterpret_cast<uint32_t*>(ctx->scratch);
+    MT_HIP_CHECK(hipMemcpyAsync(&flag, tk + 1, sizeof(flag), hipMemcpyDeviceToHost, ctx->stream));
+    MT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (flag) {
+        int32_t rc = mt_buf_fill_i32(ctx, reinterpret_cast<int32_t*>(tk + 1), 0, 1);
+        if (rc) return rc;
+        mt_set_error("scan/compaction look-back exceeded its spin limit; results invalid");
+        return MT_ERR_STATE;
+    }
+    return MT_OK;
+}
+
+// identifyMeshLights + prefix sum + prepareLightInfo (LightConfiguration.cl:1-55).
+__global__ void k_emitter_flags(const int* __restrict__ faces, int ntris,
+                                const float* __restrict__ materials, int* __restrict__ pred)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntris) return;
+    const float* m = materials + 36 * (size_t)(faces[10 * i + 9] & 0xFFFF);
+    float s = m[32]; // emission_param.x (isSurfaceEmitter, Material.cl:27-31)
+    float r = m[28] * s, g = m[29] * s, b = m[30] * s;
+    float lum = (0.2989f * r + 0.5866f * g) + 0.1145f * b;
+    pred[i] = lum != 0.0f ? 1 : 0;
+}
+
+extern "C" int32_t mt_mesh_lights(mt_context* ctx, const mt_scene* scene, const float* materials,
+                                  int32_t* light_faces, int32_t* d_count)
+{
+    MT_ARG_CHECK(ctx && scene && materials && light_faces && d_count, "mt_mesh_lights: NULL argument");
+    int n = scene->desc.num_tris;
+    int* tmp = nullptr;
+    MT_HIP_CHECK(hipMalloc(&tmp, sizeof(int) * ((size_t)n + 1)));
+    k_emitter_flags<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(scene->desc.faces, n, materials, tmp);
+    int32_t rc = mt_buf_fill_i32(ctx, tmp + n, n, 1);
+    if (!rc) rc = mt_compact(ctx, tmp, 0, tmp + n, n, light_faces, d_count, nullptr, 0);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    hipFree(tmp);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+        mt_set_error("mt_mesh_lights: %s", hipGetErrorString(e));
+        return MT_ERR_HIP;
+    }
+    return MT_OK;
+}

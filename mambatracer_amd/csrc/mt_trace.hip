@@ -1,0 +1,676 @@
+// Stage A: stackless BVH closest-hit / any-hit traversal on gfx950.
+//
+// Restates intersectMesh (src/cl/kernel/NormalBVH.cl:14-92), testOcclusion
+// (:235-317), fastTriangleIntersection (Primitive.cl:3-29) and
+// intersectBoundT (Util.cl:707-732) with the same visiting order: at each
+// interior node both child boxes are tested, the nearer (entry distance,
+// ties to the left) is taken first and the other is remembered by a 1 bit
+// on a bitstack; backtracking walks parent links and flips to the sibling.
+// Keeping that order (not just the result) keeps tie-breaks and the
+// tMax-dependent culling identical, so primitive ids match bit for bit.
+// The bitstack is 128 bits (two 64-bit words) instead of the reference's
+// 64 (NormalBVH.cl:20), so trees up to depth 128 traverse correctly; scenes
+// deeper than that are rejected at mt_scene_create.
+//
+// Two memory layouts:
+//  * MT_TRACE_REFERENCE reads the reference's flat arrays (7-int nodes,
+//    8-float boxes, 10-int faces, float4 points) as IntersectPrimitives does.
+//  * MT_TRACE_PACKED (default) reads a layout derived once per scene: each
+//    interior node is one 64-byte record holding both child boxes, both child
+//    references and its own parent/sibling; each leaf triangle is one 48-byte
+//    record (p1, e1 = p2 - p1, e2 = p3 - p1, primitive id) stored in leaf
+//    order.  e1/e2 are the same float subtractions the reference performs
+//    per test, so every intersection rounds identically.  An interior step
+//    fetches 64 B instead of 148 B, a leaf 48 B instead of 116 B.
+#include "mt_devmath.h"
+#include "mt_internal.h"
+
+#include <cstdio>
+
+namespace {
+
+struct RayT {
+    v3 o, d, inv;
+    int sx, sy, sz;
+    float tmin, tmax;
+};
+
+__device__ __forceinline__ void ray_init(RayT& r, float4 a, float4 b)
+{ // initGlobalRay, Util.cl:577-600
+    r.o = mk3(a.x, a.y, a.z);
+    r.d = mk3(b.x, b.y, b.z);
+    r.inv = mk3(1.0f / b.x, 1.0f / b.y, 1.0f / b.z);
+    r.sx = r.inv.x < 0.0f;
+    r.sy = r.inv.y < 0.0f;
+    r.sz = r.inv.z < 0.0f;
+    r.tmin = a.w;
+    r.tmax = b.w;
+}
+
+// intersectBoundT, Util.cl:707-732.  Conditional assignments (not fminf/fmaxf)
+// keep the reference's NaN behaviour.
+__device__ __forceinline__ bool box_hit(const RayT& r, float mnx, float mny, float mnz,
+                                        float mxx, float mxy, float mxz, float& tnear)
+{
+    float tmin = ((r.sx ? mxx : mnx) - r.o.x) * r.inv.x;
+    float tmax = ((r.sx ? mnx : mxx) - r.o.x) * r.inv.x;
+    float tymin = ((r.sy ? mxy : mny) - r.o.y) * r.inv.y;
+    float tymax = ((r.sy ? mny : mxy) - r.o.y) * r.inv.y;
+    if ((tmin > tymax) || (tymin > tmax)) return false;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    float tzmin = ((r.sz ? mxz : mnz) - r.o.z) * r.inv.z;
+    float tzmax = ((r.sz ? mnz : mxz) - r.o.z) * r.inv.z;
+    if ((tmin > tzmax) || (tzmin > tmax)) return false;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    tmax *= 1.00000024f;
+    tnear = tmin;
+    return (tmin < r.tmax) && (tmax > r.tmin);
+}
+
+// fastTriangleIntersection, Primitive.cl:3-29, on p1 and the two edges.
+__device__ __forceinline__ float tri_hit(const RayT& r, v3 p1, v3 e1, v3 e2)
+{
+    v3 h = cross3(r.d, e2);
+    float a = dot3(e1, h);
+    if ((double)a > -0.0000001 && (double)a < 0.0000001) return r.tmax; // double compare, :13
+    float f = 1.0f / a;
+    v3 s = sub3(r.o, p1);
+    float b1 = f * dot3(s, h);
+    v3 q = cross3(s, e1);
+    float b2 = f * dot3(r.d, q);
+    float t = f * dot3(e2, q);
+    if (b1 < 0.0f || b1 > 1.0f || b2 < 0.0f || b1 + b2 > 1.0f || t < 0.0f || t > r.tmax) return r.tmax;
+    return t;
+}
+
+struct BitStack {
+    uint64_t lo, hi;
+    __device__ __forceinline__ void push0()
+    {
+        hi = (hi << 1) | (lo >> 63);
+        lo <<= 1;
+    }
+    __device__ __forceinline__ void pop()
+    {
+        lo = (lo >> 1) | (hi << 63);
+        hi >>= 1;
+    }
+    __device__ __forceinline__ bool empty() const { return (lo | hi) == 0; }
+};
+
+// -------------------------------------------------------------------------
+// Reference layout: NormalBVH.cl:14-92 over nodes[7], bounds[8], faces[10].
+// -------------------------------------------------------------------------
+template <bool ANY, bool STATS>
+__device__ __forceinline__ int trace_reference(const int* __restrict__ nodes,
+                                               const float* __restrict__ bounds,
+                                               const int* __restrict__ faces,
+                                               const float4* __restrict__ points, RayT& r,
+                                               int& child, int* st)
+{
+    int hit = 0;
+    int node_id = 0;
+    BitStack bs{0, 0};
+    int parent_id = 0, sibling_id = 0;
+    for (;;) {
+        while (!nodes[7 * node_id + 5]) {
+            const int* n = nodes + 7 * node_id;
+            parent_id = n[4];
+            sibling_id = n[1];
+            int left = n[2], right = n[3];
+            const float* lb = bounds + 8 * nodes[7 * left];
+            const float* rb = bounds + 8 * nodes[7 * right];
+            float lt, rt;
+            bool lh = box_hit(r, lb[0], lb[1], lb[2], lb[4], lb[5], lb[6], lt);
+            bool rh = box_hit(r, rb[0], rb[1], rb[2], rb[4], rb[5], rb[6], rt);
+            if (STATS) st[0]++;
+            if (!lh && !rh) break;
+            bs.push0();
+            if (lh && rh) {
+                node_id = (rt < lt) ? right : left;
+                bs.lo |= 1;
+            } else {
+                node_id = lh ? left : right;
+            }
+        }
+        if (nodes[7 * node_id + 5]) {
+            const int* n = nodes + 7 * node_id;
+            if (STATS) st[1]++;
+            int c = n[6];
+            const int* F = faces + 10 * c;
+            v3 p1 = xyz(points[F[0]]);
+            v3 p2 = xyz(points[F[1]]);
+            v3 p3 = xyz(points[F[2]]);
+            float t = tri_hit(r, p1, sub3(p2, p1), sub3(p3, p1));
+            if ((r.tmin < t) && (t < r.tmax)) {
+                if (ANY) return 1;
+                hit = 1;
+                r.tmax = t;
+                child = c;
+            }
+            parent_id = n[4];
+            sibling_id = n[1];
+        }
+        while ((bs.lo & 1) == 0) {
+            if (bs.empty()) return hit;
+            node_id = parent_id;
+            const int* n = nodes + 7 * node_id;
+            parent_id = n[4];
+            sibling_id = n[1];
+            bs.pop();
+            if (STATS) st[2]++;
+        }
+        node_id = sibling_id;
+        bs.lo ^= 1;
+    }
+}
+
+// -------------------------------------------------------------------------
+// Packed layout, same visiting order.  cur >= 0: interior node record;
+// cur < 0: leaf ~rank.  parent/sib track the current node's links, which the
+// reference reads from the node records (leaves are not records here).
+// -------------------------------------------------------------------------
+template <bool ANY, bool STATS>
+__device__ __forceinline__ int trace_packed(const float4* __restrict__ pn,
+                                            const float4* __restrict__ pt, int root, RayT& r,
+                                            int& child, int* st)
+{
+    int hit = 0;
+    int cur = root;
+    BitStack bs{0, 0};
+    int parent = 0, sib = 0;
+    for (;;) {
+        while (cur >= 0) {
+            const float4* N = pn + 4 * (size_t)cur;
+            float4 a = N[0], b = N[1], c = N[2];
+            int4 d = *reinterpret_cast<const int4*>(N + 3);
+            parent = d.z;
+            sib = d.w;
+            float lt, rt;
+            bool lh = box_hit(r, a.x, a.y, a.z, a.w, b.x, b.y, lt);
+            bool rh = box_hit(r, b.z, b.w, c.x, c.y, c.z, c.w, rt);
+            if (STATS) st[0]++;
+            if (!lh && !rh) break;
+            bs.push0();
+            int nxt;
+            if (lh && rh) {
+                nxt = (rt < lt) ? d.y : d.x;
+                bs.lo |= 1;
+            } else {
+                nxt = lh ? d.x : d.y;
+            }
+            if (nxt < 0) { // entering a leaf: its parent is cur, its sibling the other child
+                parent = cur;
+                sib = (nxt == d.x) ? d.y : d.x;
+            }
+            cur = nxt;
+        }
+        if (cur < 0) {
+            if (STATS) st[1]++;
+            const float4* T = pt + 3 * (size_t)(~cur);
+            float4 t0 = T[0], t1 = T[1], t2 = T[2];
+            float t = tri_hit(r, mk3(t0.x, t0.y, t0.z), mk3(t0.w, t1.x, t1.y), mk3(t1.z, t1.w, t2.x));
+            if ((r.tmin < t) && (t < r.tmax)) {
+                if (ANY) return 1;
+                hit = 1;
+                r.tmax = t;
+                child = __float_as_int(t2.y);
+            }
+        }
+        while ((bs.lo & 1) == 0) {
+            if (bs.empty()) return hit;
+            cur = parent;
+            int2 d = *reinterpret_cast<const int2*>(reinterpret_cast<const int*>(pn + 4 * (size_t)cur + 3) + 2);
+            parent = d.x;
+            sib = d.y;
+            bs.pop();
+            if (STATS) st[2]++;
+        }
+        int from = cur;
+        cur = sib;
+        sib = from;
+        bs.lo ^= 1;
+    }
+}
+
+struct TraceArgs {
+    const int* nodes;
+    const float* bounds;
+    const int* faces;
+    const float4* points;
+    const float4* pn;
+    const float4* pt;
+    int root;
+    const float4* o_tmin;
+    const float4* d_tmax;
+    const int* d_count;
+    int capacity;
+    int* prim;
+    float* tout;
+    int* occluded;
+    int* stats;
+};
+
+template <bool ANY, bool STATS, bool PACKED>
+__global__ __launch_bounds__(256) void k_trace(TraceArgs A)
+{
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    int count = *A.d_count;
+    if (count > A.capacity) count = A.capacity;
+    if (id >= count) return;
+    RayT r;
+    ray_init(r, A.o_tmin[id], A.d_tmax[id]);
+    int st[3] = {0, 0, 0};
+    int child = -1;
+    int h = PACKED ? trace_packed<ANY, STATS>(A.pn, A.pt, A.root, r, child, st)
+                   : trace_reference<ANY, STATS>(A.nodes, A.bounds, A.faces, A.points, r, child, st);
+    if (ANY) {
+        A.occluded[id] = h;
+    } else {
+        A.prim[id] = h ? child : -1;
+        A.tout[id] = r.tmax;
+    }
+    if (STATS) {
+        A.stats[3 * id + 0] = st[0];
+        A.stats[3 * id + 1] = st[1];
+        A.stats[3 * id + 2] = st[2];
+    }
+}
+
+template <bool ANY>
+int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
+                     const float* d_tmax, const int32_t* d_count, int32_t capacity,
+                     int32_t* prim, float* t, int32_t* occluded, int32_t* stats, int32_t mode)
+{
+    MT_ARG_CHECK(ctx && sc && o_tmin && d_tmax && d_count, "trace: NULL argument");
+    MT_ARG_CHECK(capacity >= 0, "trace: negative capacity");
+    MT_ARG_CHECK(mode == MT_TRACE_PACKED || mode == MT_TRACE_REFERENCE, "trace: unknown mode %d", mode);
+    if (ANY) MT_ARG_CHECK(occluded != nullptr, "trace_any: occluded is NULL");
+    else MT_ARG_CHECK(prim && t, "trace_closest: prim/t NULL");
+    if (capacity == 0) return MT_OK;
+    TraceArgs A;
+    A.nodes = sc->desc.nodes;
+    A.bounds = sc->desc.bounds;
+    A.faces = sc->desc.faces;
+    A.points = reinterpret_cast<const float4*>(sc->desc.points);
+    A.pn = sc->pnodes;
+    A.pt = sc->ptris;
+    A.root = sc->root_ref;
+    A.o_tmin = reinterpret_cast<const float4*>(o_tmin);
+    A.d_tmax = reinterpret_cast<const float4*>(d_tmax);
+    A.d_count = d_count;
+    A.capacity = capacity;
+    A.prim = prim;
+    A.tout = t;
+    A.occluded = occluded;
+    A.stats = stats;
+    dim3 grid(mt_blocks(capacity, 256)), block(256);
+    bool packed = mode == MT_TRACE_PACKED;
+    if (stats) {
+        if (packed) k_trace<ANY, true, true><<<grid, block, 0, ctx->stream>>>(A);
+        else k_trace<ANY, true, false><<<grid, block, 0, ctx->stream>>>(A);
+    } else {
+        if (packed) k_trace<ANY, false, true><<<grid, block, 0, ctx->stream>>>(A);
+        else k_trace<ANY, false, false><<<grid, block, 0, ctx->stream>>>(A);
+    }
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+// -------------------------------------------------------------------------
+// Scene preparation
+// -------------------------------------------------------------------------
+__global__ void k_leaf_flags(const int* __restrict__ nodes, int n, int* __restrict__ flags)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = nodes[7 * i + 5] ? 1 : 0;
+}
+
+// Range checks of every index the traversal follows.  err bit 1: node link
+// out of range; 2: face vertex out of range; 4: leaf primitive out of range;
+// 8: bound index != own index (the reference's invariant, CNormalBVH.java:88).
+__global__ void k_validate(const int* __restrict__ nodes, int nn, const int* __restrict__ faces,
+                           int ntris, int npoints, int* err)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int e = 0;
+    if (i < nn) {
+        const int* n = nodes + 7 * i;
+        if (n[0] != i) e |= 8;
+        if (n[5]) {
+            if (n[6] < 0 || n[6] >= ntris) e |= 4;
+        } else {
+            if (n[2] <= 0 || n[2] >= nn || n[3] <= 0 || n[3] >= nn) e |= 1;
+        }
+        if (n[1] < 0 || n[1] >= nn || n[4] < 0 || n[4] >= nn) e |= 1;
+    }
+    if (i < ntris) {
+        const int* F = faces + 10 * i;
+        for (int k = 0; k < 3; k++)
+            if (F[k] < 0 || F[k] >= npoints) e |= 2;
+    }
+    if (e) atomicOr(err, e);
+}
+
+// Leaf depth by walking parent links (bounded): err |= 16 when a leaf is
+// deeper than 128 or its parent chain does not reach the root.
+__global__ void k_depth(const int* __restrict__ nodes, int nn, int* err)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn || !nodes[7 * i + 5] || i == 0) return;
+    int d = 0, cur = i;
+    while (cur != 0 && d <= 128) {
+        cur = nodes[7 * cur + 4];
+        d++;
+    }
+    if (cur != 0 || d > 128) atomicOr(err, 16);
+}
+
+__global__ void k_pack(const int* __restrict__ nodes, const float* __restrict__ bounds,
+                       const int* __restrict__ faces, const float4* __restrict__ points, int nn,
+                       const int* __restrict__ leaf_rank, float4* __restrict__ pn,
+                       float4* __restrict__ pt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const int* n = nodes + 7 * i;
+    auto ref = [&](int j) -> int {
+        return nodes[7 * j + 5] ? ~leaf_rank[j] : j - leaf_rank[j];
+    };
+    if (n[5]) {
+        int c = n[6];
+        const int* F = faces + 10 * c;
+        float4 q1 = points[F[0]], q2 = points[F[1]], q3 = points[F[2]];
+        v3 p1 = xyz(q1), e1 = sub3(xyz(q2), p1), e2 = sub3(xyz(q3), p1);
+        float4* T = pt + 3 * (size_t)leaf_rank[i];
+        T[0] = make_float4(p1.x, p1.y, p1.z, e1.x);
+        T[1] = make_float4(e1.y, e1.z, e2.x, e2.y);
+        T[2] = make_float4(e2.z, __int_as_float(c), 0.0f, 0.0f);
+    } else {
+        const float* lb = bounds + 8 * nodes[7 * n[2]];
+        const float* rb = bounds + 8 * nodes[7 * n[3]];
+        float4* N = pn + 4 * (size_t)(i - leaf_rank[i]);
+        N[0] = make_float4(lb[0], lb[1], lb[2], lb[4]);
+        N[1] = make_float4(lb[5], lb[6], rb[0], rb[1]);
+        N[2] = make_float4(rb[2], rb[4], rb[5], rb[6]);
+        int parent = n[4];
+        int4 d = make_int4(ref(n[2]), ref(n[3]), parent - leaf_rank[parent], ref(n[1]));
+        N[3] = *reinterpret_cast<float4*>(&d);
+    }
+}
+
+// -------------------------------------------------------------------------
+// Hit shading (IntersectPrimitives, NormalBVH.cl:124-232)
+// -------------------------------------------------------------------------
+__device__ __forceinline__ void barycentrics(v3 p, v3 p1, v3 p2, v3 p3, float& b1, float& b2)
+{ // triangleBarycentrics, Primitive.cl:31-53
+    v3 e1 = sub3(p2, p1), e2 = sub3(p3, p1), e = sub3(p, p1);
+    float d00 = dot3(e1, e1), d01 = dot3(e1, e2), d11 = dot3(e2, e2);
+    float d20 = dot3(e, e1), d21 = dot3(e, e2);
+    float denom = (d00 * d11 - d01 * d01);
+    if (denom == 0.0f) {
+        b1 = 0.0f;
+        b2 = 0.0f;
+        return;
+    }
+    float inv = 1.0f / denom;
+    b1 = (d11 * d20 - d01 * d21) * inv;
+    b2 = (d00 * d21 - d01 * d20) * inv;
+}
+
+__global__ __launch_bounds__(256) void k_shade(const float4* __restrict__ points,
+                                               const float4* __restrict__ normals,
+                                               const float2* __restrict__ uvs,
+                                               const int* __restrict__ faces,
+                                               const float4* __restrict__ o_tmin,
+                                               const float4* __restrict__ d_tmax,
+                                               const int* __restrict__ prim,
+                                               const float* __restrict__ tin,
+                                               const int* __restrict__ d_count, int capacity,
+                                               float4* p_out, float4* n_out, float4* ng_out,
+                                               float2* uv_out, int* mat_out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int count = *d_count;
+    if (count > capacity) count = capacity;
+    if (i >= count) return;
+    int c = prim[i];
+    if (c < 0) {
+        mat_out[i] = -1;
+        float4 z = make_float4(0, 0, 0, 0);
+        p_out[i] = z;
+        n_out[i] = z;
+        ng_out[i] = z;
+        uv_out[i] = make_float2(0, 0);
+        return;
+    }
+    const int* F = faces + 10 * c;
+    float4 dd = d_tmax[i], oo = o_tmin[i];
+    v3 d = mk3(dd.x, dd.y, dd.z), o = mk3(oo.x, oo.y, oo.z);
+    v3 p1 = xyz(points[F[0]]), p2 = xyz(points[F[1]]), p3 = xyz(points[F[2]]);
+    float tt = tin[i];
+    v3 p = mk3(o.x + tt * d.x, o.y + tt * d.y, o.z + tt * d.z); // getPoint, Util.cl:558-565
+    float texu = 0.0f, texv = 0.0f;
+    bool has_uv = F[3] > -1 && uvs;
+    if (has_uv) { // triangleBarycentricsFromUVMesh, Primitive.cl:56-71
+        v3 f1 = sub3(p1, p), f2 = sub3(p2, p), f3 = sub3(p3, p);
+        float a = len3(cross3(sub3(p1, p2), sub3(p1, p3)));
+        float a1 = len3(cross3(f2, f3)) / a;
+        float a2 = len3(cross3(f3, f1)) / a;
+        float a3 = len3(cross3(f1, f2)) / a;
+        float2 u1 = uvs[F[3]], u2 = uvs[F[4]], u3 = uvs[F[5]];
+        texu = (u1.x * a1 + u2.x * a2) + u3.x * a3;
+        texv = (u1.y * a1 + u2.y * a2) + u3.y * a3;
+    }
+    float u, v;
+    barycentrics(p, p1, p2, p3, u, v);
+    v3 n;
+    if (F[6] > -1 && normals) {
+        v3 n1 = xyz(normals[F[6]]), n2 = xyz(normals[F[7]]), n3 = xyz(normals[F[8]]);
+        float w0 = (1.0f - u) - v;
+        n = add3(add3(scl3(n1, w0), scl3(n2, u)), scl3(n3, v));
+        if (dot3(n, d) > 0.00001f) {
+            n = neg3(n);
+            n1 = neg3(n1);
+            n2 = neg3(n2);
+            n3 = neg3(n3);
+        }
+        n = norm3(n);
+        // Hanika shadow-terminator offset, NormalBVH.cl:175-193
+        v3 tu = sub3(p, p1), tv = sub3(p, p2), tw = sub3(p, p3);
+        v3 nn1 = norm3(n1), nn2 = norm3(n2), nn3 = norm3(n3);
+        float du = fminf(0.0f, dot3(tu, nn1));
+        float dv = fminf(0.0f, dot3(tv, nn2));
+        float dw = fminf(0.0f, dot3(tw, nn3));
+        tu = sub3(tu, scl3(nn1, du));
+        tv = sub3(tv, scl3(nn2, dv));
+        tw = sub3(tw, scl3(nn3, dw));
+        p = add3(add3(add3(p, scl3(tu, w0)), scl3(tv, u)), scl3(tw, v));
+    } else {
+        n = norm3(cross3(sub3(p2, p1), sub3(p3, p1))); // getNormal, Util.cl:349-357
+        if (dot3(n, d) > 0.00001f) n = neg3(n);
+    }
+    v3 ng = norm3(cross3(sub3(p2, p1), sub3(p3, p1)));
+    if (dot3(ng, d) > 0.00001f) ng = neg3(ng);
+    p_out[i] = make_float4(p.x, p.y, p.z, 0.0f);
+    n_out[i] = make_float4(n.x, n.y, n.z, 0.0f);
+    ng_out[i] = make_float4(ng.x, ng.y, ng.z, 0.0f);
+    uv_out[i] = has_uv ? make_float2(texu, texv) : make_float2(u, v);
+    mat_out[i] = F[9] & 0xFFFF;
+}
+
+__global__ __launch_bounds__(256) void k_shadow_rays(const float4* __restrict__ o_tmin,
+                                                     const float4* __restrict__ d_tmax,
+                                                     const int* __restrict__ prim,
+                                                     const float* __restrict__ tin, float lx,
+                                                     float ly, float lz,
+                                                     const int* __restrict__ d_count,
+                                                     int capacity, float4* so, float4* sd,
+                                                     int* valid)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int count = *d_count;
+    if (count > capacity) count = capacity;
+    if (i >= count) return;
+    bool ok = prim[i] >= 0;
+    float tt = ok ? tin[i] : 0.0f;
+    float4 o = o_tmin[i], d = d_tmax[i];
+    float px = o.x + tt * d.x, py = o.y + tt * d.y, pz = o.z + tt * d.z;
+    float vx = lx - px, vy = ly - py, vz = lz - pz;
+    float dist = sqrtf((vx * vx + vy * vy) + vz * vz);
+    so[i] = make_float4(px, py, pz, 0.0001f);
+    sd[i] = make_float4(vx / dist, vy / dist, vz / dist, dist - 2 * MT_EPS_RAY);
+    valid[i] = ok;
+}
+
+} // namespace
+
+extern "C" int32_t mt_scene_create(mt_context* ctx, const mt_scene_desc* desc, mt_scene** out)
+{
+    MT_ARG_CHECK(ctx && desc && out, "mt_scene_create: NULL argument");
+    *out = nullptr;
+    MT_ARG_CHECK(desc->num_tris >= 1 && desc->num_points >= 3, "mt_scene_create: empty scene");
+    MT_ARG_CHECK(desc->num_tris <= (1 << 30), "mt_scene_create: too many triangles");
+    MT_ARG_CHECK(desc->points && desc->faces && desc->nodes && desc->bounds,
+                 "mt_scene_create: missing scene array");
+    if (desc->max_depth > 128) {
+        mt_set_error("mt_scene_create: BVH depth %d exceeds the 128-level traversal bitstack",
+                     desc->max_depth);
+        return MT_ERR_DEPTH;
+    }
+    int nn = 2 * desc->num_tris - 1;
+    int ninterior = desc->num_tris - 1;
+    mt_scene* sc = new mt_scene();
+    sc->ctx = ctx;
+    sc->desc = *desc;
+    sc->num_nodes = nn;
+    sc->num_interior = ninterior;
+    sc->pnodes = nullptr;
+    sc->ptris = nullptr;
+    int* work = nullptr; // [nn] flags, [nn] ranks, [2] err/total
+    auto fail = [&](int32_t code) {
+        if (work) hipFree(work);
+        if (sc->pnodes) hipFree(sc->pnodes);
+        if (sc->ptris) hipFree(sc->ptris);
+        delete sc;
+        return code;
+    };
+    if (hipMalloc(&work, sizeof(int) * (2 * (size_t)nn + 4)) != hipSuccess ||
+        hipMalloc(&sc->pnodes, sizeof(float4) * 4 * (size_t)(ninterior > 0 ? ninterior : 1)) != hipSuccess ||
+        hipMalloc(&sc->ptris, sizeof(float4) * 3 * (size_t)desc->num_tris) != hipSuccess) {
+        mt_set_error("mt_scene_create: device allocation failed (%d triangles)", desc->num_tris);
+        return fail(MT_ERR_ALLOC);
+    }
+    int* flags = work;
+    int* ranks = work + nn;
+    int* err = work + 2 * (size_t)nn;
+    int* total = err + 1;
+    hipStream_t s = ctx->stream;
+    int32_t rc = mt_buf_fill_i32(ctx, err, 0, 2);
+    if (rc) return fail(rc);
+    k_validate<<<mt_blocks(nn, 256), 256, 0, s>>>(desc->nodes, nn, desc->faces, desc->num_tris,
+                                                  desc->num_points, err);
+    int herr = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        mt_set_error("mt_scene_create: validation launch failed");
+        return fail(MT_ERR_HIP);
+    }
+    if (herr) {
+        mt_set_error("mt_scene_create: invalid scene arrays (error mask 0x%x: 1 node link, "
+                     "2 face vertex, 4 leaf primitive, 8 bound index)", herr);
+        return fail(MT_ERR_ARG);
+    }
+    k_depth<<<mt_blocks(nn, 256), 256, 0, s>>>(desc->nodes, nn, err);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        mt_set_error("mt_scene_create: depth check launch failed");
+        return fail(MT_ERR_HIP);
+    }
+    if (herr) {
+        mt_set_error("mt_scene_create: BVH deeper than 128 levels or parent links broken");
+        return fail(MT_ERR_DEPTH);
+    }
+    k_leaf_flags<<<mt_blocks(nn, 256), 256, 0, s>>>(desc->nodes, nn, flags);
+    rc = mt_buf_fill_i32(ctx, total, nn, 1); // device-side length for the scan
+    if (rc) return fail(rc);
+    rc = mt_scan_excl_i32(ctx, flags, ranks, total, nn, total + 1);
+    if (rc) return fail(rc);
+    k_pack<<<mt_blocks(nn, 256), 256, 0, s>>>(desc->nodes, desc->bounds, desc->faces,
+                                              reinterpret_cast<const float4*>(desc->points), nn,
+                                              ranks, sc->pnodes, sc->ptris);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        mt_set_error("mt_scene_create: packing failed");
+        return fail(MT_ERR_HIP);
+    }
+    hipFree(work);
+    sc->root_ref = desc->num_tris == 1 ? ~0 : 0;
+    *out = sc;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_scene_destroy(mt_scene* sc)
+{
+    if (!sc) return MT_OK;
+    hipStreamSynchronize(sc->ctx->stream);
+    if (sc->pnodes) hipFree(sc->pnodes);
+    if (sc->ptris) hipFree(sc->ptris);
+    delete sc;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_trace_closest(mt_context* ctx, const mt_scene* scene, const float* o_tmin,
+                                    const float* d_tmax, const int32_t* d_count, int32_t capacity,
+                                    int32_t* prim, float* t, int32_t* stats, int32_t mode)
+{
+    return launch_trace<false>(ctx, scene, o_tmin, d_tmax, d_count, capacity, prim, t, nullptr,
+                               stats, mode);
+}
+
+extern "C" int32_t mt_trace_any(mt_context* ctx, const mt_scene* scene, const float* o_tmin,
+                                const float* d_tmax, const int32_t* d_count, int32_t capacity,
+                                int32_t* occluded, int32_t* stats, int32_t mode)
+{
+    return launch_trace<true>(ctx, scene, o_tmin, d_tmax, d_count, capacity, nullptr, nullptr,
+                              occluded, stats, mode);
+}
+
+extern "C" int32_t mt_shade_hits(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
+                                 const float* d_tmax, const int32_t* prim, const float* t,
+                                 const int32_t* d_count, int32_t capacity, float* p, float* n,
+                                 float* ng, float* uv, int32_t* mat)
+{
+    MT_ARG_CHECK(ctx && sc && o_tmin && d_tmax && prim && t && d_count && p && n && ng && uv && mat,
+                 "mt_shade_hits: NULL argument");
+    if (capacity <= 0) return MT_OK;
+    k_shade<<<mt_blocks(capacity, 256), 256, 0, ctx->stream>>>(
+        reinterpret_cast<const float4*>(sc->desc.points),
+        reinterpret_cast<const float4*>(sc->desc.normals),
+        reinterpret_cast<const float2*>(sc->desc.uvs), sc->desc.faces,
+        reinterpret_cast<const float4*>(o_tmin), reinterpret_cast<const float4*>(d_tmax), prim, t,
+        d_count, capacity, reinterpret_cast<float4*>(p), reinterpret_cast<float4*>(n),
+        reinterpret_cast<float4*>(ng), reinterpret_cast<float2*>(uv), mat);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_shadow_rays(mt_context* ctx, const float* o_tmin, const float* d_tmax,
+                                  const int32_t* prim, const float* t, const float* light_xyz,
+                                  const int32_t* d_count, int32_t capacity, float* s_o_tmin,
+                                  float* s_d_tmax, int32_t* valid)
+{
+    MT_ARG_CHECK(ctx && o_tmin && d_tmax && prim && t && light_xyz && d_count && s_o_tmin &&
+                     s_d_tmax && valid,
+                 "mt_shadow_rays: NULL argument");
+    if (capacity <= 0) return MT_OK;
+    k_shadow_rays<<<mt_blocks(capacity, 256), 256, 0, ctx->stream>>>(
+        reinterpret_cast<const float4*>(o_tmin), reinterpret_cast<const float4*>(d_tmax), prim, t,
+        light_xyz[0], light_xyz[1], light_xyz[2], d_count, capacity,
+        reinterpret_cast<float4*>(s_o_tmin), reinterpret_cast<float4*>(s_d_tmax), valid);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
