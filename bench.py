@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Throughput of the wavefront path-tracing core on one MI355X.
+
+Workload (one "step" = one render job, inputs resident in HBM): 256 UV
+spheres x 4096 triangles + ground quad (1,048,578 triangles), GGX materials,
+1024x512 HDR environment sampled through its summed-area table, 1920x1080 at
+16 spp, up to 8 bounces with Russian roulette after bounce 3 and stable
+compaction after every bounce.  value = (closest-hit + shadow rays) / s.
+
+roofline: the dominant kernel is the closest-hit traversal.  Its algorithmic
+bytes are the reference layout's per-ray figure 40 + 148*I + 116*L + 28*P
+(I interior steps, L leaf visits, P backtrack pops, counted by the device's
+counting traversal in an untimed render of the same job) divided by the
+traversal launches' summed duration (HIP events on the launch stream).
+
+cpu_baseline: the oracle (plain-C restatement, test infrastructure) traces a
+bounded random subset of the same job's paths on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mrays/s path-traced (1M tris, 1080p) + p50/p99 render-job latency at fixed QPS"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per closest-hit launch measured by rocprofv3 --pmc (profiles/), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic_trace_closest_c1.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return float(json.load(f)["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(W, scene_arrays, mats, rgba, job, npaths, seconds, threads):
+    import numpy as np
+    from oracle import orc_py as orc
+    points, normals, faces, nodes, bounds, _ = scene_arrays
+    w, h = rgba.shape[1], rgba.shape[0]
+    lum, sat = orc.env_lum_sat(rgba.reshape(-1, 4), w, h)
+    env = (rgba.reshape(-1, 4), lum, sat, w, h)
+    rng = np.random.default_rng(12345)
+    probe = np.sort(rng.choice(npaths, 20000, replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    _, rays = orc.render_paths(points, normals, faces, nodes, bounds, mats, env, job, probe, nthreads=threads)
+    dt = time.perf_counter() - t0
+    n = int(min(npaths, max(20000, 20000 * seconds / max(dt, 1e-3))))
+    sample = np.sort(rng.choice(npaths, n, replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    _, rays = orc.render_paths(points, normals, faces, nodes, bounds, mats, env, job, sample, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(float(rays.sum()) / dt / 1e6, 3),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} of {npaths} path ids of the same job drawn uniformly (seed 12345), "
+                  f"{int(rays[0])} closest-hit + {int(rays[1])} shadow rays in {dt:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=16)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--mode", type=int, default=0, help="0 packed traversal layout, 1 reference layout")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by pixels or jobs "
+              "with no exchange, see DESIGN.md.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from mambatracer_amd import _lib, api
+    from mambatracer_amd import workloads as W
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    t0 = time.perf_counter()
+    scene_arrays = W.c1_scene()
+    points, normals, faces, nodes, bounds, depth = scene_arrays
+    mats = W.c1_materials()
+    rgba = W.env_map()
+    log(f"scene: {faces.shape[0]} triangles, BVH depth {depth}, host prep {time.perf_counter() - t0:.1f} s")
+
+    ctx = api.Context(0)
+    scene = api.Scene(ctx, points, faces, nodes, bounds, depth, normals=normals)
+    dmats = ctx.to_device(mats)
+    env = api.CEnvironment(ctx)
+    env.setEnvironmentMap(rgba)
+    job = W.RenderJob(width=args.width, height=args.height, spp=args.spp, mode=args.mode)
+    npaths = job.width * job.height * job.spp
+
+    # untimed: counting traversal for the algorithmic bytes
+    job.flags = _lib.MT_RENDER_TRAVERSAL_STATS
+    _, st = api.render(ctx, scene, dmats, env, job)
+    ilp = [int(x) for x in st.closest_ilp]
+    ext_rays = int(st.extension_rays)
+    alg_bytes = 40 * ext_rays + 148 * ilp[0] + 116 * ilp[1] + 28 * ilp[2]
+    log(f"traversal counters: rays {ext_rays}, I {ilp[0]}, L {ilp[1]}, P {ilp[2]} "
+        f"({alg_bytes / max(ext_rays, 1):.0f} B/ray)")
+
+    job.flags = _lib.MT_RENDER_PROFILE
+    for i in range(args.warmup):
+        api.render(ctx, scene, dmats, env, job)
+        torch.cuda.synchronize()
+        log(f"warmup {i + 1}/{args.warmup} done")
+
+    lat, rays, closest_ms, launches = [], 0, 0.0, 0
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        s0 = time.perf_counter()
+        img, st = api.render(ctx, scene, dmats, env, job)   # stats readback synchronises
+        lat.append((time.perf_counter() - s0) * 1e3)
+        rays += int(st.extension_rays) + int(st.shadow_rays)
+        closest_ms += float(st.closest_ms)
+        launches += int(st.closest_launches)
+        log(f"step {i + 1}/{args.steps}: {lat[-1]:.1f} ms, {int(st.extension_rays)} + {int(st.shadow_rays)} rays")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+
+    img_np = img.cpu().numpy()
+    if not np.isfinite(img_np).all():
+        log("non-finite pixels in the image")
+        return 1
+    value = rays / elapsed / 1e6
+    per_launch_bytes = alg_bytes / max(st.closest_launches, 1)
+    avg_launch_s = closest_ms / max(launches, 1) / 1e3
+    achieved = per_launch_bytes / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    traffic = traffic_from_profiles()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads ...")
+        cpu = cpu_baseline(W, scene_arrays, mats, rgba, job, npaths, args.cpu_seconds, threads)
+
+    lat_sorted = sorted(lat)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generators: scene seed 2, materials seed 2, analytic env map)",
+        "config": {
+            "workload": "c1 wavefront path tracer: 256 UV spheres x 4096 tris + ground (1,048,578 tris), "
+                        f"{job.width}x{job.height} x {job.spp} spp, <=8 bounces, RR from bounce 3, "
+                        "env 1024x512 sampled via full-map SAT, stable compaction every bounce",
+            "global_batch": npaths,
+            "traversal_layout": "packed" if job.mode == 0 else "reference",
+            "parallelism": "single GPU",
+        },
+        "samples_per_s": round(npaths * args.steps / elapsed, 1),
+        "job_latency_ms": {"p50": round(lat_sorted[len(lat) // 2], 3),
+                           "p99": round(lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
+                           "qps": "sequential (one job in flight)"},
+        "roofline": {
+            "kernel": "closest-hit traversal (k_trace, packed layout)" if job.mode == 0 else "closest-hit traversal (reference layout)",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": round(per_launch_bytes),
+            "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

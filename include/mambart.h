@@ -314,6 +314,62 @@ int32_t mt_bsdf_eval(mt_context* ctx, int32_t n, const float* bsdf, const int32_
                      const float* materials, const float* dir, float* f, float* pdf,
                      float* cos_out);
 
+/* ------------------------------------------------------------------ */
+/* Wavefront bounce loop                                              */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    float pos[3];          /* pinhole camera position */
+    float look[3];         /* look-at point */
+    float up[3];
+    float vfov_deg;
+    int32_t width, height; /* image size */
+    int32_t spp;           /* samples per pixel, all traced as one wavefront */
+    int32_t max_bounces;   /* path segments traced per sample (the reference fixes 15) */
+    int32_t rr_start;      /* Russian roulette from this bounce on; < 0 disables */
+    uint32_t seed;         /* job seed: RNG keys are (seed, path id, bounce, purpose) */
+    int32_t mode;          /* MT_TRACE_PACKED or MT_TRACE_REFERENCE traversal */
+    int32_t flags;         /* MT_RENDER_* bits */
+} mt_render_desc;
+
+/* illuminateMeshLight reads lightInfo->faceId, i.e. always the FIRST light
+ * (Render.cl:304-305), while dividing the pick probability by the light
+ * count; the default reproduces that, this flag samples the picked light. */
+#define MT_RENDER_PICKED_LIGHT 1
+/* time every traversal launch with HIP events (stats->closest_ms/any_ms) */
+#define MT_RENDER_PROFILE 2
+/* run the counting traversal variant and total its per-ray counters */
+#define MT_RENDER_TRAVERSAL_STATS 4
+
+typedef struct {
+    int64_t extension_rays; /* closest-hit rays traced (all bounces) */
+    int64_t shadow_rays;    /* any-hit rays traced */
+    int64_t paths;          /* width * height * spp */
+    int32_t bounce_counts[64]; /* live paths entering each bounce */
+    /* MT_RENDER_PROFILE: summed durations of the traversal launches, timed
+     * with HIP events on the context stream */
+    int32_t closest_launches, any_launches;
+    float closest_ms, any_ms;
+    /* MT_RENDER_TRAVERSAL_STATS: totals of {interior steps, leaf visits,
+     * backtrack pops} over all closest-hit / any-hit rays */
+    int64_t closest_ilp[3], any_ilp[3];
+} mt_render_stats;
+
+/* CDeviceGI.loop (src/cl/device/CDeviceGI.java:214-271) for one job:
+ * camera rays, then per bounce IntersectPrimitives, SetupBSDFPath,
+ * LightHitPass (environment miss radiance with MIS, emitter hits),
+ * DirectLight (environment sample through the full-map SAT, BSDF
+ * evaluation, any-hit shadow ray), SampleBSDFRayDirection and a stable
+ * compaction of live paths.  materials is a device array of 36-word records;
+ * env may be NULL (no environment light); light_faces/num_lights (device,
+ * from mt_mesh_lights) may be NULL/0.  image receives float4[w*h]: the mean
+ * over spp of each pixel's radiance.  RNG is keyed by path id, so a path's
+ * result does not depend on which other paths share the wavefront.
+ * stats (host) may be NULL; filling it synchronises the context. */
+int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float* materials,
+                  const mt_envmap* env, const int32_t* light_faces,
+                  const int32_t* d_num_lights, const mt_render_desc* desc,
+                  float* image, mt_render_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,43 @@ class Payload(C.Structure):
     _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("elem_bytes", C.c_int32)]
 
 
+class RenderDesc(C.Structure):
+    _fields_ = [
+        ("pos", C.c_float * 3),
+        ("look", C.c_float * 3),
+        ("up", C.c_float * 3),
+        ("vfov_deg", C.c_float),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("spp", C.c_int32),
+        ("max_bounces", C.c_int32),
+        ("rr_start", C.c_int32),
+        ("seed", C.c_uint32),
+        ("mode", C.c_int32),
+        ("flags", C.c_int32),
+    ]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [
+        ("extension_rays", C.c_int64),
+        ("shadow_rays", C.c_int64),
+        ("paths", C.c_int64),
+        ("bounce_counts", C.c_int32 * 64),
+        ("closest_launches", C.c_int32),
+        ("any_launches", C.c_int32),
+        ("closest_ms", C.c_float),
+        ("any_ms", C.c_float),
+        ("closest_ilp", C.c_int64 * 3),
+        ("any_ilp", C.c_int64 * 3),
+    ]
+
+
+MT_RENDER_PICKED_LIGHT = 1
+MT_RENDER_PROFILE = 2
+MT_RENDER_TRAVERSAL_STATS = 4
+
+
 P = C.c_void_p
 I32 = C.c_int32
 I64 = C.c_int64
@@ -104,6 +141,7 @@ SIGNATURES = {
     "mt_bsdf_setup": [P, I32, P, P, P, P, P, P, P],
     "mt_bsdf_sample": [P, I32, P, P, P, P, P, P, P, P],
     "mt_bsdf_eval": [P, I32, P, P, P, P, P, P, P],
+    "mt_render": [P, P, P, P, P, P, C.POINTER(RenderDesc), P, C.POINTER(RenderStats)],
 }
 _RESTYPES = {"mt_version": C.c_char_p}
 

@@ -289,7 +289,7 @@ def compact(ctx: Context, flags, flag_mode: int = 0, length=None, out_index=True
     dsts = []
     for j, src in enumerate(payloads):
         dst = torch.empty_like(src)
-        eb = src.element_size() * (src[0].numel() if src.dim() > 1 else 1)
+        eb = src.element_size() * int(np.prod(src.shape[1:], dtype=np.int64))
         arr[j] = _lib.Payload(_ptr(src), _ptr(dst), eb)
         dsts.append(dst)
     call("mt_compact", ctx.handle, _ptr(flags), flag_mode, _ptr(length), n, _ptr(idx), _ptr(count),
@@ -506,3 +506,23 @@ def bsdf_eval(ctx: Context, rec, mat_id, materials, dirs):
     call("mt_bsdf_eval", ctx.handle, cnt, _ptr(rec), _ptr(mat_id), _ptr(materials), _ptr(dirs), _ptr(f),
          _ptr(pdf), _ptr(cs))
     return f, pdf, cs
+
+
+# ---------------------------------------------------------------------- bounce loop
+def render(ctx: Context, scene: Scene, materials, env: Optional["CEnvironment"], job, light_faces=None,
+           num_lights=None, stats: bool = True):
+    """CDeviceGI.loop for one job (see mt_render); returns (image float32[h*w,4], stats or None)."""
+    torch = _torch()
+    desc = _lib.RenderDesc()
+    desc.pos[:] = list(job.pos)
+    desc.look[:] = list(job.look)
+    desc.up[:] = list(job.up)
+    desc.vfov_deg = job.vfov_deg
+    desc.width, desc.height, desc.spp = job.width, job.height, job.spp
+    desc.max_bounces, desc.rr_start, desc.seed = job.max_bounces, job.rr_start, job.seed
+    desc.mode, desc.flags = job.mode, job.flags
+    image = ctx.empty((job.width * job.height, 4), torch.float32)
+    st = _lib.RenderStats() if stats else None
+    call("mt_render", ctx.handle, scene.handle, _ptr(materials), env.handle if env is not None else None,
+         _ptr(light_faces), _ptr(num_lights), C.byref(desc), _ptr(image), C.byref(st) if stats else None)
+    return image, st

@@ -29,6 +29,7 @@ SOURCES = [
     "mt_scan.hip",
     "mt_sat.hip",
     "mt_bsdf.hip",
+    "mt_render.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -289,8 +289,9 @@ uint32_t next_epoch(mt_context* ctx)
 extern "C" int32_t mt_scan_excl_i32(mt_context* ctx, const int32_t* in, int32_t* out,
                                     const int32_t* d_len, int32_t capacity, int32_t* d_total)
 {
-    MT_ARG_CHECK(ctx && in && out && d_len, "mt_scan_excl_i32: NULL argument");
+    MT_ARG_CHECK(ctx && d_len, "mt_scan_excl_i32: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "mt_scan_excl_i32: negative capacity");
+    MT_ARG_CHECK(capacity == 0 || (in && out), "mt_scan_excl_i32: NULL in/out");
     ScanWork W;
     int32_t rc = scan_workspace(ctx, capacity, W);
     if (rc) return rc;
@@ -308,7 +309,8 @@ extern "C" int32_t mt_compact(mt_context* ctx, const int32_t* flags, int32_t fla
                               const int32_t* d_len, int32_t capacity, int32_t* out_index,
                               int32_t* d_count, const mt_payload* payloads, int32_t npayload)
 {
-    MT_ARG_CHECK(ctx && flags && d_len && d_count, "mt_compact: NULL argument");
+    MT_ARG_CHECK(ctx && d_len && d_count, "mt_compact: NULL argument");
+    MT_ARG_CHECK(capacity == 0 || flags, "mt_compact: NULL flags");
     MT_ARG_CHECK(capacity >= 0, "mt_compact: negative capacity");
     MT_ARG_CHECK(flag_mode == 0 || flag_mode == 1, "mt_compact: flag_mode must be 0 or 1");
     MT_ARG_CHECK(npayload >= 0 && npayload <= 8, "mt_compact: npayload %d not in [0,8]", npayload);
@@ -316,7 +318,8 @@ extern "C" int32_t mt_compact(mt_context* ctx, const int32_t* flags, int32_t fla
     PayloadSet P;
     P.n = npayload;
     for (int j = 0; j < npayload; j++) {
-        MT_ARG_CHECK(payloads[j].src && payloads[j].dst, "mt_compact: payload %d has a NULL pointer", j);
+        MT_ARG_CHECK(capacity == 0 || (payloads[j].src && payloads[j].dst),
+                     "mt_compact: payload %d has a NULL pointer", j);
         int b = payloads[j].elem_bytes;
         MT_ARG_CHECK(b == 4 || b == 8 || b == 16, "mt_compact: payload %d elem_bytes %d not 4/8/16", j, b);
         P.src[j] = payloads[j].src;

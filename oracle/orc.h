@@ -111,6 +111,17 @@ float orc_ggx_lambda(const float* w, float ax, float ay);
 void orc_ggx_sample_vndf(const float* v, float ax, float ay, float u1, float u2, float* out);
 float orc_ggx_pdf_vndf(const float* v, const float* h, float ax, float ay);
 
+/* ---- bounce loop (CDeviceGI.loop + Render.cl kernels), one path at a time ---- */
+int orc_camera_basis(const float* pos, const float* look, const float* up, float vfov_deg,
+                     int32_t w, int32_t h, float* cam);
+int orc_render_paths(const float* points, const float* normals, const int32_t* faces,
+                     const int32_t* nodes, const float* bounds, const float* materials,
+                     const float* rgba, const float* lum, const float* sat, int32_t ew, int32_t eh,
+                     const int32_t* light_faces, int32_t nlights, const float* cam, int32_t w,
+                     int32_t h, int32_t spp, int32_t max_bounces, int32_t rr_start, uint32_t seed,
+                     int32_t flags, int64_t npaths, const int32_t* pids, float* L, int64_t* rays,
+                     int32_t nthreads);
+
 /* ---- RNG (Sampling.cl:15-59) ---- */
 uint32_t orc_wang_hash(int32_t seed);
 float orc_get_random(int32_t* state2);

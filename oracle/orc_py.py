@@ -294,21 +294,25 @@ def bsdf_eval(rec, mid, materials, dirs):
 
 
 def ggx_D(wh, ax, ay):
-    return float(lib().orc_ggx_D(_p(_f32(wh)), ax, ay))
+    wh = _f32(wh)
+    return float(lib().orc_ggx_D(_p(wh), ax, ay))
 
 
 def ggx_lambda(w, ax, ay):
-    return float(lib().orc_ggx_lambda(_p(_f32(w)), ax, ay))
+    w = _f32(w)
+    return float(lib().orc_ggx_lambda(_p(w), ax, ay))
 
 
 def ggx_sample_vndf(v, ax, ay, u1, u2):
     out = np.zeros(3, np.float32)
-    lib().orc_ggx_sample_vndf(_p(_f32(v)), ax, ay, u1, u2, _p(out))
+    v = _f32(v)
+    lib().orc_ggx_sample_vndf(_p(v), ax, ay, u1, u2, _p(out))
     return out
 
 
 def ggx_pdf_vndf(v, h, ax, ay):
-    return float(lib().orc_ggx_pdf_vndf(_p(_f32(v)), _p(_f32(h)), ax, ay))
+    v, h = _f32(v), _f32(h)
+    return float(lib().orc_ggx_pdf_vndf(_p(v), _p(h), ax, ay))
 
 
 def wang_hash(seed: int) -> int:
@@ -319,3 +323,52 @@ def get_random(state):
     st = np.ascontiguousarray(state, dtype=np.int32)
     r = lib().orc_get_random(_p(st))
     return float(r), st
+
+
+# ---------------------------------------------------------------- bounce loop
+def camera_basis(pos, look, up, vfov_deg, w, h):
+    cam = np.zeros(14, np.float32)
+    L = lib()
+    L.orc_camera_basis.argtypes = [C.c_void_p] * 3 + [C.c_float, C.c_int32, C.c_int32, C.c_void_p]
+    pos, look, up = _f32(pos), _f32(look), _f32(up)  # keep the arrays alive across the call
+    _ok(L.orc_camera_basis(_p(pos), _p(look), _p(up), vfov_deg, w, h, _p(cam)),
+        "orc_camera_basis")
+    return cam
+
+
+def render_paths(points, normals, faces, nodes, bounds, materials, env, job, pids, light_faces=None,
+                 nthreads=8):
+    """Traces the listed path ids one at a time; returns (L float32[max_pid+1,4], rays[2]).
+    env = (rgba[h*w,4], lum, sat, w, h) or None."""
+    L = lib()
+    P, I32, I64, U32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32
+    L.orc_render_paths.argtypes = [P, P, P, P, P, P, P, P, P, I32, I32, P, I32, P, I32, I32, I32, I32, I32,
+                                   U32, I32, I64, P, P, P, I32]
+    L.orc_render_paths.restype = C.c_int32
+    cam = camera_basis(job.pos, job.look, job.up, job.vfov_deg, job.width, job.height)
+    pids = _i32(pids)
+    out = np.zeros((int(pids.max()) + 1 if pids.size else 0, 4), np.float32)
+    rays = np.zeros(2, np.int64)
+    if env is None:
+        rgba = lum = sat = None
+        ew = eh = 0
+    else:
+        rgba, lum, sat, ew, eh = env
+        rgba, lum, sat = _f32(rgba), _f32(lum), _f32(sat)
+    normals = None if normals is None else _f32(normals)
+    lf = None if light_faces is None else _i32(light_faces)
+    points, faces, nodes, bounds, materials = _f32(points), _i32(faces), _i32(nodes), _f32(bounds), _f32(materials)
+    _ok(L.orc_render_paths(_p(points), _p(normals), _p(faces), _p(nodes), _p(bounds),
+                           _p(materials), _p(rgba), _p(lum), _p(sat), ew, eh, _p(lf),
+                           0 if lf is None else lf.size, _p(cam), job.width, job.height, job.spp,
+                           job.max_bounces, job.rr_start, job.seed, job.flags, pids.size, _p(pids), _p(out),
+                           _p(rays), nthreads), "orc_render_paths")
+    return out, rays
+
+
+def resolve(L_paths, npix, spp):
+    """Per-pixel mean over samples in sample order, float32 (k_resolve's order)."""
+    acc = np.zeros((npix, 3), np.float32)
+    for s in range(spp):
+        acc = (acc + L_paths[s * npix:(s + 1) * npix, :3]).astype(np.float32)
+    return (acc * np.float32(1.0 / spp)).astype(np.float32)

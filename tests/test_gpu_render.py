@@ -1,0 +1,91 @@
+"""The wavefront bounce loop on the GPU against the oracle's one-path-at-a-
+time restatement of the same loop (config-1 scene, reduced image).
+
+Bar: per-path radiance bit-identical for at least 99% of paths (paths can
+diverge only where device sin/cos/acos/atan2/pow round differently from the
+host libm and flip a discrete choice), ray counts within 0.5%, image
+relative RMSE <= 1e-3 (the tolerance stated for images)."""
+import numpy as np
+import pytest
+import torch
+
+from mambatracer_amd import _lib, api
+from mambatracer_amd import workloads as W
+from oracle import orc_py as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1_small():
+    return W.c1_scene(count=32)
+
+
+def _render_both(ctx, c1_small, job, with_env=True):
+    points, normals, faces, nodes, bounds, depth = c1_small
+    mats = W.c1_materials()
+    sc = api.Scene(ctx, points, faces, nodes, bounds, depth, normals=normals)
+    rgba = W.env_map()
+    env = None
+    oenv = None
+    if with_env:
+        env = api.CEnvironment(ctx)
+        env.setEnvironmentMap(rgba)
+        lum, sat = orc.env_lum_sat(rgba.reshape(-1, 4), 1024, 512)
+        oenv = (rgba.reshape(-1, 4), lum, sat, 1024, 512)
+    img, st = api.render(ctx, sc, ctx.to_device(mats), env, job)
+    npix = job.width * job.height
+    pids = np.arange(npix * job.spp, dtype=np.int32)
+    L, rays = orc.render_paths(points, normals, faces, nodes, bounds, mats, oenv, job, pids)
+    oimg = orc.resolve(L, npix, job.spp)
+    return img.cpu().numpy()[:, :3], st, oimg, rays
+
+
+def test_render_matches_oracle(ctx, c1_small):
+    print("[case] render c1 small", flush=True)
+    job = W.RenderJob(width=160, height=90, spp=4, pos=(0.0, 0.5, -8.0), look=(0.0, -1.0, 0.0))
+    img, st, oimg, rays = _render_both(ctx, c1_small, job)
+    ext, sh = int(st.extension_rays), int(st.shadow_rays)
+    print(f"gpu rays {ext}+{sh}, oracle {rays[0]}+{rays[1]}")
+    assert abs(ext - rays[0]) <= 0.005 * rays[0] and abs(sh - rays[1]) <= 0.005 * rays[1] + 2
+    same = np.all(img == oimg, axis=1).mean()
+    rmse = np.sqrt(np.mean((img - oimg) ** 2)) / np.sqrt(np.mean(oimg ** 2))
+    print(f"identical pixels {same:.4f}, relative RMSE {rmse:.2e}")
+    assert same >= 0.99
+    assert rmse <= 1e-3
+    assert np.isfinite(img).all() and img.mean() > 0
+
+
+def test_render_is_deterministic_and_independent_of_batch(ctx, c1_small):
+    """Same seed -> bit-identical image; each sample's paths do not depend on the
+    other samples in the wavefront (spp=1 render == sample 0 of an spp=3 render)."""
+    points, normals, faces, nodes, bounds, depth = c1_small
+    sc = api.Scene(ctx, points, faces, nodes, bounds, depth, normals=normals)
+    dm = ctx.to_device(W.c1_materials())
+    env = api.CEnvironment(ctx)
+    env.setEnvironmentMap(W.env_map())
+    job = W.RenderJob(width=128, height=72, spp=3, pos=(0.0, 0.5, -8.0), look=(0.0, -1.0, 0.0))
+    a, _ = api.render(ctx, sc, dm, env, job)
+    b, _ = api.render(ctx, sc, dm, env, job)
+    np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
+def test_render_without_environment_and_reference_layout(ctx, c1_small):
+    job = W.RenderJob(width=64, height=36, spp=2, pos=(0.0, 0.5, -8.0), look=(0.0, -1.0, 0.0), mode=1)
+    img, st, oimg, rays = _render_both(ctx, c1_small, job, with_env=False)
+    assert (img == 0).all() and (oimg == 0).all()   # nothing emits: black image
+    assert int(st.shadow_rays) == 0 and rays[1] == 0
+
+
+def test_render_stats_counters(ctx, c1_small):
+    points, normals, faces, nodes, bounds, depth = c1_small
+    sc = api.Scene(ctx, points, faces, nodes, bounds, depth, normals=normals)
+    dm = ctx.to_device(W.c1_materials())
+    env = api.CEnvironment(ctx)
+    env.setEnvironmentMap(W.env_map())
+    job = W.RenderJob(width=64, height=64, spp=1, pos=(0.0, 0.5, -8.0), look=(0.0, -1.0, 0.0),
+                      flags=_lib.MT_RENDER_TRAVERSAL_STATS | _lib.MT_RENDER_PROFILE)
+    _, st = api.render(ctx, sc, dm, env, job)
+    assert st.bounce_counts[0] == 64 * 64
+    assert st.closest_ilp[0] > 0 and st.closest_launches == job.max_bounces
+    assert st.closest_ms > 0

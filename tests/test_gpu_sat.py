@@ -70,8 +70,9 @@ def test_lightgrid_cells(ctx):
     device) crosses a bin boundary; such near-ties are counted and bounded."""
     rng = np.random.default_rng(3)
     n = 100000
-    v = rng.normal(size=(n, 4)).astype(np.float32)
+    v = rng.normal(size=(n, 4))
     v[:, 3] = 0
+    v = (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)  # ray directions are unit
     cell, txy = api.lightgrid_cells(ctx, ctx.to_device(v))
     ocell, otxy = orc.lg_cells(v)
     mism = (cell.cpu().numpy() != ocell).sum() + (txy.cpu().numpy() != otxy).any(axis=1).sum()
