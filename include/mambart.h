@@ -159,6 +159,7 @@ int32_t mt_scene_destroy(mt_scene* scene);
 /* ------------------------------------------------------------------ */
 #define MT_TRACE_PACKED 0    /* packed 64-B node / 48-B triangle layout (default) */
 #define MT_TRACE_REFERENCE 1 /* reads the reference's flat arrays directly */
+#define MT_TRACE_PACKED_NO_STACK 2 /* packed, backtracking by parent links only */
 
 /* Closest hit (intersectMesh with bailout = false, NormalBVH.cl:14-92, as
  * run by IntersectPrimitives :94-123; launched at CDeviceGI.java:230 and

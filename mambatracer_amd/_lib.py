@@ -22,6 +22,7 @@ ERRORS = {
 }
 MT_TRACE_PACKED = 0
 MT_TRACE_REFERENCE = 1
+MT_TRACE_PACKED_NO_STACK = 2
 MT_BSDF_WORDS = 20
 
 

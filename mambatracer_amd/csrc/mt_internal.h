@@ -72,6 +72,9 @@ struct mt_envmap {
     const float* lum;
     const float* sat;
     int32_t w, h;
+    // library-owned sampling tables (see EnvView in mt_sat.h), one allocation
+    float* tab;
+    float *marg, *cond, *fint, *pmarg;
 };
 
 int32_t mt_ctx_scratch(mt_context* ctx, size_t bytes, void** out);
@@ -90,4 +93,23 @@ inline int mt_blocks(int64_t n, int threads)
 {
     int64_t b = (n + threads - 1) / threads;
     return b < 1 ? 1 : (int)b;
+}
+
+// Work-groups for a persistent launch: what the device keeps resident at
+// once (CUs x occupancy of `kernel`), or fewer when `work` items need fewer.
+template <class K>
+int mt_persistent_blocks(mt_context* ctx, K kernel, int threads, int64_t work)
+{
+    static thread_local int cached_dev = -1, cus = 0;
+    if (cached_dev != ctx->device) {
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess)
+            cus = 256;
+        cached_dev = ctx->device;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    int64_t need = (work + threads - 1) / threads;
+    int64_t cap = (int64_t)cus * per_cu;
+    return (int)(need < cap ? (need < 1 ? 1 : need) : cap);
 }

@@ -162,12 +162,11 @@ struct ShadeArgs {
     uint32_t seed;
 };
 
-__global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
+// One slot of the fused shading stage; returns whether a shadow ray was set.
+__device__ __forceinline__ bool shade_slot(const ShadeArgs& A, int slot)
 {
-    int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    int count = *A.d_count;
     bool has_shadow = false;
-    if (slot < count) {
+    {
         const SceneView& V = A.V;
         int pid = A.queue[slot];
         float4 ro = A.o_tmin[slot], rd = A.d_tmax[slot];
@@ -349,21 +348,35 @@ __global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
         A.s_d_tmax[slot] = sd;
         A.s_contrib[slot] = scontrib;
     }
-    uint64_t m = __ballot(has_shadow);
-    if (m && __lane_id() == 0) atomicAdd(A.shadow_count, (int)__popcll(m));
+    return has_shadow;
+}
+
+// Persistent: waves stride over the live slots [0, *d_count).
+__global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
+{
+    const int count = *A.d_count;
+    const int lane = __lane_id();
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64; base < count; base += nw * 64) {
+        int slot = base + lane;
+        bool sh = slot < count ? shade_slot(A, slot) : false;
+        uint64_t m = __ballot(sh);
+        if (m && lane == 0) atomicAdd(A.shadow_count, (int)__popcll(m));
+    }
 }
 
 __global__ __launch_bounds__(RT) void k_nee_accum(const int* __restrict__ queue, const int* __restrict__ d_count,
                                                   const float4* __restrict__ contrib,
                                                   const int* __restrict__ occluded, PathState S)
 {
-    int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= *d_count) return;
-    float4 c = contrib[slot];
-    if (c.w == 0.0f || occluded[slot]) return;
-    int pid = queue[slot];
-    float4 L = S.L[pid];
-    S.L[pid] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
+    const int count = *d_count;
+    for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < count; slot += gridDim.x * blockDim.x) {
+        float4 c = contrib[slot];
+        if (c.w == 0.0f || occluded[slot]) continue;
+        int pid = queue[slot];
+        float4 L = S.L[pid];
+        S.L[pid] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
+    }
 }
 
 __global__ __launch_bounds__(RT) void k_resolve(const float4* __restrict__ L, int npix, int spp,
@@ -496,10 +509,10 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
     V.mats = reinterpret_cast<const MtMaterial*>(materials);
     V.light_faces = light_faces;
     V.num_lights = d_num_lights;
-    if (env) V.env = EnvView{reinterpret_cast<const float4*>(env->rgba), env->lum, env->sat, env->w, env->h};
-    else V.env = EnvView{nullptr, nullptr, nullptr, 0, 0};
+    V.env = env_view(env);
 
     int grid = mt_blocks(np, RT);
+    const int gshade = grid, gnee = grid; // one slot per lane; empty work-groups retire at once
     unsigned long long* tot = reinterpret_cast<unsigned long long*>(base + oTot);
     int* ilp = want_ilp ? I(oIlp) : nullptr;
     hipEvent_t ev[4 * 64];
@@ -543,7 +556,7 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
         A.rr_start = desc->rr_start;
         A.flags = desc->flags;
         A.seed = desc->seed;
-        k_shade<<<grid, RT, 0, st>>>(A);
+        k_shade<<<gshade, RT, 0, st>>>(A);
         if ((rc = chk())) break;
         if (profile) hipEventRecord(ev[4 * b + 2], st);
         rc = mt_trace_any(ctx, scene, reinterpret_cast<float*>(F4(oSo)), reinterpret_cast<float*>(F4(oSd)), dcount,
@@ -551,7 +564,7 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
         if (rc) break;
         if (profile) hipEventRecord(ev[4 * b + 3], st);
         if (want_ilp) k_sum_ilp<<<grid, RT, 0, st>>>(ilp, dcount, tot + 3);
-        k_nee_accum<<<grid, RT, 0, st>>>(q[cur], dcount, F4(oSc), I(oOcc), S);
+        k_nee_accum<<<gnee, RT, 0, st>>>(q[cur], dcount, F4(oSc), I(oOcc), S);
         if ((rc = chk())) break;
         if (b + 1 < nb) {
             mt_payload P[3] = {{q[cur], q[1 - cur], 4}, {ro[cur], ro[1 - cur], 16}, {rd[cur], rd[1 - cur], 16}};

@@ -207,12 +207,61 @@ __device__ __forceinline__ void r_sample(const SatRegion& R, const float* __rest
 }
 
 // ---------------------------------------------------------------- environment light
+// Optional exact tables of the full-map quantities the sampler evaluates
+// from SAT corners (each entry is the same float expression, computed once):
+//   marg[y]   = getMarginal(y),                 y in [0, h]
+//   cond[y][x]= getConditional(x, y),           x in [0, w], row stride w+1
+//   fint[y]   = getFuncIntConditional(y),       y in [0, h]
+//   pmarg[y]  = getPdfContinuousMarginal(y),    y in [0, h]
 struct EnvView {
     const float4* rgba;
     const float* lum;
     const float* sat;
     int w, h;
+    const float* marg;  // null: evaluate from the SAT
+    const float* cond;
+    const float* fint;
+    const float* pmarg;
 };
+
+inline EnvView env_view(const mt_envmap* env)
+{
+    if (!env) return EnvView{nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr};
+    return EnvView{reinterpret_cast<const float4*>(env->rgba), env->lum, env->sat, env->w, env->h,
+                   env->marg, env->cond, env->fint, env->pmarg};
+}
+
+__device__ __forceinline__ void env_sample_tables(const EnvView& E, float u0, float u1, float& uu,
+                                                  float& vv, int& ox, int& oy, float& pdf)
+{ // sampleContinuous (SAT.cl:244-323) reading the tables above
+    int b = 0, e = E.h;
+    while (b != e) {
+        int mid = b + (e - b) / 2;
+        bool right = !(u1 < E.marg[mid]);
+        b = right ? mid + 1 : b;
+        e = right ? e : mid;
+    }
+    oy = b - 1 > 0 ? b - 1 : 0;
+    float m0 = E.marg[oy];
+    float du = (u1 - m0) / (E.marg[oy + 1] - m0);
+    float pdf_v = E.pmarg[oy];
+    vv = ((float)oy + du) / (float)E.h;
+    const float* C = E.cond + (size_t)oy * (E.w + 1);
+    b = 0;
+    e = E.w;
+    while (b != e) {
+        int mid = b + (e - b) / 2;
+        bool right = !(u0 < C[mid]);
+        b = right ? mid + 1 : b;
+        e = right ? e : mid;
+    }
+    ox = b - 1 > 0 ? b - 1 : 0;
+    float c0 = C[ox];
+    float du2 = (u0 - c0) / (C[ox + 1] - c0);
+    float pdf_u = E.lum[(size_t)oy * E.w + ox] / E.fint[oy];
+    uu = ((float)ox + du2) / (float)E.w;
+    pdf = pdf_u * pdf_v;
+}
 
 // illuminateEnvironmentLight, Light.cl:195-227: direction, radiance, solid-angle pdf
 __device__ __forceinline__ float4 env_illuminate(const EnvView& E, float u0, float u1, v3& dir,
@@ -221,7 +270,8 @@ __device__ __forceinline__ float4 env_illuminate(const EnvView& E, float u0, flo
     SatRegion R{E.w, E.h, 0, 0, E.w, E.h};
     float uu, vv, pdf;
     int ox, oy;
-    r_sample(R, E.sat, E.lum, u0, u1, uu, vv, ox, oy, pdf);
+    if (E.marg) env_sample_tables(E, u0, u1, uu, vv, ox, oy, pdf);
+    else r_sample(R, E.sat, E.lum, u0, u1, uu, vv, ox, oy, pdf);
     dir = sph_direction(uu, vv);
     texel = sph_index(E.w, E.h, dir);
     float s = sinf(vv * MT_PI);
@@ -237,7 +287,11 @@ __device__ __forceinline__ float4 env_radiance(const EnvView& E, v3 d, float& pd
     int x, y;
     sph_xy(E.w, E.h, d, x, y);
     texel = sph_index(E.w, E.h, d);
-    float p = r_pdf_cond(R, E.sat, E.lum, x, y) * r_pdf_marg(R, E.sat, y);
+    float p;
+    if (E.marg && x >= 0 && y >= 0 && x <= E.w && y <= E.h) // getFunc outside the map reads 0
+        p = (r_func(R, E.lum, x, y) / E.fint[y]) * E.pmarg[y];
+    else
+        p = r_pdf_cond(R, E.sat, E.lum, x, y) * r_pdf_marg(R, E.sat, y);
     float v = (float)y / (float)E.h;
     pdfw = p / (((2.0f * MT_PI) * MT_PI) * sinf(v * MT_PI));
     return E.rgba[texel];

@@ -207,6 +207,26 @@ __global__ __launch_bounds__(256) void k_env_cols(float* __restrict__ sat, int w
     }
 }
 
+// Exact full-map sampling tables: the same expressions the sampler evaluates
+// from SAT corners, stored once (marg/fint/pmarg: h+1 entries, cond: h rows
+// of w+1).
+__global__ void k_env_tables(const float* __restrict__ sat, int w, int h, float* __restrict__ marg,
+                             float* __restrict__ cond, float* __restrict__ fint, float* __restrict__ pmarg)
+{
+    SatRegion R{w, h, 0, 0, w, h};
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t ncond = (int64_t)h * (w + 1);
+    if (i < ncond) {
+        int y = (int)(i / (w + 1)), x = (int)(i % (w + 1));
+        cond[i] = r_cond(R, sat, x, y);
+    } else if (i < ncond + h + 1) {
+        int y = (int)(i - ncond);
+        marg[y] = r_marg(R, sat, y);
+        fint[y] = r_func_int(R, sat, y);
+        pmarg[y] = r_pdf_marg(R, sat, y);
+    }
+}
+
 __global__ void k_region_sample(const float* __restrict__ func, const float* __restrict__ sat, SatRegion R,
                                 int n, const float2* __restrict__ u, float2* __restrict__ uv,
                                 int2* __restrict__ off, float* __restrict__ pdf)
@@ -409,12 +429,35 @@ extern "C" int32_t mt_env_create(mt_context* ctx, const float* rgba, const float
     e->sat = sat;
     e->w = w;
     e->h = h;
+    int64_t ncond = (int64_t)h * (w + 1);
+    size_t nf = (size_t)ncond + 3 * (size_t)(h + 1);
+    if (hipMalloc(&e->tab, sizeof(float) * nf) != hipSuccess) {
+        delete e;
+        mt_set_error("mt_env_create: cannot allocate sampling tables");
+        return MT_ERR_ALLOC;
+    }
+    e->cond = e->tab;
+    e->marg = e->tab + ncond;
+    e->fint = e->marg + (h + 1);
+    e->pmarg = e->fint + (h + 1);
+    k_env_tables<<<mt_blocks(ncond + h + 1, 256), 256, 0, ctx->stream>>>(sat, w, h, e->marg, e->cond, e->fint,
+                                                                         e->pmarg);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        hipFree(e->tab);
+        delete e;
+        mt_set_error("mt_env_create: %s", hipGetErrorString(err));
+        return MT_ERR_HIP;
+    }
     *out = e;
     return MT_OK;
 }
 
 extern "C" int32_t mt_env_destroy(mt_envmap* e)
 {
+    if (!e) return MT_OK;
+    hipStreamSynchronize(e->ctx->stream);
+    hipFree(e->tab);
     delete e;
     return MT_OK;
 }
@@ -441,7 +484,7 @@ extern "C" int32_t mt_env_illuminate(mt_context* ctx, const mt_envmap* env, int3
 {
     MT_ARG_CHECK(ctx && env && u && dir && contrib && pdfw && texel, "mt_env_illuminate: NULL argument");
     if (n <= 0) return MT_OK;
-    EnvView E{reinterpret_cast<const float4*>(env->rgba), env->lum, env->sat, env->w, env->h};
+    EnvView E = env_view(env);
     k_env_illuminate<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(E, n, reinterpret_cast<const float2*>(u),
                                                                  reinterpret_cast<float4*>(dir),
                                                                  reinterpret_cast<float4*>(contrib), pdfw,
@@ -455,7 +498,7 @@ extern "C" int32_t mt_env_radiance(mt_context* ctx, const mt_envmap* env, int32_
 {
     MT_ARG_CHECK(ctx && env && dir && contrib && pdfw && texel, "mt_env_radiance: NULL argument");
     if (n <= 0) return MT_OK;
-    EnvView E{reinterpret_cast<const float4*>(env->rgba), env->lum, env->sat, env->w, env->h};
+    EnvView E = env_view(env);
     k_env_radiance<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(E, n, reinterpret_cast<const float4*>(dir),
                                                                reinterpret_cast<float4*>(contrib), pdfw,
                                                                texel);

@@ -127,13 +127,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int* __restrict__ i
     __shared__ uint32_t s_wave[SCAN_ROWS][SCAN_WAVES];
     __shared__ uint32_t s_prefix;
     const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    int len = *d_len;
+    if (len > capacity) len = capacity;
+    // Work-groups beyond the tiles the live length needs leave without a
+    // ticket (so the ticket counter is not serialised over empty groups);
+    // the remaining groups draw exactly tiles 0 .. ntiles-1.
+    if ((int64_t)blockIdx.x * SCAN_TILE >= len) return;
     if (tid == 0) s_tile = (int)atomicAdd(&W.ticket[0], 1u);
     __syncthreads();
     const int tile = s_tile;
-    int len = *d_len;
-    if (len > capacity) len = capacity;
     const int64_t base = (int64_t)tile * SCAN_TILE;
-    if (base >= len) return;
     uint32_t v[SCAN_ROWS], ex[SCAN_ROWS];
 #pragma unroll
     for (int k = 0; k < SCAN_ROWS; k++) {
@@ -213,13 +216,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_compact(const int* __restrict_
     __shared__ uint32_t s_wave[SCAN_ROWS][SCAN_WAVES];
     __shared__ uint32_t s_prefix;
     const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    int len = *d_len;
+    if (len > capacity) len = capacity;
+    // Work-groups beyond the tiles the live length needs leave without a
+    // ticket (so the ticket counter is not serialised over empty groups);
+    // the remaining groups draw exactly tiles 0 .. ntiles-1.
+    if ((int64_t)blockIdx.x * SCAN_TILE >= len) return;
     if (tid == 0) s_tile = (int)atomicAdd(&W.ticket[0], 1u);
     __syncthreads();
     const int tile = s_tile;
-    int len = *d_len;
-    if (len > capacity) len = capacity;
     const int64_t base = (int64_t)tile * SCAN_TILE;
-    if (base >= len) return;
     uint64_t mask[SCAN_ROWS];
 #pragma unroll
     for (int k = 0; k < SCAN_ROWS; k++) {

@@ -98,6 +98,61 @@ struct BitStack {
         hi >>= 1;
     }
     __device__ __forceinline__ bool empty() const { return (lo | hi) == 0; }
+    // zero bits above the lowest set bit (the pops the reference performs)
+    __device__ __forceinline__ int ctz() const
+    {
+        return lo ? __builtin_ctzll(lo) : 64 + __builtin_ctzll(hi);
+    }
+    __device__ __forceinline__ void shr(int z)
+    {
+        if (z == 0) return;
+        if (z < 64) {
+            lo = (lo >> z) | (hi << (64 - z));
+            hi >>= z;
+        } else {
+            lo = hi >> (z - 64);
+            hi = 0;
+        }
+    }
+};
+
+// Short stack of postponed far children, newest first.  Each 1 bit of the
+// bitstack is a both-children-hit decision whose far child is still to be
+// visited; the stack holds the most recent MT_SHORT_STACK of them with the
+// node at which each was postponed.  Reaching the next pending far child
+// then costs no memory access: the reference's pops (walking parent links
+// up to the lowest 1 bit) are counted as ctz(bitstack) without being
+// fetched.  When older entries have been dropped, the parent walk below
+// finds them exactly as the reference does.
+#ifndef MT_SHORT_STACK
+#define MT_SHORT_STACK 8
+#endif
+struct ShortStack {
+    int ref[MT_SHORT_STACK];
+    int par[MT_SHORT_STACK];
+    int n;
+    __device__ __forceinline__ void push(int r, int p)
+    {
+#pragma unroll
+        for (int k = MT_SHORT_STACK - 1; k > 0; k--) {
+            ref[k] = ref[k - 1];
+            par[k] = par[k - 1];
+        }
+        ref[0] = r;
+        par[0] = p;
+        n = n < MT_SHORT_STACK ? n + 1 : MT_SHORT_STACK;
+    }
+    __device__ __forceinline__ void pop(int& r, int& p)
+    {
+        r = ref[0];
+        p = par[0];
+#pragma unroll
+        for (int k = 0; k < MT_SHORT_STACK - 1; k++) {
+            ref[k] = ref[k + 1];
+            par[k] = par[k + 1];
+        }
+        n--;
+    }
 };
 
 // -------------------------------------------------------------------------
@@ -172,7 +227,7 @@ __device__ __forceinline__ int trace_reference(const int* __restrict__ nodes,
 // cur < 0: leaf ~rank.  parent/sib track the current node's links, which the
 // reference reads from the node records (leaves are not records here).
 // -------------------------------------------------------------------------
-template <bool ANY, bool STATS>
+template <bool ANY, bool STATS, bool SHORT_STACK>
 __device__ __forceinline__ int trace_packed(const float4* __restrict__ pn,
                                             const float4* __restrict__ pt, int root, RayT& r,
                                             int& child, int* st)
@@ -180,6 +235,8 @@ __device__ __forceinline__ int trace_packed(const float4* __restrict__ pn,
     int hit = 0;
     int cur = root;
     BitStack bs{0, 0};
+    ShortStack ss;
+    ss.n = 0;
     int parent = 0, sib = 0;
     for (;;) {
         while (cur >= 0) {
@@ -196,7 +253,9 @@ __device__ __forceinline__ int trace_packed(const float4* __restrict__ pn,
             bs.push0();
             int nxt;
             if (lh && rh) {
-                nxt = (rt < lt) ? d.y : d.x;
+                bool right_first = rt < lt;
+                nxt = right_first ? d.y : d.x;
+                if (SHORT_STACK) ss.push(right_first ? d.x : d.y, cur);
                 bs.lo |= 1;
             } else {
                 nxt = lh ? d.x : d.y;
@@ -218,6 +277,16 @@ __device__ __forceinline__ int trace_packed(const float4* __restrict__ pn,
                 r.tmax = t;
                 child = __float_as_int(t2.y);
             }
+        }
+        if (bs.empty()) return hit;
+        if (SHORT_STACK && ss.n > 0) {
+            // the lowest 1 bit's far child is on top of the short stack
+            int z = bs.ctz();
+            if (STATS) st[2] += z;
+            bs.shr(z);
+            bs.lo ^= 1;
+            ss.pop(cur, parent); // a leaf keeps its parent for a later parent walk
+            continue;
         }
         while ((bs.lo & 1) == 0) {
             if (bs.empty()) return hit;
@@ -253,7 +322,12 @@ struct TraceArgs {
     int* stats;
 };
 
-template <bool ANY, bool STATS, bool PACKED>
+// One ray per lane over a capacity-sized grid; slots >= *d_count exit at
+// once (the hardware dispatcher retires such work-groups in microseconds,
+// cheaper than a persistent grid drawing rays from one global counter).
+// LAYOUT: 0 packed + short stack, 1 reference arrays, 2 packed without the
+// short stack (the reference's parent-link pops, kept for comparison).
+template <bool ANY, bool STATS, int LAYOUT>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
 {
     int id = blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,8 +338,9 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
     ray_init(r, A.o_tmin[id], A.d_tmax[id]);
     int st[3] = {0, 0, 0};
     int child = -1;
-    int h = PACKED ? trace_packed<ANY, STATS>(A.pn, A.pt, A.root, r, child, st)
-                   : trace_reference<ANY, STATS>(A.nodes, A.bounds, A.faces, A.points, r, child, st);
+    int h = LAYOUT == 0   ? trace_packed<ANY, STATS, true>(A.pn, A.pt, A.root, r, child, st)
+            : LAYOUT == 2 ? trace_packed<ANY, STATS, false>(A.pn, A.pt, A.root, r, child, st)
+                          : trace_reference<ANY, STATS>(A.nodes, A.bounds, A.faces, A.points, r, child, st);
     if (ANY) {
         A.occluded[id] = h;
     } else {
@@ -279,6 +354,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
     }
 }
 
+
 template <bool ANY>
 int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
                      const float* d_tmax, const int32_t* d_count, int32_t capacity,
@@ -286,7 +362,8 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
 {
     MT_ARG_CHECK(ctx && sc && o_tmin && d_tmax && d_count, "trace: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "trace: negative capacity");
-    MT_ARG_CHECK(mode == MT_TRACE_PACKED || mode == MT_TRACE_REFERENCE, "trace: unknown mode %d", mode);
+    MT_ARG_CHECK(mode == MT_TRACE_PACKED || mode == MT_TRACE_REFERENCE || mode == MT_TRACE_PACKED_NO_STACK,
+                 "trace: unknown mode %d", mode);
     if (ANY) MT_ARG_CHECK(occluded != nullptr, "trace_any: occluded is NULL");
     else MT_ARG_CHECK(prim && t, "trace_closest: prim/t NULL");
     if (capacity == 0) return MT_OK;
@@ -307,13 +384,15 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
     A.occluded = occluded;
     A.stats = stats;
     dim3 grid(mt_blocks(capacity, 256)), block(256);
-    bool packed = mode == MT_TRACE_PACKED;
+    hipStream_t s = ctx->stream;
     if (stats) {
-        if (packed) k_trace<ANY, true, true><<<grid, block, 0, ctx->stream>>>(A);
-        else k_trace<ANY, true, false><<<grid, block, 0, ctx->stream>>>(A);
+        if (mode == MT_TRACE_PACKED) k_trace<ANY, true, 0><<<grid, block, 0, s>>>(A);
+        else if (mode == MT_TRACE_REFERENCE) k_trace<ANY, true, 1><<<grid, block, 0, s>>>(A);
+        else k_trace<ANY, true, 2><<<grid, block, 0, s>>>(A);
     } else {
-        if (packed) k_trace<ANY, false, true><<<grid, block, 0, ctx->stream>>>(A);
-        else k_trace<ANY, false, false><<<grid, block, 0, ctx->stream>>>(A);
+        if (mode == MT_TRACE_PACKED) k_trace<ANY, false, 0><<<grid, block, 0, s>>>(A);
+        else if (mode == MT_TRACE_REFERENCE) k_trace<ANY, false, 1><<<grid, block, 0, s>>>(A);
+        else k_trace<ANY, false, 2><<<grid, block, 0, s>>>(A);
     }
     MT_LAUNCH_CHECK();
     return MT_OK;

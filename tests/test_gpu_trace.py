@@ -15,7 +15,7 @@ from tests.conftest import c0_rays
 
 pytestmark = pytest.mark.gpu
 
-MODES = [api._lib.MT_TRACE_PACKED, api._lib.MT_TRACE_REFERENCE]
+MODES = [api._lib.MT_TRACE_PACKED, api._lib.MT_TRACE_REFERENCE, api._lib.MT_TRACE_PACKED_NO_STACK]
 
 
 def _scene(ctx, arrs, normals=None):
