@@ -157,9 +157,13 @@ int32_t mt_scene_destroy(mt_scene* scene);
 /* ------------------------------------------------------------------ */
 /* A. Traversal                                                       */
 /* ------------------------------------------------------------------ */
-#define MT_TRACE_PACKED 0    /* packed 64-B node / 48-B triangle layout (default) */
+#define MT_TRACE_PACKED 0    /* packed 64-B node / 48-B triangle layout with a short
+                                stack; closest hit one ray per lane, any hit with
+                                per-lane ray refill (default) */
 #define MT_TRACE_REFERENCE 1 /* reads the reference's flat arrays directly */
-#define MT_TRACE_PACKED_NO_STACK 2 /* packed, backtracking by parent links only */
+#define MT_TRACE_PACKED_NO_STACK 2 /* packed, one ray per lane, parent-link backtracking */
+#define MT_TRACE_PACKED_STACK 3    /* packed, one ray per lane, short stack */
+#define MT_TRACE_PACKED_REFILL 4   /* packed, short stack, persistent waves with refill */
 
 /* Closest hit (intersectMesh with bailout = false, NormalBVH.cl:14-92, as
  * run by IntersectPrimitives :94-123; launched at CDeviceGI.java:230 and

@@ -23,6 +23,8 @@ ERRORS = {
 MT_TRACE_PACKED = 0
 MT_TRACE_REFERENCE = 1
 MT_TRACE_PACKED_NO_STACK = 2
+MT_TRACE_PACKED_STACK = 3
+MT_TRACE_PACKED_REFILL = 4
 MT_BSDF_WORDS = 20
 
 

@@ -190,19 +190,6 @@ struct PayloadSet {
     int n;
 };
 
-__device__ __forceinline__ void move_payload(const PayloadSet& P, int64_t from, int64_t to)
-{
-    for (int j = 0; j < P.n; j++) {
-        int b = P.bytes[j];
-        if (b == 4)
-            reinterpret_cast<uint32_t*>(P.dst[j])[to] = reinterpret_cast<const uint32_t*>(P.src[j])[from];
-        else if (b == 8)
-            reinterpret_cast<uint2*>(P.dst[j])[to] = reinterpret_cast<const uint2*>(P.src[j])[from];
-        else
-            reinterpret_cast<uint4*>(P.dst[j])[to] = reinterpret_cast<const uint4*>(P.src[j])[from];
-    }
-}
-
 __global__ __launch_bounds__(SCAN_THREADS) void k_compact(const int* __restrict__ flags,
                                                           int flag_mode,
                                                           const int* __restrict__ d_len,
@@ -260,13 +247,47 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_compact(const int* __restrict_
     }
     __syncthreads();
     const uint32_t pre = s_prefix;
+    int64_t rank[SCAN_ROWS];
 #pragma unroll
     for (int k = 0; k < SCAN_ROWS; k++) {
-        if ((mask[k] >> lane) & 1ull) {
-            int64_t idx = base + k * SCAN_THREADS + tid;
-            int64_t r = (int64_t)(pre + s_wave[k][wave] + lane_rank(mask[k]));
-            if (out_index) out_index[r] = (int)idx;
-            move_payload(P, idx, r);
+        rank[k] = (int64_t)(pre + s_wave[k][wave] + lane_rank(mask[k]));
+        if (out_index && ((mask[k] >> lane) & 1ull)) out_index[rank[k]] = (int)(base + k * SCAN_THREADS + tid);
+    }
+    // Each payload stream: all rows' loads issue before any store, so a
+    // stream costs one memory round trip (the streams may alias as far as the
+    // compiler knows, so loads and stores of different streams stay ordered).
+    for (int j = 0; j < P.n; j++) {
+        const int b = P.bytes[j];
+        if (b == 16) {
+            const uint4* src = reinterpret_cast<const uint4*>(P.src[j]);
+            uint4* dst = reinterpret_cast<uint4*>(P.dst[j]);
+            uint4 v[SCAN_ROWS];
+#pragma unroll
+            for (int k = 0; k < SCAN_ROWS; k++)
+                if ((mask[k] >> lane) & 1ull) v[k] = src[base + k * SCAN_THREADS + tid];
+#pragma unroll
+            for (int k = 0; k < SCAN_ROWS; k++)
+                if ((mask[k] >> lane) & 1ull) dst[rank[k]] = v[k];
+        } else if (b == 8) {
+            const uint2* src = reinterpret_cast<const uint2*>(P.src[j]);
+            uint2* dst = reinterpret_cast<uint2*>(P.dst[j]);
+            uint2 v[SCAN_ROWS];
+#pragma unroll
+            for (int k = 0; k < SCAN_ROWS; k++)
+                if ((mask[k] >> lane) & 1ull) v[k] = src[base + k * SCAN_THREADS + tid];
+#pragma unroll
+            for (int k = 0; k < SCAN_ROWS; k++)
+                if ((mask[k] >> lane) & 1ull) dst[rank[k]] = v[k];
+        } else {
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(P.src[j]);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(P.dst[j]);
+            uint32_t v[SCAN_ROWS];
+#pragma unroll
+            for (int k = 0; k < SCAN_ROWS; k++)
+                if ((mask[k] >> lane) & 1ull) v[k] = src[base + k * SCAN_THREADS + tid];
+#pragma unroll
+            for (int k = 0; k < SCAN_ROWS; k++)
+                if ((mask[k] >> lane) & 1ull) dst[rank[k]] = v[k];
         }
     }
 }

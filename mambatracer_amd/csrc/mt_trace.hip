@@ -227,6 +227,99 @@ __device__ __forceinline__ int trace_reference(const int* __restrict__ nodes,
 // cur < 0: leaf ~rank.  parent/sib track the current node's links, which the
 // reference reads from the node records (leaves are not records here).
 // -------------------------------------------------------------------------
+// Resumable traversal state of one ray (packed layout).
+struct TravState {
+    int cur, parent, sib;
+    BitStack bs;
+    ShortStack ss;
+    int hit, child;
+    int st[3];
+    __device__ __forceinline__ void init(int root)
+    {
+        cur = root;
+        parent = 0;
+        sib = 0;
+        bs.lo = 0;
+        bs.hi = 0;
+        ss.n = 0;
+        hit = 0;
+        child = -1;
+        st[0] = st[1] = st[2] = 0;
+    }
+};
+
+// One outer iteration of the packed traversal: descend to a leaf (or a node
+// whose children both miss), test the leaf, move to the next pending subtree.
+// Returns true when the ray is finished.  The sequence of nodes visited over
+// all iterations is exactly the reference's (see trace_packed below).
+template <bool ANY, bool STATS, bool SHORT_STACK>
+__device__ __forceinline__ bool trace_packed_step(const float4* __restrict__ pn, const float4* __restrict__ pt,
+                                                  RayT& r, TravState& S)
+{
+    while (S.cur >= 0) {
+        const float4* N = pn + 4 * (size_t)S.cur;
+        float4 a = N[0], b = N[1], c = N[2];
+        int4 d = *reinterpret_cast<const int4*>(N + 3);
+        S.parent = d.z;
+        S.sib = d.w;
+        float lt, rt;
+        bool lh = box_hit(r, a.x, a.y, a.z, a.w, b.x, b.y, lt);
+        bool rh = box_hit(r, b.z, b.w, c.x, c.y, c.z, c.w, rt);
+        if (STATS) S.st[0]++;
+        if (!lh && !rh) break;
+        S.bs.push0();
+        int nxt;
+        if (lh && rh) {
+            bool right_first = rt < lt;
+            nxt = right_first ? d.y : d.x;
+            if (SHORT_STACK) S.ss.push(right_first ? d.x : d.y, S.cur);
+            S.bs.lo |= 1;
+        } else {
+            nxt = lh ? d.x : d.y;
+        }
+        if (nxt < 0) {
+            S.parent = S.cur;
+            S.sib = (nxt == d.x) ? d.y : d.x;
+        }
+        S.cur = nxt;
+    }
+    if (S.cur < 0) {
+        if (STATS) S.st[1]++;
+        const float4* T = pt + 3 * (size_t)(~S.cur);
+        float4 t0 = T[0], t1 = T[1], t2 = T[2];
+        float t = tri_hit(r, mk3(t0.x, t0.y, t0.z), mk3(t0.w, t1.x, t1.y), mk3(t1.z, t1.w, t2.x));
+        if ((r.tmin < t) && (t < r.tmax)) {
+            S.hit = 1;
+            if (ANY) return true;
+            r.tmax = t;
+            S.child = __float_as_int(t2.y);
+        }
+    }
+    if (S.bs.empty()) return true;
+    if (SHORT_STACK && S.ss.n > 0) {
+        int z = S.bs.ctz();
+        if (STATS) S.st[2] += z;
+        S.bs.shr(z);
+        S.bs.lo ^= 1;
+        S.ss.pop(S.cur, S.parent);
+        return false;
+    }
+    while ((S.bs.lo & 1) == 0) {
+        if (S.bs.empty()) return true;
+        S.cur = S.parent;
+        int2 d = *reinterpret_cast<const int2*>(reinterpret_cast<const int*>(pn + 4 * (size_t)S.cur + 3) + 2);
+        S.parent = d.x;
+        S.sib = d.y;
+        S.bs.pop();
+        if (STATS) S.st[2]++;
+    }
+    int from = S.cur;
+    S.cur = S.sib;
+    S.sib = from;
+    S.bs.lo ^= 1;
+    return false;
+}
+
 template <bool ANY, bool STATS, bool SHORT_STACK>
 __device__ __forceinline__ int trace_packed(const float4* __restrict__ pn,
                                             const float4* __restrict__ pt, int root, RayT& r,
@@ -320,7 +413,105 @@ struct TraceArgs {
     float* tout;
     int* occluded;
     int* stats;
+    unsigned* ctr; // refill kernel: [0] next ray, [1] waves finished; zero between launches
 };
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
+{ // set bits of `mask` below this lane (v_mbcnt_lo/hi)
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <bool ANY, bool STATS>
+__device__ __forceinline__ void store_result(const TraceArgs& A, int id, const RayT& r, const TravState& S)
+{
+    if (ANY) {
+        A.occluded[id] = S.hit;
+    } else {
+        A.prim[id] = S.hit ? S.child : -1;
+        A.tout[id] = r.tmax;
+    }
+    if (STATS) {
+        A.stats[3 * id + 0] = S.st[0];
+        A.stats[3 * id + 1] = S.st[1];
+        A.stats[3 * id + 2] = S.st[2];
+    }
+}
+
+// Persistent traversal with per-lane ray refill.  Incoherent rays differ a
+// lot in length, and a wave that owns 64 rays runs until its longest one
+// ends.  Here a wave takes chunks of REFILL_CHUNK consecutive rays with one
+// global atomic and, whenever at least REFILL_MIN of its lanes have finished
+// (or all have), hands the chunk's next rays to the idle lanes (ballot +
+// mbcnt), so lanes stay busy until the rays run out.  Each ray is traversed
+// by trace_packed_step in the reference's visiting order; only the mapping of
+// rays to lanes changes.  The last wave to leave clears the counters.
+constexpr int REFILL_CHUNK = 1024;
+constexpr int REFILL_MIN = 16;
+
+template <bool ANY, bool STATS>
+__global__ __launch_bounds__(256) void k_trace_refill(TraceArgs A)
+{
+    int count = *A.d_count;
+    if (count > A.capacity) count = A.capacity;
+    const int lane = __lane_id();
+    int cnext = 0, cend = 0; // wave-uniform chunk cursor
+    bool drained = false;    // wave-uniform: the global counter is past count
+    bool active = false;
+    int id = 0;
+    RayT r;
+    TravState S;
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        if (idle && !drained && (idle == ~0ull || __popcll(idle) >= REFILL_MIN)) {
+            uint64_t want = idle;
+            while (want) {
+                if (cnext >= cend) {
+                    unsigned base = 0;
+                    if (lane == 0) base = atomicAdd(&A.ctr[0], (unsigned)REFILL_CHUNK);
+                    base = __shfl(base, 0);
+                    if (base >= (unsigned)count) {
+                        drained = true;
+                        break;
+                    }
+                    cnext = (int)base;
+                    cend = (int)base + REFILL_CHUNK < count ? (int)base + REFILL_CHUNK : count;
+                }
+                int take = cend - cnext;
+                int n = __popcll(want);
+                take = take < n ? take : n;
+                bool got = false;
+                if ((want >> lane) & 1ull) {
+                    int rk = (int)lane_rank(want);
+                    if (rk < take) {
+                        id = cnext + rk;
+                        ray_init(r, A.o_tmin[id], A.d_tmax[id]);
+                        S.init(A.root);
+                        active = true;
+                        got = true;
+                    }
+                }
+                cnext += take;
+                want &= ~__ballot(got);
+            }
+        }
+        if (!__any(active)) {
+            if (drained) break;
+            continue;
+        }
+        if (active && trace_packed_step<ANY, STATS, true>(A.pn, A.pt, r, S)) {
+            store_result<ANY, STATS>(A, id, r, S);
+            active = false;
+        }
+    }
+    if (lane == 0) {
+        unsigned waves = gridDim.x * (blockDim.x >> 6);
+        unsigned done = atomicAdd(&A.ctr[1], 1u);
+        if (done == waves - 1) {
+            atomicExch(&A.ctr[0], 0u);
+            atomicExch(&A.ctr[1], 0u);
+        }
+    }
+}
 
 // One ray per lane over a capacity-sized grid; slots >= *d_count exit at
 // once (the hardware dispatcher retires such work-groups in microseconds,
@@ -362,8 +553,7 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
 {
     MT_ARG_CHECK(ctx && sc && o_tmin && d_tmax && d_count, "trace: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "trace: negative capacity");
-    MT_ARG_CHECK(mode == MT_TRACE_PACKED || mode == MT_TRACE_REFERENCE || mode == MT_TRACE_PACKED_NO_STACK,
-                 "trace: unknown mode %d", mode);
+    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_REFILL, "trace: unknown mode %d", mode);
     if (ANY) MT_ARG_CHECK(occluded != nullptr, "trace_any: occluded is NULL");
     else MT_ARG_CHECK(prim && t, "trace_closest: prim/t NULL");
     if (capacity == 0) return MT_OK;
@@ -385,12 +575,23 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
     A.stats = stats;
     dim3 grid(mt_blocks(capacity, 256)), block(256);
     hipStream_t s = ctx->stream;
-    if (stats) {
-        if (mode == MT_TRACE_PACKED) k_trace<ANY, true, 0><<<grid, block, 0, s>>>(A);
+    // Default: closest hit one ray per lane (8 waves/SIMD); any hit, whose rays
+    // end at the first hit and vary most in length, with per-lane refill.
+    if (mode == MT_TRACE_PACKED) mode = ANY ? MT_TRACE_PACKED_REFILL : MT_TRACE_PACKED_STACK;
+    if (mode == MT_TRACE_PACKED_REFILL) {
+        void* scr = nullptr;
+        int32_t rc = mt_ctx_scratch(ctx, 256, &scr);
+        if (rc) return rc;
+        A.ctr = reinterpret_cast<unsigned*>(scr) + 2; // words 0-1: scan look-back ticket and flag
+        auto kern = stats ? k_trace_refill<ANY, true> : k_trace_refill<ANY, false>;
+        int blocks = mt_persistent_blocks(ctx, kern, 256, capacity);
+        kern<<<blocks, block, 0, s>>>(A);
+    } else if (stats) {
+        if (mode == MT_TRACE_PACKED_STACK) k_trace<ANY, true, 0><<<grid, block, 0, s>>>(A);
         else if (mode == MT_TRACE_REFERENCE) k_trace<ANY, true, 1><<<grid, block, 0, s>>>(A);
         else k_trace<ANY, true, 2><<<grid, block, 0, s>>>(A);
     } else {
-        if (mode == MT_TRACE_PACKED) k_trace<ANY, false, 0><<<grid, block, 0, s>>>(A);
+        if (mode == MT_TRACE_PACKED_STACK) k_trace<ANY, false, 0><<<grid, block, 0, s>>>(A);
         else if (mode == MT_TRACE_REFERENCE) k_trace<ANY, false, 1><<<grid, block, 0, s>>>(A);
         else k_trace<ANY, false, 2><<<grid, block, 0, s>>>(A);
     }

@@ -15,7 +15,8 @@ from tests.conftest import c0_rays
 
 pytestmark = pytest.mark.gpu
 
-MODES = [api._lib.MT_TRACE_PACKED, api._lib.MT_TRACE_REFERENCE, api._lib.MT_TRACE_PACKED_NO_STACK]
+MODES = [api._lib.MT_TRACE_PACKED, api._lib.MT_TRACE_REFERENCE, api._lib.MT_TRACE_PACKED_NO_STACK,
+         api._lib.MT_TRACE_PACKED_STACK, api._lib.MT_TRACE_PACKED_REFILL]
 
 
 def _scene(ctx, arrs, normals=None):
