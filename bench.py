@@ -85,6 +85,15 @@ def main() -> int:
     ap.add_argument("--mode", type=int, default=0, help="0 packed traversal layout, 1 reference layout")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["c1", "c3", "c4"], default="c1",
+                    help="c1: one 1080p render per step (default line); c3: 256 uneven jobs batched vs "
+                         "one at a time; c4: Poisson-arrival serving loop")
+    ap.add_argument("--jobs", type=int, default=256, help="c3: number of jobs")
+    ap.add_argument("--rate", type=float, default=40.0, help="c4: arrivals per second")
+    ap.add_argument("--duration", type=float, default=5.0, help="c4: seconds of arrivals")
+    ap.add_argument("--queue", type=int, default=1 << 23, help="c3/c4: wavefront queue capacity")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="copy 1 GiB device to device and exit (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by pixels or jobs "
@@ -100,6 +109,16 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copied {a.numel() * 4} bytes", flush=True)
+        return 0
+    if args.workload != "c1":
+        return bench_jobs(args)
     t0 = time.perf_counter()
     scene_arrays = W.c1_scene()
     points, normals, faces, nodes, bounds, depth = scene_arrays
@@ -201,6 +220,80 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_jobs(args) -> int:
+    """c3 (all jobs at t=0) or c4 (Poisson arrivals): the multi-job engine
+    against one-job-at-a-time plain renders of the same requests, images
+    compared bit for bit."""
+    import numpy as np
+    import torch
+
+    from mambatracer_amd import api, serve
+    from mambatracer_amd import workloads as W
+
+    t0 = time.perf_counter()
+    ctx = api.Context(0)
+    dmats = ctx.to_device(W.c1_materials())
+    env = api.CEnvironment(ctx)
+    env.setEnvironmentMap(W.env_map())
+    reqs = []
+    if args.workload == "c3":
+        specs, ntris = W.c3_specs(n_jobs=args.jobs)
+        scenes = []
+        for spec, n in zip(specs, ntris):
+            scenes.append(api.Scene(ctx, *W.soup_scene(n, spec.seed)))
+            reqs.append(serve.Request(0.0, scenes[-1], dmats, env, W.soup_job(spec), tag=spec.index))
+        desc = (f"c3: {len(specs)} jobs submitted at t=0, side log-uniform 64-1024, spp 1-64, each on its own "
+                f"soup of 1k-500k triangles ({sum(ntris)} total), c1 materials and sky, <=8 bounces")
+    else:
+        sizes = W.c4_soup_sizes()
+        scenes = [api.Scene(ctx, *W.soup_scene(n, 5_000 + i)) for i, n in enumerate(sizes)]
+        points, normals, faces, nodes, bounds, depth = W.c1_scene()
+        scenes.append(api.Scene(ctx, points, faces, nodes, bounds, depth, normals=normals))
+        for t, spec in W.c4_arrivals(args.rate, args.duration, n_soups=len(sizes)):
+            job = W.soup_job(spec) if spec.side > 0 else W.RenderJob(seed=spec.seed)
+            reqs.append(serve.Request(t, scenes[spec.scene], dmats, env, job, tag=spec.index))
+        desc = (f"c4: Poisson arrivals at {args.rate} jobs/s for {args.duration} s ({len(reqs)} jobs); 90% c3-sized "
+                f"jobs on {len(sizes)} resident soups, 10% c1 1920x1080x16spp jobs on the 1M-triangle scene")
+    torch.cuda.synchronize()
+    paths = sum(r.job.width * r.job.height * r.job.spp for r in reqs)
+    log(f"{len(reqs)} jobs, {paths} paths, setup {time.perf_counter() - t0:.1f} s")
+
+    # warm both paths (code objects, allocator) on two small jobs
+    warm = [serve.Request(0.0, reqs[k].scene, dmats, env, W.soup_job(W.JobSpec(k, 0, 32, 1, 7)), tag=k)
+            for k in range(2)]
+    serve.serve_sequential(ctx, warm)
+    serve.serve_batched(ctx, warm, queue_capacity=args.queue)
+
+    seq, seq_rays = serve.serve_sequential(ctx, reqs, keep_images=True)
+    s_seq = serve.summary(seq, seq_rays)
+    log(f"sequential: {s_seq}")
+    bat, st = serve.serve_batched(ctx, reqs, queue_capacity=args.queue, keep_images=True)
+    s_bat = serve.summary(bat, int(st.extension_rays) + int(st.shadow_rays))
+    log(f"batched: {s_bat}, iterations {int(st.iterations)}, arena peak {int(st.arena_peak) / 1e9:.2f} GB")
+    ref = {d.tag: d.image for d in seq}
+    mismatched = sum(0 if torch.equal(ref[d.tag], d.image) else 1 for d in bat)
+    ctx.sync()
+    metric = "jobs/s, continuous batching vs one job at a time" if args.workload == "c3" else \
+        "p50/p99 render-job latency at fixed QPS, continuous batching vs one job at a time"
+    out = {
+        "metric": metric,
+        "value": s_bat["jobs_per_s"] if args.workload == "c3" else s_bat["latency_ms"]["p99"],
+        "unit": "jobs/s" if args.workload == "c3" else "ms (p99)",
+        "n_gpus": 1,
+        "higher_is_better": args.workload == "c3",
+        "dtype": "f32",
+        "data": "synthetic (seeded generators)",
+        "config": {"workload": desc, "queue_capacity": args.queue, "paths": paths},
+        "batched": s_bat,
+        "sequential": s_seq,
+        "speedup_jobs_per_s": round(s_bat["jobs_per_s"] / s_seq["jobs_per_s"], 3),
+        "images_bit_identical": len(bat) - mismatched,
+        "images_compared": len(bat),
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if mismatched == 0 else 1
 
 
 if __name__ == "__main__":

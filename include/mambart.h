@@ -157,9 +157,8 @@ int32_t mt_scene_destroy(mt_scene* scene);
 /* ------------------------------------------------------------------ */
 /* A. Traversal                                                       */
 /* ------------------------------------------------------------------ */
-#define MT_TRACE_PACKED 0    /* packed 64-B node / 48-B triangle layout with a short
-                                stack; closest hit one ray per lane, any hit with
-                                per-lane ray refill (default) */
+#define MT_TRACE_PACKED 0    /* default: packed 64-B node / 48-B triangle layout,
+                                one ray per lane, short stack (= PACKED_STACK) */
 #define MT_TRACE_REFERENCE 1 /* reads the reference's flat arrays directly */
 #define MT_TRACE_PACKED_NO_STACK 2 /* packed, one ray per lane, parent-link backtracking */
 #define MT_TRACE_PACKED_STACK 3    /* packed, one ray per lane, short stack */
@@ -374,6 +373,66 @@ int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float* materials
                   const mt_envmap* env, const int32_t* light_faces,
                   const int32_t* d_num_lights, const mt_render_desc* desc,
                   float* image, mt_render_stats* stats);
+
+/* ------------------------------------------------------------------ */
+/* Multi-job wavefront engine (continuous batching)                   */
+/* ------------------------------------------------------------------ */
+/* The reference renders one job per device at a time: CDeviceGI.start
+ * hands execute() -> loop() to its render thread (CDeviceGI.java:190-204,
+ * 424-431) and every launch covers that job's paths only.  mt_batch runs
+ * many jobs through ONE set of wavefront queues: each iteration refills the
+ * free queue slots with new camera paths of the admitted jobs (oldest job
+ * first), traces all live rays against their own job's scene in one
+ * closest-hit and one any-hit launch, shades every slot with its job's
+ * materials and lights, and compacts the survivors.  A path computes the
+ * same bits as under mt_render (same per-path code, RNG keyed by job seed
+ * and path id), so every job's image is identical to a plain mt_render of
+ * that job, alone or mixed with any others.
+ *
+ * Jobs are admitted in submission order when a job slot is free and their
+ * per-path radiance buffer (mt_batch_job_bytes) fits in the engine's arena;
+ * scenes, materials, environments and images stay owned by the caller and
+ * must outlive the job. */
+typedef struct mt_batch mt_batch;
+
+typedef struct {
+    const mt_scene* scene;        /* as mt_render */
+    const float* materials;
+    const mt_envmap* env;
+    const int32_t* light_faces;
+    const int32_t* d_num_lights;
+    mt_render_desc render;        /* mode must be MT_TRACE_PACKED or
+                                     MT_TRACE_PACKED_STACK; flags may hold
+                                     MT_RENDER_PICKED_LIGHT only */
+    float* image;                 /* device float4[w*h], written at completion */
+    int64_t tag;                  /* caller's id, returned by mt_batch_step */
+} mt_job_desc;
+
+typedef struct {
+    int64_t iterations;           /* wavefront iterations run */
+    int64_t extension_rays;       /* closest-hit rays traced */
+    int64_t shadow_rays;          /* any-hit rays with a light sample */
+    int64_t jobs_submitted, jobs_admitted, jobs_completed;
+    int32_t active_jobs, pending_jobs;
+    uint64_t arena_used, arena_peak; /* bytes of per-job buffers held */
+} mt_batch_stats;
+
+/* queue_capacity: live paths per iteration (1 .. 2^30); arena_bytes: device
+ * memory for per-job radiance buffers; max_jobs: jobs admitted at once
+ * (1 .. 1<<20). */
+int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint64_t arena_bytes, int32_t max_jobs,
+                        mt_batch** out);
+int32_t mt_batch_destroy(mt_batch* batch);
+/* Arena bytes a job holds while admitted (16 B per path, 256-B aligned). */
+int32_t mt_batch_job_bytes(const mt_render_desc* desc, uint64_t* bytes);
+/* Queues a job (validated now; MT_ERR_CAPACITY if it can never fit). */
+int32_t mt_batch_submit(mt_batch* batch, const mt_job_desc* job);
+/* Admits what fits, runs one wavefront iteration and synchronises.  Writes
+ * the tags of jobs whose image is complete (at most max_done; the rest
+ * follow on later calls) to done_tags, their number to *n_done, and
+ * *busy = 1 while any job is pending or in flight. */
+int32_t mt_batch_step(mt_batch* batch, int32_t max_done, int64_t* done_tags, int32_t* n_done, int32_t* busy);
+int32_t mt_batch_get_stats(mt_batch* batch, mt_batch_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,34 @@ MT_RENDER_PROFILE = 2
 MT_RENDER_TRAVERSAL_STATS = 4
 
 
+class JobDesc(C.Structure):
+    _fields_ = [
+        ("scene", C.c_void_p),
+        ("materials", C.c_void_p),
+        ("env", C.c_void_p),
+        ("light_faces", C.c_void_p),
+        ("d_num_lights", C.c_void_p),
+        ("render", RenderDesc),
+        ("image", C.c_void_p),
+        ("tag", C.c_int64),
+    ]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("extension_rays", C.c_int64),
+        ("shadow_rays", C.c_int64),
+        ("jobs_submitted", C.c_int64),
+        ("jobs_admitted", C.c_int64),
+        ("jobs_completed", C.c_int64),
+        ("active_jobs", C.c_int32),
+        ("pending_jobs", C.c_int32),
+        ("arena_used", C.c_uint64),
+        ("arena_peak", C.c_uint64),
+    ]
+
+
 P = C.c_void_p
 I32 = C.c_int32
 I64 = C.c_int64
@@ -145,6 +173,12 @@ SIGNATURES = {
     "mt_bsdf_sample": [P, I32, P, P, P, P, P, P, P, P],
     "mt_bsdf_eval": [P, I32, P, P, P, P, P, P, P],
     "mt_render": [P, P, P, P, P, P, C.POINTER(RenderDesc), P, C.POINTER(RenderStats)],
+    "mt_batch_create": [P, I32, U64, I32, PP],
+    "mt_batch_destroy": [P],
+    "mt_batch_job_bytes": [C.POINTER(RenderDesc), C.POINTER(C.c_uint64)],
+    "mt_batch_submit": [P, C.POINTER(JobDesc)],
+    "mt_batch_step": [P, I32, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+    "mt_batch_get_stats": [P, C.POINTER(BatchStats)],
 }
 _RESTYPES = {"mt_version": C.c_char_p}
 

@@ -513,6 +513,16 @@ def render(ctx: Context, scene: Scene, materials, env: Optional["CEnvironment"],
            num_lights=None, stats: bool = True):
     """CDeviceGI.loop for one job (see mt_render); returns (image float32[h*w,4], stats or None)."""
     torch = _torch()
+    desc = render_desc(job)
+    image = ctx.empty((job.width * job.height, 4), torch.float32)
+    st = _lib.RenderStats() if stats else None
+    call("mt_render", ctx.handle, scene.handle, _ptr(materials), env.handle if env is not None else None,
+         _ptr(light_faces), _ptr(num_lights), C.byref(desc), _ptr(image), C.byref(st) if stats else None)
+    return image, st
+
+
+def render_desc(job) -> "_lib.RenderDesc":
+    """mt_render_desc of a job (any object with the RenderJob fields)."""
     desc = _lib.RenderDesc()
     desc.pos[:] = list(job.pos)
     desc.look[:] = list(job.look)
@@ -521,8 +531,79 @@ def render(ctx: Context, scene: Scene, materials, env: Optional["CEnvironment"],
     desc.width, desc.height, desc.spp = job.width, job.height, job.spp
     desc.max_bounces, desc.rr_start, desc.seed = job.max_bounces, job.rr_start, job.seed
     desc.mode, desc.flags = job.mode, job.flags
-    image = ctx.empty((job.width * job.height, 4), torch.float32)
-    st = _lib.RenderStats() if stats else None
-    call("mt_render", ctx.handle, scene.handle, _ptr(materials), env.handle if env is not None else None,
-         _ptr(light_faces), _ptr(num_lights), C.byref(desc), _ptr(image), C.byref(st) if stats else None)
-    return image, st
+    return desc
+
+
+class Batch:
+    """Multi-job wavefront engine (mt_batch): jobs share one set of queues.
+
+    The reference renders one job at a time per device (CDeviceGI.start ->
+    execute -> loop, CDeviceGI.java:190-204, 424-431); here any number of
+    jobs are in flight together and each job's image equals its plain
+    `render` bit for bit.
+    """
+
+    def __init__(self, ctx: Context, queue_capacity: int = 1 << 22, arena_bytes: int = 8 << 30,
+                 max_jobs: int = 1024):
+        self.ctx = ctx
+        h = C.c_void_p()
+        call("mt_batch_create", ctx.handle, int(queue_capacity), int(arena_bytes), int(max_jobs), C.byref(h))
+        self.handle = h
+        self._keep = {}  # tag -> objects the device reads until the job completes
+        self._done = (C.c_int64 * 256)()
+
+    @staticmethod
+    def job_bytes(job) -> int:
+        b = C.c_uint64(0)
+        call("mt_batch_job_bytes", C.byref(render_desc(job)), C.byref(b))
+        return int(b.value)
+
+    def submit(self, tag: int, scene: Scene, materials, env: Optional["CEnvironment"], job, image,
+               light_faces=None, num_lights=None) -> None:
+        """Queues a job; `image` (device float32[h*w,4]) is written when it completes."""
+        d = _lib.JobDesc()
+        d.scene = scene.handle
+        d.materials = _ptr(materials)
+        d.env = env.handle if env is not None else None
+        d.light_faces = _ptr(light_faces)
+        d.d_num_lights = _ptr(num_lights)
+        d.render = render_desc(job)
+        d.image = _ptr(image)
+        d.tag = int(tag)
+        call("mt_batch_submit", self.handle, C.byref(d))
+        self._keep[int(tag)] = (scene, materials, env, image, light_faces, num_lights)
+
+    def step(self):
+        """One wavefront iteration; returns (completed tags, busy)."""
+        n = C.c_int32(0)
+        busy = C.c_int32(0)
+        call("mt_batch_step", self.handle, len(self._done), self._done, C.byref(n), C.byref(busy))
+        tags = [int(self._done[i]) for i in range(n.value)]
+        for t in tags:
+            self._keep.pop(t, None)
+        return tags, bool(busy.value)
+
+    def run(self):
+        """Steps until every submitted job is complete; returns tags in completion order."""
+        order = []
+        busy = True
+        while busy:
+            tags, busy = self.step()
+            order.extend(tags)
+        return order
+
+    def stats(self) -> "_lib.BatchStats":
+        s = _lib.BatchStats()
+        call("mt_batch_get_stats", self.handle, C.byref(s))
+        return s
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.load().mt_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -30,6 +30,7 @@ SOURCES = [
     "mt_sat.hip",
     "mt_bsdf.hip",
     "mt_render.hip",
+    "mt_batch.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -79,6 +79,20 @@ struct mt_envmap {
 
 int32_t mt_ctx_scratch(mt_context* ctx, size_t bytes, void** out);
 
+// One scene's packed traversal layout, as a row of a per-job table.
+struct MtSceneRef {
+    const float4* pn;
+    const float4* pt;
+    int32_t root;
+    int32_t pad;
+};
+
+// Closest hit (any = false: prim, t) or any hit (occluded) over a queue that
+// mixes jobs: slot i's scene is refs[tag[2*i] >> shift].
+int32_t mt_trace_jobs(mt_context* ctx, bool any, const MtSceneRef* refs, const int32_t* tag, int32_t shift,
+                      const float4* o_tmin, const float4* d_tmax, const int32_t* d_count, int32_t capacity,
+                      int32_t* prim, float* t, int32_t* occluded);
+
 // Light-grid geometry (LightGrid.cl:9-37, CLightGrid.java:47-58)
 constexpr int LG_NU = 1600;
 constexpr int LG_NV = 1600;

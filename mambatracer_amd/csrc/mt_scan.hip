@@ -39,6 +39,38 @@ struct ScanWork {
     uint64_t* status; // [ntiles]
 };
 
+// Tile I/O goes through buffer resources ("windows"): a descriptor whose
+// range check drops stores and zeroes loads at byte offsets at or past its
+// size.  Tails and lanes with nothing to write then need no branches, so
+// the compiler keeps all of a tile's loads in flight together instead of
+// serialising one guarded load per row.
+constexpr int RSRC_WORD3 = 0x00020000; // raw (untyped) buffer, 32-bit data format
+constexpr uint32_t OUT_OF_RANGE = 0x7fffff00u;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t window(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, RSRC_WORD3);
+}
+
+template <int B> struct BufIO;
+template <> struct BufIO<4> {
+    typedef unsigned int T;
+    static __device__ T ld(__amdgpu_buffer_rsrc_t r, uint32_t o) { return __builtin_amdgcn_raw_buffer_load_b32(r, o, 0, 0); }
+    static __device__ void st(T v, __amdgpu_buffer_rsrc_t r, uint32_t o) { __builtin_amdgcn_raw_buffer_store_b32(v, r, o, 0, 0); }
+};
+template <> struct BufIO<8> {
+    typedef u32x2 T;
+    static __device__ T ld(__amdgpu_buffer_rsrc_t r, uint32_t o) { return __builtin_amdgcn_raw_buffer_load_b64(r, o, 0, 0); }
+    static __device__ void st(T v, __amdgpu_buffer_rsrc_t r, uint32_t o) { __builtin_amdgcn_raw_buffer_store_b64(v, r, o, 0, 0); }
+};
+template <> struct BufIO<16> {
+    typedef u32x4 T;
+    static __device__ T ld(__amdgpu_buffer_rsrc_t r, uint32_t o) { return __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0); }
+    static __device__ void st(T v, __amdgpu_buffer_rsrc_t r, uint32_t o) { __builtin_amdgcn_raw_buffer_store_b128(v, r, o, 0, 0); }
+};
+
 __device__ __forceinline__ uint64_t pack_status(uint32_t epoch, uint32_t flag, uint32_t v)
 {
     return ((uint64_t)(epoch & 0x3FFFFFFFu) << 34) | ((uint64_t)flag << 32) | (uint64_t)v;
@@ -135,14 +167,15 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int* __restrict__ i
     if ((int64_t)blockIdx.x * SCAN_TILE >= len) return;
     if (tid == 0) s_tile = (int)atomicAdd(&W.ticket[0], 1u);
     __syncthreads();
-    const int tile = s_tile;
+    const int tile = __builtin_amdgcn_readfirstlane(s_tile);
     const int64_t base = (int64_t)tile * SCAN_TILE;
+    const int64_t rest = (int64_t)len - base;
+    const uint32_t live = (uint32_t)(rest < SCAN_TILE ? rest : SCAN_TILE);
+    // Loads past the live length read 0 (buffer range check), the identity.
+    const __amdgpu_buffer_rsrc_t rin = window(in + base, live * 4);
     uint32_t v[SCAN_ROWS], ex[SCAN_ROWS];
 #pragma unroll
-    for (int k = 0; k < SCAN_ROWS; k++) {
-        int64_t idx = base + k * SCAN_THREADS + tid;
-        v[k] = idx < len ? (uint32_t)in[idx] : 0u;
-    }
+    for (int k = 0; k < SCAN_ROWS; k++) v[k] = BufIO<4>::ld(rin, (uint32_t)(k * SCAN_THREADS + tid) * 4);
 #pragma unroll
     for (int k = 0; k < SCAN_ROWS; k++) {
         uint32_t x = v[k];
@@ -176,11 +209,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int* __restrict__ i
     }
     __syncthreads();
     const uint32_t pre = s_prefix;
+    const __amdgpu_buffer_rsrc_t rout = window(out + base, live * 4);
 #pragma unroll
-    for (int k = 0; k < SCAN_ROWS; k++) {
-        int64_t idx = base + k * SCAN_THREADS + tid;
-        if (idx < len) out[idx] = (int)(pre + s_wave[k][wave] + ex[k]);
-    }
+    for (int k = 0; k < SCAN_ROWS; k++)
+        BufIO<4>::st(pre + s_wave[k][wave] + ex[k], rout, (uint32_t)(k * SCAN_THREADS + tid) * 4);
 }
 
 struct PayloadSet {
@@ -189,6 +221,28 @@ struct PayloadSet {
     int bytes[8];
     int n;
 };
+
+// Payload moves go through buffer windows: a tile's source window
+// [base, base+live) and destination window [pre, pre+agg).  Lanes without a
+// survivor store to an out-of-range offset, so every load and store is
+// unconditional: no branches, all of a stream's loads in flight before its
+// first store.
+// dst_off[k]: byte offset of row k's survivor in the destination window
+// divided by the element size, or OUT_OF_RANGE for a lane without one.
+template <int B>
+__device__ __forceinline__ void move_stream(const void* src, void* dst, int64_t base, uint32_t live,
+                                            uint32_t pre, uint32_t agg, const uint32_t (&dst_off)[SCAN_ROWS],
+                                            int tid)
+{
+    const __amdgpu_buffer_rsrc_t rs = window(static_cast<const char*>(src) + base * B, live * B);
+    const __amdgpu_buffer_rsrc_t rd = window(static_cast<char*>(dst) + (int64_t)pre * B, agg * B);
+    typename BufIO<B>::T v[SCAN_ROWS];
+#pragma unroll
+    for (int k = 0; k < SCAN_ROWS; k++) v[k] = BufIO<B>::ld(rs, (uint32_t)(k * SCAN_THREADS + tid) * B);
+#pragma unroll
+    for (int k = 0; k < SCAN_ROWS; k++)
+        BufIO<B>::st(v[k], rd, dst_off[k] == OUT_OF_RANGE ? OUT_OF_RANGE : dst_off[k] * B);
+}
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_compact(const int* __restrict__ flags,
                                                           int flag_mode,
@@ -201,7 +255,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_compact(const int* __restrict_
 {
     __shared__ int s_tile;
     __shared__ uint32_t s_wave[SCAN_ROWS][SCAN_WAVES];
-    __shared__ uint32_t s_prefix;
+    __shared__ uint32_t s_prefix, s_agg;
     const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     int len = *d_len;
     if (len > capacity) len = capacity;
@@ -211,17 +265,18 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_compact(const int* __restrict_
     if ((int64_t)blockIdx.x * SCAN_TILE >= len) return;
     if (tid == 0) s_tile = (int)atomicAdd(&W.ticket[0], 1u);
     __syncthreads();
-    const int tile = s_tile;
+    const int tile = __builtin_amdgcn_readfirstlane(s_tile);
     const int64_t base = (int64_t)tile * SCAN_TILE;
+    const int64_t rest = (int64_t)len - base;
+    const uint32_t live = (uint32_t)(rest < SCAN_TILE ? rest : SCAN_TILE);
+    const __amdgpu_buffer_rsrc_t rflag = window(flags + base, live * 4);
+    int x[SCAN_ROWS];
+#pragma unroll
+    for (int k = 0; k < SCAN_ROWS; k++) x[k] = (int)BufIO<4>::ld(rflag, (uint32_t)(k * SCAN_THREADS + tid) * 4);
     uint64_t mask[SCAN_ROWS];
 #pragma unroll
     for (int k = 0; k < SCAN_ROWS; k++) {
-        int64_t idx = base + k * SCAN_THREADS + tid;
-        bool f = false;
-        if (idx < len) {
-            int x = flags[idx];
-            f = flag_mode == 0 ? (x != 0) : (x >= 0);
-        }
+        const bool f = (uint32_t)(k * SCAN_THREADS + tid) < live && (flag_mode == 0 ? (x[k] != 0) : (x[k] >= 0));
         mask[k] = __ballot(f);
         if (lane == 0) s_wave[k][wave] = (uint32_t)__popcll(mask[k]);
     }
@@ -241,54 +296,34 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_compact(const int* __restrict_
         uint32_t agg = s_prefix;
         uint32_t pre = look_back(W, tile, agg, epoch);
         if (lane == 0) {
+            s_agg = agg;
             s_prefix = pre;
             if ((int64_t)len - 1 < base + SCAN_TILE) *d_count = (int)(pre + agg);
         }
     }
     __syncthreads();
-    const uint32_t pre = s_prefix;
-    int64_t rank[SCAN_ROWS];
+    // Block-uniform values, read into scalar registers for the descriptors.
+    const uint32_t pre = __builtin_amdgcn_readfirstlane(s_prefix);
+    const uint32_t agg = __builtin_amdgcn_readfirstlane(s_agg);
+    uint32_t dst_off[SCAN_ROWS];
 #pragma unroll
-    for (int k = 0; k < SCAN_ROWS; k++) {
-        rank[k] = (int64_t)(pre + s_wave[k][wave] + lane_rank(mask[k]));
-        if (out_index && ((mask[k] >> lane) & 1ull)) out_index[rank[k]] = (int)(base + k * SCAN_THREADS + tid);
+    for (int k = 0; k < SCAN_ROWS; k++)
+        dst_off[k] = ((mask[k] >> lane) & 1ull) ? s_wave[k][wave] + lane_rank(mask[k]) : OUT_OF_RANGE;
+    if (out_index) {
+        const __amdgpu_buffer_rsrc_t rd = window(out_index + pre, agg * 4);
+#pragma unroll
+        for (int k = 0; k < SCAN_ROWS; k++)
+            BufIO<4>::st((unsigned)(base + k * SCAN_THREADS + tid), rd,
+                         dst_off[k] == OUT_OF_RANGE ? OUT_OF_RANGE : dst_off[k] * 4);
     }
-    // Each payload stream: all rows' loads issue before any store, so a
-    // stream costs one memory round trip (the streams may alias as far as the
-    // compiler knows, so loads and stores of different streams stay ordered).
     for (int j = 0; j < P.n; j++) {
         const int b = P.bytes[j];
-        if (b == 16) {
-            const uint4* src = reinterpret_cast<const uint4*>(P.src[j]);
-            uint4* dst = reinterpret_cast<uint4*>(P.dst[j]);
-            uint4 v[SCAN_ROWS];
-#pragma unroll
-            for (int k = 0; k < SCAN_ROWS; k++)
-                if ((mask[k] >> lane) & 1ull) v[k] = src[base + k * SCAN_THREADS + tid];
-#pragma unroll
-            for (int k = 0; k < SCAN_ROWS; k++)
-                if ((mask[k] >> lane) & 1ull) dst[rank[k]] = v[k];
-        } else if (b == 8) {
-            const uint2* src = reinterpret_cast<const uint2*>(P.src[j]);
-            uint2* dst = reinterpret_cast<uint2*>(P.dst[j]);
-            uint2 v[SCAN_ROWS];
-#pragma unroll
-            for (int k = 0; k < SCAN_ROWS; k++)
-                if ((mask[k] >> lane) & 1ull) v[k] = src[base + k * SCAN_THREADS + tid];
-#pragma unroll
-            for (int k = 0; k < SCAN_ROWS; k++)
-                if ((mask[k] >> lane) & 1ull) dst[rank[k]] = v[k];
-        } else {
-            const uint32_t* src = reinterpret_cast<const uint32_t*>(P.src[j]);
-            uint32_t* dst = reinterpret_cast<uint32_t*>(P.dst[j]);
-            uint32_t v[SCAN_ROWS];
-#pragma unroll
-            for (int k = 0; k < SCAN_ROWS; k++)
-                if ((mask[k] >> lane) & 1ull) v[k] = src[base + k * SCAN_THREADS + tid];
-#pragma unroll
-            for (int k = 0; k < SCAN_ROWS; k++)
-                if ((mask[k] >> lane) & 1ull) dst[rank[k]] = v[k];
-        }
+        if (b == 16)
+            move_stream<16>(P.src[j], P.dst[j], base, live, pre, agg, dst_off, tid);
+        else if (b == 8)
+            move_stream<8>(P.src[j], P.dst[j], base, live, pre, agg, dst_off, tid);
+        else
+            move_stream<4>(P.src[j], P.dst[j], base, live, pre, agg, dst_off, tid);
     }
 }
 

@@ -545,6 +545,30 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
     }
 }
 
+// Rays of many jobs in one queue: ray id's scene is refs[tag[2*id] >> shift]
+// (the job word of a wavefront slot).  Same per-ray traversal as k_trace
+// LAYOUT 0, so a ray's result does not depend on which queue carries it.
+template <bool ANY>
+__global__ __launch_bounds__(256) void k_trace_jobs(const MtSceneRef* __restrict__ refs, const int* __restrict__ tag,
+                                                    int shift, TraceArgs A)
+{
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    int count = *A.d_count;
+    if (count > A.capacity) count = A.capacity;
+    if (id >= count) return;
+    const MtSceneRef R = refs[tag[2 * id] >> shift];
+    RayT r;
+    ray_init(r, A.o_tmin[id], A.d_tmax[id]);
+    int st[3];
+    int child = -1;
+    int h = trace_packed<ANY, false, true>(R.pn, R.pt, R.root, r, child, st);
+    if (ANY) {
+        A.occluded[id] = h;
+    } else {
+        A.prim[id] = h ? child : -1;
+        A.tout[id] = r.tmax;
+    }
+}
 
 template <bool ANY>
 int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
@@ -575,9 +599,11 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
     A.stats = stats;
     dim3 grid(mt_blocks(capacity, 256)), block(256);
     hipStream_t s = ctx->stream;
-    // Default: closest hit one ray per lane (8 waves/SIMD); any hit, whose rays
-    // end at the first hit and vary most in length, with per-lane refill.
-    if (mode == MT_TRACE_PACKED) mode = ANY ? MT_TRACE_PACKED_REFILL : MT_TRACE_PACKED_STACK;
+    // Default: one ray per lane with the short stack (8 waves/SIMD).  Refill
+    // wins only on long incoherent batches (16M random any-hit rays through a
+    // 4M-triangle soup: 18.3 vs 21.3 ms); on a path tracer's bounce batches it
+    // loses (1080p x 16 spp render: 25.8 vs 23.5 ms), so it is opt-in.
+    if (mode == MT_TRACE_PACKED) mode = MT_TRACE_PACKED_STACK;
     if (mode == MT_TRACE_PACKED_REFILL) {
         void* scr = nullptr;
         int32_t rc = mt_ctx_scratch(ctx, 256, &scr);
@@ -917,6 +943,26 @@ extern "C" int32_t mt_trace_any(mt_context* ctx, const mt_scene* scene, const fl
 {
     return launch_trace<true>(ctx, scene, o_tmin, d_tmax, d_count, capacity, nullptr, nullptr,
                               occluded, stats, mode);
+}
+
+int32_t mt_trace_jobs(mt_context* ctx, bool any, const MtSceneRef* refs, const int32_t* tag, int32_t shift,
+                      const float4* o_tmin, const float4* d_tmax, const int32_t* d_count, int32_t capacity,
+                      int32_t* prim, float* t, int32_t* occluded)
+{
+    if (capacity == 0) return MT_OK;
+    TraceArgs A{};
+    A.o_tmin = o_tmin;
+    A.d_tmax = d_tmax;
+    A.d_count = d_count;
+    A.capacity = capacity;
+    A.prim = prim;
+    A.tout = t;
+    A.occluded = occluded;
+    dim3 grid(mt_blocks(capacity, 256)), block(256);
+    if (any) k_trace_jobs<true><<<grid, block, 0, ctx->stream>>>(refs, tag, shift, A);
+    else k_trace_jobs<false><<<grid, block, 0, ctx->stream>>>(refs, tag, shift, A);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
 }
 
 extern "C" int32_t mt_shade_hits(mt_context* ctx, const mt_scene* sc, const float* o_tmin,

@@ -1,0 +1,121 @@
+"""Render-request loop over the multi-job wavefront engine (continuous
+batching), and the one-job-at-a-time loop it is measured against.
+
+The reference has one render job per device at a time (CDeviceGI.start ->
+renderThread -> execute -> loop, src/cl/device/CDeviceGI.java:190-204,
+424-431).  `serve_batched` accepts jobs as they arrive, submits them to an
+`api.Batch` (admission by arena footprint happens in the engine, in arrival
+order), keeps stepping the shared queues and timestamps each job when its
+image is complete.  `serve_sequential` runs the same arrivals through plain
+`api.render` calls, first come first served.
+
+A request is a `Request`: arrival time (seconds from the start of the run),
+the resident scene, materials, environment and lights it renders with, and
+the RenderJob.  Both loops return one `Done` record per request.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Any, List, Optional
+
+import numpy as np
+
+from . import api
+
+
+@dataclass
+class Request:
+    arrival: float
+    scene: Any
+    materials: Any
+    env: Any
+    job: Any
+    light_faces: Any = None
+    num_lights: Any = None
+    tag: int = 0
+
+
+@dataclass
+class Done:
+    tag: int
+    arrival: float
+    complete: float
+    image: Any = field(repr=False, default=None)
+
+    @property
+    def latency(self) -> float:
+        return self.complete - self.arrival
+
+
+def _wait_until(t0: float, t: float) -> None:
+    while True:
+        dt = t - (time.perf_counter() - t0)
+        if dt <= 0:
+            return
+        time.sleep(min(dt, 0.0005))
+
+
+def serve_batched(ctx: api.Context, requests: List[Request], queue_capacity: int = 1 << 23,
+                  arena_bytes: int = 32 << 30, max_jobs: int = 1024, keep_images: bool = False,
+                  batch: Optional[api.Batch] = None):
+    """Continuous batching: returns (Done records in completion order, engine stats)."""
+    torch = api._torch()
+    reqs = sorted(requests, key=lambda r: r.arrival)
+    own = batch is None
+    b = api.Batch(ctx, queue_capacity, arena_bytes, max_jobs) if own else batch
+    images, arrivals = {}, {}
+    out: List[Done] = []
+    t0 = time.perf_counter()
+    i, busy = 0, False
+    while i < len(reqs) or busy:
+        now = time.perf_counter() - t0
+        while i < len(reqs) and reqs[i].arrival <= now:
+            r = reqs[i]
+            img = ctx.empty((r.job.width * r.job.height, 4), torch.float32)
+            b.submit(r.tag, r.scene, r.materials, r.env, r.job, img, r.light_faces, r.num_lights)
+            images[r.tag], arrivals[r.tag] = img, r.arrival
+            i += 1
+            busy = True
+        if not busy:
+            _wait_until(t0, reqs[i].arrival)
+            continue
+        tags, busy = b.step()
+        t = time.perf_counter() - t0
+        for tag in tags:
+            img = images.pop(tag)
+            out.append(Done(tag, arrivals.pop(tag), t, img if keep_images else None))
+    stats = b.stats()
+    if own:
+        b.close()
+    return out, stats
+
+
+def serve_sequential(ctx: api.Context, requests: List[Request], keep_images: bool = False):
+    """One job at a time with plain mt_render launches, first come first served."""
+    reqs = sorted(requests, key=lambda r: r.arrival)
+    out: List[Done] = []
+    rays = 0
+    t0 = time.perf_counter()
+    for r in reqs:
+        _wait_until(t0, r.arrival)
+        img, st = api.render(ctx, r.scene, r.materials, r.env, r.job, light_faces=r.light_faces,
+                             num_lights=r.num_lights, stats=True)   # stats read-back synchronises
+        rays += int(st.extension_rays) + int(st.shadow_rays)
+        out.append(Done(r.tag, r.arrival, time.perf_counter() - t0, img if keep_images else None))
+    return out, rays
+
+
+def summary(done: List[Done], rays: int) -> dict:
+    """Latency percentiles (ms), jobs/s and Mrays/s over the run's span."""
+    lat = np.array([d.latency for d in done]) * 1e3
+    span = max(d.complete for d in done) - min(d.arrival for d in done)
+    return {
+        "jobs": len(done),
+        "span_s": round(span, 4),
+        "jobs_per_s": round(len(done) / span, 3),
+        "mrays_per_s": round(rays / span / 1e6, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3),
+                       "max": round(float(lat.max()), 3)},
+    }

@@ -122,3 +122,80 @@ def c2_light(bounds, seed: int = 3):
     rng = np.random.default_rng(seed)
     lo, hi = bounds[0, :3].astype(np.float64), bounds[0, 4:7].astype(np.float64)
     return (lo + (hi - lo) * rng.random(3)).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# Batched jobs (c3) and the serving loop (c4)
+# ---------------------------------------------------------------------------
+@dataclass
+class JobSpec:
+    """One render job of the multi-job workloads."""
+    index: int
+    scene: int          # index into the scene list the job renders
+    side: int           # square image side
+    spp: int
+    seed: int
+
+
+def _log_uniform(rng, lo: float, hi: float) -> float:
+    return math.exp(rng.uniform(math.log(lo), math.log(hi)))
+
+
+def soup_scene(num_tris: int, seed: int, n_materials: int = 257):
+    """A c0-style random-triangle soup (centroids in [-0.8,0.8]^3, edges
+    0.01-0.05) without the room box, so the environment lights it; material
+    ids drawn uniformly from the c1 material table."""
+    points, faces = api.gen_soup(seed, num_tris, -0.8, 0.8, 0.01, 0.05, first_vertex=0, mat=0)
+    faces[:, 9] = np.random.default_rng(seed).integers(0, n_materials, size=num_tris, dtype=np.int32)
+    nodes, bounds, depth = api.CNormalBVH.build(points, faces)
+    return points, faces, nodes, bounds, depth
+
+
+def c3_specs(n_jobs: int = 256, seed: int = 4, side=(64, 1024), spp=(1, 64), tris=(1_000, 500_000)):
+    """c3: jobs of uneven size, each with its own soup: image side log-uniform
+    in [side], spp uniform integer in [spp], triangle count log-uniform in
+    [tris].  Returns (job specs, per-scene triangle counts)."""
+    rng = np.random.default_rng(seed)
+    specs, ntris = [], []
+    for i in range(n_jobs):
+        s = int(round(_log_uniform(rng, side[0], side[1])))
+        k = int(rng.integers(spp[0], spp[1] + 1))
+        n = int(round(_log_uniform(rng, tris[0], tris[1])))
+        specs.append(JobSpec(i, i, s, k, seed * 100_000 + i))
+        ntris.append(n)
+    return specs, ntris
+
+
+def soup_job(spec: JobSpec) -> RenderJob:
+    """Render job of a soup scene: the c0 camera (pinhole at (0,0,-0.95)
+    looking +z, 60 deg), c1 path settings (8 bounces, roulette from 3)."""
+    return RenderJob(width=spec.side, height=spec.side, spp=spec.spp, max_bounces=8, rr_start=3,
+                     seed=spec.seed, pos=(0.0, 0.0, -0.95), look=(0.0, 0.0, 1.0), up=(0.0, 1.0, 0.0),
+                     vfov_deg=60.0)
+
+
+def c4_arrivals(rate: float, duration: float, seed: int = 5, n_soups: int = 31, large_frac: float = 0.1,
+                side=(64, 1024), spp=(1, 64)):
+    """c4: Poisson arrivals at `rate` jobs/s for `duration` s.  A job is a
+    c1-scene 1080p x 16 spp render with probability `large_frac` (scene index
+    n_soups), else a c3-sized job on one of `n_soups` resident soups.
+    Returns a list of (arrival time s, JobSpec)."""
+    rng = np.random.default_rng(seed)
+    t, out, i = 0.0, [], 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            return out
+        if rng.random() < large_frac:
+            spec = JobSpec(i, n_soups, -1, 16, seed * 100_000 + i)
+        else:
+            s = int(round(_log_uniform(rng, side[0], side[1])))
+            k = int(rng.integers(spp[0], spp[1] + 1))
+            spec = JobSpec(i, int(rng.integers(0, n_soups)), s, k, seed * 100_000 + i)
+        out.append((t, spec))
+        i += 1
+
+
+def c4_soup_sizes(n_soups: int = 31, seed: int = 5, tris=(1_000, 500_000)):
+    rng = np.random.default_rng(seed + 1)
+    return [int(round(_log_uniform(rng, tris[0], tris[1]))) for _ in range(n_soups)]

@@ -1,0 +1,148 @@
+"""The multi-job wavefront engine (mt_batch) against plain one-job renders.
+
+A path runs the same per-path code under both drivers and its RNG is keyed
+by (job seed, path id, bounce, purpose), so the bar is bit-exact: every
+job's image from the shared queues equals its own mt_render image, whether
+the job runs alone or mixed with others, whatever the queue capacity, arena
+size and admission order.  (mt_render itself is held to the oracle in
+test_gpu_render.py.)"""
+import numpy as np
+import pytest
+import torch
+
+from mambatracer_amd import _lib, api
+from mambatracer_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def world(ctx):
+    """Three soups of different sizes, c1 materials plus one emitter, the c1 sky."""
+    mats = W.c1_materials()
+    emit = W.material_records(np.full((1, 3), 0.5), 0.04, 0.5, 0.5, emission=np.full((1, 3), 1.0),
+                              emission_power=20.0)
+    mats = np.concatenate([mats, emit])
+    scenes = []
+    for n, seed in [(1500, 41), (9000, 42), (40000, 43)]:
+        points, faces, nodes, bounds, depth = W.soup_scene(n, seed)
+        scenes.append(api.Scene(ctx, points, faces, nodes, bounds, depth))
+    # a fourth scene whose triangles are partly emitters (mesh lights)
+    points, faces, nodes, bounds, depth = W.soup_scene(5000, 44)
+    faces[::50, 9] = mats.shape[0] - 1
+    scenes.append(api.Scene(ctx, points, faces, nodes, bounds, depth))
+    dm = ctx.to_device(mats)
+    lf, nl = api.mesh_lights(ctx, scenes[3], dm)
+    env = api.CEnvironment(ctx)
+    env.setEnvironmentMap(W.env_map())
+    torch.cuda.synchronize()
+    assert int(nl.item()) > 10
+    return scenes, dm, env, (lf, nl)
+
+
+def _jobs():
+    specs = [(0, 40, 3), (1, 64, 2), (2, 33, 5), (3, 48, 2), (0, 17, 1), (2, 80, 1), (3, 24, 4)]
+    out = []
+    for i, (sc, side, spp) in enumerate(specs):
+        j = W.soup_job(W.JobSpec(i, sc, side, spp, 9000 + i))
+        if i == 6:
+            j.flags = _lib.MT_RENDER_PICKED_LIGHT
+        out.append((sc, j))
+    return out
+
+
+def _plain(ctx, world, sc, job):
+    scenes, dm, env, (lf, nl) = world
+    lights = (lf, nl) if sc == 3 else (None, None)
+    img, _ = api.render(ctx, scenes[sc], dm, env, job, light_faces=lights[0], num_lights=lights[1],
+                        stats=False)
+    return img.cpu().numpy()
+
+
+def _batched(ctx, world, jobs, capacity, arena, max_jobs=64):
+    scenes, dm, env, (lf, nl) = world
+    b = api.Batch(ctx, queue_capacity=capacity, arena_bytes=arena, max_jobs=max_jobs)
+    imgs = {}
+    for tag, (sc, job) in jobs:
+        imgs[tag] = ctx.empty((job.width * job.height, 4), torch.float32)
+        lights = (lf, nl) if sc == 3 else (None, None)
+        b.submit(tag, scenes[sc], dm, env, job, imgs[tag], light_faces=lights[0], num_lights=lights[1])
+    order = b.run()
+    st = b.stats()
+    b.close()
+    torch.cuda.synchronize()
+    return {t: v.cpu().numpy() for t, v in imgs.items()}, order, st
+
+
+@pytest.mark.parametrize("capacity", [5000, 1 << 16])
+def test_batch_matches_plain_renders_bit_exact(ctx, world, capacity):
+    print(f"[case] batch mix capacity={capacity}", flush=True)
+    jobs = list(enumerate(_jobs()))
+    got, order, st = _batched(ctx, world, jobs, capacity, 256 << 20)
+    assert sorted(order) == list(range(len(jobs)))
+    assert st.jobs_completed == len(jobs) and st.active_jobs == 0 and st.pending_jobs == 0
+    assert st.arena_used == 0
+    rays = 0
+    for tag, (sc, job) in jobs:
+        ref = _plain(ctx, world, sc, job)
+        np.testing.assert_array_equal(got[tag], ref, err_msg=f"job {tag}")
+        assert np.isfinite(ref).all() and ref[:, :3].mean() > 0
+        rays += job.width * job.height * job.spp
+    assert st.extension_rays >= rays          # at least one ray per path
+    if capacity == 5000:
+        assert st.iterations > 10             # the queue refilled many times
+
+
+def test_job_alone_equals_job_in_mix(ctx, world):
+    print("[case] alone vs mix", flush=True)
+    jobs = list(enumerate(_jobs()))
+    mix, _, _ = _batched(ctx, world, jobs, 1 << 14, 256 << 20)
+    for tag in (1, 5):
+        alone, _, _ = _batched(ctx, world, [jobs[tag]], 1 << 14, 256 << 20)
+        np.testing.assert_array_equal(alone[tag], mix[tag])
+
+
+def test_admission_by_arena_footprint(ctx, world):
+    """An arena that holds one job at a time: jobs are admitted in order as
+    space frees up, complete in submission order, and images are unchanged."""
+    print("[case] arena admission", flush=True)
+    jobs = list(enumerate(_jobs()))[:4]
+    need = [api.Batch.job_bytes(j) for _, (_, j) in jobs]
+    arena = max(need)
+    got, order, st = _batched(ctx, world, jobs, 1 << 14, arena)
+    assert order == [0, 1, 2, 3]
+    assert st.arena_peak <= arena
+    for tag, (sc, job) in jobs:
+        np.testing.assert_array_equal(got[tag], _plain(ctx, world, sc, job))
+
+
+def test_job_slots_reused(ctx, world):
+    """max_jobs = 2 slots for 7 jobs: slots and arena ranges are recycled."""
+    print("[case] slot reuse", flush=True)
+    jobs = list(enumerate(_jobs()))
+    got, order, st = _batched(ctx, world, jobs, 1 << 15, 256 << 20, max_jobs=2)
+    assert sorted(order) == list(range(len(jobs)))
+    for tag in (0, 4, 6):
+        sc, job = jobs[tag][1]
+        np.testing.assert_array_equal(got[tag], _plain(ctx, world, sc, job))
+
+
+def test_batch_rejects_bad_jobs(ctx, world):
+    scenes, dm, env, _ = world
+    b = api.Batch(ctx, queue_capacity=1024, arena_bytes=1 << 20, max_jobs=4)
+    img = ctx.empty((64 * 64, 4), torch.float32)
+    big = W.soup_job(W.JobSpec(0, 0, 512, 4, 1))           # 16 MB of radiance > 1 MB arena
+    with pytest.raises(_lib.MambartError) as e:
+        b.submit(0, scenes[0], dm, env, big, ctx.empty((512 * 512, 4), torch.float32))
+    assert e.value.code == -5
+    bad = W.soup_job(W.JobSpec(1, 0, 64, 1, 1))
+    bad.mode = _lib.MT_TRACE_REFERENCE
+    with pytest.raises(_lib.MambartError):
+        b.submit(1, scenes[0], dm, env, bad, img)
+    bad = W.soup_job(W.JobSpec(2, 0, 64, 1, 1))
+    bad.max_bounces = 64
+    with pytest.raises(_lib.MambartError):
+        b.submit(2, scenes[0], dm, env, bad, img)
+    tags, busy = b.step()                                    # nothing queued: idle
+    assert tags == [] and not busy
+    b.close()

@@ -57,6 +57,42 @@ def test_render_matches_oracle(ctx, c1_small):
     assert np.isfinite(img).all() and img.mean() > 0
 
 
+@pytest.mark.parametrize("flags", [0, _lib.MT_RENDER_PICKED_LIGHT])
+def test_render_mesh_lights_matches_oracle(ctx, c1_small, flags):
+    """Emitter hits (LightHitPass emitter branch) and DirectLight on area
+    lights picked from the mesh-light list, beside the environment."""
+    print(f"[case] render mesh lights flags={flags}", flush=True)
+    points, normals, faces, nodes, bounds, depth = c1_small
+    mats = W.c1_materials()
+    mats[:4, 28:31] = [[1.0, 0.9, 0.7]]   # spheres 0-3 emit
+    mats[:4, 32] = 1.0
+    mats[:4, 33] = 5.0
+    sc = api.Scene(ctx, points, faces, nodes, bounds, depth, normals=normals)
+    dm = ctx.to_device(mats)
+    lf, nl = api.mesh_lights(ctx, sc, dm)
+    rgba = W.env_map()
+    env = api.CEnvironment(ctx)
+    env.setEnvironmentMap(rgba)
+    job = W.RenderJob(width=96, height=54, spp=4, pos=(0.0, 0.5, -8.0), look=(0.0, -1.0, 0.0), flags=flags)
+    img, st = api.render(ctx, sc, dm, env, job, light_faces=lf, num_lights=nl)
+    torch.cuda.synchronize()
+    n = int(nl.item())
+    assert n == 4 * 4096
+    hlf = lf.cpu().numpy()[:n].copy()
+    lum, sat = orc.env_lum_sat(rgba.reshape(-1, 4), 1024, 512)
+    npix = job.width * job.height
+    L, rays = orc.render_paths(points, normals, faces, nodes, bounds, mats, (rgba.reshape(-1, 4), lum, sat, 1024, 512),
+                               job, np.arange(npix * job.spp, dtype=np.int32), light_faces=hlf)
+    oimg = orc.resolve(L, npix, job.spp)
+    img = img.cpu().numpy()[:, :3]
+    ext, sh = int(st.extension_rays), int(st.shadow_rays)
+    assert abs(ext - rays[0]) <= 0.005 * rays[0] and abs(sh - rays[1]) <= 0.005 * rays[1] + 2
+    close = np.all(np.abs(img - oimg) <= 1e-4 * np.abs(oimg) + 1e-7, axis=1).mean()
+    rmse = np.sqrt(np.mean((img - oimg) ** 2)) / np.sqrt(np.mean(oimg ** 2))
+    print(f"pixels within 1e-4 relative {close:.4f}, relative RMSE {rmse:.2e}")
+    assert close >= 0.95 and rmse <= 1e-3
+
+
 def test_render_is_deterministic_and_independent_of_batch(ctx, c1_small):
     """Same seed -> bit-identical image; each sample's paths do not depend on the
     other samples in the wavefront (spp=1 render == sample 0 of an spp=3 render)."""
