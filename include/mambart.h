@@ -10,7 +10,7 @@
  * src/cl/algorithms/CAdaptiveEnvironment.java:69-145); each entry point below
  * replaces one such launch sequence and cites it.
  *
- * Conventions (kept from the reference, src/cl/kernel/*.cl):
+ * Conventions (kept from the reference's OpenCL kernels in src/cl/kernel):
  *  - The caller owns every buffer; kernels never allocate caller-visible
  *    memory.  Buffers are device pointers (mt_buf_alloc, or any HIP
  *    allocation on the context's device).
@@ -146,6 +146,16 @@ typedef struct {
     int32_t num_points;
     int32_t max_depth;     /* from mt_bvh_build; > 128 is rejected */
 } mt_scene_desc;
+
+/* CNormalBVH.build (CNormalBVH.java:45-148) on the device: from device
+ * points/faces (layout above) into device nodes/bounds, the same tree as
+ * mt_bvh_build (node numbering, leaf primitives, bounds; a zero bound may
+ * differ in sign).  Builds one level of the tree per round and reads each
+ * round's node count back, so the call synchronises the context's stream.
+ * max_depth (host) receives the deepest leaf (root = 0). */
+int32_t mt_bvh_build_device(mt_context* ctx, int32_t ntris, const float* points,
+                            const int32_t* faces, int32_t* nodes, float* bounds,
+                            int32_t* max_depth);
 
 /* Validates the description and derives the packed traversal layout on the
  * device (the reference-layout arrays are left untouched and stay

@@ -143,6 +143,7 @@ SIGNATURES = {
     "mt_gen_camera_rays": [P, P, P, F32, I32, I32, P, P],
     "mt_gen_random_rays": [U64, I64, I64, P, P, P, P, I32],
     "mt_gen_spheres": [U64, I32, F32, F32, F32, F32, F32, F32, P, P, P],
+    "mt_bvh_build_device": [P, I32, P, P, P, P, C.POINTER(C.c_int32)],
     "mt_scene_create": [P, C.POINTER(SceneDesc), PP],
     "mt_scene_destroy": [P],
     "mt_trace_closest": [P, P, P, P, P, I32, P, P, P, I32],

@@ -108,6 +108,19 @@ class CNormalBVH:
              C.byref(depth), nthreads)
         return nodes, bounds, int(depth.value)
 
+    @staticmethod
+    def build_device(ctx: "Context", points, faces):
+        """Same tree on the device from device points/faces; returns device
+        (nodes int32[2N-1,7], bounds float32[2N-1,8]) and the depth."""
+        torch = _torch()
+        n = int(faces.shape[0])
+        nodes = ctx.empty((2 * n - 1, 7), torch.int32)
+        bounds = ctx.empty((2 * n - 1, 8), torch.float32)
+        depth = C.c_int32(0)
+        call("mt_bvh_build_device", ctx.handle, n, _ptr(points), _ptr(faces), _ptr(nodes), _ptr(bounds),
+             C.byref(depth))
+        return nodes, bounds, int(depth.value)
+
 
 def gen_soup(seed: int, n: int, c_lo: float, c_hi: float, e_lo: float, e_hi: float,
              first_vertex: int = 0, mat: int = 0):

@@ -31,6 +31,7 @@ SOURCES = [
     "mt_bsdf.hip",
     "mt_render.hip",
     "mt_batch.hip",
+    "mt_bvh.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -75,7 +75,13 @@ struct mt_envmap {
     // library-owned sampling tables (see EnvView in mt_sat.h), one allocation
     float* tab;
     float *marg, *cond, *fint, *pmarg;
+    int *gmarg, *gcond; // search guides over marg / cond rows (same allocation)
 };
+
+// Guide intervals of the environment sampling tables: the marginal CDF gets
+// ENV_GUIDE_M + 2 entries, every conditional row ENV_GUIDE_C + 2.
+constexpr int ENV_GUIDE_M = 1024;
+constexpr int ENV_GUIDE_C = 256;
 
 int32_t mt_ctx_scratch(mt_context* ctx, size_t bytes, void** out);
 

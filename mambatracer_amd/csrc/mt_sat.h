@@ -222,39 +222,60 @@ struct EnvView {
     const float* cond;
     const float* fint;
     const float* pmarg;
+    const int* gmarg;   // search guides (null: plain binary search)
+    const int* gcond;
 };
 
 inline EnvView env_view(const mt_envmap* env)
 {
-    if (!env) return EnvView{nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr};
+    if (!env) return EnvView{nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     return EnvView{reinterpret_cast<const float4*>(env->rgba), env->lum, env->sat, env->w, env->h,
-                   env->marg, env->cond, env->fint, env->pmarg};
+                   env->marg, env->cond, env->fint, env->pmarg, env->gmarg, env->gcond};
+}
+
+// upperBoundMarginal / upperBoundConditional (SAT.cl:206-242): the number of
+// leading entries of a[b, e) with !(u < a[i]), by bisection.
+__device__ __forceinline__ int cdf_search(const float* __restrict__ a, int b, int e, float u)
+{
+    while (b != e) {
+        int mid = b + (e - b) / 2;
+        bool right = !(u < a[mid]);
+        b = right ? mid + 1 : b;
+        e = right ? e : mid;
+    }
+    return b;
+}
+
+// The same search over a[0, n) narrowed by a guide: g[k] = cdf_search(a, 0,
+// n, k / m) for k in [0, m+1] (m a power of two, so u * m and k / m are
+// exact).  The predicate !(u < a[i]) is monotone in u, and in i for a
+// non-decreasing CDF, so for k = floor(u * m) the answer lies in
+// [g[k], g[k+1]] and bisecting that range returns the full search's index.
+__device__ __forceinline__ int cdf_search_guided(const float* __restrict__ a, int n, const int* __restrict__ g,
+                                                 int m, float u)
+{
+    const float s = u * (float)m;
+    if (!(s >= 0.0f && s <= (float)m)) return cdf_search(a, 0, n, u);
+    const int k = (int)s;
+    const int lo = g[k], hi = g[k + 1];
+    // a table that is not non-decreasing can give an inverted bracket:
+    // search the whole table instead
+    if (!(0 <= lo && lo <= hi && hi <= n)) return cdf_search(a, 0, n, u);
+    return cdf_search(a, lo, hi, u);
 }
 
 __device__ __forceinline__ void env_sample_tables(const EnvView& E, float u0, float u1, float& uu,
                                                   float& vv, int& ox, int& oy, float& pdf)
 { // sampleContinuous (SAT.cl:244-323) reading the tables above
-    int b = 0, e = E.h;
-    while (b != e) {
-        int mid = b + (e - b) / 2;
-        bool right = !(u1 < E.marg[mid]);
-        b = right ? mid + 1 : b;
-        e = right ? e : mid;
-    }
+    int b = E.gmarg ? cdf_search_guided(E.marg, E.h, E.gmarg, ENV_GUIDE_M, u1) : cdf_search(E.marg, 0, E.h, u1);
     oy = b - 1 > 0 ? b - 1 : 0;
     float m0 = E.marg[oy];
     float du = (u1 - m0) / (E.marg[oy + 1] - m0);
     float pdf_v = E.pmarg[oy];
     vv = ((float)oy + du) / (float)E.h;
     const float* C = E.cond + (size_t)oy * (E.w + 1);
-    b = 0;
-    e = E.w;
-    while (b != e) {
-        int mid = b + (e - b) / 2;
-        bool right = !(u0 < C[mid]);
-        b = right ? mid + 1 : b;
-        e = right ? e : mid;
-    }
+    b = E.gcond ? cdf_search_guided(C, E.w, E.gcond + (size_t)oy * (ENV_GUIDE_C + 2), ENV_GUIDE_C, u0)
+                : cdf_search(C, 0, E.w, u0);
     ox = b - 1 > 0 ? b - 1 : 0;
     float c0 = C[ox];
     float du2 = (u0 - c0) / (C[ox + 1] - c0);

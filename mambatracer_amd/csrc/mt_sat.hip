@@ -227,6 +227,23 @@ __global__ void k_env_tables(const float* __restrict__ sat, int w, int h, float*
     }
 }
 
+// Search guides over the tables above: gmarg[k] = cdf_search(marg, 0, h,
+// k / ENV_GUIDE_M) for k in [0, ENV_GUIDE_M + 1]; row y of gcond likewise
+// over cond row y (w entries) with ENV_GUIDE_C intervals.
+__global__ void k_env_guides(const float* __restrict__ marg, const float* __restrict__ cond, int w, int h,
+                             int* __restrict__ gmarg, int* __restrict__ gcond)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nm = ENV_GUIDE_M + 2, nc = (int64_t)h * (ENV_GUIDE_C + 2);
+    if (i < nm) {
+        gmarg[i] = cdf_search(marg, 0, h, (float)i / (float)ENV_GUIDE_M);
+    } else if (i < nm + nc) {
+        const int64_t j = i - nm;
+        const int y = (int)(j / (ENV_GUIDE_C + 2)), k = (int)(j % (ENV_GUIDE_C + 2));
+        gcond[j] = cdf_search(cond + (size_t)y * (w + 1), 0, w, (float)k / (float)ENV_GUIDE_C);
+    }
+}
+
 __global__ void k_region_sample(const float* __restrict__ func, const float* __restrict__ sat, SatRegion R,
                                 int n, const float2* __restrict__ u, float2* __restrict__ uv,
                                 int2* __restrict__ off, float* __restrict__ pdf)
@@ -430,7 +447,8 @@ extern "C" int32_t mt_env_create(mt_context* ctx, const float* rgba, const float
     e->w = w;
     e->h = h;
     int64_t ncond = (int64_t)h * (w + 1);
-    size_t nf = (size_t)ncond + 3 * (size_t)(h + 1);
+    const int64_t ngm = ENV_GUIDE_M + 2, ngc = (int64_t)h * (ENV_GUIDE_C + 2);
+    size_t nf = (size_t)ncond + 3 * (size_t)(h + 1) + (size_t)(ngm + ngc);
     if (hipMalloc(&e->tab, sizeof(float) * nf) != hipSuccess) {
         delete e;
         mt_set_error("mt_env_create: cannot allocate sampling tables");
@@ -440,8 +458,11 @@ extern "C" int32_t mt_env_create(mt_context* ctx, const float* rgba, const float
     e->marg = e->tab + ncond;
     e->fint = e->marg + (h + 1);
     e->pmarg = e->fint + (h + 1);
+    e->gmarg = reinterpret_cast<int*>(e->pmarg + (h + 1));
+    e->gcond = e->gmarg + ngm;
     k_env_tables<<<mt_blocks(ncond + h + 1, 256), 256, 0, ctx->stream>>>(sat, w, h, e->marg, e->cond, e->fint,
                                                                          e->pmarg);
+    k_env_guides<<<mt_blocks(ngm + ngc, 256), 256, 0, ctx->stream>>>(e->marg, e->cond, w, h, e->gmarg, e->gcond);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         hipFree(e->tab);
