@@ -119,6 +119,8 @@ __device__ __forceinline__ int seg_scan(int v, int s, int lane, Op op)
     return v;
 }
 
+// Last lane of its run; call with the whole wave active (a shuffle reads
+// nothing useful from an inactive lane).
 __device__ __forceinline__ bool run_tail(int s, int lane)
 {
     int ns = __shfl_down(s, 1);
@@ -164,7 +166,8 @@ __global__ __launch_bounds__(BB) void k_bounds(BuildState S)
     for (int k = 6; k < 9; k++) v[k] = seg_scan(v[k], s, lane, mn);
 #pragma unroll
     for (int k = 9; k < 12; k++) v[k] = seg_scan(v[k], s, lane, mx);
-    if (s >= 0 && run_tail(s, lane)) {
+    const bool tail = run_tail(s, lane); // all lanes take part in the shuffle
+    if (s >= 0 && tail) {
         int* K = S.key + 12 * (size_t)s;
         for (int k = 0; k < 3; k++) atomicMin(&K[k], v[k]);
         for (int k = 3; k < 6; k++) atomicMax(&K[k], v[k]);
@@ -225,7 +228,8 @@ __global__ __launch_bounds__(BB) void k_flags(BuildState S)
     if (__ballot(s >= 0) == 0) return;
     int left = (s >= 0 && !right) ? 1 : 0;
     left = seg_scan(left, s, lane, [](int x, int y) { return x + y; });
-    if (s >= 0 && run_tail(s, lane) && left) atomicAdd(&S.nleft[s], left);
+    const bool tail = run_tail(s, lane); // all lanes take part in the shuffle
+    if (s >= 0 && tail && left) atomicAdd(&S.nleft[s], left);
 }
 
 __global__ void k_node_b(BuildState S, const int* __restrict__ list, int count, int* __restrict__ next,
