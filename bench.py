@@ -92,6 +92,7 @@ def main() -> int:
     ap.add_argument("--rate", type=float, default=40.0, help="c4: arrivals per second")
     ap.add_argument("--duration", type=float, default=5.0, help="c4: seconds of arrivals")
     ap.add_argument("--queue", type=int, default=1 << 23, help="c3/c4: wavefront queue capacity")
+    ap.add_argument("--policy", type=int, default=0, help="c3/c4: 0 oldest job first, 1 smallest job first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB device to device and exit (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -269,7 +270,7 @@ def bench_jobs(args) -> int:
     seq, seq_rays = serve.serve_sequential(ctx, reqs, keep_images=True)
     s_seq = serve.summary(seq, seq_rays)
     log(f"sequential: {s_seq}")
-    bat, st = serve.serve_batched(ctx, reqs, queue_capacity=args.queue, keep_images=True)
+    bat, st = serve.serve_batched(ctx, reqs, queue_capacity=args.queue, keep_images=True, policy=args.policy)
     s_bat = serve.summary(bat, int(st.extension_rays) + int(st.shadow_rays))
     log(f"batched: {s_bat}, iterations {int(st.iterations)}, arena peak {int(st.arena_peak) / 1e9:.2f} GB")
     ref = {d.tag: d.image for d in seq}
@@ -285,7 +286,8 @@ def bench_jobs(args) -> int:
         "higher_is_better": args.workload == "c3",
         "dtype": "f32",
         "data": "synthetic (seeded generators)",
-        "config": {"workload": desc, "queue_capacity": args.queue, "paths": paths},
+        "config": {"workload": desc, "queue_capacity": args.queue, "paths": paths,
+                   "policy": "smallest job first" if args.policy == 1 else "oldest job first"},
         "batched": s_bat,
         "sequential": s_seq,
         "speedup_jobs_per_s": round(s_bat["jobs_per_s"] / s_seq["jobs_per_s"], 3),

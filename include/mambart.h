@@ -76,6 +76,8 @@ const char* mt_version(void);
  * device's null stream). */
 #define MT_CTX_OWN_STREAM 1
 int32_t mt_ctx_create(int32_t device, void* stream, int32_t flags, mt_context** out);
+/* Objects made on a context (scenes, light grids, environments, batches)
+ * keep a pointer to it: destroy them first. */
 int32_t mt_ctx_destroy(mt_context* ctx);
 int32_t mt_ctx_sync(mt_context* ctx); /* OpenCLConfiguration.finish() */
 
@@ -443,6 +445,12 @@ int32_t mt_batch_submit(mt_batch* batch, const mt_job_desc* job);
  * *busy = 1 while any job is pending or in flight. */
 int32_t mt_batch_step(mt_batch* batch, int32_t max_done, int64_t* done_tags, int32_t* n_done, int32_t* busy);
 int32_t mt_batch_get_stats(mt_batch* batch, mt_batch_stats* stats);
+/* Which admitted jobs get free queue slots first: the oldest (default), or
+ * the one with the fewest paths (short jobs finish within max_bounces
+ * iterations of admission, ahead of long ones).  Images do not depend on it. */
+#define MT_BATCH_OLDEST_FIRST 0
+#define MT_BATCH_SMALLEST_FIRST 1
+int32_t mt_batch_set_policy(mt_batch* batch, int32_t policy);
 
 #ifdef __cplusplus
 }

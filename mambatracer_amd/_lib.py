@@ -180,7 +180,10 @@ SIGNATURES = {
     "mt_batch_submit": [P, C.POINTER(JobDesc)],
     "mt_batch_step": [P, I32, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "mt_batch_get_stats": [P, C.POINTER(BatchStats)],
+    "mt_batch_set_policy": [P, I32],
 }
+MT_BATCH_OLDEST_FIRST = 0
+MT_BATCH_SMALLEST_FIRST = 1
 _RESTYPES = {"mt_version": C.c_char_p}
 
 _lib = None

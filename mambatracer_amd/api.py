@@ -21,6 +21,7 @@ missing library or a non-zero status raises.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from typing import Optional, Sequence
 
 import numpy as np
@@ -63,6 +64,12 @@ class Context:
         # so torch copies and library kernels are ordered without extra syncs
         call("mt_ctx_create", device, C.c_void_p(stream.cuda_stream), 0, C.byref(h))
         self.handle = h
+        # library objects made on this context; the C ABI requires them to be
+        # destroyed before the context, so close() closes them first
+        self._deps = weakref.WeakSet()
+
+    def _own(self, obj) -> None:
+        self._deps.add(obj)
 
     def sync(self) -> None:
         call("mt_ctx_sync", self.handle)
@@ -72,6 +79,8 @@ class Context:
 
     def close(self) -> None:
         if self.handle:
+            for d in list(getattr(self, "_deps", ())):
+                d.close()
             _lib.load().mt_ctx_destroy(self.handle)
             self.handle = None
 
@@ -190,6 +199,7 @@ class Scene:
         torch.cuda.synchronize(ctx.torch_device)
         call("mt_scene_create", ctx.handle, C.byref(desc), C.byref(h))
         self.handle = h
+        ctx._own(self)
 
     def close(self) -> None:
         if getattr(self, "handle", None):
@@ -352,6 +362,7 @@ class CAdaptiveEnvironment:
         h = C.c_void_p()
         call("mt_lightgrid_create", ctx.handle, C.byref(h))
         self.handle = h
+        ctx._own(self)
         torch = _torch()
         a, f, s = C.c_void_p(), C.c_void_p(), C.c_void_p()
         call("mt_lightgrid_arrays", h, C.byref(a), C.byref(f), C.byref(s))
@@ -434,6 +445,7 @@ class CEnvironment:
     def __init__(self, ctx: Context):
         self.ctx = ctx
         self.handle = None
+        ctx._own(self)
 
     def setEnvironmentMap(self, rgba: np.ndarray) -> None:
         torch = _torch()
@@ -557,11 +569,13 @@ class Batch:
     """
 
     def __init__(self, ctx: Context, queue_capacity: int = 1 << 22, arena_bytes: int = 8 << 30,
-                 max_jobs: int = 1024):
+                 max_jobs: int = 1024, policy: int = _lib.MT_BATCH_OLDEST_FIRST):
         self.ctx = ctx
         h = C.c_void_p()
         call("mt_batch_create", ctx.handle, int(queue_capacity), int(arena_bytes), int(max_jobs), C.byref(h))
         self.handle = h
+        ctx._own(self)
+        call("mt_batch_set_policy", h, int(policy))
         self._keep = {}  # tag -> objects the device reads until the job completes
         self._done = (C.c_int64 * 256)()
 

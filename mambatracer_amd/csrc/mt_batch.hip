@@ -253,6 +253,7 @@ struct mt_batch {
     std::map<uint64_t, uint64_t> free_blocks; // arena offset -> length
     std::deque<int64_t> finished;             // tags not yet reported
     bool active_dirty;
+    int policy;
     mt_batch_stats st;
 };
 
@@ -354,7 +355,15 @@ int32_t iterate(mt_batch* b)
     hipStream_t s = b->ctx->stream;
     const int n_active = (int)b->active_jobs.size();
     if (b->active_dirty) {
-        for (int i = 0; i < n_active; i++) b->h_active[i] = b->active_jobs[i].slot;
+        // hand-out order of free queue slots: admission order, or fewest
+        // paths first (ties by admission order)
+        std::vector<int> order(n_active);
+        for (int i = 0; i < n_active; i++) order[i] = i;
+        if (b->policy == MT_BATCH_SMALLEST_FIRST)
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+                return b->active_jobs[x].npaths < b->active_jobs[y].npaths;
+            });
+        for (int i = 0; i < n_active; i++) b->h_active[i] = b->active_jobs[order[i]].slot;
         MT_HIP_CHECK(hipMemcpyAsync(b->active, b->h_active, sizeof(int) * (size_t)std::max(n_active, 1),
                                     hipMemcpyHostToDevice, s));
         b->active_dirty = false;
@@ -488,6 +497,7 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     b->cur = 0;
     b->active_dirty = true;
     b->st = mt_batch_stats{};
+    b->policy = MT_BATCH_OLDEST_FIRST;
     for (int k = max_jobs - 1; k >= 0; k--) b->free_slots.push_back(k);
     b->free_blocks[0] = b->arena_bytes;
     k_zero_counts<<<1, 64, 0, ctx->stream>>>(b->cnt, b->tot);
@@ -583,6 +593,16 @@ extern "C" int32_t mt_batch_step(mt_batch* b, int32_t max_done, int64_t* done_ta
     }
     *n_done = nd;
     *busy = (!b->active_jobs.empty() || !b->pending.empty() || !b->finished.empty()) ? 1 : 0;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_batch_set_policy(mt_batch* b, int32_t policy)
+{
+    MT_ARG_CHECK(b, "mt_batch_set_policy: NULL batch");
+    MT_ARG_CHECK(policy == MT_BATCH_OLDEST_FIRST || policy == MT_BATCH_SMALLEST_FIRST,
+                 "mt_batch_set_policy: unknown policy %d", policy);
+    b->policy = policy;
+    b->active_dirty = true;
     return MT_OK;
 }
 

@@ -48,25 +48,27 @@ __device__ __forceinline__ void ray_init(RayT& r, float4 a, float4 b)
 }
 
 // intersectBoundT, Util.cl:707-732.  Conditional assignments (not fminf/fmaxf)
-// keep the reference's NaN behaviour.
+// keep the reference's NaN behaviour.  The reference's two early returns
+// become terms of one conjunction (all six slab distances are computed
+// either way), so the test has no branches: a wave's lanes never split
+// inside it, and the node's loads all issue before the first compare.
 __device__ __forceinline__ bool box_hit(const RayT& r, float mnx, float mny, float mnz,
                                         float mxx, float mxy, float mxz, float& tnear)
 {
-    float tmin = ((r.sx ? mxx : mnx) - r.o.x) * r.inv.x;
-    float tmax = ((r.sx ? mnx : mxx) - r.o.x) * r.inv.x;
-    float tymin = ((r.sy ? mxy : mny) - r.o.y) * r.inv.y;
-    float tymax = ((r.sy ? mny : mxy) - r.o.y) * r.inv.y;
-    if ((tmin > tymax) || (tymin > tmax)) return false;
-    if (tymin > tmin) tmin = tymin;
-    if (tymax < tmax) tmax = tymax;
-    float tzmin = ((r.sz ? mxz : mnz) - r.o.z) * r.inv.z;
-    float tzmax = ((r.sz ? mnz : mxz) - r.o.z) * r.inv.z;
-    if ((tmin > tzmax) || (tzmin > tmax)) return false;
-    if (tzmin > tmin) tmin = tzmin;
-    if (tzmax < tmax) tmax = tzmax;
-    tmax *= 1.00000024f;
-    tnear = tmin;
-    return (tmin < r.tmax) && (tmax > r.tmin);
+    const float tmin = ((r.sx ? mxx : mnx) - r.o.x) * r.inv.x;
+    const float tmax = ((r.sx ? mnx : mxx) - r.o.x) * r.inv.x;
+    const float tymin = ((r.sy ? mxy : mny) - r.o.y) * r.inv.y;
+    const float tymax = ((r.sy ? mny : mxy) - r.o.y) * r.inv.y;
+    const float tzmin = ((r.sz ? mxz : mnz) - r.o.z) * r.inv.z;
+    const float tzmax = ((r.sz ? mnz : mxz) - r.o.z) * r.inv.z;
+    const bool xy = !((tmin > tymax) | (tymin > tmax));
+    const float t1 = tymin > tmin ? tymin : tmin;
+    const float u1 = tymax < tmax ? tymax : tmax;
+    const bool xyz = !((t1 > tzmax) | (tzmin > u1));
+    const float t2 = tzmin > t1 ? tzmin : t1;
+    const float u2 = (tzmax < u1 ? tzmax : u1) * 1.00000024f;
+    tnear = t2;
+    return xy & xyz & (t2 < r.tmax) & (u2 > r.tmin);
 }
 
 // fastTriangleIntersection, Primitive.cl:3-29, on p1 and the two edges.

@@ -58,12 +58,13 @@ def _wait_until(t0: float, t: float) -> None:
 
 def serve_batched(ctx: api.Context, requests: List[Request], queue_capacity: int = 1 << 23,
                   arena_bytes: int = 32 << 30, max_jobs: int = 1024, keep_images: bool = False,
-                  batch: Optional[api.Batch] = None):
-    """Continuous batching: returns (Done records in completion order, engine stats)."""
+                  batch: Optional[api.Batch] = None, policy: int = 0):
+    """Continuous batching: returns (Done records in completion order, engine stats).
+    policy: MT_BATCH_OLDEST_FIRST (0) or MT_BATCH_SMALLEST_FIRST (1)."""
     torch = api._torch()
     reqs = sorted(requests, key=lambda r: r.arrival)
     own = batch is None
-    b = api.Batch(ctx, queue_capacity, arena_bytes, max_jobs) if own else batch
+    b = api.Batch(ctx, queue_capacity, arena_bytes, max_jobs, policy=policy) if own else batch
     images, arrivals = {}, {}
     out: List[Done] = []
     t0 = time.perf_counter()

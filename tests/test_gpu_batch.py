@@ -59,9 +59,9 @@ def _plain(ctx, world, sc, job):
     return img.cpu().numpy()
 
 
-def _batched(ctx, world, jobs, capacity, arena, max_jobs=64):
+def _batched(ctx, world, jobs, capacity, arena, max_jobs=64, policy=_lib.MT_BATCH_OLDEST_FIRST):
     scenes, dm, env, (lf, nl) = world
-    b = api.Batch(ctx, queue_capacity=capacity, arena_bytes=arena, max_jobs=max_jobs)
+    b = api.Batch(ctx, queue_capacity=capacity, arena_bytes=arena, max_jobs=max_jobs, policy=policy)
     imgs = {}
     for tag, (sc, job) in jobs:
         imgs[tag] = ctx.empty((job.width * job.height, 4), torch.float32)
@@ -127,6 +127,19 @@ def test_job_slots_reused(ctx, world):
         np.testing.assert_array_equal(got[tag], _plain(ctx, world, sc, job))
 
 
+def test_smallest_first_policy(ctx, world):
+    """A short job queued behind a long one finishes first under
+    MT_BATCH_SMALLEST_FIRST; images do not depend on the policy."""
+    print("[case] smallest first", flush=True)
+    big = W.soup_job(W.JobSpec(0, 2, 96, 4, 7001))      # 36 864 paths, 9 queue fills
+    small = W.soup_job(W.JobSpec(1, 0, 16, 1, 7002))    # 256 paths
+    jobs = [(0, (2, big)), (1, (0, small))]
+    got, order, _ = _batched(ctx, world, jobs, 4096, 64 << 20, policy=_lib.MT_BATCH_SMALLEST_FIRST)
+    assert order == [1, 0]
+    np.testing.assert_array_equal(got[0], _plain(ctx, world, 2, big))
+    np.testing.assert_array_equal(got[1], _plain(ctx, world, 0, small))
+
+
 def test_batch_rejects_bad_jobs(ctx, world):
     scenes, dm, env, _ = world
     b = api.Batch(ctx, queue_capacity=1024, arena_bytes=1 << 20, max_jobs=4)
@@ -145,4 +158,6 @@ def test_batch_rejects_bad_jobs(ctx, world):
         b.submit(2, scenes[0], dm, env, bad, img)
     tags, busy = b.step()                                    # nothing queued: idle
     assert tags == [] and not busy
+    with pytest.raises(_lib.MambartError):
+        _lib.call("mt_batch_set_policy", b.handle, 7)
     b.close()
