@@ -72,19 +72,22 @@ __device__ __forceinline__ bool box_hit(const RayT& r, float mnx, float mny, flo
 }
 
 // fastTriangleIntersection, Primitive.cl:3-29, on p1 and the two edges.
+// Both early returns of the reference give tMax; here every quantity is
+// computed and one select picks t or tMax, so p1 is fetched with the edges
+// instead of after the determinant test.
 __device__ __forceinline__ float tri_hit(const RayT& r, v3 p1, v3 e1, v3 e2)
 {
-    v3 h = cross3(r.d, e2);
-    float a = dot3(e1, h);
-    if ((double)a > -0.0000001 && (double)a < 0.0000001) return r.tmax; // double compare, :13
-    float f = 1.0f / a;
-    v3 s = sub3(r.o, p1);
-    float b1 = f * dot3(s, h);
-    v3 q = cross3(s, e1);
-    float b2 = f * dot3(r.d, q);
-    float t = f * dot3(e2, q);
-    if (b1 < 0.0f || b1 > 1.0f || b2 < 0.0f || b1 + b2 > 1.0f || t < 0.0f || t > r.tmax) return r.tmax;
-    return t;
+    const v3 h = cross3(r.d, e2);
+    const float a = dot3(e1, h);
+    const bool parallel = ((double)a > -0.0000001) & ((double)a < 0.0000001); // double compare, :13
+    const float f = 1.0f / a;
+    const v3 s = sub3(r.o, p1);
+    const float b1 = f * dot3(s, h);
+    const v3 q = cross3(s, e1);
+    const float b2 = f * dot3(r.d, q);
+    const float t = f * dot3(e2, q);
+    const bool out = (b1 < 0.0f) | (b1 > 1.0f) | (b2 < 0.0f) | (b1 + b2 > 1.0f) | (t < 0.0f) | (t > r.tmax);
+    return (parallel | out) ? r.tmax : t;
 }
 
 struct BitStack {
