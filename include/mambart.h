@@ -387,6 +387,32 @@ int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float* materials
                   float* image, mt_render_stats* stats);
 
 /* ------------------------------------------------------------------ */
+/* Image pipeline                                                     */
+/* ------------------------------------------------------------------ */
+/* CImage.initBuffers (CImage.java:117-122; InitFloat4DataXYZ and
+ * InitIntDataRGB, Initialize.cl:1-11,91-98): accum[i] = (0,0,0,1) and
+ * argb[i] = 0xFF000000; either pointer may be NULL. */
+int32_t mt_image_init(mt_context* ctx, float* accum, int32_t* argb, int32_t n);
+/* CImage.processImage (CImage.java:124-135): averageAccum (Image.cl:29-65)
+ * frame = accum / frame_count and per-pixel log(0.01 + Lw); when
+ * update_totals, *d_total_log / *d_total_count = sum of the logs / count of
+ * pixels with Lw > 0 (fixed-order float sum; the reference recomputes them
+ * while frameCount < 3); then updateFrameImage (Image.cl:68-118): Reinhard
+ * tone map of the luminance scaled by exposure / exp(mean log), xyY -> RGB,
+ * gamma, argb = getIntARGB.  Pixels with Lw <= 0 keep their argb value.
+ * frame: float4[n] out; argb: int32[n]; totals: device scalars. */
+int32_t mt_image_process(mt_context* ctx, const float* accum, float frame_count, int32_t n,
+                         float exposure, float gamma, int32_t update_totals, float* frame,
+                         int32_t* argb, float* d_total_log, int32_t* d_total_count);
+/* PrefixSumFloat.execute (PrefixSumFloat.java:95-102; kernels
+ * Util.cl:982-1042): out = exclusive scan of in[0..*d_len), *d_total = the
+ * sum (d_total may be NULL).  The reference's global pass converts partial
+ * sums to int (Util.cl:1035); this scan is a float scan in a fixed order
+ * (deterministic run to run). */
+int32_t mt_scan_excl_f32(mt_context* ctx, const float* in, float* out, const int32_t* d_len,
+                         int32_t capacity, float* d_total);
+
+/* ------------------------------------------------------------------ */
 /* Multi-job wavefront engine (continuous batching)                   */
 /* ------------------------------------------------------------------ */
 /* The reference renders one job per device at a time: CDeviceGI.start

@@ -174,6 +174,9 @@ SIGNATURES = {
     "mt_bsdf_sample": [P, I32, P, P, P, P, P, P, P, P],
     "mt_bsdf_eval": [P, I32, P, P, P, P, P, P, P],
     "mt_render": [P, P, P, P, P, P, C.POINTER(RenderDesc), P, C.POINTER(RenderStats)],
+    "mt_image_init": [P, P, P, I32],
+    "mt_image_process": [P, P, F32, I32, F32, F32, I32, P, P, P, P],
+    "mt_scan_excl_f32": [P, P, P, P, I32, P],
     "mt_batch_create": [P, I32, U64, I32, PP],
     "mt_batch_destroy": [P],
     "mt_batch_job_bytes": [C.POINTER(RenderDesc), C.POINTER(C.c_uint64)],

@@ -299,6 +299,68 @@ class PrefixSumInteger:
         return int(self.ctotal.item())
 
 
+class PrefixSumFloat:
+    """PrefixSumFloat (PrefixSumFloat.java:49-117): exclusive float scan + total."""
+
+    def __init__(self, ctx: Context, cdata):
+        torch = _torch()
+        self.ctx = ctx
+        self.cdata = cdata
+        self.length = int(cdata.shape[0])
+        self.clength = ctx.scalar_i32(self.length)
+        self.cprefixsum = ctx.empty((self.length,), torch.float32)
+        self.ctotal = torch.zeros((1,), dtype=torch.float32, device=ctx.torch_device)
+
+    def execute(self) -> None:
+        call("mt_scan_excl_f32", self.ctx.handle, _ptr(self.cdata), _ptr(self.cprefixsum),
+             _ptr(self.clength), self.length, _ptr(self.ctotal))
+
+    def getPrefixSum(self):
+        return self.cprefixsum
+
+    def getCTotal(self):
+        return self.ctotal
+
+
+class CImage:
+    """CImage (CImage.java:60-150): accumulation buffer -> tone-mapped ARGB frame."""
+
+    def __init__(self, ctx: Context, width: int, height: int):
+        torch = _torch()
+        self.ctx = ctx
+        self.width, self.height = width, height
+        self.frameSize = width * height
+        self.cFrameAccum = ctx.empty((self.frameSize, 4), torch.float32)
+        self.cFrameBuffer = ctx.empty((self.frameSize, 4), torch.float32)
+        self.cFrameARGB = ctx.empty((self.frameSize,), torch.int32)
+        self.cTotalLogLuminance = torch.zeros((1,), dtype=torch.float32, device=ctx.torch_device)
+        self.cTotalNumber = ctx.scalar_i32(0)
+        self.frameCount = 1.0
+        self.gamma = 2.2
+        self.exposure = 0.18
+        self.initBuffers()
+
+    def initBuffers(self) -> None:
+        call("mt_image_init", self.ctx.handle, _ptr(self.cFrameAccum), _ptr(self.cFrameARGB), self.frameSize)
+        self.frameCount = 1.0
+
+    def processImage(self) -> None:
+        # the log-luminance totals are recomputed only for the first frames
+        call("mt_image_process", self.ctx.handle, _ptr(self.cFrameAccum), float(self.frameCount), self.frameSize,
+             float(self.exposure), float(self.gamma), 1 if self.frameCount < 3 else 0,
+             _ptr(self.cFrameBuffer), _ptr(self.cFrameARGB), _ptr(self.cTotalLogLuminance),
+             _ptr(self.cTotalNumber))
+
+    def setGamma(self, value: float) -> None:
+        self.gamma = float(value)
+
+    def setExposure(self, value: float) -> None:
+        self.exposure = float(value)
+
+    def getFrameARGB(self):
+        return self.cFrameARGB
+
+
 def compact(ctx: Context, flags, flag_mode: int = 0, length=None, out_index=True,
             payloads: Sequence = ()):
     """Stable compaction; returns (out_index or None, d_count, [dst tensors])."""

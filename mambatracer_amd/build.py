@@ -32,6 +32,7 @@ SOURCES = [
     "mt_render.hip",
     "mt_batch.hip",
     "mt_bvh.hip",
+    "mt_image.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -55,6 +55,8 @@ extern "C" int32_t mt_ctx_create(int32_t device, void* stream, int32_t flags, mt
     c->device = device;
     c->scratch = nullptr;
     c->scratch_bytes = 0;
+    c->temp = nullptr;
+    c->temp_bytes = 0;
     c->epoch = 0;
     if (!(flags & MT_CTX_OWN_STREAM)) {
         c->stream = (hipStream_t)stream;
@@ -78,6 +80,7 @@ extern "C" int32_t mt_ctx_destroy(mt_context* ctx)
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     if (ctx->scratch) hipFree(ctx->scratch);
+    if (ctx->temp) hipFree(ctx->temp);
     if (ctx->own_stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return MT_OK;
@@ -87,6 +90,31 @@ extern "C" int32_t mt_ctx_sync(mt_context* ctx)
 {
     MT_ARG_CHECK(ctx != nullptr, "mt_ctx_sync: ctx is NULL");
     MT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MT_OK;
+}
+
+int32_t mt_ctx_temp(mt_context* ctx, size_t bytes, void** out)
+{
+    if (bytes <= ctx->temp_bytes) {
+        *out = ctx->temp;
+        return MT_OK;
+    }
+    size_t want = bytes < (size_t)1 << 20 ? (size_t)1 << 20 : bytes;
+    MT_HIP_CHECK(hipStreamSynchronize(ctx->stream)); // earlier work may still read the old buffer
+    if (ctx->temp) {
+        MT_HIP_CHECK(hipFree(ctx->temp));
+        ctx->temp = nullptr;
+        ctx->temp_bytes = 0;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        mt_set_error("temporary allocation of %zu bytes failed: %s", want, hipGetErrorString(e));
+        return MT_ERR_ALLOC;
+    }
+    ctx->temp = p;
+    ctx->temp_bytes = want;
+    *out = p;
     return MT_OK;
 }
 

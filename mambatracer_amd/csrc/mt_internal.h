@@ -42,6 +42,10 @@ struct mt_context {
     void* scratch;
     size_t scratch_bytes;
     uint32_t epoch;   // tags look-back status words so they never need clearing
+    // Second workspace for plain per-call temporaries (never the status
+    // words above, which must only ever hold epoch-tagged values).
+    void* temp;
+    size_t temp_bytes;
 };
 
 // Packed traversal layout derived from the reference arrays.
@@ -84,6 +88,7 @@ constexpr int ENV_GUIDE_M = 1024;
 constexpr int ENV_GUIDE_C = 256;
 
 int32_t mt_ctx_scratch(mt_context* ctx, size_t bytes, void** out);
+int32_t mt_ctx_temp(mt_context* ctx, size_t bytes, void** out);
 
 // One scene's packed traversal layout, as a row of a per-job table.
 struct MtSceneRef {

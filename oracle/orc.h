@@ -72,6 +72,14 @@ int orc_shadow_rays(int64_t n, const float* o_tmin, const float* d_tmax,
 /* ---- D: scan and compaction ---- */
 int orc_scan_excl_i32(const int32_t* in, int64_t n, int32_t* out, int32_t* total);
 int orc_compact_stable(const int32_t* pred, int64_t n, int32_t* out_idx, int32_t* count);
+int32_t orc_scan_excl_f32(const float* in, int64_t n, float* out, float* total);
+
+/* ---- image pipeline (CImage.processImage) ---- */
+int32_t orc_image_process(const float* accum, float frame_count, int32_t n, float exposure, float gamma,
+                          int32_t update_totals, float* frame, int32_t* argb_io, float* total_log,
+                          int32_t* total_count);
+int32_t orc_image_init_argb(void);
+int32_t orc_argb(float r, float g, float b, float a);
 
 /* ---- C: light grid, SATs, environment light ---- */
 int orc_sat_tiles(const float* func, float* sat, int32_t nu, int32_t nv,

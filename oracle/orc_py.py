@@ -174,6 +174,50 @@ def scan_excl(x):
     return out, int(tot.value)
 
 
+def scan_excl_f32(x):
+    """orc_scan_excl_f32: the device float scan's exact summation order."""
+    x = _f32(x)
+    out = np.zeros_like(x)
+    tot = C.c_float(0)
+    f = lib().orc_scan_excl_f32
+    f.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+    f.restype = C.c_int32
+    _ok(f(_p(x), x.size, _p(out), C.byref(tot)), "orc_scan_excl_f32")
+    return out, float(tot.value)
+
+
+def image_process(accum, frame_count, exposure, gamma, argb=None, totals=None):
+    """orc_image_process: CImage.processImage on float32 accum[n,4].  `totals`
+    = (total_log, count) to reuse (update_totals off) or None to compute.
+    Returns (frame[n,4], argb[n], (total_log, count))."""
+    accum = _f32(accum)
+    n = accum.shape[0]
+    frame = np.zeros((n, 4), np.float32)
+    argb = np.full(n, image_init_argb(), np.int32) if argb is None else _i32(argb).copy()
+    tl = C.c_float(0.0 if totals is None else totals[0])
+    tc = C.c_int32(0 if totals is None else totals[1])
+    f = lib().orc_image_process
+    f.argtypes = [C.c_void_p, C.c_float, C.c_int32, C.c_float, C.c_float, C.c_int32, C.c_void_p, C.c_void_p,
+                  C.c_void_p, C.c_void_p]
+    f.restype = C.c_int32
+    _ok(f(_p(accum), frame_count, n, exposure, gamma, 1 if totals is None else 0, _p(frame), _p(argb),
+          C.byref(tl), C.byref(tc)), "orc_image_process")
+    return frame, argb, (float(tl.value), int(tc.value))
+
+
+def image_init_argb():
+    f = lib().orc_image_init_argb
+    f.restype = C.c_int32
+    return int(f())
+
+
+def argb(r, g, b, a):
+    f = lib().orc_argb
+    f.argtypes = [C.c_float] * 4
+    f.restype = C.c_int32
+    return int(f(r, g, b, a))
+
+
 def compact_stable(pred):
     pred = _i32(pred)
     out = np.zeros(pred.size, np.int32)
