@@ -135,6 +135,30 @@ int32_t mt_gen_spheres(uint64_t seed, int32_t count, float mu, float sigma,
                        float* points, float* normals, int32_t* faces);
 
 /* ------------------------------------------------------------------ */
+/* Input formats (host)                                               */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int32_t num_points, num_normals, num_uvs, num_tris, num_groups;
+} mt_obj_counts;
+
+/* OBJParser.readString (un-vendored coordinate.parser.obj.OBJParser, used
+ * at TracerAPI.java:321-322): Wavefront OBJ text into the layouts above —
+ * points/normals float4 (w = 0), uvs float2, faces int32[10] with -1 for an
+ * absent uv/normal; polygons are fanned into triangles (corner 0, k, k+1);
+ * negative indices count from the end.  Material index = rank of the
+ * face's group among groups with faces (`o`, `g`, `usemtl` start one).
+ * Call with points == NULL to get the counts, then with arrays of at least
+ * those sizes in *counts (normals/uvs may be NULL). */
+int32_t mt_obj_parse(const char* text, uint64_t len, mt_obj_counts* counts, float* points,
+                     float* normals, float* uvs, int32_t* faces);
+/* HDRBitmapReader (un-vendored bitmap.reader.HDRBitmapReader, used at
+ * UserInterfaceFXMLController.java:176-179): Radiance RGBE (.hdr, "-Y h +X
+ * w", flat, old-style or adaptive run-length scanlines) into float RGBA
+ * rows (a = 1), value = mantissa * 2^(exponent - 136).  rgba == NULL
+ * returns only the size. */
+int32_t mt_hdr_decode(const uint8_t* data, uint64_t len, int32_t* width, int32_t* height, float* rgba);
+
+/* ------------------------------------------------------------------ */
 /* Device scene                                                       */
 /* ------------------------------------------------------------------ */
 typedef struct {

@@ -48,6 +48,11 @@ class SceneDesc(C.Structure):
     ]
 
 
+class ObjCounts(C.Structure):
+    _fields_ = [("num_points", C.c_int32), ("num_normals", C.c_int32), ("num_uvs", C.c_int32),
+                ("num_tris", C.c_int32), ("num_groups", C.c_int32)]
+
+
 class Payload(C.Structure):
     _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("elem_bytes", C.c_int32)]
 
@@ -144,6 +149,8 @@ SIGNATURES = {
     "mt_gen_random_rays": [U64, I64, I64, P, P, P, P, I32],
     "mt_gen_spheres": [U64, I32, F32, F32, F32, F32, F32, F32, P, P, P],
     "mt_bvh_build_device": [P, I32, P, P, P, P, C.POINTER(C.c_int32)],
+    "mt_obj_parse": [C.c_char_p, U64, C.POINTER(ObjCounts), P, P, P, P],
+    "mt_hdr_decode": [C.c_char_p, U64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), P],
     "mt_scene_create": [P, C.POINTER(SceneDesc), PP],
     "mt_scene_destroy": [P],
     "mt_trace_closest": [P, P, P, P, P, I32, P, P, P, I32],

@@ -696,3 +696,44 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------- formats
+class OBJParser:
+    """OBJParser.readString (un-vendored; used at TracerAPI.java:321-322) into
+    the reference's mesh arrays: (points float32[V,4], normals float32[N,4],
+    uvs float32[T,2], faces int32[F,10], number of material groups)."""
+
+    def readString(self, text):
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        cnt = _lib.ObjCounts()
+        call("mt_obj_parse", data, len(data), C.byref(cnt), None, None, None, None)
+        points = np.zeros((cnt.num_points, 4), np.float32)
+        normals = np.zeros((cnt.num_normals, 4), np.float32)
+        uvs = np.zeros((cnt.num_uvs, 2), np.float32)
+        faces = np.zeros((cnt.num_tris, 10), np.int32)
+        if cnt.num_points == 0:
+            return points, normals, uvs, faces, 0
+        call("mt_obj_parse", data, len(data), C.byref(cnt), _ptr(points), _ptr(normals) if cnt.num_normals else None,
+             _ptr(uvs) if cnt.num_uvs else None, _ptr(faces))
+        return points, normals, uvs, faces, int(cnt.num_groups)
+
+    def read(self, path):
+        with open(path, "rb") as f:
+            return self.readString(f.read())
+
+
+class HDRBitmapReader:
+    """HDRBitmapReader (un-vendored; used at UserInterfaceFXMLController.java:
+    176-179): Radiance RGBE bytes -> float32[h, w, 4] (a = 1)."""
+
+    def decode(self, data: bytes) -> np.ndarray:
+        w, h = C.c_int32(0), C.c_int32(0)
+        call("mt_hdr_decode", data, len(data), C.byref(w), C.byref(h), None)
+        out = np.zeros((h.value, w.value, 4), np.float32)
+        call("mt_hdr_decode", data, len(data), C.byref(w), C.byref(h), _ptr(out))
+        return out
+
+    def load(self, path) -> np.ndarray:
+        with open(path, "rb") as f:
+            return self.decode(f.read())

@@ -33,6 +33,7 @@ SOURCES = [
     "mt_batch.hip",
     "mt_bvh.hip",
     "mt_image.hip",
+    "mt_io.cpp",
 ]
 
 HIPCC_FLAGS = [
