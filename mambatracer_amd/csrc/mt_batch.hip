@@ -23,7 +23,8 @@
 //   3. closest hit over the queue, each ray against its job's scene;
 //   4. k_shade_jobs: shade_path per slot with its job's scene view; ended
 //      paths count into their job's done counter (wave-aggregated atomics);
-//   5. any hit over the shadow rays; 6. k_nee_jobs adds unoccluded light;
+//      shadow rays go to a dense queue with their slot's tag;
+//   5. any hit over the shadow queue; 6. k_nee_jobs adds unoccluded light;
 //   7. stable compaction of live slots (4 streams) into the other copy.
 // The host then reads the done counters (4 B per job slot), resolves the
 // images of finished jobs (k_resolve, fixed sample order) and admits waiting
@@ -113,6 +114,7 @@ __global__ __launch_bounds__(64) void k_refill(const int* __restrict__ active, i
     }
     if (lane == 0) {
         cnt[1] = live + (int)carry;
+        cnt[2] = 0; // shadow queue of this iteration
         tot[0] += (unsigned long long)(live + carry);
     }
 }
@@ -154,8 +156,9 @@ struct ShadeJobsArgs {
     const int* prim;
     const float* tin;
     int* alive;
-    float4 *s_o, *s_d, *s_c;
-    unsigned long long* tot;
+    float4 *s_o, *s_d, *s_c; // dense shadow queue: rays and contributions
+    int2* s_tag;             // the entry's slot tag (job word, path id)
+    int* s_count;            // queue length (cnt[2])
 };
 
 __global__ __launch_bounds__(BT) void k_shade_jobs(ShadeJobsArgs A)
@@ -166,6 +169,8 @@ __global__ __launch_bounds__(BT) void k_shade_jobs(ShadeJobsArgs A)
     const int lane = __lane_id();
     bool ended = false, has_shadow = false;
     int job = -1;
+    float4 so, sd, sc;
+    int2 stag;
     if (slot < count) {
         const int2 tg = A.Q.tag[slot];
         job = tg.x >> BOUNCE_BITS;
@@ -186,14 +191,20 @@ __global__ __launch_bounds__(BT) void k_shade_jobs(ShadeJobsArgs A)
             J.L[pid] = make_float4(L.x + P.Ladd.x, L.y + P.Ladd.y, L.z + P.Ladd.z, L.w);
         }
         A.alive[slot] = alive ? 1 : 0;
-        A.s_o[slot] = P.so;
-        A.s_d[slot] = P.sd;
-        A.s_c[slot] = P.contrib;
         ended = !alive;
         has_shadow = P.has_shadow;
+        so = P.so;
+        sd = P.sd;
+        sc = P.contrib;
+        stag = make_int2(tg.x, pid);
     }
-    const uint64_t sm = __ballot(has_shadow);
-    if (sm && lane == 0) atomicAdd(&A.tot[1], (unsigned long long)__popcll(sm));
+    const int k = wave_append(has_shadow, A.s_count);
+    if (has_shadow) {
+        A.s_o[k] = so;
+        A.s_d[k] = sd;
+        A.s_c[k] = sc;
+        A.s_tag[k] = stag;
+    }
     // ended paths per job: one atomic per (wave, job)
     uint64_t em = __ballot(ended);
     while (em) {
@@ -205,15 +216,17 @@ __global__ __launch_bounds__(BT) void k_shade_jobs(ShadeJobsArgs A)
     }
 }
 
+// Unoccluded shadow-queue entries add their contribution to their path;
+// thread 0 adds the iteration's shadow rays to the running total.
 __global__ __launch_bounds__(BT) void k_nee_jobs(const JobRec* __restrict__ jobs, const int* __restrict__ cnt,
-                                                 const int2* __restrict__ tag, const float4* __restrict__ contrib,
-                                                 const int* __restrict__ occluded)
+                                                 const int2* __restrict__ s_tag, const float4* __restrict__ contrib,
+                                                 const int* __restrict__ occluded, unsigned long long* __restrict__ tot)
 {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= cnt[1]) return;
-    const float4 c = contrib[slot];
-    if (c.w == 0.0f || occluded[slot]) return;
-    const int2 tg = tag[slot];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) tot[1] += (unsigned long long)cnt[2];
+    if (k >= cnt[2] || occluded[k]) return;
+    const float4 c = contrib[k];
+    const int2 tg = s_tag[k];
     float4* L = jobs[tg.x >> BOUNCE_BITS].L;
     const float4 v = L[tg.y];
     L[tg.y] = make_float4(v.x + c.x, v.y + c.y, v.z + c.z, v.w);
@@ -231,13 +244,14 @@ struct mt_batch {
     MtSceneRef* refs;
     int *npaths, *issued, *done;
     int *active, *gen_start, *gen_pid0, *gen_job;
-    int* cnt;                 // [0] live after compaction, [1] live + new
+    int* cnt;                 // [0] live after compaction, [1] live + new, [2] shadow queue length
     unsigned long long* tot;  // [0] extension rays, [1] shadow rays
     Queue q[2];
     int cur;
     int *prim, *alive, *occ;
     float* t;
     float4 *s_o, *s_d, *s_c;
+    int2* s_tag;
     int* h_active;            // pinned
     int* h_done;              // pinned
     unsigned long long* h_tot; // pinned
@@ -390,13 +404,14 @@ int32_t iterate(mt_batch* b)
     A.s_o = b->s_o;
     A.s_d = b->s_d;
     A.s_c = b->s_c;
-    A.tot = b->tot;
+    A.s_tag = b->s_tag;
+    A.s_count = b->cnt + 2;
     k_shade_jobs<<<grid, BT, 0, s>>>(A);
     MT_LAUNCH_CHECK();
-    rc = mt_trace_jobs(b->ctx, true, b->refs, &q.tag->x, BOUNCE_BITS, b->s_o, b->s_d, b->cnt + 1, Qc, nullptr,
+    rc = mt_trace_jobs(b->ctx, true, b->refs, &b->s_tag->x, BOUNCE_BITS, b->s_o, b->s_d, b->cnt + 2, Qc, nullptr,
                        nullptr, b->occ);
     if (rc) return rc;
-    k_nee_jobs<<<grid, BT, 0, s>>>(b->jobs, b->cnt, q.tag, b->s_c, b->occ);
+    k_nee_jobs<<<grid, BT, 0, s>>>(b->jobs, b->cnt, b->s_tag, b->s_c, b->occ, b->tot);
     MT_LAUNCH_CHECK();
     mt_payload P[4] = {{q.tag, nq.tag, 8}, {q.ro, nq.ro, 16}, {q.rd, nq.rd, 16}, {q.T, nq.T, 16}};
     rc = mt_compact(b->ctx, b->alive, 0, b->cnt + 1, Qc, nullptr, b->cnt, P, 4);
@@ -447,7 +462,7 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
         for (int m = 1; m < 4; m++) oQ[k][m] = take(16 * n);
     }
     size_t oPrim = take(4 * n), oT = take(4 * n), oAlive = take(4 * n), oOcc = take(4 * n);
-    size_t oSo = take(16 * n), oSd = take(16 * n), oSc = take(16 * n);
+    size_t oSo = take(16 * n), oSd = take(16 * n), oSc = take(16 * n), oStag = take(8 * n);
     if (hipMalloc(&b->mem, bytes) != hipSuccess) {
         delete b;
         mt_set_error("mt_batch_create: cannot allocate %.2f GB of queue state", bytes / 1e9);
@@ -494,6 +509,7 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     b->s_o = reinterpret_cast<float4*>(m + oSo);
     b->s_d = reinterpret_cast<float4*>(m + oSd);
     b->s_c = reinterpret_cast<float4*>(m + oSc);
+    b->s_tag = reinterpret_cast<int2*>(m + oStag);
     b->cur = 0;
     b->active_dirty = true;
     b->st = mt_batch_stats{};

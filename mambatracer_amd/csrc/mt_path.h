@@ -332,6 +332,21 @@ __device__ __forceinline__ void shade_path(const SceneView& V, uint32_t seed, in
     out.Ladd = Ladd;
 }
 
+// Position of this lane's entry when the wave's flagged lanes append to a
+// dense queue: one atomic per wave, lane order inside the wave.  Call with
+// the whole wave active.  Queue order across waves follows arrival, which
+// no result depends on (each path owns at most one entry per bounce).
+__device__ __forceinline__ int wave_append(bool flag, int* counter)
+{
+    const uint64_t m = __ballot(flag);
+    int base = 0;
+    if (m) {
+        if (__lane_id() == 0) base = atomicAdd(counter, (int)__popcll(m));
+        base = __shfl(base, 0);
+    }
+    return base + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Per-pixel mean over samples in sample order (averageAccum, Image.cl:29-45,
 // without tone mapping): deterministic, no float atomics.
 __global__ __launch_bounds__(256) void k_resolve(const float4* __restrict__ L, int npix, int spp,

@@ -14,7 +14,9 @@
 //   2. one fused shading kernel per slot (shade_path): hit shading,
 //      setupBsdf, LightHitPass, DirectLight's light sample + EvaluateBsdf +
 //      shadow-ray setup, SampleBsdf for the next ray and Russian roulette;
-//   3. any hit over the shadow rays (mt_trace_any);
+//      shadow rays are appended to a dense queue (one atomic per wave), so
+//      the any-hit launch has no empty lanes;
+//   3. any hit over the shadow queue (mt_trace_any);
 //   4. unoccluded direct light added to the path radiance;
 //   5. stable compaction of live slots (pid and next ray) into the other
 //      queue buffer (mt_compact; CCompact.execute in the reference).
@@ -57,37 +59,43 @@ struct ShadeArgs {
     const int* prim;
     const float* tin;
     int* alive;           // out: 1 if the slot's path continues
-    float4* s_o_tmin;     // out: shadow rays (tMin = inf marks "none")
+    float4* s_o_tmin;     // out: dense queue of shadow rays
     float4* s_d_tmax;
-    float4* s_contrib;    // out: unoccluded direct-light contribution
-    int* shadow_count;
+    float4* s_contrib;    // out: unoccluded contribution, w = the ray's slot (int bits)
+    int* shadow_count;    // queue length
     int bounce;
     int rr_start;
     int flags;
     uint32_t seed;
 };
 
-// One slot of the fused shading stage; returns whether a shadow ray was set.
-__device__ __forceinline__ bool shade_slot(const ShadeArgs& A, int slot)
+// One slot of the fused shading stage; its shadow ray, if any, goes to the
+// next free entry of the dense shadow queue.
+__device__ __forceinline__ void shade_slot(const ShadeArgs& A, int slot, bool live)
 {
-    const int pid = A.queue[slot];
     PathOut P;
-    shade_path(A.V, A.seed, A.rr_start, A.flags, pid, A.bounce, A.o_tmin[slot], A.d_tmax[slot], A.S.T[pid],
-               A.prim[slot], A.tin[slot], P);
-    if (P.alive) {
-        A.S.T[pid] = P.T;
-        A.o_tmin[slot] = P.ro;
-        A.d_tmax[slot] = P.rd;
+    P.has_shadow = false;
+    if (live) {
+        const int pid = A.queue[slot];
+        shade_path(A.V, A.seed, A.rr_start, A.flags, pid, A.bounce, A.o_tmin[slot], A.d_tmax[slot], A.S.T[pid],
+                   A.prim[slot], A.tin[slot], P);
+        if (P.alive) {
+            A.S.T[pid] = P.T;
+            A.o_tmin[slot] = P.ro;
+            A.d_tmax[slot] = P.rd;
+        }
+        if (P.Ladd.x != 0.0f || P.Ladd.y != 0.0f || P.Ladd.z != 0.0f) {
+            float4 L = A.S.L[pid];
+            A.S.L[pid] = make_float4(L.x + P.Ladd.x, L.y + P.Ladd.y, L.z + P.Ladd.z, L.w);
+        }
+        A.alive[slot] = P.alive ? 1 : 0;
     }
-    if (P.Ladd.x != 0.0f || P.Ladd.y != 0.0f || P.Ladd.z != 0.0f) {
-        float4 L = A.S.L[pid];
-        A.S.L[pid] = make_float4(L.x + P.Ladd.x, L.y + P.Ladd.y, L.z + P.Ladd.z, L.w);
+    const int k = wave_append(P.has_shadow, A.shadow_count);
+    if (P.has_shadow) {
+        A.s_o_tmin[k] = P.so;
+        A.s_d_tmax[k] = P.sd;
+        A.s_contrib[k] = make_float4(P.contrib.x, P.contrib.y, P.contrib.z, __int_as_float(slot));
     }
-    A.alive[slot] = P.alive ? 1 : 0;
-    A.s_o_tmin[slot] = P.so;
-    A.s_d_tmax[slot] = P.sd;
-    A.s_contrib[slot] = P.contrib;
-    return P.has_shadow;
 }
 
 // Waves stride over the live slots [0, *d_count).
@@ -96,23 +104,20 @@ __global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
     const int count = *A.d_count;
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64; base < count; base += nw * 64) {
-        int slot = base + lane;
-        bool sh = slot < count ? shade_slot(A, slot) : false;
-        uint64_t m = __ballot(sh);
-        if (m && lane == 0) atomicAdd(A.shadow_count, (int)__popcll(m));
-    }
+    for (int base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64; base < count; base += nw * 64)
+        shade_slot(A, base + lane, base + lane < count);
 }
 
+// Unoccluded shadow-queue entries add their contribution to their path.
 __global__ __launch_bounds__(RT) void k_nee_accum(const int* __restrict__ queue, const int* __restrict__ d_count,
                                                   const float4* __restrict__ contrib,
                                                   const int* __restrict__ occluded, PathState S)
 {
     const int count = *d_count;
-    for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < count; slot += gridDim.x * blockDim.x) {
-        float4 c = contrib[slot];
-        if (c.w == 0.0f || occluded[slot]) continue;
-        int pid = queue[slot];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
+        if (occluded[k]) continue;
+        const float4 c = contrib[k];
+        const int pid = queue[__float_as_int(c.w)];
         float4 L = S.L[pid];
         S.L[pid] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
     }
@@ -261,12 +266,13 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
         k_shade<<<gshade, RT, 0, st>>>(A);
         if ((rc = chk())) break;
         if (profile) hipEventRecord(ev[4 * b + 2], st);
-        rc = mt_trace_any(ctx, scene, reinterpret_cast<float*>(F4(oSo)), reinterpret_cast<float*>(F4(oSd)), dcount,
-                          np, I(oOcc), ilp, desc->mode);
+        // any hit over the dense shadow queue (length shc[b], on the device)
+        rc = mt_trace_any(ctx, scene, reinterpret_cast<float*>(F4(oSo)), reinterpret_cast<float*>(F4(oSd)),
+                          shc + b, np, I(oOcc), ilp, desc->mode);
         if (rc) break;
         if (profile) hipEventRecord(ev[4 * b + 3], st);
-        if (want_ilp) k_sum_ilp<<<grid, RT, 0, st>>>(ilp, dcount, tot + 3);
-        k_nee_accum<<<gnee, RT, 0, st>>>(q[cur], dcount, F4(oSc), I(oOcc), S);
+        if (want_ilp) k_sum_ilp<<<grid, RT, 0, st>>>(ilp, shc + b, tot + 3);
+        k_nee_accum<<<gnee, RT, 0, st>>>(q[cur], shc + b, F4(oSc), I(oOcc), S);
         if ((rc = chk())) break;
         if (b + 1 < nb) {
             mt_payload P[3] = {{q[cur], q[1 - cur], 4}, {ro[cur], ro[1 - cur], 16}, {rd[cur], rd[1 - cur], 16}};
