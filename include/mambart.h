@@ -339,8 +339,10 @@ int32_t mt_env_radiance(mt_context* ctx, const mt_envmap* env, int32_t n,
 /* ------------------------------------------------------------------ */
 #define MT_BSDF_WORDS 20 /* frame mX,mY,mZ (9), localDirFix (3), localGeomN (3),
                             diffProb, glossyProb, 3 reserved */
-/* setupBsdf (Bsdf.cl:130-160) for n hits: bsdf float[20*n], mat_id (-1 if
- * the incoming direction is grazing). */
+/* setupBsdf (Bsdf.cl:130-160) for n hits: ray_d, isect_n, isect_ng are
+ * float4[n] (xyz used), mat int32[n] (material index, or -1 for a miss);
+ * bsdf float[20*n], mat_id (-1 for a miss or a grazing incoming
+ * direction). */
 int32_t mt_bsdf_setup(mt_context* ctx, int32_t n, const float* ray_d,
                       const float* isect_n, const float* isect_ng, const int32_t* mat,
                       const float* materials, float* bsdf, int32_t* mat_id);

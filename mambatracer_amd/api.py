@@ -46,6 +46,18 @@ def _torch():
     return torch
 
 
+def _need(x, name: str, n: int, tail=(), dtype="float32"):
+    """Raises unless x is a contiguous device tensor of shape (n, *tail) and
+    the given dtype: the C ABI reads fixed strides (float4 rays, float[3]
+    directions ...) and cannot see a tensor's shape."""
+    torch = _torch()
+    want = getattr(torch, dtype)
+    if not isinstance(x, torch.Tensor) or x.dtype != want or not x.is_contiguous() or x.device.type != "cuda" \
+            or tuple(x.shape) != (n, *tail):
+        got = f"{type(x).__name__} {getattr(x, 'dtype', '')} {tuple(getattr(x, 'shape', ()))}"
+        raise ValueError(f"{name}: need a contiguous {dtype} device tensor of shape {(n, *tail)}, got {got}")
+
+
 # ---------------------------------------------------------------------- context
 class Context:
     """One device, one in-order stream (torch's current stream by default)."""
@@ -218,6 +230,8 @@ def intersect_primitives(ctx: Context, scene: Scene, o_tmin, d_tmax, count=None,
     """Closest hit: returns (prim int32[n], t float32[n], stats int32[n,3] or None)."""
     torch = _torch()
     n = o_tmin.shape[0]
+    _need(o_tmin, "o_tmin", n, (4,))
+    _need(d_tmax, "d_tmax", n, (4,))
     if count is None:
         count = ctx.scalar_i32(n)
     prim = ctx.empty((n,), torch.int32)
@@ -233,6 +247,8 @@ def intersect_occlusion(ctx: Context, scene: Scene, o_tmin, d_tmax, count=None, 
     """Any hit: returns (occluded int32[n], stats or None)."""
     torch = _torch()
     n = o_tmin.shape[0]
+    _need(o_tmin, "o_tmin", n, (4,))
+    _need(d_tmax, "d_tmax", n, (4,))
     if count is None:
         count = ctx.scalar_i32(n)
     occ = ctx.empty((n,), torch.int32)
@@ -563,8 +579,12 @@ def sat_sample_region(ctx: Context, func, sat, nu, nv, minx, miny, sx, sy, u):
 
 # ---------------------------------------------------------------------- B
 def bsdf_setup(ctx: Context, ray_d, n, ng, mat, materials):
+    """ray_d, n, ng: float32[cnt, 4] (xyz used); mat: int32[cnt] (-1 = miss)."""
     torch = _torch()
     cnt = int(ray_d.shape[0])
+    for x, nm in ((ray_d, "ray_d"), (n, "n"), (ng, "ng")):
+        _need(x, f"bsdf_setup {nm}", cnt, (4,))
+    _need(mat, "bsdf_setup mat", cnt, (), "int32")
     rec = ctx.empty((cnt, _lib.MT_BSDF_WORDS), torch.float32)
     mid = ctx.empty((cnt,), torch.int32)
     call("mt_bsdf_setup", ctx.handle, cnt, _ptr(ray_d), _ptr(n), _ptr(ng), _ptr(mat), _ptr(materials),
@@ -575,6 +595,9 @@ def bsdf_setup(ctx: Context, ray_d, n, ng, mat, materials):
 def bsdf_sample(ctx: Context, rec, mat_id, materials, u):
     torch = _torch()
     cnt = int(rec.shape[0])
+    _need(rec, "bsdf_sample rec", cnt, (_lib.MT_BSDF_WORDS,))
+    _need(mat_id, "bsdf_sample mat_id", cnt, (), "int32")
+    _need(u, "bsdf_sample u", cnt, (3,))
     f = ctx.empty((cnt, 3), torch.float32)
     d = ctx.empty((cnt, 3), torch.float32)
     pdf = ctx.empty((cnt,), torch.float32)
@@ -587,6 +610,9 @@ def bsdf_sample(ctx: Context, rec, mat_id, materials, u):
 def bsdf_eval(ctx: Context, rec, mat_id, materials, dirs):
     torch = _torch()
     cnt = int(rec.shape[0])
+    _need(rec, "bsdf_eval rec", cnt, (_lib.MT_BSDF_WORDS,))
+    _need(mat_id, "bsdf_eval mat_id", cnt, (), "int32")
+    _need(dirs, "bsdf_eval dirs", cnt, (3,))
     f = ctx.empty((cnt, 3), torch.float32)
     pdf = ctx.empty((cnt,), torch.float32)
     cs = ctx.empty((cnt,), torch.float32)

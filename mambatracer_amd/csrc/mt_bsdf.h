@@ -148,7 +148,7 @@ __device__ __forceinline__ MtBsdf mt_bsdf_setup_one(v3 ray_d, v3 n, v3 ng, int m
     B.geom = mk3(0, 0, 0);
     B.diff_prob = 0.0f;
     B.glossy_prob = 0.0f;
-    if (fabsf(B.fix.z) < MT_EPS_COSINE) {
+    if (fabsf(B.fix.z) < MT_EPS_COSINE || mat < 0) { // grazing, or no surface (a miss)
         B.mat = -1;
         return B;
     }

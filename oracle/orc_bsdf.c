@@ -201,7 +201,8 @@ int orc_bsdf_setup(int32_t n, const float* ray_d, const float* isect_n,
         rec[3] = F.my.x; rec[4] = F.my.y; rec[5] = F.my.z;
         rec[6] = F.mz.x; rec[7] = F.mz.y; rec[8] = F.mz.z;
         rec[9] = fix.x; rec[10] = fix.y; rec[11] = fix.z;
-        if (fabsf(fix.z) < EPS_COSINE) { mat_id[i] = -1; continue; }
+        /* grazing, or no surface (mat -1: a miss; the reference never sets up those) */
+        if (fabsf(fix.z) < EPS_COSINE || mat[i] < 0) { mat_id[i] = -1; continue; }
         f4 g = to_local(&F, ng);
         rec[12] = g.x; rec[13] = g.y; rec[14] = g.z;
         mat_id[i] = mat[i];

@@ -47,6 +47,7 @@ def _setup(ctx, n=100000, seed=0):
     ray_d[:, :3] = np.where((np.sum(ray_d[:, :3] * nrm[:, :3], 1) > 0)[:, None], -ray_d[:, :3], ray_d[:, :3])
     ng = nrm.copy()
     mat = rng.integers(0, 64, size=n).astype(np.int32)
+    mat[::97] = -1                 # misses: no material is read, mat_id -1
     rec, mid = api.bsdf_setup(ctx, ctx.to_device(ray_d), ctx.to_device(nrm), ctx.to_device(ng), ctx.to_device(mat),
                               ctx.to_device(M))
     orec, omid = orc.bsdf_setup(ray_d, nrm, ng, mat, M)
