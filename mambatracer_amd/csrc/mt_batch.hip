@@ -8,7 +8,9 @@
 //   job table   JobRec[max_jobs]: scene view, camera, sizes, seed and the
 //               job's per-path radiance buffer (a slice of the arena);
 //               MtSceneRef[max_jobs]: the packed traversal layout per job;
-//               npaths/issued/done int32[max_jobs] counters.
+//               npaths/issued int32[max_jobs] counters; done counters
+//               int32[DONE_SEGS][max_jobs] (work-group b counts into row
+//               b % DONE_SEGS) summed per job into done_tot[max_jobs].
 //   queue       two copies (ping-pong) of four SoA streams per slot:
 //               tag int2 (job << 6 | bounce, path id), ray o/tmin and
 //               d/tmax float4, throughput float4 (w = last BSDF pdf);
@@ -23,10 +25,12 @@
 //   3. closest hit over the queue, each ray against its job's scene;
 //   4. k_shade_jobs: shade_path per slot with its job's scene view; ended
 //      paths count into their job's done counter (wave-aggregated atomics);
-//      shadow rays go to a dense queue with their slot's tag;
+//      shadow rays go to a segmented queue (MtSegQueue) with their slot's
+//      tag;
 //   5. any hit over the shadow queue; 6. k_nee_jobs adds unoccluded light;
-//   7. stable compaction of live slots (4 streams) into the other copy.
-// The host then reads the done counters (4 B per job slot), resolves the
+//   7. stable compaction of live slots (4 streams) into the other copy;
+//   8. k_done_sum folds the done rows of the active jobs.
+// The host then reads the done totals (4 B per job slot), resolves the
 // images of finished jobs (k_resolve, fixed sample order) and admits waiting
 // jobs into the freed slots and arena space.
 #include <algorithm>
@@ -40,6 +44,10 @@ namespace {
 
 constexpr int BT = 256;
 constexpr int BOUNCE_BITS = 6; // max_bounces <= 63
+// Rows of done counters: a job's ended paths are counted by many waves of
+// one launch, and one counter per job would serialise their atomics.
+constexpr int DONE_SEGS = 64;
+static_assert(SHADOW_SEGS % 64 == 0, "k_refill clears the segment counters one wave-width at a time");
 
 struct JobRec {
     SceneView V;
@@ -54,16 +62,31 @@ struct Queue {
     float4 *ro, *rd, *T;
 };
 
-__global__ void k_set_job(JobRec* jobs, MtSceneRef* refs, int* npaths, int* issued, int* done, int slot,
-                          JobRec rec, MtSceneRef ref)
+// One wave of DONE_SEGS lanes.
+__global__ void k_set_job(JobRec* jobs, MtSceneRef* refs, int* npaths, int* issued, int* done, int* done_tot,
+                          int max_jobs, int slot, JobRec rec, MtSceneRef ref)
 {
+    if (threadIdx.x < DONE_SEGS) done[(size_t)threadIdx.x * max_jobs + slot] = 0;
     if (threadIdx.x == 0) {
         jobs[slot] = rec;
         refs[slot] = ref;
         npaths[slot] = rec.npaths;
         issued[slot] = 0;
-        done[slot] = 0;
+        done_tot[slot] = 0;
     }
+}
+
+// done_tot[j] = sum of job j's DONE_SEGS rows, for the active jobs.
+__global__ __launch_bounds__(BT) void k_done_sum(const int* __restrict__ active, int n_active,
+                                                 const int* __restrict__ done, int max_jobs,
+                                                 int* __restrict__ done_tot)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_active) return;
+    const int j = active[i];
+    int s = 0;
+    for (int r = 0; r < DONE_SEGS; r++) s += done[(size_t)r * max_jobs + j];
+    done_tot[j] = s;
 }
 
 __global__ void k_zero_counts(int* cnt, unsigned long long* tot)
@@ -79,9 +102,10 @@ __global__ __launch_bounds__(64) void k_refill(const int* __restrict__ active, i
                                                const int* __restrict__ npaths, int* __restrict__ issued,
                                                int capacity, int* __restrict__ cnt, int* __restrict__ gen_start,
                                                int* __restrict__ gen_pid0, int* __restrict__ gen_job,
-                                               unsigned long long* __restrict__ tot)
+                                               int* __restrict__ s_seg, unsigned long long* __restrict__ tot)
 {
     const int lane = threadIdx.x;
+    for (int p = lane; p < SHADOW_SEGS; p += 64) s_seg[p] = 0; // shadow queue of this iteration
     const int live = cnt[0];
     const int64_t avail = (int64_t)capacity - live;
     int64_t carry = 0;
@@ -114,7 +138,6 @@ __global__ __launch_bounds__(64) void k_refill(const int* __restrict__ active, i
     }
     if (lane == 0) {
         cnt[1] = live + (int)carry;
-        cnt[2] = 0; // shadow queue of this iteration
         tot[0] += (unsigned long long)(live + carry);
     }
 }
@@ -150,15 +173,17 @@ __global__ __launch_bounds__(BT) void k_generate(const JobRec* __restrict__ jobs
 
 struct ShadeJobsArgs {
     const JobRec* jobs;
-    int* done;
+    int* done;               // [DONE_SEGS][max_jobs]
+    int max_jobs;
     const int* cnt;
     Queue Q;
     const int* prim;
     const float* tin;
     int* alive;
-    float4 *s_o, *s_d, *s_c; // dense shadow queue: rays and contributions
+    float4 *s_o, *s_d, *s_c; // segmented shadow queue: rays and contributions
     int2* s_tag;             // the entry's slot tag (job word, path id)
-    int* s_count;            // queue length (cnt[2])
+    int* s_seg;              // SHADOW_SEGS segment lengths
+    int seg_cap;
 };
 
 __global__ __launch_bounds__(BT) void k_shade_jobs(ShadeJobsArgs A)
@@ -198,33 +223,41 @@ __global__ __launch_bounds__(BT) void k_shade_jobs(ShadeJobsArgs A)
         sc = P.contrib;
         stag = make_int2(tg.x, pid);
     }
-    const int k = wave_append(has_shadow, A.s_count);
+    const int seg = blockIdx.x % SHADOW_SEGS;
+    const int k = seg * A.seg_cap + wave_append(has_shadow, A.s_seg + seg);
     if (has_shadow) {
         A.s_o[k] = so;
         A.s_d[k] = sd;
         A.s_c[k] = sc;
         A.s_tag[k] = stag;
     }
-    // ended paths per job: one atomic per (wave, job)
+    // ended paths per job: one atomic per (wave, job) into this work-group's row
+    int* done_row = A.done + (size_t)(blockIdx.x % DONE_SEGS) * A.max_jobs;
     uint64_t em = __ballot(ended);
     while (em) {
         const int leader = __builtin_ctzll(em);
         const int lj = __shfl(job, leader);
         const uint64_t same = __ballot(ended && job == lj);
-        if (lane == leader) atomicAdd(&A.done[lj], (int)__popcll(same));
+        if (lane == leader) atomicAdd(&done_row[lj], (int)__popcll(same));
         em &= ~same;
     }
 }
 
 // Unoccluded shadow-queue entries add their contribution to their path;
-// thread 0 adds the iteration's shadow rays to the running total.
-__global__ __launch_bounds__(BT) void k_nee_jobs(const JobRec* __restrict__ jobs, const int* __restrict__ cnt,
-                                                 const int2* __restrict__ s_tag, const float4* __restrict__ contrib,
+// work-group 0 adds the iteration's shadow rays to the running total.
+__global__ __launch_bounds__(BT) void k_nee_jobs(const JobRec* __restrict__ jobs, const int* __restrict__ s_seg,
+                                                 int seg_cap, const int2* __restrict__ s_tag,
+                                                 const float4* __restrict__ contrib,
                                                  const int* __restrict__ occluded, unsigned long long* __restrict__ tot)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) tot[1] += (unsigned long long)cnt[2];
-    if (k >= cnt[2] || occluded[k]) return;
+    if (blockIdx.x == 0) {
+        int c = 0;
+        for (int p = threadIdx.x; p < SHADOW_SEGS; p += BT) c += s_seg[p];
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+        if (__lane_id() == 0 && c) atomicAdd(&tot[1], (unsigned long long)c);
+    }
+    if (!seg_live(s_seg, seg_cap, k) || occluded[k]) return;
     const float4 c = contrib[k];
     const int2 tg = s_tag[k];
     float4* L = jobs[tg.x >> BOUNCE_BITS].L;
@@ -242,9 +275,11 @@ struct mt_batch {
     char* arena; // per-job radiance buffers
     JobRec* jobs;
     MtSceneRef* refs;
-    int *npaths, *issued, *done;
+    int *npaths, *issued, *done, *done_tot;
     int *active, *gen_start, *gen_pid0, *gen_job;
-    int* cnt;                 // [0] live after compaction, [1] live + new, [2] shadow queue length
+    int* cnt;                 // [0] live after compaction, [1] live + new
+    int* s_seg;               // shadow queue segment lengths
+    int seg_cap;              // shadow queue segment capacity
     unsigned long long* tot;  // [0] extension rays, [1] shadow rays
     Queue q[2];
     int cur;
@@ -352,7 +387,8 @@ int32_t admit(mt_batch* b)
         rec.flags = d.render.flags;
         rec.seed = d.render.seed;
         MtSceneRef ref{d.scene->pnodes, d.scene->ptris, d.scene->root_ref, 0};
-        k_set_job<<<1, 64, 0, b->ctx->stream>>>(b->jobs, b->refs, b->npaths, b->issued, b->done, J.slot, rec, ref);
+        k_set_job<<<1, DONE_SEGS, 0, b->ctx->stream>>>(b->jobs, b->refs, b->npaths, b->issued, b->done, b->done_tot,
+                                                       b->max_jobs, J.slot, rec, ref);
         MT_LAUNCH_CHECK();
         b->st.arena_used += J.len;
         if (b->st.arena_used > b->st.arena_peak) b->st.arena_peak = b->st.arena_used;
@@ -387,15 +423,16 @@ int32_t iterate(mt_batch* b)
     Queue& q = b->q[b->cur];
     Queue& nq = b->q[1 - b->cur];
     k_refill<<<1, 64, 0, s>>>(b->active, n_active, b->npaths, b->issued, Qc, b->cnt, b->gen_start, b->gen_pid0,
-                              b->gen_job, b->tot);
+                              b->gen_job, b->s_seg, b->tot);
     k_generate<<<grid, BT, 0, s>>>(b->jobs, b->cnt, b->gen_start, b->gen_pid0, b->gen_job, n_active, q);
     MT_LAUNCH_CHECK();
     int32_t rc = mt_trace_jobs(b->ctx, false, b->refs, &q.tag->x, BOUNCE_BITS, q.ro, q.rd, b->cnt + 1, Qc, b->prim,
-                               b->t, nullptr);
+                               b->t, nullptr, MtSegQueue{nullptr, 0, 0});
     if (rc) return rc;
     ShadeJobsArgs A;
     A.jobs = b->jobs;
     A.done = b->done;
+    A.max_jobs = b->max_jobs;
     A.cnt = b->cnt;
     A.Q = q;
     A.prim = b->prim;
@@ -405,17 +442,22 @@ int32_t iterate(mt_batch* b)
     A.s_d = b->s_d;
     A.s_c = b->s_c;
     A.s_tag = b->s_tag;
-    A.s_count = b->cnt + 2;
+    A.s_seg = b->s_seg;
+    A.seg_cap = b->seg_cap;
     k_shade_jobs<<<grid, BT, 0, s>>>(A);
     MT_LAUNCH_CHECK();
-    rc = mt_trace_jobs(b->ctx, true, b->refs, &b->s_tag->x, BOUNCE_BITS, b->s_o, b->s_d, b->cnt + 2, Qc, nullptr,
-                       nullptr, b->occ);
+    const MtSegQueue sq{b->s_seg, SHADOW_SEGS, b->seg_cap};
+    rc = mt_trace_jobs(b->ctx, true, b->refs, &b->s_tag->x, BOUNCE_BITS, b->s_o, b->s_d, nullptr, 0, nullptr,
+                       nullptr, b->occ, sq);
     if (rc) return rc;
-    k_nee_jobs<<<grid, BT, 0, s>>>(b->jobs, b->cnt, b->s_tag, b->s_c, b->occ, b->tot);
+    k_nee_jobs<<<mt_blocks((int64_t)SHADOW_SEGS * b->seg_cap, BT), BT, 0, s>>>(b->jobs, b->s_seg, b->seg_cap,
+                                                                               b->s_tag, b->s_c, b->occ, b->tot);
     MT_LAUNCH_CHECK();
     mt_payload P[4] = {{q.tag, nq.tag, 8}, {q.ro, nq.ro, 16}, {q.rd, nq.rd, 16}, {q.T, nq.T, 16}};
     rc = mt_compact(b->ctx, b->alive, 0, b->cnt + 1, Qc, nullptr, b->cnt, P, 4);
     if (rc) return rc;
+    k_done_sum<<<mt_blocks(n_active, BT), BT, 0, s>>>(b->active, n_active, b->done, b->max_jobs, b->done_tot);
+    MT_LAUNCH_CHECK();
     b->cur = 1 - b->cur;
     b->st.iterations++;
     return MT_OK;
@@ -453,16 +495,18 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     size_t bytes = 0;
     auto take = [&](size_t sz) { size_t off = bytes; bytes += (sz + 255) & ~(size_t)255; return off; };
     size_t oJobs = take(sizeof(JobRec) * J), oRefs = take(sizeof(MtSceneRef) * J);
-    size_t oNp = take(4 * J), oIss = take(4 * J), oDone = take(4 * J);
+    size_t oNp = take(4 * J), oIss = take(4 * J), oDone = take(4 * J * DONE_SEGS), oDtot = take(4 * J);
     size_t oAct = take(4 * J), oGs = take(4 * J), oGp = take(4 * J), oGj = take(4 * J);
-    size_t oCnt = take(16), oTot = take(16);
+    size_t oCnt = take(16), oTot = take(16), oSeg = take(4 * SHADOW_SEGS);
+    b->seg_cap = shadow_seg_cap(queue_capacity, BT);
+    const size_t sq = (size_t)SHADOW_SEGS * b->seg_cap;
     size_t oQ[2][4];
     for (int k = 0; k < 2; k++) {
         oQ[k][0] = take(8 * n);
         for (int m = 1; m < 4; m++) oQ[k][m] = take(16 * n);
     }
-    size_t oPrim = take(4 * n), oT = take(4 * n), oAlive = take(4 * n), oOcc = take(4 * n);
-    size_t oSo = take(16 * n), oSd = take(16 * n), oSc = take(16 * n), oStag = take(8 * n);
+    size_t oPrim = take(4 * n), oT = take(4 * n), oAlive = take(4 * n), oOcc = take(4 * sq);
+    size_t oSo = take(16 * sq), oSd = take(16 * sq), oSc = take(16 * sq), oStag = take(8 * sq);
     if (hipMalloc(&b->mem, bytes) != hipSuccess) {
         delete b;
         mt_set_error("mt_batch_create: cannot allocate %.2f GB of queue state", bytes / 1e9);
@@ -490,6 +534,8 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     b->npaths = reinterpret_cast<int*>(m + oNp);
     b->issued = reinterpret_cast<int*>(m + oIss);
     b->done = reinterpret_cast<int*>(m + oDone);
+    b->done_tot = reinterpret_cast<int*>(m + oDtot);
+    b->s_seg = reinterpret_cast<int*>(m + oSeg);
     b->active = reinterpret_cast<int*>(m + oAct);
     b->gen_start = reinterpret_cast<int*>(m + oGs);
     b->gen_pid0 = reinterpret_cast<int*>(m + oGp);
@@ -570,7 +616,7 @@ extern "C" int32_t mt_batch_step(mt_batch* b, int32_t max_done, int64_t* done_ta
     if (!b->active_jobs.empty()) {
         rc = iterate(b);
         if (rc) return rc;
-        MT_HIP_CHECK(hipMemcpyAsync(b->h_done, b->done, 4 * (size_t)b->max_jobs, hipMemcpyDeviceToHost, s));
+        MT_HIP_CHECK(hipMemcpyAsync(b->h_done, b->done_tot, 4 * (size_t)b->max_jobs, hipMemcpyDeviceToHost, s));
         MT_HIP_CHECK(hipMemcpyAsync(b->h_tot, b->tot, 16, hipMemcpyDeviceToHost, s));
         MT_HIP_CHECK(hipStreamSynchronize(s));
         b->st.extension_rays = (int64_t)b->h_tot[0];

@@ -347,6 +347,24 @@ __device__ __forceinline__ int wave_append(bool flag, int* counter)
     return base + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Shadow-ray queues are segmented (see MtSegQueue): work-group b of a
+// launch appends to segment b % SHADOW_SEGS, whose capacity is the slots of
+// the work-groups that map to it.
+constexpr int SHADOW_SEGS = 256;
+
+inline int shadow_seg_cap(int64_t slots, int group)
+{
+    const int64_t groups = (slots + group - 1) / group;
+    return (int)((groups + SHADOW_SEGS - 1) / SHADOW_SEGS) * group;
+}
+
+// Whether entry `id` of a segmented queue holds an item.
+__device__ __forceinline__ bool seg_live(const int* __restrict__ counts, int seg_cap, int id)
+{
+    const int p = id / seg_cap;
+    return p < SHADOW_SEGS && id - p * seg_cap < counts[p];
+}
+
 // Per-pixel mean over samples in sample order (averageAccum, Image.cl:29-45,
 // without tone mapping): deterministic, no float atomics.
 __global__ __launch_bounds__(256) void k_resolve(const float4* __restrict__ L, int npix, int spp,

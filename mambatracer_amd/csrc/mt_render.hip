@@ -14,15 +14,18 @@
 //   2. one fused shading kernel per slot (shade_path): hit shading,
 //      setupBsdf, LightHitPass, DirectLight's light sample + EvaluateBsdf +
 //      shadow-ray setup, SampleBsdf for the next ray and Russian roulette;
-//      shadow rays are appended to a dense queue (one atomic per wave), so
-//      the any-hit launch has no empty lanes;
-//   3. any hit over the shadow queue (mt_trace_any);
+//      shadow rays are appended to a segmented queue (one atomic per wave on
+//      the counter of the work-group's segment), so the any-hit launch has
+//      empty lanes only at segment tails and no counter is contended;
+//   3. any hit over the shadow queue (mt_trace_any_seg);
 //   4. unoccluded direct light added to the path radiance;
 //   5. stable compaction of live slots (pid and next ray) into the other
 //      queue buffer (mt_compact; CCompact.execute in the reference).
 // After the last bounce the per-path radiance is averaged per pixel in a
 // fixed order, so the image is deterministic: no float atomics.
 #include "mt_path.h"
+
+#include <vector>
 
 namespace {
 
@@ -59,10 +62,11 @@ struct ShadeArgs {
     const int* prim;
     const float* tin;
     int* alive;           // out: 1 if the slot's path continues
-    float4* s_o_tmin;     // out: dense queue of shadow rays
+    float4* s_o_tmin;     // out: segmented queue of shadow rays
     float4* s_d_tmax;
     float4* s_contrib;    // out: unoccluded contribution, w = the ray's slot (int bits)
-    int* shadow_count;    // queue length
+    int* shadow_count;    // SHADOW_SEGS segment lengths
+    int seg_cap;
     int bounce;
     int rr_start;
     int flags;
@@ -70,7 +74,7 @@ struct ShadeArgs {
 };
 
 // One slot of the fused shading stage; its shadow ray, if any, goes to the
-// next free entry of the dense shadow queue.
+// next free entry of its work-group's shadow-queue segment.
 __device__ __forceinline__ void shade_slot(const ShadeArgs& A, int slot, bool live)
 {
     PathOut P;
@@ -90,7 +94,8 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, int slot, bool li
         }
         A.alive[slot] = P.alive ? 1 : 0;
     }
-    const int k = wave_append(P.has_shadow, A.shadow_count);
+    const int seg = blockIdx.x % SHADOW_SEGS;
+    const int k = seg * A.seg_cap + wave_append(P.has_shadow, A.shadow_count + seg);
     if (P.has_shadow) {
         A.s_o_tmin[k] = P.so;
         A.s_d_tmax[k] = P.sd;
@@ -98,29 +103,27 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, int slot, bool li
     }
 }
 
-// Waves stride over the live slots [0, *d_count).
+// One slot per lane over a capacity-sized grid (the segment capacities
+// assume exactly that); work-groups past *d_count leave at once.
 __global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
 {
     const int count = *A.d_count;
-    const int lane = __lane_id();
-    const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64; base < count; base += nw * 64)
-        shade_slot(A, base + lane, base + lane < count);
+    if ((int64_t)blockIdx.x * blockDim.x >= count) return;
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    shade_slot(A, slot, slot < count);
 }
 
 // Unoccluded shadow-queue entries add their contribution to their path.
-__global__ __launch_bounds__(RT) void k_nee_accum(const int* __restrict__ queue, const int* __restrict__ d_count,
-                                                  const float4* __restrict__ contrib,
+__global__ __launch_bounds__(RT) void k_nee_accum(const int* __restrict__ queue, const int* __restrict__ seg_counts,
+                                                  int seg_cap, const float4* __restrict__ contrib,
                                                   const int* __restrict__ occluded, PathState S)
 {
-    const int count = *d_count;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
-        if (occluded[k]) continue;
-        const float4 c = contrib[k];
-        const int pid = queue[__float_as_int(c.w)];
-        float4 L = S.L[pid];
-        S.L[pid] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
-    }
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!seg_live(seg_counts, seg_cap, k) || occluded[k]) return;
+    const float4 c = contrib[k];
+    const int pid = queue[__float_as_int(c.w)];
+    float4 L = S.L[pid];
+    S.L[pid] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
 }
 
 __global__ void k_zero_i32(int* p, int n)
@@ -129,13 +132,15 @@ __global__ void k_zero_i32(int* p, int n)
     if (i < n) p[i] = 0;
 }
 
-// Sums per-slot traversal counters {I, L, P} of the slots < *d_count.
+// Sums per-entry traversal counters {I, L, P} of the live entries: i <
+// *d_count, or (seg_counts set) the live entries of a segmented queue.
 __global__ __launch_bounds__(RT) void k_sum_ilp(const int* __restrict__ st, const int* __restrict__ d_count,
+                                                const int* __restrict__ seg_counts, int seg_cap,
                                                 unsigned long long* __restrict__ tot)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long a = 0, b = 0, c = 0;
-    if (i < *d_count) {
+    if (seg_counts ? seg_live(seg_counts, seg_cap, i) : i < *d_count) {
         a = (unsigned)st[3 * i];
         b = (unsigned)st[3 * i + 1];
         c = (unsigned)st[3 * i + 2];
@@ -181,12 +186,15 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
     size_t oT = take(16 * n), oL = take(16 * n);
     size_t oA_o = take(16 * n), oA_d = take(16 * n), oB_o = take(16 * n), oB_d = take(16 * n);
     size_t oQA = take(4 * n), oQB = take(4 * n);
-    size_t oPrim = take(4 * n), oT_ = take(4 * n), oAlive = take(4 * n), oOcc = take(4 * n);
-    size_t oSo = take(16 * n), oSd = take(16 * n), oSc = take(16 * n);
-    size_t oCnt = take(4 * 64), oShc = take(4 * 64);
+    // shadow rays: a segmented queue filled by k_shade's work-groups
+    const int seg_cap = shadow_seg_cap(np, RT);
+    const size_t sq = (size_t)SHADOW_SEGS * seg_cap;
+    size_t oPrim = take(4 * n), oT_ = take(4 * n), oAlive = take(4 * n), oOcc = take(4 * sq);
+    size_t oSo = take(16 * sq), oSd = take(16 * sq), oSc = take(16 * sq);
+    size_t oCnt = take(4 * 64), oShc = take(4 * 64 * (size_t)SHADOW_SEGS);
     const bool want_ilp = (desc->flags & MT_RENDER_TRAVERSAL_STATS) != 0;
     const bool profile = (desc->flags & MT_RENDER_PROFILE) != 0;
-    size_t oIlp = want_ilp ? take(12 * n) : 0, oTot = take(8 * 8);
+    size_t oIlp = want_ilp ? take(12 * (sq > n ? sq : n)) : 0, oTot = take(8 * 8);
     char* base = nullptr;
     if (hipMalloc(&base, bytes) != hipSuccess) {
         mt_set_error("mt_render: cannot allocate %.2f GB of wavefront state", bytes / 1e9);
@@ -219,7 +227,8 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
     V.env = env_view(env);
 
     int grid = mt_blocks(np, RT);
-    const int gshade = grid, gnee = grid; // one slot per lane; empty work-groups retire at once
+    const int gshade = grid;              // one slot per lane; empty work-groups retire at once
+    const int gshadow = mt_blocks((int64_t)sq, RT);
     unsigned long long* tot = reinterpret_cast<unsigned long long*>(base + oTot);
     int* ilp = want_ilp ? I(oIlp) : nullptr;
     hipEvent_t ev[4 * 64];
@@ -232,7 +241,8 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
                 break;
             }
     }
-    k_zero_i32<<<1, 128, 0, st>>>(counts, 128); // counts[64] and shadow counts[64] are adjacent
+    k_zero_i32<<<1, 64, 0, st>>>(counts, 64);
+    k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(shc, 64 * SHADOW_SEGS);
     k_zero_i32<<<1, 64, 0, st>>>(reinterpret_cast<int*>(tot), 16);
     k_camera<<<grid, RT, 0, st>>>(C, (int)npix, np, desc->seed, ro[0], rd[0], q[0], S, counts);
     if (rc == MT_OK) rc = chk();
@@ -244,7 +254,7 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
                               dcount, np, I(oPrim), reinterpret_cast<float*>(base + oT_), ilp, desc->mode);
         if (rc) break;
         if (profile) hipEventRecord(ev[4 * b + 1], st);
-        if (want_ilp) k_sum_ilp<<<grid, RT, 0, st>>>(ilp, dcount, tot);
+        if (want_ilp) k_sum_ilp<<<grid, RT, 0, st>>>(ilp, dcount, nullptr, 0, tot);
         ShadeArgs A;
         A.V = V;
         A.S = S;
@@ -258,7 +268,9 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
         A.s_o_tmin = F4(oSo);
         A.s_d_tmax = F4(oSd);
         A.s_contrib = F4(oSc);
-        A.shadow_count = shc + b;
+        int* seg_counts = shc + (size_t)b * SHADOW_SEGS;
+        A.shadow_count = seg_counts;
+        A.seg_cap = seg_cap;
         A.bounce = b;
         A.rr_start = desc->rr_start;
         A.flags = desc->flags;
@@ -266,13 +278,13 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
         k_shade<<<gshade, RT, 0, st>>>(A);
         if ((rc = chk())) break;
         if (profile) hipEventRecord(ev[4 * b + 2], st);
-        // any hit over the dense shadow queue (length shc[b], on the device)
-        rc = mt_trace_any(ctx, scene, reinterpret_cast<float*>(F4(oSo)), reinterpret_cast<float*>(F4(oSd)),
-                          shc + b, np, I(oOcc), ilp, desc->mode);
+        // any hit over the segmented shadow queue (segment lengths on the device)
+        rc = mt_trace_any_seg(ctx, scene, F4(oSo), F4(oSd), MtSegQueue{seg_counts, SHADOW_SEGS, seg_cap}, I(oOcc),
+                              ilp, desc->mode);
         if (rc) break;
         if (profile) hipEventRecord(ev[4 * b + 3], st);
-        if (want_ilp) k_sum_ilp<<<grid, RT, 0, st>>>(ilp, shc + b, tot + 3);
-        k_nee_accum<<<gnee, RT, 0, st>>>(q[cur], shc + b, F4(oSc), I(oOcc), S);
+        if (want_ilp) k_sum_ilp<<<gshadow, RT, 0, st>>>(ilp, nullptr, seg_counts, seg_cap, tot + 3);
+        k_nee_accum<<<gshadow, RT, 0, st>>>(q[cur], seg_counts, seg_cap, F4(oSc), I(oOcc), S);
         if ((rc = chk())) break;
         if (b + 1 < nb) {
             mt_payload P[3] = {{q[cur], q[1 - cur], 4}, {ro[cur], ro[1 - cur], 16}, {rd[cur], rd[1 - cur], 16}};
@@ -286,9 +298,12 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
         rc = chk();
     }
     if (rc == MT_OK && stats) {
-        int hc[128];
+        int hc[64];
+        std::vector<int> hs((size_t)nb * SHADOW_SEGS);
         unsigned long long ht[6];
         hipError_t e = hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(hs.data(), shc, hs.size() * sizeof(int), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipMemcpyAsync(ht, tot, sizeof(ht), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) {
@@ -299,10 +314,8 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
             stats->shadow_rays = 0;
             stats->paths = np;
             for (int b = 0; b < 64; b++) stats->bounce_counts[b] = b < nb ? hc[b] : 0;
-            for (int b = 0; b < nb; b++) {
-                stats->extension_rays += hc[b];
-                stats->shadow_rays += hc[64 + b];
-            }
+            for (int b = 0; b < nb; b++) stats->extension_rays += hc[b];
+            for (int v : hs) stats->shadow_rays += v;
             for (int k = 0; k < 3; k++) {
                 stats->closest_ilp[k] = (int64_t)ht[k];
                 stats->any_ilp[k] = (int64_t)ht[3 + k];

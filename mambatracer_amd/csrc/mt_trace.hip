@@ -419,7 +419,21 @@ struct TraceArgs {
     int* occluded;
     int* stats;
     unsigned* ctr; // refill kernel: [0] next ray, [1] waves finished; zero between launches
+    MtSegQueue seg; // segmented queue (seg.counts set) instead of *d_count
 };
+
+// Whether queue entry `id` holds a ray (plain queue: id < *d_count clamped
+// to the capacity; segmented: below its segment's count).
+__device__ __forceinline__ bool entry_live(const TraceArgs& A, int id)
+{
+    if (A.seg.counts) {
+        const int p = id / A.seg.seg_cap;
+        return p < A.seg.nseg && id - p * A.seg.seg_cap < A.seg.counts[p];
+    }
+    int count = *A.d_count;
+    if (count > A.capacity) count = A.capacity;
+    return id < count;
+}
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
 { // set bits of `mask` below this lane (v_mbcnt_lo/hi)
@@ -527,9 +541,7 @@ template <bool ANY, bool STATS, int LAYOUT>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
 {
     int id = blockIdx.x * blockDim.x + threadIdx.x;
-    int count = *A.d_count;
-    if (count > A.capacity) count = A.capacity;
-    if (id >= count) return;
+    if (!entry_live(A, id)) return;
     RayT r;
     ray_init(r, A.o_tmin[id], A.d_tmax[id]);
     int st[3] = {0, 0, 0};
@@ -558,9 +570,7 @@ __global__ __launch_bounds__(256) void k_trace_jobs(const MtSceneRef* __restrict
                                                     int shift, TraceArgs A)
 {
     int id = blockIdx.x * blockDim.x + threadIdx.x;
-    int count = *A.d_count;
-    if (count > A.capacity) count = A.capacity;
-    if (id >= count) return;
+    if (!entry_live(A, id)) return;
     const MtSceneRef R = refs[tag[2 * id] >> shift];
     RayT r;
     ray_init(r, A.o_tmin[id], A.d_tmax[id]);
@@ -578,15 +588,21 @@ __global__ __launch_bounds__(256) void k_trace_jobs(const MtSceneRef* __restrict
 template <bool ANY>
 int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
                      const float* d_tmax, const int32_t* d_count, int32_t capacity,
-                     int32_t* prim, float* t, int32_t* occluded, int32_t* stats, int32_t mode)
+                     int32_t* prim, float* t, int32_t* occluded, int32_t* stats, int32_t mode,
+                     MtSegQueue seg = MtSegQueue{nullptr, 0, 0})
 {
-    MT_ARG_CHECK(ctx && sc && o_tmin && d_tmax && d_count, "trace: NULL argument");
+    MT_ARG_CHECK(ctx && sc && o_tmin && d_tmax && (d_count || seg.counts), "trace: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "trace: negative capacity");
     MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_REFILL, "trace: unknown mode %d", mode);
     if (ANY) MT_ARG_CHECK(occluded != nullptr, "trace_any: occluded is NULL");
     else MT_ARG_CHECK(prim && t, "trace_closest: prim/t NULL");
+    if (seg.counts) {
+        capacity = seg.nseg * seg.seg_cap;
+        if (mode == MT_TRACE_PACKED_REFILL) mode = MT_TRACE_PACKED_STACK; // the refill kernel reads *d_count
+    }
     if (capacity == 0) return MT_OK;
-    TraceArgs A;
+    TraceArgs A{};
+    A.seg = seg;
     A.nodes = sc->desc.nodes;
     A.bounds = sc->desc.bounds;
     A.faces = sc->desc.faces;
@@ -950,10 +966,20 @@ extern "C" int32_t mt_trace_any(mt_context* ctx, const mt_scene* scene, const fl
                               occluded, stats, mode);
 }
 
+int32_t mt_trace_any_seg(mt_context* ctx, const mt_scene* scene, const float4* o_tmin, const float4* d_tmax,
+                         MtSegQueue seg, int32_t* occluded, int32_t* stats, int32_t mode)
+{
+    MT_ARG_CHECK(seg.counts && seg.nseg > 0 && seg.seg_cap > 0, "trace_any_seg: bad segment table");
+    return launch_trace<true>(ctx, scene, reinterpret_cast<const float*>(o_tmin),
+                              reinterpret_cast<const float*>(d_tmax), nullptr, 0, nullptr, nullptr, occluded, stats,
+                              mode, seg);
+}
+
 int32_t mt_trace_jobs(mt_context* ctx, bool any, const MtSceneRef* refs, const int32_t* tag, int32_t shift,
                       const float4* o_tmin, const float4* d_tmax, const int32_t* d_count, int32_t capacity,
-                      int32_t* prim, float* t, int32_t* occluded)
+                      int32_t* prim, float* t, int32_t* occluded, MtSegQueue seg)
 {
+    if (seg.counts) capacity = seg.nseg * seg.seg_cap;
     if (capacity == 0) return MT_OK;
     TraceArgs A{};
     A.o_tmin = o_tmin;
@@ -963,6 +989,7 @@ int32_t mt_trace_jobs(mt_context* ctx, bool any, const MtSceneRef* refs, const i
     A.prim = prim;
     A.tout = t;
     A.occluded = occluded;
+    A.seg = seg;
     dim3 grid(mt_blocks(capacity, 256)), block(256);
     if (any) k_trace_jobs<true><<<grid, block, 0, ctx->stream>>>(refs, tag, shift, A);
     else k_trace_jobs<false><<<grid, block, 0, ctx->stream>>>(refs, tag, shift, A);
