@@ -412,6 +412,29 @@ int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float* materials
                   const int32_t* d_num_lights, const mt_render_desc* desc,
                   float* image, mt_render_stats* stats);
 
+/* Captured render.  The reference's render thread re-issues the same
+ * launch sequence for every frame (CDeviceGI.loop, CDeviceGI.java:214-271,
+ * once per frame from execute(), :424-431).  mt_render_graph_create lays out
+ * one job's wavefront state once (library-owned, freed by destroy) and
+ * records mt_render's whole launch sequence into a HIP graph;
+ * mt_render_graph_launch(g, seed) replays it on the context stream as one
+ * launch, with `seed` in place of desc->seed, and produces exactly the
+ * image mt_render produces for that seed.  The scene, materials,
+ * environment, light list and image must outlive the graph and keep their
+ * addresses.  desc->flags may not hold MT_RENDER_PROFILE.
+ * mt_render_graph_stats fills the ray counts (and, with
+ * MT_RENDER_TRAVERSAL_STATS, the traversal totals) of the last replay;
+ * it synchronises the context.  Replays of one graph are serialised on the
+ * context stream; one graph must not be replayed on two streams at once. */
+typedef struct mt_render_graph mt_render_graph;
+int32_t mt_render_graph_create(mt_context* ctx, const mt_scene* scene, const float* materials,
+                               const mt_envmap* env, const int32_t* light_faces,
+                               const int32_t* d_num_lights, const mt_render_desc* desc,
+                               float* image, mt_render_graph** out);
+int32_t mt_render_graph_launch(mt_render_graph* g, uint32_t seed);
+int32_t mt_render_graph_stats(mt_render_graph* g, mt_render_stats* stats);
+int32_t mt_render_graph_destroy(mt_render_graph* g);
+
 /* ------------------------------------------------------------------ */
 /* Image pipeline                                                     */
 /* ------------------------------------------------------------------ */

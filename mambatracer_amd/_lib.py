@@ -181,6 +181,10 @@ SIGNATURES = {
     "mt_bsdf_sample": [P, I32, P, P, P, P, P, P, P, P],
     "mt_bsdf_eval": [P, I32, P, P, P, P, P, P, P],
     "mt_render": [P, P, P, P, P, P, C.POINTER(RenderDesc), P, C.POINTER(RenderStats)],
+    "mt_render_graph_create": [P, P, P, P, P, P, C.POINTER(RenderDesc), P, PP],
+    "mt_render_graph_launch": [P, C.c_uint32],
+    "mt_render_graph_stats": [P, C.POINTER(RenderStats)],
+    "mt_render_graph_destroy": [P],
     "mt_image_init": [P, P, P, I32],
     "mt_image_process": [P, P, F32, I32, F32, F32, I32, P, P, P, P],
     "mt_scan_excl_f32": [P, P, P, P, I32, P],

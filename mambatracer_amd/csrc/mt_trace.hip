@@ -629,7 +629,8 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
         void* scr = nullptr;
         int32_t rc = mt_ctx_scratch(ctx, 256, &scr);
         if (rc) return rc;
-        A.ctr = reinterpret_cast<unsigned*>(scr) + 2; // words 0-1: scan look-back ticket and flag
+        // words 0-2: scan look-back ticket, error flag and epoch
+        A.ctr = reinterpret_cast<unsigned*>(scr) + 4;
         auto kern = stats ? k_trace_refill<ANY, true> : k_trace_refill<ANY, false>;
         int blocks = mt_persistent_blocks(ctx, kern, 256, capacity);
         kern<<<blocks, block, 0, s>>>(A);

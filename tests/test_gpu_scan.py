@@ -10,7 +10,9 @@ from oracle import orc_py as orc
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [0, 1, 2, 63, 64, 65, 200, 2047, 2048, 2049, 4096 * 3 + 17, 1 << 20, 3_000_001]
+# tiles are 8192 elements: sizes straddle one, two and many tiles
+SIZES = [0, 1, 2, 63, 64, 65, 200, 2047, 2048, 2049, 4096 * 3 + 17, 8191, 8192, 8193, 8192 * 3 + 17, 1 << 20,
+         3_000_001]
 
 
 @pytest.mark.parametrize("n", SIZES)
