@@ -5,13 +5,18 @@ Workload (one "step" = one render job, inputs resident in HBM): 256 UV
 spheres x 4096 triangles + ground quad (1,048,578 triangles), GGX materials,
 1024x512 HDR environment sampled through its summed-area table, 1920x1080 at
 16 spp, up to 8 bounces with Russian roulette after bounce 3 and stable
-compaction after every bounce.  value = (closest-hit + shadow rays) / s.
+compaction after every bounce.  value = (closest-hit + shadow rays) / s over
+the timed steps, each one replay of the job's captured render
+(mt_render_graph, as a render thread re-runs a frame) plus its ray-count
+readback.  The same job through uncaptured mt_render is timed after it and
+reported beside it; both produce the same image bit for bit (checked).
 
 roofline: the dominant kernel is the closest-hit traversal.  Its algorithmic
 bytes are the reference layout's per-ray figure 40 + 148*I + 116*L + 28*P
 (I interior steps, L leaf visits, P backtrack pops, counted by the device's
 counting traversal in an untimed render of the same job) divided by the
-traversal launches' summed duration (HIP events on the launch stream).
+traversal launches' summed duration (HIP events on the launch stream around
+every closest-hit launch of the uncaptured renders).
 
 cpu_baseline: the oracle (plain-C restatement, test infrastructure) traces a
 bounded random subset of the same job's paths on the host cores.
@@ -55,11 +60,14 @@ def cpu_baseline(W, scene_arrays, mats, rgba, job, npaths, seconds, threads):
     lum, sat = orc.env_lum_sat(rgba.reshape(-1, 4), w, h)
     env = (rgba.reshape(-1, 4), lum, sat, w, h)
     rng = np.random.default_rng(12345)
-    probe = np.sort(rng.choice(npaths, 20000, replace=False)).astype(np.int32)
+    # untimed warm-up (threads, page faults), then a timed probe sizes the sample
+    warm = np.sort(rng.choice(npaths, 20000, replace=False)).astype(np.int32)
+    orc.render_paths(points, normals, faces, nodes, bounds, mats, env, job, warm, nthreads=threads)
+    probe = np.sort(rng.choice(npaths, 400000, replace=False)).astype(np.int32)
     t0 = time.perf_counter()
-    _, rays = orc.render_paths(points, normals, faces, nodes, bounds, mats, env, job, probe, nthreads=threads)
+    orc.render_paths(points, normals, faces, nodes, bounds, mats, env, job, probe, nthreads=threads)
     dt = time.perf_counter() - t0
-    n = int(min(npaths, max(20000, 20000 * seconds / max(dt, 1e-3))))
+    n = int(min(npaths, max(400000, 400000 * seconds / max(dt, 1e-3))))
     sample = np.sort(rng.choice(npaths, n, replace=False)).astype(np.int32)
     t0 = time.perf_counter()
     _, rays = orc.render_paths(points, normals, faces, nodes, bounds, mats, env, job, sample, nthreads=threads)
@@ -77,8 +85,8 @@ def cpu_baseline(W, scene_arrays, mats, rgba, job, npaths, seconds, threads):
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--spp", type=int, default=16)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
@@ -144,32 +152,57 @@ def main() -> int:
     log(f"traversal counters: rays {ext_rays}, I {ilp[0]}, L {ilp[1]}, P {ilp[2]} "
         f"({alg_bytes / max(ext_rays, 1):.0f} B/ray)")
 
-    job.flags = _lib.MT_RENDER_PROFILE
+    # The job's launch sequence captured once (mt_render_graph: state laid
+    # out once, one graph launch per frame); each step replays it with the
+    # job's seed and reads its ray counts.
+    job.flags = 0
+    graph = api.RenderGraph(ctx, scene, dmats, env, job)
     for i in range(args.warmup):
-        api.render(ctx, scene, dmats, env, job)
-        torch.cuda.synchronize()
+        graph.launch(job.seed)
+        graph.stats()
         log(f"warmup {i + 1}/{args.warmup} done")
 
-    lat, rays, closest_ms, launches = [], 0, 0.0, 0
+    lat, rays = [], 0
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for i in range(args.steps):
         s0 = time.perf_counter()
-        img, st = api.render(ctx, scene, dmats, env, job)   # stats readback synchronises
+        graph.launch(job.seed)
+        st = graph.stats()   # synchronises
         lat.append((time.perf_counter() - s0) * 1e3)
         rays += int(st.extension_rays) + int(st.shadow_rays)
-        closest_ms += float(st.closest_ms)
-        launches += int(st.closest_launches)
         log(f"step {i + 1}/{args.steps}: {lat[-1]:.1f} ms, {int(st.extension_rays)} + {int(st.shadow_rays)} rays")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
+    img = graph.image
+
+    # The same job through mt_render (uncaptured), with every traversal
+    # launch bracketed by HIP events on the context stream: the closest-hit
+    # kernel's average launch duration for the roofline, and the uncaptured
+    # throughput for comparison.
+    job.flags = _lib.MT_RENDER_PROFILE
+    api.render(ctx, scene, dmats, env, job)
+    e_rays, closest_ms, launches = 0, 0.0, 0
+    torch.cuda.synchronize()
+    e_start = time.perf_counter()
+    for i in range(args.steps):
+        eimg, est = api.render(ctx, scene, dmats, env, job)
+        e_rays += int(est.extension_rays) + int(est.shadow_rays)
+        closest_ms += float(est.closest_ms)
+        launches += int(est.closest_launches)
+    torch.cuda.synchronize()
+    e_elapsed = time.perf_counter() - e_start
+    if not torch.equal(eimg, img):
+        log("captured and uncaptured renders differ")
+        return 1
+    log(f"uncaptured: {e_elapsed * 1e3 / args.steps:.2f} ms per job")
 
     img_np = img.cpu().numpy()
     if not np.isfinite(img_np).all():
         log("non-finite pixels in the image")
         return 1
     value = rays / elapsed / 1e6
-    per_launch_bytes = alg_bytes / max(st.closest_launches, 1)
+    per_launch_bytes = alg_bytes / max(est.closest_launches, 1)
     avg_launch_s = closest_ms / max(launches, 1) / 1e3
     achieved = per_launch_bytes / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     traffic = traffic_from_profiles()
@@ -206,6 +239,11 @@ def main() -> int:
         "job_latency_ms": {"p50": round(lat_sorted[len(lat) // 2], 3),
                            "p99": round(lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
                            "qps": "sequential (one job in flight)"},
+        "timed_path": "captured render (mt_render_graph) replayed once per step",
+        "uncaptured": {"Mrays_per_s": round(e_rays / e_elapsed / 1e6, 3),
+                       "ms_per_job": round(e_elapsed * 1e3 / args.steps, 3),
+                       "note": "mt_render per step (allocates its state per call), HIP events on "
+                               "every traversal launch; source of the roofline launch duration"},
         "roofline": {
             "kernel": "closest-hit traversal (k_trace, packed layout)" if job.mode == 0 else "closest-hit traversal (reference layout)",
             "bound": "hbm",
