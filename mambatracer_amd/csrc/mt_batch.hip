@@ -53,7 +53,7 @@ struct JobRec {
     SceneView V;
     Camera C;
     float4* L;         // per-path radiance, pid-indexed
-    int npix, npaths, max_bounces, rr_start, flags;
+    int npaths, max_bounces, rr_start, flags;
     uint32_t seed;
 };
 
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(BT) void k_generate(const JobRec* __restrict__ jobs
     const int pid = gen_pid0[lo] + (off - gen_start[lo]);
     const JobRec& J = jobs[j];
     float4 o, d;
-    camera_ray(J.C, pid, J.npix, J.seed, o, d);
+    camera_ray(J.C, pid, J.seed, o, d);
     Q.ro[slot] = o;
     Q.rd[slot] = d;
     Q.T[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f); // InitPathData, Initialize.cl:25-48
@@ -380,7 +380,6 @@ int32_t admit(mt_batch* b)
         rec.V.num_lights = d.d_num_lights;
         rec.V.env = env_view(d.env);
         rec.L = reinterpret_cast<float4*>(b->arena + off);
-        rec.npix = d.render.width * d.render.height;
         rec.npaths = J.npaths;
         rec.max_bounces = d.render.max_bounces;
         rec.rr_start = d.render.rr_start;
@@ -630,8 +629,9 @@ extern "C" int32_t mt_batch_step(mt_batch* b, int32_t max_done, int64_t* done_ta
             }
             const mt_render_desc& r = J.d.render;
             const int npix = r.width * r.height;
-            k_resolve<<<mt_blocks(npix, BT), BT, 0, s>>>(reinterpret_cast<const float4*>(b->arena + J.off), npix,
-                                                         r.spp, reinterpret_cast<float4*>(J.d.image));
+            k_resolve<<<mt_blocks(npix, BT), BT, 0, s>>>(reinterpret_cast<const float4*>(b->arena + J.off),
+                                                         r.width, r.height, r.spp,
+                                                         reinterpret_cast<float4*>(J.d.image));
             MT_LAUNCH_CHECK();
             // stream order keeps the freed slot and arena range untouched
             // until this resolve has read them

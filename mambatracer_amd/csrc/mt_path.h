@@ -32,7 +32,7 @@ struct Camera {
     float ox, oy, oz;
     float fx, fy, fz, rx, ry, rz, ux, uy, uz;
     float tan_half, aspect;
-    int w, h;
+    int w, h, spp;
 };
 
 // Pinhole basis from a render description (host, double precision).
@@ -57,6 +57,7 @@ inline int32_t camera_from_desc(const mt_render_desc* desc, Camera& C)
     C.aspect = (float)desc->width / (float)desc->height;
     C.w = desc->width;
     C.h = desc->height;
+    C.spp = desc->spp;
     return MT_OK;
 }
 
@@ -80,13 +81,46 @@ __device__ __forceinline__ void seed_for(uint32_t seed, int pid, int bounce, int
     y = (int32_t)((uint32_t)pid + (uint32_t)key * 998244353u);
 }
 
-// Primary ray of path `pid` (pid = sample * pixels + pixel), jittered inside
-// its pixel; tMin 0.001 and tMax inf as initGlobalRay (Util.cl:577-587).
-__device__ __forceinline__ void camera_ray(const Camera& C, int pid, int npix, uint32_t seed, float4& o_tmin,
+// Path-id convention: pid = k * spp + sample, where k numbers the pixels
+// in bands of 8 rows, column by column inside a band (the last band holds
+// h % 8 rows).  Consecutive ids are the samples of one pixel, then the
+// pixels of 8x8 tiles, so the camera pass (slot = pid) traces neighbouring
+// rays in every wave and the resolve reads each pixel's samples
+// contiguously.  A path's result depends only on its id, never on its slot.
+__host__ __device__ inline void path_pixel(int pid, int w, int h, int spp, int& x, int& y)
+{
+    const int k = pid / spp;
+    const int full = (h >> 3) * 8 * w; // pixels in full bands
+    int band, rows, r;
+    if (k < full) {
+        band = k / (8 * w);
+        rows = 8;
+        r = k - band * 8 * w;
+    } else {
+        band = h >> 3;
+        rows = h - 8 * band;
+        r = k - full;
+    }
+    x = r / rows;
+    y = band * 8 + (r - x * rows);
+}
+
+// First path id of pixel (x, y) (its samples follow it).
+__host__ __device__ inline int pixel_first_path(int x, int y, int w, int h, int spp)
+{
+    const int band = y >> 3, full_bands = h >> 3;
+    const int k = band < full_bands ? band * 8 * w + x * 8 + (y & 7)
+                                    : full_bands * 8 * w + x * (h - 8 * full_bands) + (y - 8 * band);
+    return k * spp;
+}
+
+// Primary ray of path `pid` (pixel from path_pixel), jittered inside its
+// pixel; tMin 0.001 and tMax inf as initGlobalRay (Util.cl:577-587).
+__device__ __forceinline__ void camera_ray(const Camera& C, int pid, uint32_t seed, float4& o_tmin,
                                            float4& d_tmax)
 {
-    int pixel = pid % npix;
-    int px = pixel % C.w, py = pixel / C.w;
+    int px, py;
+    path_pixel(pid, C.w, C.h, C.spp, px, py);
     int32_t sx, sy;
     seed_for(seed, pid, 0, 0, sx, sy);
     float j0 = mt_get_random(sx, sy);
@@ -367,14 +401,15 @@ __device__ __forceinline__ bool seg_live(const int* __restrict__ counts, int seg
 
 // Per-pixel mean over samples in sample order (averageAccum, Image.cl:29-45,
 // without tone mapping): deterministic, no float atomics.
-__global__ __launch_bounds__(256) void k_resolve(const float4* __restrict__ L, int npix, int spp,
+__global__ __launch_bounds__(256) void k_resolve(const float4* __restrict__ L, int w, int h, int spp,
                                                  float4* __restrict__ image)
 {
     int px = blockIdx.x * blockDim.x + threadIdx.x;
-    if (px >= npix) return;
+    if (px >= w * h) return;
+    const float4* Lp = L + pixel_first_path(px % w, px / w, w, h, spp);
     float r = 0.0f, g = 0.0f, b = 0.0f;
     for (int s = 0; s < spp; s++) {
-        float4 v = L[(size_t)s * npix + px];
+        float4 v = Lp[s];
         r += v.x;
         g += v.y;
         b += v.z;

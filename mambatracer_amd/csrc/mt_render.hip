@@ -2,8 +2,9 @@
 // (src/cl/device/CDeviceGI.java:214-271); the per-path work of a bounce is
 // shade_path (mt_path.h).
 //
-// Layout.  Path state is SoA indexed by path id (pid = sample * pixels +
-// pixel): throughput.xyz + last BSDF pdf in one float4, accumulated
+// Layout.  Path state is SoA indexed by path id (pid = pixel rank * spp +
+// sample, pixels ranked in 8x8 tiles: path_pixel in mt_path.h; the camera
+// pass puts path pid in queue slot pid): throughput.xyz + last BSDF pdf in one float4, accumulated
 // radiance in another.  The live paths of a bounce form a queue of pids with
 // their rays (two float4 streams) and a device-resident count; every stage
 // reads the count on the device (the reference's `global int* count`), so
@@ -43,16 +44,16 @@ __global__ void k_set_seed(uint32_t* d_seed, uint32_t seed)
     if (threadIdx.x == 0) *d_seed = seed;
 }
 
-__global__ __launch_bounds__(RT) void k_camera(Camera C, int npix, int npaths, const uint32_t* __restrict__ d_seed,
+__global__ __launch_bounds__(RT) void k_camera(Camera C, int npaths, const uint32_t* __restrict__ d_seed,
                                                float4* __restrict__ o_tmin, float4* __restrict__ d_tmax,
                                                int* __restrict__ queue, PathState S, int* __restrict__ count0)
 {
-    int pid = blockIdx.x * blockDim.x + threadIdx.x;
+    int pid = blockIdx.x * blockDim.x + threadIdx.x; // slot = path id (see path_pixel)
     if (pid == 0) *count0 = npaths;
     if (pid >= npaths) return;
     const uint32_t seed = *d_seed;
     float4 o, d;
-    camera_ray(C, pid, npix, seed, o, d);
+    camera_ray(C, pid, seed, o, d);
     o_tmin[pid] = o;
     d_tmax[pid] = d;
     queue[pid] = pid;
@@ -312,7 +313,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
     k_zero_i32<<<1, 64, 0, st>>>(counts, 64);
     k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(shc, 64 * SHADOW_SEGS);
     k_zero_i32<<<1, 64, 0, st>>>(reinterpret_cast<int*>(tot), 16);
-    k_camera<<<grid, RT, 0, st>>>(R.C, (int)R.npix, np, seed_ptr(R), ro[0], rd[0], q[0], S, counts);
+    k_camera<<<grid, RT, 0, st>>>(R.C, np, seed_ptr(R), ro[0], rd[0], q[0], S, counts);
     int32_t rc = chk();
     int cur = 0;
     for (int b = 0; b < nb && rc == MT_OK; b++) {
@@ -362,7 +363,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         }
     }
     if (rc == MT_OK) {
-        k_resolve<<<mt_blocks(R.npix, RT), RT, 0, st>>>(S.L, (int)R.npix, desc->spp,
+        k_resolve<<<mt_blocks(R.npix, RT), RT, 0, st>>>(S.L, desc->width, desc->height, desc->spp,
                                                          reinterpret_cast<float4*>(R.image));
         rc = chk();
     }

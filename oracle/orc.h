@@ -120,6 +120,9 @@ void orc_ggx_sample_vndf(const float* v, float ax, float ay, float u1, float u2,
 float orc_ggx_pdf_vndf(const float* v, const float* h, float ax, float ay);
 
 /* ---- bounce loop (CDeviceGI.loop + Render.cl kernels), one path at a time ---- */
+/* pixel of path `pid`: pid = rank * spp + sample, pixels ranked in bands of 8
+ * rows, column by column inside a band (the last band holds h % 8 rows) */
+void orc_path_pixel(int32_t pid, int32_t w, int32_t h, int32_t spp, int32_t* x, int32_t* y);
 int orc_camera_basis(const float* pos, const float* look, const float* up, float vfov_deg,
                      int32_t w, int32_t h, float* cam);
 int orc_render_paths(const float* points, const float* normals, const int32_t* faces,

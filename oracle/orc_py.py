@@ -410,9 +410,34 @@ def render_paths(points, normals, faces, nodes, bounds, materials, env, job, pid
     return out, rays
 
 
-def resolve(L_paths, npix, spp):
+def path_pixel(pids, w, h, spp):
+    """(x, y) of each path id under the bounce loop's convention (orc_path_pixel)."""
+    L = lib()
+    L.orc_path_pixel.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32)]
+    L.orc_path_pixel.restype = None
+    x, y = C.c_int32(), C.c_int32()
+    out = np.empty((len(pids), 2), np.int32)
+    for i, p in enumerate(pids):
+        L.orc_path_pixel(int(p), w, h, spp, C.byref(x), C.byref(y))
+        out[i] = (x.value, y.value)
+    return out
+
+
+def pixel_first_paths(w, h, spp):
+    """First path id of every pixel (row-major), the inverse of path_pixel:
+    pixel rank k in bands of 8 rows walked column by column, times spp."""
+    y, x = np.divmod(np.arange(w * h, dtype=np.int64), w)
+    fb = h // 8
+    band = y // 8
+    k = np.where(band < fb, band * 8 * w + x * 8 + (y % 8), fb * 8 * w + x * (h - 8 * fb) + (y - 8 * fb))
+    return k * spp
+
+
+def resolve(L_paths, w, h, spp):
     """Per-pixel mean over samples in sample order, float32 (k_resolve's order)."""
-    acc = np.zeros((npix, 3), np.float32)
+    first = pixel_first_paths(w, h, spp)
+    acc = np.zeros((w * h, 3), np.float32)
     for s in range(spp):
-        acc = (acc + L_paths[s * npix:(s + 1) * npix, :3]).astype(np.float32)
+        acc = (acc + L_paths[first + s, :3]).astype(np.float32)
     return (acc * np.float32(1.0 / spp)).astype(np.float32)

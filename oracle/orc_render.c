@@ -101,11 +101,30 @@ static void area_light(const orc_job* J, int tri, oarea* A)
     A->inten[0] = m[28] * pw; A->inten[1] = m[29] * pw; A->inten[2] = m[30] * pw;
 }
 
+/* Path-id convention of the bounce loop: pid = k * spp + sample, k the
+ * pixel's rank when the image is walked in bands of 8 rows, column by
+ * column inside a band (the last band holds h % 8 rows). */
+void orc_path_pixel(int32_t pid, int32_t w, int32_t h, int32_t spp, int32_t* x, int32_t* y)
+{
+    int32_t k = pid / spp;
+    int32_t full = (h / 8) * 8 * w, band, rows, r;
+    if (k < full) {
+        band = k / (8 * w);
+        rows = 8;
+        r = k - band * 8 * w;
+    } else {
+        band = h / 8;
+        rows = h - 8 * band;
+        r = k - full;
+    }
+    *x = r / rows;
+    *y = band * 8 + r % rows;
+}
+
 static void trace_path(orc_job* J, int32_t pid)
 {
-    int npix = J->w * J->h;
-    int pixel = pid % npix;
-    int px = pixel % J->w, py = pixel / J->w;
+    int32_t px, py;
+    orc_path_pixel(pid, J->w, J->h, J->spp, &px, &py);
     const float* C = J->cam;
     int32_t st[2];
     seed_for(J->seed, pid, 0, 0, st);

@@ -38,7 +38,7 @@ def _render_both(ctx, c1_small, job, with_env=True):
     npix = job.width * job.height
     pids = np.arange(npix * job.spp, dtype=np.int32)
     L, rays = orc.render_paths(points, normals, faces, nodes, bounds, mats, oenv, job, pids)
-    oimg = orc.resolve(L, npix, job.spp)
+    oimg = orc.resolve(L, job.width, job.height, job.spp)
     return img.cpu().numpy()[:, :3], st, oimg, rays
 
 
@@ -83,7 +83,7 @@ def test_render_mesh_lights_matches_oracle(ctx, c1_small, flags):
     npix = job.width * job.height
     L, rays = orc.render_paths(points, normals, faces, nodes, bounds, mats, (rgba.reshape(-1, 4), lum, sat, 1024, 512),
                                job, np.arange(npix * job.spp, dtype=np.int32), light_faces=hlf)
-    oimg = orc.resolve(L, npix, job.spp)
+    oimg = orc.resolve(L, job.width, job.height, job.spp)
     img = img.cpu().numpy()[:, :3]
     ext, sh = int(st.extension_rays), int(st.shadow_rays)
     assert abs(ext - rays[0]) <= 0.005 * rays[0] and abs(sh - rays[1]) <= 0.005 * rays[1] + 2

@@ -77,7 +77,7 @@ def test_graph_matches_oracle(ctx, scene_small):
     pids = np.arange(npix * job.spp, dtype=np.int32)
     L, rays = orc.render_paths(points, normals, faces, nodes, bounds, mats, (rgba.reshape(-1, 4), lum, sat, 1024, 512),
                                job, pids)
-    oimg = orc.resolve(L, npix, job.spp)
+    oimg = orc.resolve(L, job.width, job.height, job.spp)
     ext, sh = int(st.extension_rays), int(st.shadow_rays)
     assert abs(ext - rays[0]) <= 0.005 * rays[0] and abs(sh - rays[1]) <= 0.005 * rays[1] + 2
     rmse = np.sqrt(np.mean((img - oimg) ** 2)) / np.sqrt(np.mean(oimg ** 2))
