@@ -25,6 +25,7 @@
 // After the last bounce the per-path radiance is averaged per pixel in a
 // fixed order, so the image is deterministic: no float atomics.
 #include "mt_path.h"
+#include "mt_traverse.h"
 
 #include <vector>
 
@@ -61,9 +62,56 @@ __global__ __launch_bounds__(RT) void k_camera(Camera C, int npaths, const uint3
     S.L[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// Bounce 0 of the fused loop: path pid = slot traces its camera ray
+// (generated here, never stored) to its closest hit; same per-ray traversal
+// as mt_trace_closest's default mode, so prim/t are the same bits.
+__global__ __launch_bounds__(RT) void k_trace_camera(Camera C, int npaths, const uint32_t* __restrict__ d_seed,
+                                                     const float4* __restrict__ pn, const float4* __restrict__ pt,
+                                                     int root, int* __restrict__ prim, float* __restrict__ tout,
+                                                     int* __restrict__ count0)
+{
+    const int pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid == 0) *count0 = npaths;
+    if (pid >= npaths) return;
+    float4 o, d;
+    camera_ray(C, pid, *d_seed, o, d);
+    RayT r;
+    ray_init(r, o, d);
+    int st[3] = {0, 0, 0};
+    int child = -1;
+    const int h = trace_packed<false, false, true>(pn, pt, root, r, child, st);
+    prim[pid] = h ? child : -1;
+    tout[pid] = r.tmax;
+}
+
+// Any hit over the segmented shadow queue with the direct-light add fused
+// in: an unoccluded entry adds its contribution to its path's radiance (what
+// k_nee_accum does after mt_trace_any_seg).  queue == nullptr: slot = pid.
+__global__ __launch_bounds__(RT) void k_trace_shadow(const float4* __restrict__ pn, const float4* __restrict__ pt,
+                                                     int root, const int* __restrict__ seg_counts, int seg_cap,
+                                                     const float4* __restrict__ s_o, const float4* __restrict__ s_d,
+                                                     const float4* __restrict__ contrib, const int* __restrict__ queue,
+                                                     float4* __restrict__ L)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!seg_live(seg_counts, seg_cap, k)) return;
+    RayT r;
+    ray_init(r, s_o[k], s_d[k]);
+    int st[3] = {0, 0, 0};
+    int child = -1;
+    if (trace_packed<true, false, true>(pn, pt, root, r, child, st)) return;
+    const float4 c = contrib[k];
+    const int slot = __float_as_int(c.w);
+    const int pid = queue ? queue[slot] : slot;
+    const float4 v = L[pid];
+    L[pid] = make_float4(v.x + c.x, v.y + c.y, v.z + c.z, v.w);
+}
+
 struct ShadeArgs {
     SceneView V;
     PathState S;
+    Camera C;             // first: the camera (bounce 0 of the fused loop)
+    int first;            // 1: slot = pid, ray = camera ray, throughput 1, radiance not yet stored
     const int* queue;
     const int* d_count;
     float4* o_tmin;       // in: this bounce's rays; out: next rays (same slot)
@@ -89,15 +137,27 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, uint32_t seed, in
     PathOut P;
     P.has_shadow = false;
     if (live) {
-        const int pid = A.queue[slot];
-        shade_path(A.V, seed, A.rr_start, A.flags, pid, A.bounce, A.o_tmin[slot], A.d_tmax[slot], A.S.T[pid],
-                   A.prim[slot], A.tin[slot], P);
+        int pid;
+        float4 ro, rd, T4;
+        if (A.first) { // what k_camera would have stored: InitPathData, Initialize.cl:25-48
+            pid = slot;
+            camera_ray(A.C, pid, seed, ro, rd);
+            T4 = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        } else {
+            pid = A.queue[slot];
+            ro = A.o_tmin[slot];
+            rd = A.d_tmax[slot];
+            T4 = A.S.T[pid];
+        }
+        shade_path(A.V, seed, A.rr_start, A.flags, pid, A.bounce, ro, rd, T4, A.prim[slot], A.tin[slot], P);
         if (P.alive) {
             A.S.T[pid] = P.T;
             A.o_tmin[slot] = P.ro;
             A.d_tmax[slot] = P.rd;
         }
-        if (P.Ladd.x != 0.0f || P.Ladd.y != 0.0f || P.Ladd.z != 0.0f) {
+        if (A.first) { // radiance starts at 0 (the same sums as k_camera's zero plus Ladd)
+            A.S.L[pid] = make_float4(0.0f + P.Ladd.x, 0.0f + P.Ladd.y, 0.0f + P.Ladd.z, 0.0f);
+        } else if (P.Ladd.x != 0.0f || P.Ladd.y != 0.0f || P.Ladd.z != 0.0f) {
             float4 L = A.S.L[pid];
             A.S.L[pid] = make_float4(L.x + P.Ladd.x, L.y + P.Ladd.y, L.z + P.Ladd.z, L.w);
         }
@@ -313,20 +373,37 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
     k_zero_i32<<<1, 64, 0, st>>>(counts, 64);
     k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(shc, 64 * SHADOW_SEGS);
     k_zero_i32<<<1, 64, 0, st>>>(reinterpret_cast<int*>(tot), 16);
-    k_camera<<<grid, RT, 0, st>>>(R.C, np, seed_ptr(R), ro[0], rd[0], q[0], S, counts);
-    int32_t rc = chk();
+    // Fused loop (default traversal, no traversal counters): bounce 0 traces
+    // camera rays generated in place and shades with slot = pid; shadow rays
+    // add their light inside the any-hit kernel.  Otherwise camera rays are
+    // stored first and every stage is a separate launch.
+    const bool fused = (desc->mode == MT_TRACE_PACKED || desc->mode == MT_TRACE_PACKED_STACK) && !ilp;
+    int32_t rc = MT_OK;
+    if (!fused) {
+        k_camera<<<grid, RT, 0, st>>>(R.C, np, seed_ptr(R), ro[0], rd[0], q[0], S, counts);
+        rc = chk();
+    }
     int cur = 0;
     for (int b = 0; b < nb && rc == MT_OK; b++) {
         int* dcount = counts + b;
+        const bool first = fused && b == 0;
         if (profile) hipEventRecord(ev[4 * b], st);
-        rc = mt_trace_closest(ctx, scene, reinterpret_cast<float*>(ro[cur]), reinterpret_cast<float*>(rd[cur]),
-                              dcount, np, I(R.oPrim), reinterpret_cast<float*>(base + R.oT_), ilp, desc->mode);
+        if (first) {
+            k_trace_camera<<<grid, RT, 0, st>>>(R.C, np, seed_ptr(R), scene->pnodes, scene->ptris, scene->root_ref,
+                                                I(R.oPrim), reinterpret_cast<float*>(base + R.oT_), counts);
+            rc = chk();
+        } else {
+            rc = mt_trace_closest(ctx, scene, reinterpret_cast<float*>(ro[cur]), reinterpret_cast<float*>(rd[cur]),
+                                  dcount, np, I(R.oPrim), reinterpret_cast<float*>(base + R.oT_), ilp, desc->mode);
+        }
         if (rc) break;
         if (profile) hipEventRecord(ev[4 * b + 1], st);
         if (ilp) k_sum_ilp<<<grid, RT, 0, st>>>(ilp, dcount, nullptr, 0, tot);
         ShadeArgs A;
         A.V = R.V;
         A.S = S;
+        A.C = R.C;
+        A.first = first ? 1 : 0;
         A.queue = q[cur];
         A.d_count = dcount;
         A.o_tmin = ro[cur];
@@ -347,17 +424,31 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         k_shade<<<gshade, RT, 0, st>>>(A);
         if ((rc = chk())) break;
         if (profile) hipEventRecord(ev[4 * b + 2], st);
-        // any hit over the segmented shadow queue (segment lengths on the device)
-        rc = mt_trace_any_seg(ctx, scene, F4(R.oSo), F4(R.oSd), MtSegQueue{seg_counts, SHADOW_SEGS, seg_cap},
-                              I(R.oOcc), ilp, desc->mode);
-        if (rc) break;
-        if (profile) hipEventRecord(ev[4 * b + 3], st);
-        if (ilp) k_sum_ilp<<<gshadow, RT, 0, st>>>(ilp, nullptr, seg_counts, seg_cap, tot + 3);
-        k_nee_accum<<<gshadow, RT, 0, st>>>(q[cur], seg_counts, seg_cap, F4(R.oSc), I(R.oOcc), S);
-        if ((rc = chk())) break;
+        if (fused) {
+            k_trace_shadow<<<gshadow, RT, 0, st>>>(scene->pnodes, scene->ptris, scene->root_ref, seg_counts, seg_cap,
+                                                    F4(R.oSo), F4(R.oSd), F4(R.oSc), first ? nullptr : q[cur], S.L);
+            if ((rc = chk())) break;
+            if (profile) hipEventRecord(ev[4 * b + 3], st);
+        } else {
+            // any hit over the segmented shadow queue (segment lengths on the device)
+            rc = mt_trace_any_seg(ctx, scene, F4(R.oSo), F4(R.oSd), MtSegQueue{seg_counts, SHADOW_SEGS, seg_cap},
+                                  I(R.oOcc), ilp, desc->mode);
+            if (rc) break;
+            if (profile) hipEventRecord(ev[4 * b + 3], st);
+            if (ilp) k_sum_ilp<<<gshadow, RT, 0, st>>>(ilp, nullptr, seg_counts, seg_cap, tot + 3);
+            k_nee_accum<<<gshadow, RT, 0, st>>>(q[cur], seg_counts, seg_cap, F4(R.oSc), I(R.oOcc), S);
+            if ((rc = chk())) break;
+        }
         if (b + 1 < nb) {
-            mt_payload P[3] = {{q[cur], q[1 - cur], 4}, {ro[cur], ro[1 - cur], 16}, {rd[cur], rd[1 - cur], 16}};
-            rc = mt_compact(ctx, I(R.oAlive), 0, dcount, np, nullptr, counts + b + 1, P, 3);
+            if (first) {
+                // the survivors' slots are their path ids: the compaction's
+                // source indices become the next queue
+                mt_payload P[2] = {{ro[cur], ro[1 - cur], 16}, {rd[cur], rd[1 - cur], 16}};
+                rc = mt_compact(ctx, I(R.oAlive), 0, dcount, np, q[1 - cur], counts + b + 1, P, 2);
+            } else {
+                mt_payload P[3] = {{q[cur], q[1 - cur], 4}, {ro[cur], ro[1 - cur], 16}, {rd[cur], rd[1 - cur], 16}};
+                rc = mt_compact(ctx, I(R.oAlive), 0, dcount, np, nullptr, counts + b + 1, P, 3);
+            }
             if (rc) break;
             cur = 1 - cur;
         }

@@ -22,385 +22,11 @@
 //    order.  e1/e2 are the same float subtractions the reference performs
 //    per test, so every intersection rounds identically.  An interior step
 //    fetches 64 B instead of 148 B, a leaf 48 B instead of 116 B.
-#include "mt_devmath.h"
-#include "mt_internal.h"
+#include "mt_traverse.h"
 
 #include <cstdio>
 
 namespace {
-
-struct RayT {
-    v3 o, d, inv;
-    int sx, sy, sz;
-    float tmin, tmax;
-};
-
-__device__ __forceinline__ void ray_init(RayT& r, float4 a, float4 b)
-{ // initGlobalRay, Util.cl:577-600
-    r.o = mk3(a.x, a.y, a.z);
-    r.d = mk3(b.x, b.y, b.z);
-    r.inv = mk3(1.0f / b.x, 1.0f / b.y, 1.0f / b.z);
-    r.sx = r.inv.x < 0.0f;
-    r.sy = r.inv.y < 0.0f;
-    r.sz = r.inv.z < 0.0f;
-    r.tmin = a.w;
-    r.tmax = b.w;
-}
-
-// intersectBoundT, Util.cl:707-732.  Conditional assignments (not fminf/fmaxf)
-// keep the reference's NaN behaviour.  The reference's two early returns
-// become terms of one conjunction (all six slab distances are computed
-// either way), so the test has no branches: a wave's lanes never split
-// inside it, and the node's loads all issue before the first compare.
-__device__ __forceinline__ bool box_hit(const RayT& r, float mnx, float mny, float mnz,
-                                        float mxx, float mxy, float mxz, float& tnear)
-{
-    const float tmin = ((r.sx ? mxx : mnx) - r.o.x) * r.inv.x;
-    const float tmax = ((r.sx ? mnx : mxx) - r.o.x) * r.inv.x;
-    const float tymin = ((r.sy ? mxy : mny) - r.o.y) * r.inv.y;
-    const float tymax = ((r.sy ? mny : mxy) - r.o.y) * r.inv.y;
-    const float tzmin = ((r.sz ? mxz : mnz) - r.o.z) * r.inv.z;
-    const float tzmax = ((r.sz ? mnz : mxz) - r.o.z) * r.inv.z;
-    const bool xy = !((tmin > tymax) | (tymin > tmax));
-    const float t1 = tymin > tmin ? tymin : tmin;
-    const float u1 = tymax < tmax ? tymax : tmax;
-    const bool xyz = !((t1 > tzmax) | (tzmin > u1));
-    const float t2 = tzmin > t1 ? tzmin : t1;
-    const float u2 = (tzmax < u1 ? tzmax : u1) * 1.00000024f;
-    tnear = t2;
-    return xy & xyz & (t2 < r.tmax) & (u2 > r.tmin);
-}
-
-// fastTriangleIntersection, Primitive.cl:3-29, on p1 and the two edges.
-// Both early returns of the reference give tMax; here every quantity is
-// computed and one select picks t or tMax, so p1 is fetched with the edges
-// instead of after the determinant test.
-__device__ __forceinline__ float tri_hit(const RayT& r, v3 p1, v3 e1, v3 e2)
-{
-    const v3 h = cross3(r.d, e2);
-    const float a = dot3(e1, h);
-    const bool parallel = ((double)a > -0.0000001) & ((double)a < 0.0000001); // double compare, :13
-    const float f = 1.0f / a;
-    const v3 s = sub3(r.o, p1);
-    const float b1 = f * dot3(s, h);
-    const v3 q = cross3(s, e1);
-    const float b2 = f * dot3(r.d, q);
-    const float t = f * dot3(e2, q);
-    const bool out = (b1 < 0.0f) | (b1 > 1.0f) | (b2 < 0.0f) | (b1 + b2 > 1.0f) | (t < 0.0f) | (t > r.tmax);
-    return (parallel | out) ? r.tmax : t;
-}
-
-struct BitStack {
-    uint64_t lo, hi;
-    __device__ __forceinline__ void push0()
-    {
-        hi = (hi << 1) | (lo >> 63);
-        lo <<= 1;
-    }
-    __device__ __forceinline__ void pop()
-    {
-        lo = (lo >> 1) | (hi << 63);
-        hi >>= 1;
-    }
-    __device__ __forceinline__ bool empty() const { return (lo | hi) == 0; }
-    // zero bits above the lowest set bit (the pops the reference performs)
-    __device__ __forceinline__ int ctz() const
-    {
-        return lo ? __builtin_ctzll(lo) : 64 + __builtin_ctzll(hi);
-    }
-    __device__ __forceinline__ void shr(int z)
-    {
-        if (z == 0) return;
-        if (z < 64) {
-            lo = (lo >> z) | (hi << (64 - z));
-            hi >>= z;
-        } else {
-            lo = hi >> (z - 64);
-            hi = 0;
-        }
-    }
-};
-
-// Short stack of postponed far children, newest first.  Each 1 bit of the
-// bitstack is a both-children-hit decision whose far child is still to be
-// visited; the stack holds the most recent MT_SHORT_STACK of them with the
-// node at which each was postponed.  Reaching the next pending far child
-// then costs no memory access: the reference's pops (walking parent links
-// up to the lowest 1 bit) are counted as ctz(bitstack) without being
-// fetched.  When older entries have been dropped, the parent walk below
-// finds them exactly as the reference does.
-#ifndef MT_SHORT_STACK
-#define MT_SHORT_STACK 8
-#endif
-struct ShortStack {
-    int ref[MT_SHORT_STACK];
-    int par[MT_SHORT_STACK];
-    int n;
-    __device__ __forceinline__ void push(int r, int p)
-    {
-#pragma unroll
-        for (int k = MT_SHORT_STACK - 1; k > 0; k--) {
-            ref[k] = ref[k - 1];
-            par[k] = par[k - 1];
-        }
-        ref[0] = r;
-        par[0] = p;
-        n = n < MT_SHORT_STACK ? n + 1 : MT_SHORT_STACK;
-    }
-    __device__ __forceinline__ void pop(int& r, int& p)
-    {
-        r = ref[0];
-        p = par[0];
-#pragma unroll
-        for (int k = 0; k < MT_SHORT_STACK - 1; k++) {
-            ref[k] = ref[k + 1];
-            par[k] = par[k + 1];
-        }
-        n--;
-    }
-};
-
-// -------------------------------------------------------------------------
-// Reference layout: NormalBVH.cl:14-92 over nodes[7], bounds[8], faces[10].
-// -------------------------------------------------------------------------
-template <bool ANY, bool STATS>
-__device__ __forceinline__ int trace_reference(const int* __restrict__ nodes,
-                                               const float* __restrict__ bounds,
-                                               const int* __restrict__ faces,
-                                               const float4* __restrict__ points, RayT& r,
-                                               int& child, int* st)
-{
-    int hit = 0;
-    int node_id = 0;
-    BitStack bs{0, 0};
-    int parent_id = 0, sibling_id = 0;
-    for (;;) {
-        while (!nodes[7 * node_id + 5]) {
-            const int* n = nodes + 7 * node_id;
-            parent_id = n[4];
-            sibling_id = n[1];
-            int left = n[2], right = n[3];
-            const float* lb = bounds + 8 * nodes[7 * left];
-            const float* rb = bounds + 8 * nodes[7 * right];
-            float lt, rt;
-            bool lh = box_hit(r, lb[0], lb[1], lb[2], lb[4], lb[5], lb[6], lt);
-            bool rh = box_hit(r, rb[0], rb[1], rb[2], rb[4], rb[5], rb[6], rt);
-            if (STATS) st[0]++;
-            if (!lh && !rh) break;
-            bs.push0();
-            if (lh && rh) {
-                node_id = (rt < lt) ? right : left;
-                bs.lo |= 1;
-            } else {
-                node_id = lh ? left : right;
-            }
-        }
-        if (nodes[7 * node_id + 5]) {
-            const int* n = nodes + 7 * node_id;
-            if (STATS) st[1]++;
-            int c = n[6];
-            const int* F = faces + 10 * c;
-            v3 p1 = xyz(points[F[0]]);
-            v3 p2 = xyz(points[F[1]]);
-            v3 p3 = xyz(points[F[2]]);
-            float t = tri_hit(r, p1, sub3(p2, p1), sub3(p3, p1));
-            if ((r.tmin < t) && (t < r.tmax)) {
-                if (ANY) return 1;
-                hit = 1;
-                r.tmax = t;
-                child = c;
-            }
-            parent_id = n[4];
-            sibling_id = n[1];
-        }
-        while ((bs.lo & 1) == 0) {
-            if (bs.empty()) return hit;
-            node_id = parent_id;
-            const int* n = nodes + 7 * node_id;
-            parent_id = n[4];
-            sibling_id = n[1];
-            bs.pop();
-            if (STATS) st[2]++;
-        }
-        node_id = sibling_id;
-        bs.lo ^= 1;
-    }
-}
-
-// -------------------------------------------------------------------------
-// Packed layout, same visiting order.  cur >= 0: interior node record;
-// cur < 0: leaf ~rank.  parent/sib track the current node's links, which the
-// reference reads from the node records (leaves are not records here).
-// -------------------------------------------------------------------------
-// Resumable traversal state of one ray (packed layout).
-struct TravState {
-    int cur, parent, sib;
-    BitStack bs;
-    ShortStack ss;
-    int hit, child;
-    int st[3];
-    __device__ __forceinline__ void init(int root)
-    {
-        cur = root;
-        parent = 0;
-        sib = 0;
-        bs.lo = 0;
-        bs.hi = 0;
-        ss.n = 0;
-        hit = 0;
-        child = -1;
-        st[0] = st[1] = st[2] = 0;
-    }
-};
-
-// One outer iteration of the packed traversal: descend to a leaf (or a node
-// whose children both miss), test the leaf, move to the next pending subtree.
-// Returns true when the ray is finished.  The sequence of nodes visited over
-// all iterations is exactly the reference's (see trace_packed below).
-template <bool ANY, bool STATS, bool SHORT_STACK>
-__device__ __forceinline__ bool trace_packed_step(const float4* __restrict__ pn, const float4* __restrict__ pt,
-                                                  RayT& r, TravState& S)
-{
-    while (S.cur >= 0) {
-        const float4* N = pn + 4 * (size_t)S.cur;
-        float4 a = N[0], b = N[1], c = N[2];
-        int4 d = *reinterpret_cast<const int4*>(N + 3);
-        S.parent = d.z;
-        S.sib = d.w;
-        float lt, rt;
-        bool lh = box_hit(r, a.x, a.y, a.z, a.w, b.x, b.y, lt);
-        bool rh = box_hit(r, b.z, b.w, c.x, c.y, c.z, c.w, rt);
-        if (STATS) S.st[0]++;
-        if (!lh && !rh) break;
-        S.bs.push0();
-        int nxt;
-        if (lh && rh) {
-            bool right_first = rt < lt;
-            nxt = right_first ? d.y : d.x;
-            if (SHORT_STACK) S.ss.push(right_first ? d.x : d.y, S.cur);
-            S.bs.lo |= 1;
-        } else {
-            nxt = lh ? d.x : d.y;
-        }
-        if (nxt < 0) {
-            S.parent = S.cur;
-            S.sib = (nxt == d.x) ? d.y : d.x;
-        }
-        S.cur = nxt;
-    }
-    if (S.cur < 0) {
-        if (STATS) S.st[1]++;
-        const float4* T = pt + 3 * (size_t)(~S.cur);
-        float4 t0 = T[0], t1 = T[1], t2 = T[2];
-        float t = tri_hit(r, mk3(t0.x, t0.y, t0.z), mk3(t0.w, t1.x, t1.y), mk3(t1.z, t1.w, t2.x));
-        if ((r.tmin < t) && (t < r.tmax)) {
-            S.hit = 1;
-            if (ANY) return true;
-            r.tmax = t;
-            S.child = __float_as_int(t2.y);
-        }
-    }
-    if (S.bs.empty()) return true;
-    if (SHORT_STACK && S.ss.n > 0) {
-        int z = S.bs.ctz();
-        if (STATS) S.st[2] += z;
-        S.bs.shr(z);
-        S.bs.lo ^= 1;
-        S.ss.pop(S.cur, S.parent);
-        return false;
-    }
-    while ((S.bs.lo & 1) == 0) {
-        if (S.bs.empty()) return true;
-        S.cur = S.parent;
-        int2 d = *reinterpret_cast<const int2*>(reinterpret_cast<const int*>(pn + 4 * (size_t)S.cur + 3) + 2);
-        S.parent = d.x;
-        S.sib = d.y;
-        S.bs.pop();
-        if (STATS) S.st[2]++;
-    }
-    int from = S.cur;
-    S.cur = S.sib;
-    S.sib = from;
-    S.bs.lo ^= 1;
-    return false;
-}
-
-template <bool ANY, bool STATS, bool SHORT_STACK>
-__device__ __forceinline__ int trace_packed(const float4* __restrict__ pn,
-                                            const float4* __restrict__ pt, int root, RayT& r,
-                                            int& child, int* st)
-{
-    int hit = 0;
-    int cur = root;
-    BitStack bs{0, 0};
-    ShortStack ss;
-    ss.n = 0;
-    int parent = 0, sib = 0;
-    for (;;) {
-        while (cur >= 0) {
-            const float4* N = pn + 4 * (size_t)cur;
-            float4 a = N[0], b = N[1], c = N[2];
-            int4 d = *reinterpret_cast<const int4*>(N + 3);
-            parent = d.z;
-            sib = d.w;
-            float lt, rt;
-            bool lh = box_hit(r, a.x, a.y, a.z, a.w, b.x, b.y, lt);
-            bool rh = box_hit(r, b.z, b.w, c.x, c.y, c.z, c.w, rt);
-            if (STATS) st[0]++;
-            if (!lh && !rh) break;
-            bs.push0();
-            int nxt;
-            if (lh && rh) {
-                bool right_first = rt < lt;
-                nxt = right_first ? d.y : d.x;
-                if (SHORT_STACK) ss.push(right_first ? d.x : d.y, cur);
-                bs.lo |= 1;
-            } else {
-                nxt = lh ? d.x : d.y;
-            }
-            if (nxt < 0) { // entering a leaf: its parent is cur, its sibling the other child
-                parent = cur;
-                sib = (nxt == d.x) ? d.y : d.x;
-            }
-            cur = nxt;
-        }
-        if (cur < 0) {
-            if (STATS) st[1]++;
-            const float4* T = pt + 3 * (size_t)(~cur);
-            float4 t0 = T[0], t1 = T[1], t2 = T[2];
-            float t = tri_hit(r, mk3(t0.x, t0.y, t0.z), mk3(t0.w, t1.x, t1.y), mk3(t1.z, t1.w, t2.x));
-            if ((r.tmin < t) && (t < r.tmax)) {
-                if (ANY) return 1;
-                hit = 1;
-                r.tmax = t;
-                child = __float_as_int(t2.y);
-            }
-        }
-        if (bs.empty()) return hit;
-        if (SHORT_STACK && ss.n > 0) {
-            // the lowest 1 bit's far child is on top of the short stack
-            int z = bs.ctz();
-            if (STATS) st[2] += z;
-            bs.shr(z);
-            bs.lo ^= 1;
-            ss.pop(cur, parent); // a leaf keeps its parent for a later parent walk
-            continue;
-        }
-        while ((bs.lo & 1) == 0) {
-            if (bs.empty()) return hit;
-            cur = parent;
-            int2 d = *reinterpret_cast<const int2*>(reinterpret_cast<const int*>(pn + 4 * (size_t)cur + 3) + 2);
-            parent = d.x;
-            sib = d.y;
-            bs.pop();
-            if (STATS) st[2]++;
-        }
-        int from = cur;
-        cur = sib;
-        sib = from;
-        bs.lo ^= 1;
-    }
-}
 
 struct TraceArgs {
     const int* nodes;

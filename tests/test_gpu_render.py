@@ -126,3 +126,37 @@ def test_render_stats_counters(ctx, c1_small):
     assert st.bounce_counts[0] == 64 * 64
     assert st.closest_ilp[0] > 0 and st.closest_launches == job.max_bounces
     assert st.closest_ms > 0
+
+
+@pytest.mark.parametrize("mesh_lights", [False, True])
+def test_fused_and_staged_loops_agree(ctx, c1_small, mesh_lights):
+    """The default loop fuses the camera pass into bounce 0's traversal and
+    the direct-light add into the any-hit kernel; with traversal counters on,
+    or another traversal layout, every stage is a separate launch.  Same
+    paths, so images, ray counts and per-bounce counts are bit-identical."""
+    print(f"[case] fused vs staged loop, mesh lights {mesh_lights}", flush=True)
+    points, normals, faces, nodes, bounds, depth = c1_small
+    sc = api.Scene(ctx, points, faces, nodes, bounds, depth, normals=normals)
+    mats = W.c1_materials()
+    lf = nl = None
+    if mesh_lights:
+        mats[:4, 28:31] = [[1.0, 0.9, 0.7]]
+        mats[:4, 32] = 1.0
+        mats[:4, 33] = 5.0
+    dm = ctx.to_device(mats)
+    if mesh_lights:
+        lf, nl = api.mesh_lights(ctx, sc, dm)
+    env = api.CEnvironment(ctx)
+    env.setEnvironmentMap(W.env_map())
+    base = dict(width=100, height=60, spp=3, pos=(0.0, 0.5, -8.0), look=(0.0, -1.0, 0.0), seed=5)
+    runs = [W.RenderJob(**base),                                           # fused
+            W.RenderJob(**base, flags=_lib.MT_RENDER_TRAVERSAL_STATS),     # staged, same traversal
+            W.RenderJob(**base, mode=_lib.MT_TRACE_PACKED_NO_STACK),       # staged, parent-link pops
+            W.RenderJob(**base, mode=_lib.MT_TRACE_REFERENCE)]             # staged, reference arrays
+    out = [api.render(ctx, sc, dm, env, j, light_faces=lf, num_lights=nl) for j in runs]
+    ref_img, ref_st = out[0]
+    for img, st in out[1:]:
+        np.testing.assert_array_equal(img.cpu().numpy().view(np.int32), ref_img.cpu().numpy().view(np.int32))
+        assert int(st.extension_rays) == int(ref_st.extension_rays)
+        assert int(st.shadow_rays) == int(ref_st.shadow_rays)
+        assert list(st.bounce_counts) == list(ref_st.bounce_counts)
