@@ -141,7 +141,8 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, uint32_t seed, in
         float4 ro, rd, T4;
         if (A.first) { // what k_camera would have stored: InitPathData, Initialize.cl:25-48
             pid = slot;
-            camera_ray(A.C, pid, seed, ro, rd);
+            const Camera C = A.C; // a register copy: a reference into the kernel argument spills it to scratch
+            camera_ray(C, pid, seed, ro, rd);
             T4 = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
         } else {
             pid = A.queue[slot];
