@@ -40,9 +40,9 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles():
+def traffic_from_profiles(workload: str = "c1"):
     """HBM bytes per closest-hit launch measured by rocprofv3 --pmc (profiles/), or None."""
-    p = os.path.join(ROOT, "profiles", "traffic_trace_closest_c1.json")
+    p = os.path.join(ROOT, "profiles", f"traffic_trace_closest_{workload}.json")
     if not os.path.exists(p):
         return None
     try:
@@ -93,9 +93,11 @@ def main() -> int:
     ap.add_argument("--mode", type=int, default=0, help="0 packed traversal layout, 1 reference layout")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["c1", "c3", "c4"], default="c1",
-                    help="c1: one 1080p render per step (default line); c3: 256 uneven jobs batched vs "
-                         "one at a time; c4: Poisson-arrival serving loop")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c1",
+                    help="c1: one 1080p render per step (default line); c2: 64M random rays through 4M "
+                         "triangles (closest hit, shadow rays, any hit, compaction); c3: 256 uneven jobs "
+                         "batched vs one at a time; c4: Poisson-arrival serving loop")
+    ap.add_argument("--rays", type=int, default=64 << 20, help="c2: rays per step")
     ap.add_argument("--jobs", type=int, default=256, help="c3: number of jobs")
     ap.add_argument("--rate", type=float, default=40.0, help="c4: arrivals per second")
     ap.add_argument("--duration", type=float, default=5.0, help="c4: seconds of arrivals")
@@ -126,6 +128,8 @@ def main() -> int:
         torch.cuda.synchronize()
         print(f"copied {a.numel() * 4} bytes", flush=True)
         return 0
+    if args.workload == "c2":
+        return bench_c2(args)
     if args.workload != "c1":
         return bench_jobs(args)
     t0 = time.perf_counter()
@@ -256,6 +260,156 @@ def main() -> int:
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
         },
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_c2(args) -> int:
+    """c2 stress (one step): closest hit of 64*2^20 rays with origins uniform in
+    the scene box and directions uniform on the sphere, through 3.6M tiny
+    triangles clustered at the centre plus 0.4M large ones spread over a
+    200-unit box; then a shadow ray per ray toward one light
+    (tMax = dist - 2*EPS_RAY), any hit over them, and the stable compaction
+    of the hit rays (ray o/d payloads + source index).  All buffers are
+    allocated before the timed region; HIP events on the launch stream time
+    each stage.  roofline: closest-hit kernel, algorithmic bytes
+    40 + 148*I + 116*L + 28*P per ray counted over all rays by the counting
+    traversal (untimed)."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from mambatracer_amd import _lib, api
+    from mambatracer_amd import workloads as W
+
+    t0 = time.perf_counter()
+    points, faces, nodes, bounds, depth = W.c2_scene()
+    n = args.rays
+    o, d = api.gen_random_rays(3, 0, n, bounds[0, :3], bounds[0, 4:7], nthreads=16)
+    light = np.ascontiguousarray(W.c2_light(bounds), dtype=np.float32)
+    log(f"c2: {faces.shape[0]} triangles, BVH depth {depth}, {n} rays, host prep {time.perf_counter() - t0:.1f} s")
+    ctx = api.Context(0)
+    scene = api.Scene(ctx, points, faces, nodes, bounds, depth)
+    do, dd = ctx.to_device(o), ctx.to_device(d)
+    del o, d
+    dev = ctx.torch_device
+    length = ctx.scalar_i32(n)
+    prim = torch.empty(n, dtype=torch.int32, device=dev)
+    tt = torch.empty(n, dtype=torch.float32, device=dev)
+    occ = torch.empty(n, dtype=torch.int32, device=dev)
+    so, sd = torch.empty_like(do), torch.empty_like(dd)
+    valid = torch.empty(n, dtype=torch.int32, device=dev)
+    idx = torch.empty(n, dtype=torch.int32, device=dev)
+    cnt = ctx.scalar_i32(0)
+    co, cd = torch.empty_like(do), torch.empty_like(dd)
+    pay = (_lib.Payload * 2)()
+    pay[0] = _lib.Payload(do.data_ptr(), co.data_ptr(), 16)
+    pay[1] = _lib.Payload(dd.data_ptr(), cd.data_ptr(), 16)
+    P = lambda x: C.c_void_p(x.data_ptr())
+    h = ctx.handle
+
+    def closest(st=None):
+        api.call("mt_trace_closest", h, scene.handle, P(do), P(dd), P(length), n, P(prim), P(tt),
+                 P(st) if st is not None else None, 0)
+
+    def shadow():
+        api.call("mt_shadow_rays", h, P(do), P(dd), P(prim), P(tt), light.ctypes.data_as(C.c_void_p), P(length), n,
+                 P(so), P(sd), P(valid))
+
+    def anyhit(st=None):
+        api.call("mt_trace_any", h, scene.handle, P(so), P(sd), P(length), n, P(occ),
+                 P(st) if st is not None else None, 0)
+
+    def compact():
+        api.call("mt_compact", h, P(prim), 1, P(length), n, P(idx), P(cnt), pay, 2)
+
+    # untimed: traversal counters over all rays (algorithmic bytes)
+    st = torch.empty((n, 3), dtype=torch.int32, device=dev)
+    closest(st)
+    ilp_c = st.sum(0, dtype=torch.int64).tolist()
+    shadow()
+    anyhit(st)
+    ilp_a = st.sum(0, dtype=torch.int64).tolist()
+    del st
+    alg_c = 40 * n + 148 * ilp_c[0] + 116 * ilp_c[1] + 28 * ilp_c[2]
+    alg_a = 36 * n + 148 * ilp_a[0] + 116 * ilp_a[1] + 28 * ilp_a[2]
+    log(f"closest: I {ilp_c[0] / n:.1f} L {ilp_c[1] / n:.1f} P {ilp_c[2] / n:.1f} per ray ({alg_c / n:.0f} B/ray); "
+        f"any: I {ilp_a[0] / n:.1f} L {ilp_a[1] / n:.1f} P {ilp_a[2] / n:.1f} per ray")
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    stage = np.zeros(4)
+
+    def step():
+        ev[0].record()
+        closest()
+        ev[1].record()
+        shadow()
+        ev[2].record()
+        anyhit()
+        ev[3].record()
+        compact()
+        ev[4].record()
+
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        torch.cuda.synchronize()
+        stage += [ev[k].elapsed_time(ev[k + 1]) for k in range(4)]
+    elapsed = time.perf_counter() - t_start
+    ctx.check_scans()
+    hits = int((prim >= 0).sum().item())
+    survivors = int(cnt.item())
+    if survivors != hits:
+        log(f"compaction count {survivors} != hits {hits}")
+        return 1
+    stage /= args.steps
+    value = 2 * n * args.steps / elapsed / 1e9
+    achieved = alg_c / (stage[0] / 1e3) / 1e9
+    log(f"per step: closest {stage[0]:.2f} ms, shadow setup {stage[1]:.2f} ms, any {stage[2]:.2f} ms, "
+        f"compaction {stage[3]:.3f} ms; hits {hits}")
+    cbytes = n * (4 + 32) + survivors * (32 + 4)
+    traffic = traffic_from_profiles("c2")
+    out = {
+        "metric": "Grays/s, c2 stress (closest hit + any hit of 64M incoherent rays, 4M triangles)",
+        "value": round(value, 4),
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generators: scene seed 3, rays seed 3)",
+        "config": {"workload": f"c2: {faces.shape[0]} triangles (3.6M tiny in a 0.1 cube + 0.4M large in a "
+                               f"200-unit box), {n} rays uniform in the scene box, uniform directions; closest hit, "
+                               "a shadow ray per ray to one light, any hit, stable compaction of the hit rays",
+                   "rays": n, "traversal_layout": "packed, short stack"},
+        "stages_ms": {"closest": round(stage[0], 3), "shadow_setup": round(stage[1], 3),
+                      "any": round(stage[2], 3), "compaction": round(stage[3], 4)},
+        "closest_Grays_per_s": round(n / stage[0] / 1e6, 4),
+        "any_Grays_per_s": round(n / stage[2] / 1e6, 4),
+        "compaction_GB_per_s": round(cbytes / stage[3] / 1e6, 1),
+        "roofline": {
+            "kernel": "closest-hit traversal (k_trace, packed layout)",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "traffic_GB_per_s": round(traffic / (stage[0] / 1e3) / 1e9, 1) if traffic else None,
+            "algorithmic_bytes_per_launch": int(alg_c),
+            "avg_launch_ms": round(stage[0], 4),
+        },
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
     return 0

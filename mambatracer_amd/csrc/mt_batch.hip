@@ -629,7 +629,7 @@ extern "C" int32_t mt_batch_step(mt_batch* b, int32_t max_done, int64_t* done_ta
             }
             const mt_render_desc& r = J.d.render;
             const int npix = r.width * r.height;
-            k_resolve<<<mt_blocks(npix, BT), BT, 0, s>>>(reinterpret_cast<const float4*>(b->arena + J.off),
+            k_resolve<<<mt_blocks(npix, resolve_pixels_per_group(r.spp)), BT, 0, s>>>(reinterpret_cast<const float4*>(b->arena + J.off),
                                                          r.width, r.height, r.spp,
                                                          reinterpret_cast<float4*>(J.d.image));
             MT_LAUNCH_CHECK();

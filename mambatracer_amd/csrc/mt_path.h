@@ -400,22 +400,54 @@ __device__ __forceinline__ bool seg_live(const int* __restrict__ counts, int seg
 }
 
 // Per-pixel mean over samples in sample order (averageAccum, Image.cl:29-45,
-// without tone mapping): deterministic, no float atomics.
+// without tone mapping): deterministic, no float atomics.  A pixel's samples
+// are consecutive path ids, so a work-group takes a run of consecutive pixel
+// ranks, stages their paths' radiance through LDS with coalesced loads, and
+// thread t then sums pixel t's samples in order (the same sums as reading
+// them one by one).
+constexpr int RESOLVE_PATHS = 1024; // radiance values staged per round
+__host__ __device__ inline int resolve_pixels_per_group(int spp)
+{
+    const int p = spp >= RESOLVE_PATHS ? 1 : RESOLVE_PATHS / spp;
+    return p < 256 ? p : 256;
+}
+
 __global__ __launch_bounds__(256) void k_resolve(const float4* __restrict__ L, int w, int h, int spp,
                                                  float4* __restrict__ image)
 {
-    int px = blockIdx.x * blockDim.x + threadIdx.x;
-    if (px >= w * h) return;
-    const float4* Lp = L + pixel_first_path(px % w, px / w, w, h, spp);
+    __shared__ float4 s_L[RESOLVE_PATHS];
+    const int npix = w * h;
+    const int ppg = resolve_pixels_per_group(spp);
+    const int k0 = blockIdx.x * ppg;
+    if (k0 >= npix) return;
+    const int npx = npix - k0 < ppg ? npix - k0 : ppg;
+    const int t = threadIdx.x;
+    const float4* Lg = L + (size_t)k0 * spp;
+    const int total = npx * spp; // > RESOLVE_PATHS only when one pixel has more samples
     float r = 0.0f, g = 0.0f, b = 0.0f;
-    for (int s = 0; s < spp; s++) {
-        float4 v = Lp[s];
-        r += v.x;
-        g += v.y;
-        b += v.z;
+    for (int c = 0; c < total; c += RESOLVE_PATHS) {
+        const int m = total - c < RESOLVE_PATHS ? total - c : RESOLVE_PATHS;
+        __syncthreads();
+        for (int i = t; i < m; i += blockDim.x) s_L[i] = Lg[c + i];
+        __syncthreads();
+        if (t < npx) {
+            int lo = t * spp, hi = lo + spp;
+            lo = lo > c ? lo : c;
+            hi = hi < c + m ? hi : c + m;
+            for (int i = lo; i < hi; i++) {
+                const float4 v = s_L[i - c];
+                r += v.x;
+                g += v.y;
+                b += v.z;
+            }
+        }
     }
-    float inv = 1.0f / (float)spp;
-    image[px] = make_float4(r * inv, g * inv, b * inv, 1.0f);
+    if (t < npx) {
+        int x, y;
+        path_pixel((k0 + t) * spp, w, h, spp, x, y);
+        float inv = 1.0f / (float)spp;
+        image[(size_t)y * w + x] = make_float4(r * inv, g * inv, b * inv, 1.0f);
+    }
 }
 
 } // namespace

@@ -3,25 +3,28 @@
 // shade_path (mt_path.h).
 //
 // Layout.  Path state is SoA indexed by path id (pid = pixel rank * spp +
-// sample, pixels ranked in 8x8 tiles: path_pixel in mt_path.h; the camera
-// pass puts path pid in queue slot pid): throughput.xyz + last BSDF pdf in one float4, accumulated
-// radiance in another.  The live paths of a bounce form a queue of pids with
-// their rays (two float4 streams) and a device-resident count; every stage
-// reads the count on the device (the reference's `global int* count`), so
-// the host enqueues the whole render without synchronising.
+// sample, pixels ranked in 8x8 tiles: path_pixel in mt_path.h; bounce 0 puts
+// path pid in queue slot pid): throughput.xyz + last BSDF pdf in one float4,
+// accumulated radiance in another.  The live paths of a bounce form a queue
+// of pids with their rays (two float4 streams) and a device-resident count;
+// every stage reads the count on the device (the reference's `global int*
+// count`), so the host enqueues the whole render without synchronising.
 //
 // Per bounce:
-//   1. closest hit over the queue (mt_trace_closest);
+//   1. closest hit over the queue (mt_trace_closest; at bounce 0
+//      k_trace_camera generates each camera ray in place instead);
 //   2. one fused shading kernel per slot (shade_path): hit shading,
 //      setupBsdf, LightHitPass, DirectLight's light sample + EvaluateBsdf +
 //      shadow-ray setup, SampleBsdf for the next ray and Russian roulette;
 //      shadow rays are appended to a segmented queue (one atomic per wave on
 //      the counter of the work-group's segment), so the any-hit launch has
 //      empty lanes only at segment tails and no counter is contended;
-//   3. any hit over the shadow queue (mt_trace_any_seg);
-//   4. unoccluded direct light added to the path radiance;
-//   5. stable compaction of live slots (pid and next ray) into the other
+//   3. any hit over the shadow queue, an unoccluded ray adding its light to
+//      its path's radiance in the same kernel (k_trace_shadow);
+//   4. stable compaction of live slots (pid and next ray) into the other
 //      queue buffer (mt_compact; CCompact.execute in the reference).
+// With traversal counters or a non-default traversal layout the stages run
+// unfused (k_camera, mt_trace_any_seg, k_nee_accum): same paths, same bits.
 // After the last bounce the per-path radiance is averaged per pixel in a
 // fixed order, so the image is deterministic: no float atomics.
 #include "mt_path.h"
@@ -455,7 +458,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         }
     }
     if (rc == MT_OK) {
-        k_resolve<<<mt_blocks(R.npix, RT), RT, 0, st>>>(S.L, desc->width, desc->height, desc->spp,
+        k_resolve<<<mt_blocks(R.npix, resolve_pixels_per_group(desc->spp)), RT, 0, st>>>(S.L, desc->width, desc->height, desc->spp,
                                                          reinterpret_cast<float4*>(R.image));
         rc = chk();
     }
