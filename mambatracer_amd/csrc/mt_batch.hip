@@ -15,7 +15,7 @@
 //               tag int2 (job << 6 | bounce, path id), ray o/tmin and
 //               d/tmax float4, throughput float4 (w = last BSDF pdf);
 //               plus per-iteration scratch: hit prim/t, alive flag,
-//               occlusion flag, shadow ray o/d and contribution.
+//               shadow ray o/d, contribution and slot tag.
 // One iteration (all device-side, counts never leave the device):
 //   1. k_refill (one wave): free slots = capacity - live; hand them to the
 //      active jobs in admission order (wave prefix sum over the jobs'
@@ -27,9 +27,10 @@
 //      paths count into their job's done counter (wave-aggregated atomics);
 //      shadow rays go to a segmented queue (MtSegQueue) with their slot's
 //      tag;
-//   5. any hit over the shadow queue; 6. k_nee_jobs adds unoccluded light;
-//   7. stable compaction of live slots (4 streams) into the other copy;
-//   8. k_done_sum folds the done rows of the active jobs.
+//   5. k_shadow_jobs: any hit over the shadow queue, each unoccluded ray
+//      adding its light to its path's radiance;
+//   6. stable compaction of live slots (4 streams) into the other copy;
+//   7. k_done_sum folds the done rows of the active jobs.
 // The host then reads the done totals (4 B per job slot), resolves the
 // images of finished jobs (k_resolve, fixed sample order) and admits waiting
 // jobs into the freed slots and arena space.
@@ -39,6 +40,7 @@
 #include <vector>
 
 #include "mt_path.h"
+#include "mt_traverse.h"
 
 namespace {
 
@@ -243,12 +245,17 @@ __global__ __launch_bounds__(BT) void k_shade_jobs(ShadeJobsArgs A)
     }
 }
 
-// Unoccluded shadow-queue entries add their contribution to their path;
-// work-group 0 adds the iteration's shadow rays to the running total.
-__global__ __launch_bounds__(BT) void k_nee_jobs(const JobRec* __restrict__ jobs, const int* __restrict__ s_seg,
-                                                 int seg_cap, const int2* __restrict__ s_tag,
-                                                 const float4* __restrict__ contrib,
-                                                 const int* __restrict__ occluded, unsigned long long* __restrict__ tot)
+// Any hit over the segmented shadow queue, each ray against its job's scene,
+// with the direct-light add fused in: an unoccluded entry adds its
+// contribution to its path's radiance in its job's buffer.  Work-group 0
+// adds the iteration's shadow rays to the running total.
+__global__ __launch_bounds__(BT) void k_shadow_jobs(const JobRec* __restrict__ jobs,
+                                                    const MtSceneRef* __restrict__ refs,
+                                                    const int* __restrict__ s_seg, int seg_cap,
+                                                    const int2* __restrict__ s_tag, const float4* __restrict__ s_o,
+                                                    const float4* __restrict__ s_d,
+                                                    const float4* __restrict__ contrib,
+                                                    unsigned long long* __restrict__ tot)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0) {
@@ -257,10 +264,17 @@ __global__ __launch_bounds__(BT) void k_nee_jobs(const JobRec* __restrict__ jobs
         for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
         if (__lane_id() == 0 && c) atomicAdd(&tot[1], (unsigned long long)c);
     }
-    if (!seg_live(s_seg, seg_cap, k) || occluded[k]) return;
-    const float4 c = contrib[k];
+    if (!seg_live(s_seg, seg_cap, k)) return;
     const int2 tg = s_tag[k];
-    float4* L = jobs[tg.x >> BOUNCE_BITS].L;
+    const int job = tg.x >> BOUNCE_BITS;
+    const MtSceneRef R = refs[job];
+    RayT r;
+    ray_init(r, s_o[k], s_d[k]);
+    int st[3] = {0, 0, 0};
+    int child = -1;
+    if (trace_packed<true, false, true>(R.pn, R.pt, R.root, r, child, st)) return;
+    const float4 c = contrib[k];
+    float4* L = jobs[job].L;
     const float4 v = L[tg.y];
     L[tg.y] = make_float4(v.x + c.x, v.y + c.y, v.z + c.z, v.w);
 }
@@ -283,7 +297,7 @@ struct mt_batch {
     unsigned long long* tot;  // [0] extension rays, [1] shadow rays
     Queue q[2];
     int cur;
-    int *prim, *alive, *occ;
+    int *prim, *alive;
     float* t;
     float4 *s_o, *s_d, *s_c;
     int2* s_tag;
@@ -445,12 +459,8 @@ int32_t iterate(mt_batch* b)
     A.seg_cap = b->seg_cap;
     k_shade_jobs<<<grid, BT, 0, s>>>(A);
     MT_LAUNCH_CHECK();
-    const MtSegQueue sq{b->s_seg, SHADOW_SEGS, b->seg_cap};
-    rc = mt_trace_jobs(b->ctx, true, b->refs, &b->s_tag->x, BOUNCE_BITS, b->s_o, b->s_d, nullptr, 0, nullptr,
-                       nullptr, b->occ, sq);
-    if (rc) return rc;
-    k_nee_jobs<<<mt_blocks((int64_t)SHADOW_SEGS * b->seg_cap, BT), BT, 0, s>>>(b->jobs, b->s_seg, b->seg_cap,
-                                                                               b->s_tag, b->s_c, b->occ, b->tot);
+    k_shadow_jobs<<<mt_blocks((int64_t)SHADOW_SEGS * b->seg_cap, BT), BT, 0, s>>>(
+        b->jobs, b->refs, b->s_seg, b->seg_cap, b->s_tag, b->s_o, b->s_d, b->s_c, b->tot);
     MT_LAUNCH_CHECK();
     mt_payload P[4] = {{q.tag, nq.tag, 8}, {q.ro, nq.ro, 16}, {q.rd, nq.rd, 16}, {q.T, nq.T, 16}};
     rc = mt_compact(b->ctx, b->alive, 0, b->cnt + 1, Qc, nullptr, b->cnt, P, 4);
@@ -504,7 +514,7 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
         oQ[k][0] = take(8 * n);
         for (int m = 1; m < 4; m++) oQ[k][m] = take(16 * n);
     }
-    size_t oPrim = take(4 * n), oT = take(4 * n), oAlive = take(4 * n), oOcc = take(4 * sq);
+    size_t oPrim = take(4 * n), oT = take(4 * n), oAlive = take(4 * n);
     size_t oSo = take(16 * sq), oSd = take(16 * sq), oSc = take(16 * sq), oStag = take(8 * sq);
     if (hipMalloc(&b->mem, bytes) != hipSuccess) {
         delete b;
@@ -550,7 +560,6 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     b->prim = reinterpret_cast<int*>(m + oPrim);
     b->t = reinterpret_cast<float*>(m + oT);
     b->alive = reinterpret_cast<int*>(m + oAlive);
-    b->occ = reinterpret_cast<int*>(m + oOcc);
     b->s_o = reinterpret_cast<float4*>(m + oSo);
     b->s_d = reinterpret_cast<float4*>(m + oSd);
     b->s_c = reinterpret_cast<float4*>(m + oSc);
