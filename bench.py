@@ -467,24 +467,31 @@ def bench_jobs(args) -> int:
     log(f"batched: {s_bat}, iterations {int(st.iterations)}, arena peak {int(st.arena_peak) / 1e9:.2f} GB")
     ref = {d.tag: d.image for d in seq}
     mismatched = sum(0 if torch.equal(ref[d.tag], d.image) else 1 for d in bat)
+    modes = {"batched": s_bat, "sequential": s_seq}
+    if args.workload == "c4":
+        sjf, sjf_rays = serve.serve_sequential(ctx, reqs, keep_images=True, order="sjf")
+        modes["sequential_sjf"] = serve.summary(sjf, sjf_rays)
+        log(f"sequential, shortest job first: {modes['sequential_sjf']}")
+        mismatched += sum(0 if torch.equal(ref[d.tag], d.image) else 1 for d in sjf)
     ctx.sync()
     metric = "jobs/s, continuous batching vs one job at a time" if args.workload == "c3" else \
-        "p50/p99 render-job latency at fixed QPS, continuous batching vs one job at a time"
+        "p50/p99 render-job latency at fixed QPS (best serving mode; every mode reported)"
+    best = min(modes, key=lambda m: modes[m]["latency_ms"]["p99"])
     out = {
         "metric": metric,
-        "value": s_bat["jobs_per_s"] if args.workload == "c3" else s_bat["latency_ms"]["p99"],
+        "value": s_bat["jobs_per_s"] if args.workload == "c3" else modes[best]["latency_ms"]["p99"],
         "unit": "jobs/s" if args.workload == "c3" else "ms (p99)",
+        "value_mode": "batched" if args.workload == "c3" else best,
         "n_gpus": 1,
         "higher_is_better": args.workload == "c3",
         "dtype": "f32",
         "data": "synthetic (seeded generators)",
         "config": {"workload": desc, "queue_capacity": args.queue, "paths": paths,
                    "policy": "smallest job first" if args.policy == 1 else "oldest job first"},
-        "batched": s_bat,
-        "sequential": s_seq,
+        **modes,
         "speedup_jobs_per_s": round(s_bat["jobs_per_s"] / s_seq["jobs_per_s"], 3),
-        "images_bit_identical": len(bat) - mismatched,
-        "images_compared": len(bat),
+        "images_mismatched": mismatched,
+        "images_compared": len(bat) * (len(modes) - 1),
     }
     print(json.dumps(out), flush=True)
     return 0 if mismatched == 0 else 1

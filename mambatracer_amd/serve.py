@@ -7,7 +7,7 @@ renderThread -> execute -> loop, src/cl/device/CDeviceGI.java:190-204,
 `api.Batch` (admission by arena footprint happens in the engine, in arrival
 order), keeps stepping the shared queues and timestamps each job when its
 image is complete.  `serve_sequential` runs the same arrivals through plain
-`api.render` calls, first come first served.
+`api.render` calls, first come first served or shortest job first.
 
 A request is a `Request`: arrival time (seconds from the start of the run),
 the resident scene, materials, environment and lights it renders with, and
@@ -92,14 +92,34 @@ def serve_batched(ctx: api.Context, requests: List[Request], queue_capacity: int
     return out, stats
 
 
-def serve_sequential(ctx: api.Context, requests: List[Request], keep_images: bool = False):
-    """One job at a time with plain mt_render launches, first come first served."""
+def _paths(r: Request) -> int:
+    return r.job.width * r.job.height * r.job.spp
+
+
+def serve_sequential(ctx: api.Context, requests: List[Request], keep_images: bool = False, order: str = "fifo"):
+    """One job at a time with plain mt_render launches.  order "fifo": first
+    come first served; "sjf": whenever the GPU frees up, the waiting job with
+    the fewest paths runs next (ties by arrival)."""
+    if order not in ("fifo", "sjf"):
+        raise ValueError(f"unknown order {order!r}")
     reqs = sorted(requests, key=lambda r: r.arrival)
     out: List[Done] = []
     rays = 0
+    waiting: List[Request] = []
+    i = 0
     t0 = time.perf_counter()
-    for r in reqs:
-        _wait_until(t0, r.arrival)
+    while i < len(reqs) or waiting:
+        if not waiting:
+            _wait_until(t0, reqs[i].arrival)
+        now = time.perf_counter() - t0
+        while i < len(reqs) and reqs[i].arrival <= now:
+            waiting.append(reqs[i])
+            i += 1
+        if order == "sjf":
+            k = min(range(len(waiting)), key=lambda j: (_paths(waiting[j]), waiting[j].arrival))
+        else:
+            k = 0
+        r = waiting.pop(k)
         img, st = api.render(ctx, r.scene, r.materials, r.env, r.job, light_faces=r.light_faces,
                              num_lights=r.num_lights, stats=True)   # stats read-back synchronises
         rays += int(st.extension_rays) + int(st.shadow_rays)

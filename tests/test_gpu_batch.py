@@ -161,3 +161,31 @@ def test_batch_rejects_bad_jobs(ctx, world):
     with pytest.raises(_lib.MambartError):
         _lib.call("mt_batch_set_policy", b.handle, 7)
     b.close()
+
+
+def test_serving_loops_agree(ctx, world):
+    """The request loops (mambatracer_amd.serve): batched, first-come and
+    shortest-job-first one-at-a-time serving of the same arrivals return the
+    same images; shortest-job-first starts the smaller of two waiting jobs
+    first."""
+    from mambatracer_amd import serve
+    print("[case] serving loops", flush=True)
+    scenes, dm, env, (lf, nl) = world
+    reqs = []
+    for tag, (sc, job) in enumerate(_jobs()[:6]):
+        reqs.append(serve.Request(0.0 if tag < 3 else 1.0 + 0.01 * tag, scenes[sc], dm, env, job,
+                                  *((lf, nl) if sc == 3 else (None, None)), tag=tag))
+    fifo, _ = serve.serve_sequential(ctx, reqs, keep_images=True)
+    sjf, _ = serve.serve_sequential(ctx, reqs, keep_images=True, order="sjf")
+    bat, _ = serve.serve_batched(ctx, reqs, queue_capacity=1 << 15, keep_images=True)
+    ref = {d.tag: d.image.cpu().numpy() for d in fifo}
+    for done in (sjf, bat):
+        assert sorted(d.tag for d in done) == list(range(6))
+        for d in done:
+            np.testing.assert_array_equal(d.image.cpu().numpy().view(np.int32), ref[d.tag].view(np.int32))
+    # the first three arrive together: shortest job first runs them smallest first
+    first3 = [d.tag for d in sjf][:3]
+    sizes = {r.tag: r.job.width * r.job.height * r.job.spp for r in reqs}
+    assert first3 == sorted([0, 1, 2], key=lambda t: (sizes[t], t))
+    with pytest.raises(ValueError):
+        serve.serve_sequential(ctx, reqs[:1], order="lifo")
