@@ -439,8 +439,8 @@ int32_t iterate(mt_batch* b)
                               b->gen_job, b->s_seg, b->tot);
     k_generate<<<grid, BT, 0, s>>>(b->jobs, b->cnt, b->gen_start, b->gen_pid0, b->gen_job, n_active, q);
     MT_LAUNCH_CHECK();
-    int32_t rc = mt_trace_jobs(b->ctx, false, b->refs, &q.tag->x, BOUNCE_BITS, q.ro, q.rd, b->cnt + 1, Qc, b->prim,
-                               b->t, nullptr, MtSegQueue{nullptr, 0, 0});
+    int32_t rc = mt_trace_jobs(b->ctx, b->refs, &q.tag->x, BOUNCE_BITS, q.ro, q.rd, b->cnt + 1, Qc, b->prim,
+                               b->t);
     if (rc) return rc;
     ShadeJobsArgs A;
     A.jobs = b->jobs;

@@ -205,12 +205,19 @@ __global__ void k_zero_i32(int* p, int n)
     if (i < n) p[i] = 0;
 }
 
+// Totals of the traversal counters live in ILP_SLOTS rows of 8 words
+// ({I, L, P} of closest hit, then of any hit); work-group b adds into row
+// b % ILP_SLOTS, so the waves' atomics do not all meet on three addresses;
+// the host sums the rows.
+constexpr int ILP_SLOTS = 64;
+
 // Sums per-entry traversal counters {I, L, P} of the live entries: i <
 // *d_count, or (seg_counts set) the live entries of a segmented queue.
 __global__ __launch_bounds__(RT) void k_sum_ilp(const int* __restrict__ st, const int* __restrict__ d_count,
                                                 const int* __restrict__ seg_counts, int seg_cap,
-                                                unsigned long long* __restrict__ tot)
+                                                unsigned long long* __restrict__ tot_rows)
 {
+    unsigned long long* tot = tot_rows + 8 * (blockIdx.x % ILP_SLOTS);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long a = 0, b = 0, c = 0;
     if (seg_counts ? seg_live(seg_counts, seg_cap, i) : i < *d_count) {
@@ -306,7 +313,7 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
     R.want_ilp = (desc->flags & MT_RENDER_TRAVERSAL_STATS) != 0;
     R.profile = (desc->flags & MT_RENDER_PROFILE) != 0;
     R.oIlp = R.want_ilp ? take(12 * (R.sq > n ? R.sq : n)) : 0;
-    R.oTot = take(8 * 8);
+    R.oTot = take(8 * 8 * ILP_SLOTS);
     R.oSeed = take(4);
     R.bytes = bytes;
     return MT_OK;
@@ -376,7 +383,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
 
     k_zero_i32<<<1, 64, 0, st>>>(counts, 64);
     k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(shc, 64 * SHADOW_SEGS);
-    k_zero_i32<<<1, 64, 0, st>>>(reinterpret_cast<int*>(tot), 16);
+    k_zero_i32<<<mt_blocks(16 * ILP_SLOTS, RT), RT, 0, st>>>(reinterpret_cast<int*>(tot), 16 * ILP_SLOTS);
     // Fused loop (default traversal, no traversal counters): bounce 0 traces
     // camera rays generated in place and shades with slot = pid; shadow rays
     // add their light inside the any-hit kernel.  Otherwise camera rays are
@@ -473,10 +480,12 @@ int32_t read_stats(mt_context* ctx, RenderPlan& R, mt_render_stats* stats)
     const int nb = R.nb;
     int hc[64];
     std::vector<int> hs((size_t)nb * SHADOW_SEGS);
-    unsigned long long ht[6];
+    std::vector<unsigned long long> ht(8 * ILP_SLOTS);
     hipError_t e = hipMemcpyAsync(hc, R.base + R.oCnt, sizeof(hc), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(hs.data(), R.base + R.oShc, hs.size() * sizeof(int), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(ht, R.base + R.oTot, sizeof(ht), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ht.data(), R.base + R.oTot, ht.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         mt_set_error("mt_render: stats readback failed: %s", hipGetErrorString(e));
@@ -489,8 +498,11 @@ int32_t read_stats(mt_context* ctx, RenderPlan& R, mt_render_stats* stats)
     for (int b = 0; b < nb; b++) stats->extension_rays += hc[b];
     for (int v : hs) stats->shadow_rays += v;
     for (int k = 0; k < 3; k++) {
-        stats->closest_ilp[k] = (int64_t)ht[k];
-        stats->any_ilp[k] = (int64_t)ht[3 + k];
+        stats->closest_ilp[k] = stats->any_ilp[k] = 0;
+        for (int row = 0; row < ILP_SLOTS; row++) {
+            stats->closest_ilp[k] += (int64_t)ht[8 * row + k];
+            stats->any_ilp[k] += (int64_t)ht[8 * row + 3 + k];
+        }
     }
     stats->closest_launches = stats->any_launches = 0;
     stats->closest_ms = stats->any_ms = 0.0f;

@@ -191,7 +191,6 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
 // Rays of many jobs in one queue: ray id's scene is refs[tag[2*id] >> shift]
 // (the job word of a wavefront slot).  Same per-ray traversal as k_trace
 // LAYOUT 0, so a ray's result does not depend on which queue carries it.
-template <bool ANY>
 __global__ __launch_bounds__(256) void k_trace_jobs(const MtSceneRef* __restrict__ refs, const int* __restrict__ tag,
                                                     int shift, TraceArgs A)
 {
@@ -202,13 +201,9 @@ __global__ __launch_bounds__(256) void k_trace_jobs(const MtSceneRef* __restrict
     ray_init(r, A.o_tmin[id], A.d_tmax[id]);
     int st[3];
     int child = -1;
-    int h = trace_packed<ANY, false, true>(R.pn, R.pt, R.root, r, child, st);
-    if (ANY) {
-        A.occluded[id] = h;
-    } else {
-        A.prim[id] = h ? child : -1;
-        A.tout[id] = r.tmax;
-    }
+    int h = trace_packed<false, false, true>(R.pn, R.pt, R.root, r, child, st);
+    A.prim[id] = h ? child : -1;
+    A.tout[id] = r.tmax;
 }
 
 template <bool ANY>
@@ -602,11 +597,10 @@ int32_t mt_trace_any_seg(mt_context* ctx, const mt_scene* scene, const float4* o
                               mode, seg);
 }
 
-int32_t mt_trace_jobs(mt_context* ctx, bool any, const MtSceneRef* refs, const int32_t* tag, int32_t shift,
+int32_t mt_trace_jobs(mt_context* ctx, const MtSceneRef* refs, const int32_t* tag, int32_t shift,
                       const float4* o_tmin, const float4* d_tmax, const int32_t* d_count, int32_t capacity,
-                      int32_t* prim, float* t, int32_t* occluded, MtSegQueue seg)
+                      int32_t* prim, float* t)
 {
-    if (seg.counts) capacity = seg.nseg * seg.seg_cap;
     if (capacity == 0) return MT_OK;
     TraceArgs A{};
     A.o_tmin = o_tmin;
@@ -615,11 +609,8 @@ int32_t mt_trace_jobs(mt_context* ctx, bool any, const MtSceneRef* refs, const i
     A.capacity = capacity;
     A.prim = prim;
     A.tout = t;
-    A.occluded = occluded;
-    A.seg = seg;
     dim3 grid(mt_blocks(capacity, 256)), block(256);
-    if (any) k_trace_jobs<true><<<grid, block, 0, ctx->stream>>>(refs, tag, shift, A);
-    else k_trace_jobs<false><<<grid, block, 0, ctx->stream>>>(refs, tag, shift, A);
+    k_trace_jobs<<<grid, block, 0, ctx->stream>>>(refs, tag, shift, A);
     MT_LAUNCH_CHECK();
     return MT_OK;
 }
