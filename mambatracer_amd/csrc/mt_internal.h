@@ -53,7 +53,7 @@ struct mt_context {
 //   node (64 B): float4 a = (L.min.xyz, L.max.x), b = (L.max.yz, R.min.xy),
 //                float4 c = (R.min.z, R.max.xyz), int4 d = (left, right, parent, sibling)
 //   refs: >= 0 interior node (packed index), < 0 leaf ~rank
-//   tri  (48 B): float4 (p1.xyz, e1.x), (e1.yz, e2.xy), (e2.z, prim bits, 0, 0)
+//   tri  (48 B): float4 (p1.xyz, e1.x), (e1.yz, e2.xy), (e2.z, prim bits, parent, sibling)
 struct mt_scene {
     mt_context* ctx;
     mt_scene_desc desc;

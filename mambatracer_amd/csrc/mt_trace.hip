@@ -18,8 +18,8 @@
 //  * MT_TRACE_PACKED (default) reads a layout derived once per scene: each
 //    interior node is one 64-byte record holding both child boxes, both child
 //    references and its own parent/sibling; each leaf triangle is one 48-byte
-//    record (p1, e1 = p2 - p1, e2 = p3 - p1, primitive id) stored in leaf
-//    order.  e1/e2 are the same float subtractions the reference performs
+//    record (p1, e1 = p2 - p1, e2 = p3 - p1, primitive id, and the leaf's
+//    parent/sibling in its two spare words) stored in leaf order.  e1/e2 are the same float subtractions the reference performs
 //    per test, so every intersection rounds identically.  An interior step
 //    fetches 64 B instead of 148 B, a leaf 48 B instead of 116 B.
 #include "mt_traverse.h"
@@ -336,7 +336,10 @@ __global__ void k_pack(const int* __restrict__ nodes, const float* __restrict__ 
         float4* T = pt + 3 * (size_t)leaf_rank[i];
         T[0] = make_float4(p1.x, p1.y, p1.z, e1.x);
         T[1] = make_float4(e1.y, e1.z, e2.x, e2.y);
-        T[2] = make_float4(e2.z, __int_as_float(c), 0.0f, 0.0f);
+        // spare words: the leaf's parent record and sibling reference (node_links)
+        const int parent = n[4];
+        T[2] = make_float4(e2.z, __int_as_float(c), __int_as_float(parent - leaf_rank[parent]),
+                           __int_as_float(ref(n[1])));
     } else {
         const float* lb = bounds + 8 * nodes[7 * n[2]];
         const float* rb = bounds + 8 * nodes[7 * n[3]];

@@ -103,40 +103,34 @@ struct BitStack {
 
 // Short stack of postponed far children, newest first.  Each 1 bit of the
 // bitstack is a both-children-hit decision whose far child is still to be
-// visited; the stack holds the most recent MT_SHORT_STACK of them with the
-// node at which each was postponed.  Reaching the next pending far child
-// then costs no memory access: the reference's pops (walking parent links
-// up to the lowest 1 bit) are counted as ctz(bitstack) without being
-// fetched.  When older entries have been dropped, the parent walk below
-// finds them exactly as the reference does.
+// visited; the stack holds the most recent MT_SHORT_STACK of them.  Reaching
+// the next pending far child then costs no memory access: the reference's
+// pops (walking parent links up to the lowest 1 bit) are counted as
+// ctz(bitstack) without being fetched.  When older entries have been
+// dropped, the parent walk finds them exactly as the reference does, reading
+// parent and sibling links from the current node's record (an interior
+// record's last word pair, a leaf triangle's spare words), so no links are
+// carried in registers while descending.
 #ifndef MT_SHORT_STACK
 #define MT_SHORT_STACK 8
 #endif
 struct ShortStack {
     int ref[MT_SHORT_STACK];
-    int par[MT_SHORT_STACK];
     int n;
-    __device__ __forceinline__ void push(int r, int p)
+    __device__ __forceinline__ void push(int r)
     {
 #pragma unroll
-        for (int k = MT_SHORT_STACK - 1; k > 0; k--) {
-            ref[k] = ref[k - 1];
-            par[k] = par[k - 1];
-        }
+        for (int k = MT_SHORT_STACK - 1; k > 0; k--) ref[k] = ref[k - 1];
         ref[0] = r;
-        par[0] = p;
         n = n < MT_SHORT_STACK ? n + 1 : MT_SHORT_STACK;
     }
-    __device__ __forceinline__ void pop(int& r, int& p)
+    __device__ __forceinline__ int pop()
     {
-        r = ref[0];
-        p = par[0];
+        const int r = ref[0];
 #pragma unroll
-        for (int k = 0; k < MT_SHORT_STACK - 1; k++) {
-            ref[k] = ref[k + 1];
-            par[k] = par[k + 1];
-        }
+        for (int k = 0; k < MT_SHORT_STACK - 1; k++) ref[k] = ref[k + 1];
         n--;
+        return r;
     }
 };
 
@@ -213,8 +207,37 @@ __device__ __forceinline__ int trace_reference(const int* __restrict__ nodes,
 // reference reads from the node records (leaves are not records here).
 // -------------------------------------------------------------------------
 // Resumable traversal state of one ray (packed layout).
+// Parent and sibling references of node `cur`: an interior record's last
+// word pair, or the spare words of a leaf's triangle record.
+__device__ __forceinline__ int2 node_links(const float4* __restrict__ pn, const float4* __restrict__ pt, int cur)
+{
+    const float4 w = cur >= 0 ? pn[4 * (size_t)cur + 3] : pt[3 * (size_t)(~cur) + 2];
+    return make_int2(__float_as_int(w.z), __float_as_int(w.w));
+}
+
+// The reference's backtracking (NormalBVH.cl:80-90): climb parent links
+// while the lowest bitstack bit is 0, then cross to the sibling and flip the
+// bit.  Returns false when the bitstack empties (the ray is finished).
+template <bool STATS>
+__device__ __forceinline__ bool parent_walk(const float4* __restrict__ pn, const float4* __restrict__ pt, int& cur,
+                                            BitStack& bs, int* st)
+{
+    int2 l = node_links(pn, pt, cur); // (parent, sibling) of cur
+    while ((bs.lo & 1) == 0) {
+        if (bs.empty()) return false;
+        cur = l.x;
+        l = node_links(pn, pt, cur);
+        bs.pop();
+        if (STATS) st[2]++;
+    }
+    cur = l.y;
+    bs.lo ^= 1;
+    return true;
+}
+
+// Resumable traversal state of one ray (packed layout).
 struct TravState {
-    int cur, parent, sib;
+    int cur;
     BitStack bs;
     ShortStack ss;
     int hit, child;
@@ -222,8 +245,6 @@ struct TravState {
     __device__ __forceinline__ void init(int root)
     {
         cur = root;
-        parent = 0;
-        sib = 0;
         bs.lo = 0;
         bs.hi = 0;
         ss.n = 0;
@@ -245,8 +266,6 @@ __device__ __forceinline__ bool trace_packed_step(const float4* __restrict__ pn,
         const float4* N = pn + 4 * (size_t)S.cur;
         float4 a = N[0], b = N[1], c = N[2];
         int4 d = *reinterpret_cast<const int4*>(N + 3);
-        S.parent = d.z;
-        S.sib = d.w;
         float lt, rt;
         bool lh = box_hit(r, a.x, a.y, a.z, a.w, b.x, b.y, lt);
         bool rh = box_hit(r, b.z, b.w, c.x, c.y, c.z, c.w, rt);
@@ -257,14 +276,10 @@ __device__ __forceinline__ bool trace_packed_step(const float4* __restrict__ pn,
         if (lh && rh) {
             bool right_first = rt < lt;
             nxt = right_first ? d.y : d.x;
-            if (SHORT_STACK) S.ss.push(right_first ? d.x : d.y, S.cur);
+            if (SHORT_STACK) S.ss.push(right_first ? d.x : d.y);
             S.bs.lo |= 1;
         } else {
             nxt = lh ? d.x : d.y;
-        }
-        if (nxt < 0) {
-            S.parent = S.cur;
-            S.sib = (nxt == d.x) ? d.y : d.x;
         }
         S.cur = nxt;
     }
@@ -286,23 +301,10 @@ __device__ __forceinline__ bool trace_packed_step(const float4* __restrict__ pn,
         if (STATS) S.st[2] += z;
         S.bs.shr(z);
         S.bs.lo ^= 1;
-        S.ss.pop(S.cur, S.parent);
+        S.cur = S.ss.pop();
         return false;
     }
-    while ((S.bs.lo & 1) == 0) {
-        if (S.bs.empty()) return true;
-        S.cur = S.parent;
-        int2 d = *reinterpret_cast<const int2*>(reinterpret_cast<const int*>(pn + 4 * (size_t)S.cur + 3) + 2);
-        S.parent = d.x;
-        S.sib = d.y;
-        S.bs.pop();
-        if (STATS) S.st[2]++;
-    }
-    int from = S.cur;
-    S.cur = S.sib;
-    S.sib = from;
-    S.bs.lo ^= 1;
-    return false;
+    return !parent_walk<STATS>(pn, pt, S.cur, S.bs, S.st);
 }
 
 template <bool ANY, bool STATS, bool SHORT_STACK>
@@ -315,14 +317,11 @@ __device__ __forceinline__ int trace_packed(const float4* __restrict__ pn,
     BitStack bs{0, 0};
     ShortStack ss;
     ss.n = 0;
-    int parent = 0, sib = 0;
     for (;;) {
         while (cur >= 0) {
             const float4* N = pn + 4 * (size_t)cur;
             float4 a = N[0], b = N[1], c = N[2];
             int4 d = *reinterpret_cast<const int4*>(N + 3);
-            parent = d.z;
-            sib = d.w;
             float lt, rt;
             bool lh = box_hit(r, a.x, a.y, a.z, a.w, b.x, b.y, lt);
             bool rh = box_hit(r, b.z, b.w, c.x, c.y, c.z, c.w, rt);
@@ -333,14 +332,10 @@ __device__ __forceinline__ int trace_packed(const float4* __restrict__ pn,
             if (lh && rh) {
                 bool right_first = rt < lt;
                 nxt = right_first ? d.y : d.x;
-                if (SHORT_STACK) ss.push(right_first ? d.x : d.y, cur);
+                if (SHORT_STACK) ss.push(right_first ? d.x : d.y);
                 bs.lo |= 1;
             } else {
                 nxt = lh ? d.x : d.y;
-            }
-            if (nxt < 0) { // entering a leaf: its parent is cur, its sibling the other child
-                parent = cur;
-                sib = (nxt == d.x) ? d.y : d.x;
             }
             cur = nxt;
         }
@@ -363,22 +358,10 @@ __device__ __forceinline__ int trace_packed(const float4* __restrict__ pn,
             if (STATS) st[2] += z;
             bs.shr(z);
             bs.lo ^= 1;
-            ss.pop(cur, parent); // a leaf keeps its parent for a later parent walk
+            cur = ss.pop();
             continue;
         }
-        while ((bs.lo & 1) == 0) {
-            if (bs.empty()) return hit;
-            cur = parent;
-            int2 d = *reinterpret_cast<const int2*>(reinterpret_cast<const int*>(pn + 4 * (size_t)cur + 3) + 2);
-            parent = d.x;
-            sib = d.y;
-            bs.pop();
-            if (STATS) st[2]++;
-        }
-        int from = cur;
-        cur = sib;
-        sib = from;
-        bs.lo ^= 1;
+        if (!parent_walk<STATS>(pn, pt, cur, bs, st)) return hit;
     }
 }
 
