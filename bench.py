@@ -310,9 +310,11 @@ def bench_c2(args) -> int:
     P = lambda x: C.c_void_p(x.data_ptr())
     h = ctx.handle
 
+    modes = {"closest": 0, "any": 0}
+
     def closest(st=None):
         api.call("mt_trace_closest", h, scene.handle, P(do), P(dd), P(length), n, P(prim), P(tt),
-                 P(st) if st is not None else None, 0)
+                 P(st) if st is not None else None, 0 if st is not None else modes["closest"])
 
     def shadow():
         api.call("mt_shadow_rays", h, P(do), P(dd), P(prim), P(tt), light.ctypes.data_as(C.c_void_p), P(length), n,
@@ -320,7 +322,7 @@ def bench_c2(args) -> int:
 
     def anyhit(st=None):
         api.call("mt_trace_any", h, scene.handle, P(so), P(sd), P(length), n, P(occ),
-                 P(st) if st is not None else None, 0)
+                 P(st) if st is not None else None, 0 if st is not None else modes["any"])
 
     def compact():
         api.call("mt_compact", h, P(prim), 1, P(length), n, P(idx), P(cnt), pay, 2)
@@ -337,6 +339,18 @@ def bench_c2(args) -> int:
     alg_a = 36 * n + 148 * ilp_a[0] + 116 * ilp_a[1] + 28 * ilp_a[2]
     log(f"closest: I {ilp_c[0] / n:.1f} L {ilp_c[1] / n:.1f} P {ilp_c[2] / n:.1f} per ray ({alg_c / n:.0f} B/ray); "
         f"any: I {ilp_a[0] / n:.1f} L {ilp_a[1] / n:.1f} P {ilp_a[2] / n:.1f} per ray")
+
+    # untimed: the traversal mode of each stage, picked by the autotuner on
+    # this batch (every mode returns the same bits)
+    for key, any_hit, (o_, d_) in (("closest", False, (do, dd)), ("any", True, (so, sd))):
+        m, ms = C.c_int32(0), C.c_float(0.0)
+        api.call("mt_trace_autotune", h, scene.handle, P(o_), P(d_), P(length), n, 1 if any_hit else 0,
+                 P(prim) if not any_hit else None, P(tt) if not any_hit else None, P(occ) if any_hit else None,
+                 2, C.byref(m), C.byref(ms))
+        modes[key] = int(m.value)
+    log(f"autotuned traversal modes: {modes}")
+    closest()   # restore this batch's closest hits (the autotuner's last run may have been any hit)
+    shadow()
 
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
     stage = np.zeros(4)
@@ -391,14 +405,17 @@ def bench_c2(args) -> int:
         "config": {"workload": f"c2: {faces.shape[0]} triangles (3.6M tiny in a 0.1 cube + 0.4M large in a "
                                f"200-unit box), {n} rays uniform in the scene box, uniform directions; closest hit, "
                                "a shadow ray per ray to one light, any hit, stable compaction of the hit rays",
-                   "rays": n, "traversal_layout": "packed, short stack"},
+                   "rays": n, "traversal_layout": "packed",
+                   "traversal_modes": {k: {3: "short stack", 4: "persistent waves with per-lane refill"}.get(v, v)
+                                       for k, v in modes.items()}},
         "stages_ms": {"closest": round(stage[0], 3), "shadow_setup": round(stage[1], 3),
                       "any": round(stage[2], 3), "compaction": round(stage[3], 4)},
         "closest_Grays_per_s": round(n / stage[0] / 1e6, 4),
         "any_Grays_per_s": round(n / stage[2] / 1e6, 4),
         "compaction_GB_per_s": round(cbytes / stage[3] / 1e6, 1),
         "roofline": {
-            "kernel": "closest-hit traversal (k_trace, packed layout)",
+            "kernel": "closest-hit traversal (packed layout, "
+                      + ("k_trace_refill" if modes["closest"] == 4 else "k_trace") + ")",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

@@ -218,6 +218,21 @@ int32_t mt_trace_any(mt_context* ctx, const mt_scene* scene,
                      const int32_t* d_count, int32_t capacity,
                      int32_t* occluded, int32_t* stats, int32_t mode);
 
+/* Traversal autotuner (no counterpart in the reference, which has one
+ * traversal kernel).  Every packed mode returns the same bits, but their
+ * speed depends on the batch: one ray per lane with the short stack wins on
+ * coherent path-tracing bounces, persistent waves with per-lane refill on
+ * long incoherent batches.  Traces this batch (any = 0: closest hit into
+ * prim/t; any != 0: any hit into occluded) with MT_TRACE_PACKED_STACK and
+ * MT_TRACE_PACKED_REFILL, `reps` timed runs each after one warm-up, times
+ * them with HIP events on the context stream and sets *best_mode to the
+ * faster and *best_ms to its mean time.  The outputs hold the batch's
+ * result.  Synchronises the context. */
+int32_t mt_trace_autotune(mt_context* ctx, const mt_scene* scene, const float* o_tmin,
+                          const float* d_tmax, const int32_t* d_count, int32_t capacity,
+                          int32_t any, int32_t* prim, float* t, int32_t* occluded, int32_t reps,
+                          int32_t* best_mode, float* best_ms);
+
 /* IntersectPrimitives' hit shading (NormalBVH.cl:124-232) for slots with
  * prim >= 0: p (after the shadow-terminator offset), shading normal n,
  * geometric normal ng (float4 each), uv (float2), material id (-1 on

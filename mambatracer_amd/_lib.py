@@ -180,6 +180,7 @@ SIGNATURES = {
     "mt_bsdf_setup": [P, I32, P, P, P, P, P, P, P],
     "mt_bsdf_sample": [P, I32, P, P, P, P, P, P, P, P],
     "mt_bsdf_eval": [P, I32, P, P, P, P, P, P, P],
+    "mt_trace_autotune": [P, P, P, P, P, I32, I32, P, P, P, I32, C.POINTER(C.c_int32), C.POINTER(C.c_float)],
     "mt_render": [P, P, P, P, P, P, C.POINTER(RenderDesc), P, C.POINTER(RenderStats)],
     "mt_render_graph_create": [P, P, P, P, P, P, C.POINTER(RenderDesc), P, PP],
     "mt_render_graph_launch": [P, C.c_uint32],

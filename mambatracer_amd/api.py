@@ -258,6 +258,28 @@ def intersect_occlusion(ctx: Context, scene: Scene, o_tmin, d_tmax, count=None, 
     return occ, st
 
 
+def autotune_trace(ctx: Context, scene: Scene, o_tmin, d_tmax, any_hit: bool = False, count=None, reps: int = 2):
+    """Fastest packed traversal mode for this batch (mt_trace_autotune):
+    returns (mode, ms per launch); every mode gives the same results."""
+    torch = _torch()
+    n = o_tmin.shape[0]
+    _need(o_tmin, "o_tmin", n, (4,))
+    _need(d_tmax, "d_tmax", n, (4,))
+    if count is None:
+        count = ctx.scalar_i32(n)
+    prim = t = occ = None
+    if any_hit:
+        occ = ctx.empty((n,), torch.int32)
+    else:
+        prim = ctx.empty((n,), torch.int32)
+        t = ctx.empty((n,), torch.float32)
+    mode = C.c_int32(-1)
+    ms = C.c_float(0.0)
+    call("mt_trace_autotune", ctx.handle, scene.handle, _ptr(o_tmin), _ptr(d_tmax), _ptr(count), n,
+         1 if any_hit else 0, _ptr(prim), _ptr(t), _ptr(occ), int(reps), C.byref(mode), C.byref(ms))
+    return int(mode.value), float(ms.value)
+
+
 def shade_hits(ctx: Context, scene: Scene, o_tmin, d_tmax, prim, t, count=None):
     torch = _torch()
     n = o_tmin.shape[0]

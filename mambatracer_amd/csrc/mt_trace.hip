@@ -591,6 +591,60 @@ extern "C" int32_t mt_trace_any(mt_context* ctx, const mt_scene* scene, const fl
                               occluded, stats, mode);
 }
 
+extern "C" int32_t mt_trace_autotune(mt_context* ctx, const mt_scene* scene, const float* o_tmin,
+                                     const float* d_tmax, const int32_t* d_count, int32_t capacity, int32_t any,
+                                     int32_t* prim, float* t, int32_t* occluded, int32_t reps, int32_t* best_mode,
+                                     float* best_ms)
+{
+    MT_ARG_CHECK(ctx && scene && best_mode && best_ms, "mt_trace_autotune: NULL argument");
+    MT_ARG_CHECK(reps >= 1 && reps <= 100, "mt_trace_autotune: reps %d not in [1,100]", reps);
+    MT_ARG_CHECK(any ? occluded != nullptr : (prim && t), "mt_trace_autotune: NULL outputs");
+    const int32_t modes[2] = {MT_TRACE_PACKED_STACK, MT_TRACE_PACKED_REFILL};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MT_HIP_CHECK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        mt_set_error("mt_trace_autotune: hipEventCreate failed");
+        return MT_ERR_HIP;
+    }
+    auto run = [&](int32_t mode) -> int32_t {
+        return any ? launch_trace<true>(ctx, scene, o_tmin, d_tmax, d_count, capacity, nullptr, nullptr, occluded,
+                                        nullptr, mode)
+                   : launch_trace<false>(ctx, scene, o_tmin, d_tmax, d_count, capacity, prim, t, nullptr, nullptr,
+                                         mode);
+    };
+    int32_t rc = MT_OK;
+    float best = -1.0f;
+    int32_t pick = modes[0];
+    for (int32_t mode : modes) {
+        rc = run(mode); // warm-up
+        if (rc) break;
+        hipError_t e = hipEventRecord(e0, ctx->stream);
+        for (int k = 0; k < reps && rc == MT_OK; k++) rc = run(mode);
+        if (rc) break;
+        if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            mt_set_error("mt_trace_autotune: timing failed: %s", hipGetErrorString(e));
+            rc = MT_ERR_HIP;
+            break;
+        }
+        ms /= (float)reps;
+        if (best < 0.0f || ms < best) {
+            best = ms;
+            pick = mode;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    *best_mode = pick;
+    *best_ms = best;
+    return MT_OK;
+}
+
 int32_t mt_trace_any_seg(mt_context* ctx, const mt_scene* scene, const float4* o_tmin, const float4* d_tmax,
                          MtSegQueue seg, int32_t* occluded, int32_t* stats, int32_t mode)
 {

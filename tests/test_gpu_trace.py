@@ -132,3 +132,27 @@ def test_shadow_rays_and_compaction_pipeline(ctx, c0):
     assert int(cnt.item()) == ocnt
     np.testing.assert_array_equal(idx.cpu().numpy()[:ocnt], oidx)
     ctx.check_scans()
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_autotune_picks_a_packed_mode_with_identical_results(ctx, c0, any_hit):
+    """mt_trace_autotune times the short-stack and refill modes on the batch;
+    whichever it picks, its outputs equal the oracle's (every mode is exact)."""
+    print(f"[case] autotune any={any_hit}", flush=True)
+    points, faces, nodes, bounds, _ = c0
+    o, d = c0_rays(256, 256)
+    sc = _scene(ctx, c0)
+    do, dd = ctx.to_device(o), ctx.to_device(d)
+    mode, ms = api.autotune_trace(ctx, sc, do, dd, any_hit=any_hit, reps=2)
+    assert mode in (api._lib.MT_TRACE_PACKED_STACK, api._lib.MT_TRACE_PACKED_REFILL) and ms > 0
+    if any_hit:
+        occ, _ = api.intersect_occlusion(ctx, sc, do, dd, mode=mode)
+        oocc, _, _ = orc.trace_any(points, faces, nodes, bounds, o, d)
+        np.testing.assert_array_equal(occ.cpu().numpy(), oocc)
+    else:
+        prim, t, _ = api.intersect_primitives(ctx, sc, do, dd, mode=mode)
+        oprim, ot, _, _ = orc.trace_closest(points, faces, nodes, bounds, o, d)
+        np.testing.assert_array_equal(prim.cpu().numpy(), oprim)
+        np.testing.assert_array_equal(t.cpu().numpy().view(np.int32), ot.view(np.int32))
+    with pytest.raises(RuntimeError, match="reps"):
+        api.autotune_trace(ctx, sc, do, dd, any_hit=any_hit, reps=0)
