@@ -127,53 +127,101 @@ __device__ __forceinline__ bool run_tail(int s, int lane)
     return lane == 63 || ns != s;
 }
 
+// Each wave reduces BOUND_ROWS consecutive rows of 64 positions.  A run of
+// equal node id still open at the end of a row is carried into the next row
+// instead of being flushed, so a node gets one set of atomics per run per
+// wave, not per 64 positions: near the root, where one node spans every
+// position, 16x fewer atomics land on the same addresses.  Min/max and sums
+// do not depend on the order, so results are unchanged.
+constexpr int BOUND_ROWS = 16;
+
+// Lanes [0, result) of the row hold its first run.
+__device__ __forceinline__ int first_change(int s, int lane)
+{
+    const int prev = __shfl_up(s, 1);
+    const uint64_t ch = __ballot(lane > 0 && prev != s);
+    return ch ? __builtin_ctzll(ch) : 64;
+}
+
+// Keys 0-2 and 6-8 are minima, 3-5 and 9-11 maxima.
+__device__ __forceinline__ int kcomb(int k, int x, int y)
+{
+    return ((k / 3) & 1) ? (x > y ? x : y) : (x < y ? x : y);
+}
+
+__device__ __forceinline__ void flush_keys(const BuildState& S, int s, const int (&v)[12])
+{
+    int* K = S.key + 12 * (size_t)s;
+    for (int k = 0; k < 3; k++) atomicMin(&K[k], v[k]);
+    for (int k = 3; k < 6; k++) atomicMax(&K[k], v[k]);
+    for (int k = 6; k < 9; k++) atomicMin(&K[k], v[k]);
+    for (int k = 9; k < 12; k++) atomicMax(&K[k], v[k]);
+}
+
 __global__ __launch_bounds__(BB) void k_bounds(BuildState S)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = __lane_id();
-    int s = -1;
-    float4 a = make_float4(0, 0, 0, 0), b = a, c = a;
-    if (i < S.n) {
-        s = S.seg[i];
-        if (s >= 0) {
-            int p = S.obj[i];
-            a = S.pmn[p];
-            b = S.pmx[p];
-            c = S.cen[p];
-        }
-    }
-    if (__ballot(s >= 0) == 0) return;
+    const int64_t base = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * BOUND_ROWS;
+    if (base >= S.n) return; // wave-uniform
     auto mn = [](int x, int y) { return x < y ? x : y; };
     auto mx = [](int x, int y) { return x > y ? x : y; };
-    int v[12];
-    v[0] = mn(fkey(a.x), fkey(b.x));
-    v[1] = mn(fkey(a.y), fkey(b.y));
-    v[2] = mn(fkey(a.z), fkey(b.z));
-    v[3] = mx(fkey(a.x), fkey(b.x));
-    v[4] = mx(fkey(a.y), fkey(b.y));
-    v[5] = mx(fkey(a.z), fkey(b.z));
-    v[6] = fkey(c.x);
-    v[7] = fkey(c.y);
-    v[8] = fkey(c.z);
-    v[9] = v[6];
-    v[10] = v[7];
-    v[11] = v[8];
+    int cs = -1; // node of the run carried from the previous row (wave-uniform)
+    int cv[12];
+    for (int r = 0; r < BOUND_ROWS; r++) {
+        const int64_t row = base + (int64_t)r * 64;
+        if (row >= S.n) break; // wave-uniform
+        const int64_t i = row + lane;
+        int s = -1;
+        float4 a = make_float4(0, 0, 0, 0), b = a, c = a;
+        if (i < S.n) {
+            s = S.seg[i];
+            if (s >= 0) {
+                int p = S.obj[i];
+                a = S.pmn[p];
+                b = S.pmx[p];
+                c = S.cen[p];
+            }
+        }
+        int v[12];
+        v[0] = mn(fkey(a.x), fkey(b.x));
+        v[1] = mn(fkey(a.y), fkey(b.y));
+        v[2] = mn(fkey(a.z), fkey(b.z));
+        v[3] = mx(fkey(a.x), fkey(b.x));
+        v[4] = mx(fkey(a.y), fkey(b.y));
+        v[5] = mx(fkey(a.z), fkey(b.z));
+        v[6] = fkey(c.x);
+        v[7] = fkey(c.y);
+        v[8] = fkey(c.z);
+        v[9] = v[6];
+        v[10] = v[7];
+        v[11] = v[8];
 #pragma unroll
-    for (int k = 0; k < 3; k++) v[k] = seg_scan(v[k], s, lane, mn);
+        for (int k = 0; k < 3; k++) v[k] = seg_scan(v[k], s, lane, mn);
 #pragma unroll
-    for (int k = 3; k < 6; k++) v[k] = seg_scan(v[k], s, lane, mx);
+        for (int k = 3; k < 6; k++) v[k] = seg_scan(v[k], s, lane, mx);
 #pragma unroll
-    for (int k = 6; k < 9; k++) v[k] = seg_scan(v[k], s, lane, mn);
+        for (int k = 6; k < 9; k++) v[k] = seg_scan(v[k], s, lane, mn);
 #pragma unroll
-    for (int k = 9; k < 12; k++) v[k] = seg_scan(v[k], s, lane, mx);
-    const bool tail = run_tail(s, lane); // all lanes take part in the shuffle
-    if (s >= 0 && tail) {
-        int* K = S.key + 12 * (size_t)s;
-        for (int k = 0; k < 3; k++) atomicMin(&K[k], v[k]);
-        for (int k = 3; k < 6; k++) atomicMax(&K[k], v[k]);
-        for (int k = 6; k < 9; k++) atomicMin(&K[k], v[k]);
-        for (int k = 9; k < 12; k++) atomicMax(&K[k], v[k]);
+        for (int k = 9; k < 12; k++) v[k] = seg_scan(v[k], s, lane, mx);
+        const int fc = first_change(s, lane);
+        const int s0 = __shfl(s, 0);
+        if (cs >= 0) {
+            if (s0 == cs) {
+                if (lane < fc) {
+#pragma unroll
+                    for (int k = 0; k < 12; k++) v[k] = kcomb(k, v[k], cv[k]);
+                }
+            } else if (lane == 0) {
+                flush_keys(S, cs, cv);
+            }
+        }
+        const bool tail = run_tail(s, lane); // all lanes take part in the shuffle
+        if (s >= 0 && tail && lane != 63) flush_keys(S, s, v);
+        cs = __shfl(s, 63);
+#pragma unroll
+        for (int k = 0; k < 12; k++) cv[k] = __shfl(v[k], 63);
     }
+    if (cs >= 0 && lane == 0) flush_keys(S, cs, cv);
 }
 
 __global__ void k_node_a(BuildState S, const int* __restrict__ list, int count)
@@ -209,27 +257,46 @@ __global__ void k_node_a(BuildState S, const int* __restrict__ list, int count)
 }
 
 __global__ __launch_bounds__(BB) void k_flags(BuildState S)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+{ // rows and carried runs as in k_bounds; the left count per node is a sum
     const int lane = __lane_id();
-    int s = -1, right = 0;
-    if (i < S.n) {
-        s = S.seg[i];
-        if (s >= 0 && S.nkind[s] == 0) {
-            const float4 c = S.cen[S.obj[i]];
-            const int dim = S.naxis[s];
-            const float v = dim == 0 ? c.x : (dim == 1 ? c.y : c.z);
-            right = !(v < S.nsplit[s]) ? 1 : 0; // the partition keeps centroid < split on the left
-        } else {
-            s = -1;
+    const int64_t base = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * BOUND_ROWS;
+    if (base >= S.n) return; // wave-uniform
+    auto add = [](int x, int y) { return x + y; };
+    int cs = -1, cv = 0;
+    for (int r = 0; r < BOUND_ROWS; r++) {
+        const int64_t row = base + (int64_t)r * 64;
+        if (row >= S.n) break; // wave-uniform
+        const int64_t i = row + lane;
+        int s = -1, right = 0;
+        if (i < S.n) {
+            s = S.seg[i];
+            if (s >= 0 && S.nkind[s] == 0) {
+                const float4 c = S.cen[S.obj[i]];
+                const int dim = S.naxis[s];
+                const float v = dim == 0 ? c.x : (dim == 1 ? c.y : c.z);
+                right = !(v < S.nsplit[s]) ? 1 : 0; // the partition keeps centroid < split on the left
+            } else {
+                s = -1;
+            }
+            S.flag[i] = right;
         }
-        S.flag[i] = right;
+        int left = (s >= 0 && !right) ? 1 : 0;
+        left = seg_scan(left, s, lane, add);
+        const int fc = first_change(s, lane);
+        const int s0 = __shfl(s, 0);
+        if (cs >= 0) {
+            if (s0 == cs) {
+                if (lane < fc) left += cv;
+            } else if (lane == 0 && cv) {
+                atomicAdd(&S.nleft[cs], cv);
+            }
+        }
+        const bool tail = run_tail(s, lane); // all lanes take part in the shuffle
+        if (s >= 0 && tail && lane != 63 && left) atomicAdd(&S.nleft[s], left);
+        cs = __shfl(s, 63);
+        cv = __shfl(left, 63);
     }
-    if (__ballot(s >= 0) == 0) return;
-    int left = (s >= 0 && !right) ? 1 : 0;
-    left = seg_scan(left, s, lane, [](int x, int y) { return x + y; });
-    const bool tail = run_tail(s, lane); // all lanes take part in the shuffle
-    if (s >= 0 && tail && left) atomicAdd(&S.nleft[s], left);
+    if (cs >= 0 && lane == 0 && cv) atomicAdd(&S.nleft[cs], cv);
 }
 
 __global__ void k_node_b(BuildState S, const int* __restrict__ list, int count, int* __restrict__ next,
@@ -373,9 +440,9 @@ extern "C" int32_t mt_bvh_build_device(mt_context* ctx, int32_t ntris, const flo
     while (rc == MT_OK && count > 0) {
         const int lg = mt_blocks(count, BB);
         k_level_init<<<lg, BB, 0, st>>>(S, list[cur], count);
-        k_bounds<<<grid, BB, 0, st>>>(S);
+        k_bounds<<<mt_blocks(n, BB * BOUND_ROWS), BB, 0, st>>>(S);
         k_node_a<<<lg, BB, 0, st>>>(S, list[cur], count);
-        k_flags<<<grid, BB, 0, st>>>(S);
+        k_flags<<<mt_blocks(n, BB * BOUND_ROWS), BB, 0, st>>>(S);
         rc = mt_scan_excl_i32(ctx, S.flag, S.scan, cnt + 1, ntris, S.scan + n);
         if (rc) break;
         k_zero1<<<1, 64, 0, st>>>(cnt);
