@@ -299,44 +299,64 @@ __global__ __launch_bounds__(BB) void k_flags(BuildState S)
     if (cs >= 0 && lane == 0 && cv) atomicAdd(&S.nleft[cs], cv);
 }
 
-__global__ void k_node_b(BuildState S, const int* __restrict__ list, int count, int* __restrict__ next,
-                         int* __restrict__ next_count)
+// The children of the level's split nodes are appended to the next level's
+// list with one atomic per wave (the list's order does not matter: every
+// node's numbering comes from its range, not from its place in the list).
+__global__ __launch_bounds__(BB) void k_node_b(BuildState S, const int* __restrict__ list, int count,
+                                               int* __restrict__ next, int* __restrict__ next_count)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    int s = list[t];
-    if (S.nkind[s] == 1) return;
-    int start = S.nstart[s], end = S.nend[s], base = S.nbase[s];
-    int nl = S.nleft[s];
-    int mid = start + nl;
-    if (mid == start || mid == end) { // one side empty: no swaps, split the range in half
-        mid = start + (end - start) / 2;
-        S.nkind[s] = 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int64_t)blockIdx.x * blockDim.x >= count) return; // whole work-group idle
+    int s = -1;
+    bool split = false;
+    if (t < count) {
+        s = list[t];
+        split = S.nkind[s] != 1;
     }
-    S.nmid[s] = mid;
-    const int li = base, ri = base + 1;
-    int* pn = S.nodes + 7 * (size_t)s;
-    pn[2] = li;
-    pn[3] = ri;
-    int* ln = S.nodes + 7 * (size_t)li;
-    int* rn = S.nodes + 7 * (size_t)ri;
-    for (int k = 0; k < 7; k++) {
-        ln[k] = 0;
-        rn[k] = 0;
+    int li = 0, ri = 0;
+    if (split) {
+        int start = S.nstart[s], end = S.nend[s], base = S.nbase[s];
+        int nl = S.nleft[s];
+        int mid = start + nl;
+        if (mid == start || mid == end) { // one side empty: no swaps, split the range in half
+            mid = start + (end - start) / 2;
+            S.nkind[s] = 2;
+        }
+        S.nmid[s] = mid;
+        li = base;
+        ri = base + 1;
+        int* pn = S.nodes + 7 * (size_t)s;
+        pn[2] = li;
+        pn[3] = ri;
+        int* ln = S.nodes + 7 * (size_t)li;
+        int* rn = S.nodes + 7 * (size_t)ri;
+        for (int k = 0; k < 7; k++) {
+            ln[k] = 0;
+            rn[k] = 0;
+        }
+        ln[4] = s;
+        rn[4] = s;
+        ln[1] = ri;
+        rn[1] = li;
+        S.nstart[li] = start;
+        S.nend[li] = mid;
+        S.nbase[li] = base + 2;
+        S.nstart[ri] = mid;
+        S.nend[ri] = end;
+        S.nbase[ri] = base + 2 * (mid - start);
     }
-    ln[4] = s;
-    rn[4] = s;
-    ln[1] = ri;
-    rn[1] = li;
-    S.nstart[li] = start;
-    S.nend[li] = mid;
-    S.nbase[li] = base + 2;
-    S.nstart[ri] = mid;
-    S.nend[ri] = end;
-    S.nbase[ri] = base + 2 * (mid - start);
-    int w = atomicAdd(next_count, 2);
-    next[w] = li;
-    next[w + 1] = ri;
+    const uint64_t m = __ballot(split); // whole wave active here
+    int wbase = 0;
+    if (m) {
+        if (__lane_id() == 0) wbase = atomicAdd(next_count, 2 * (int)__popcll(m));
+        wbase = __shfl(wbase, 0);
+    }
+    if (split) {
+        const int w = wbase + 2 * (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        next[w] = li;
+        next[w + 1] = ri;
+    }
 }
 
 // Right part [mid, end): the k-th left element counted from the right
