@@ -93,10 +93,12 @@ def main() -> int:
     ap.add_argument("--mode", type=int, default=0, help="0 packed traversal layout, 1 reference layout")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c1",
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "compact", "bvh"], default="c1",
                     help="c1: one 1080p render per step (default line); c2: 64M random rays through 4M "
                          "triangles (closest hit, shadow rays, any hit, compaction); c3: 256 uneven jobs "
-                         "batched vs one at a time; c4: Poisson-arrival serving loop")
+                         "batched vs one at a time; c4: Poisson-arrival serving loop; compact: stable "
+                         "compaction alone on the shapes the paths use; bvh: the device BVH builder on the "
+                         "c1 and c2 scenes")
     ap.add_argument("--rays", type=int, default=64 << 20, help="c2: rays per step")
     ap.add_argument("--jobs", type=int, default=256, help="c3: number of jobs")
     ap.add_argument("--rate", type=float, default=40.0, help="c4: arrivals per second")
@@ -130,6 +132,10 @@ def main() -> int:
         return 0
     if args.workload == "c2":
         return bench_c2(args)
+    if args.workload == "compact":
+        return bench_compact(args)
+    if args.workload == "bvh":
+        return bench_bvh(args)
     if args.workload != "c1":
         return bench_jobs(args)
     t0 = time.perf_counter()
@@ -430,6 +436,129 @@ def bench_c2(args) -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_compact(args) -> int:
+    """Stable compaction (mt_compact) alone, outputs preallocated, on the
+    shapes the paths use: the 33M-slot render queue after bounce 0 (45%
+    survivors; 4 + 16 + 16 B payloads), the c2 hit rays (16M, 87%; 2 x 16 B
+    + index) and a multi-job queue (8M, 60%; 8 + 3 x 16 B).  Each case is
+    checked against torch's nonzero + gather and timed with HIP events.
+    Bytes: flags + payloads read for every slot, payloads (+ index) written
+    for the survivors; a plain device copy of 1 GiB is timed beside them."""
+    import ctypes as C
+
+    import torch
+
+    from mambatracer_amd import _lib, api
+
+    ctx = api.Context(0)
+    dev = ctx.torch_device
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+
+    def ev_time(f, reps=10):
+        best = 1e9
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            f()
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        return best
+
+    a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    copy_gbs = 2 * a.numel() * 4 / ev_time(lambda: b.copy_(a)) / 1e6
+    del a, b
+    cases = {}
+    ok_all = True
+    for name, n, p, sizes, index in [("render_queue_33M", 33177600, 0.45, (4, 16, 16), False),
+                                     ("c2_hits_16M", 1 << 24, 0.87, (16, 16), True),
+                                     ("batch_queue_8M", 8 << 20, 0.6, (8, 16, 16, 16), False)]:
+        flags = (torch.rand(n, generator=g, device=dev) < p).to(torch.int32)
+        srcs = [torch.empty((n, sz // 4), dtype=torch.int32, device=dev).random_(generator=g) for sz in sizes]
+        dsts = [torch.empty_like(x) for x in srcs]
+        idx = torch.empty(n, dtype=torch.int32, device=dev) if index else None
+        length, cnt = ctx.scalar_i32(n), ctx.scalar_i32(0)
+        arr = (_lib.Payload * len(srcs))()
+        for j, (x, y) in enumerate(zip(srcs, dsts)):
+            arr[j] = _lib.Payload(x.data_ptr(), y.data_ptr(), x.shape[1] * 4)
+        ip = C.c_void_p(idx.data_ptr()) if index else None
+        f = lambda: api.call("mt_compact", ctx.handle, C.c_void_p(flags.data_ptr()), 0, C.c_void_p(length.data_ptr()),
+                             n, ip, C.c_void_p(cnt.data_ptr()), arr, len(srcs))
+        f()
+        torch.cuda.synchronize()
+        k = int(cnt.item())
+        sel = flags.nonzero().squeeze(1)
+        ok = k == sel.numel() and all(torch.equal(y[:k], x[sel]) for x, y in zip(srcs, dsts))
+        if index:
+            ok = ok and torch.equal(idx[:k], sel.to(torch.int32))
+        ok_all &= ok
+        ms = ev_time(f)
+        psum = sum(sizes)
+        byts = n * (4 + psum) + k * (psum + (4 if index else 0))
+        cases[name] = {"ms": round(ms, 4), "GB_per_s": round(byts / ms / 1e6, 1), "survivors": k, "n": n,
+                       "payload_bytes": list(sizes), "index": index, "correct": ok}
+        log(f"{name}: {cases[name]}")
+        del flags, srcs, dsts, idx
+    ctx.check_scans()
+    best = max(c["GB_per_s"] for c in cases.values())
+    out = {"metric": "stable compaction bandwidth (flags + payload reads, survivor writes)", "value": best,
+           "unit": "GB/s", "n_gpus": 1, "higher_is_better": True, "dtype": "int32/bytes", "data": "synthetic",
+           "config": {"workload": "mt_compact on three queue shapes, outputs preallocated"},
+           "cases": cases, "copy_1GiB_GB_per_s": round(copy_gbs, 1),
+           "roofline": {"bound": "hbm", "achieved": best, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(best / HBM_PEAK_GBS, 4)}}
+    print(json.dumps(out), flush=True)
+    return 0 if ok_all else 1
+
+
+def bench_bvh(args) -> int:
+    """The device BVH builder (mt_bvh_build_device, CNormalBVH.build's
+    midpoint split) on the c1 (1,048,578 triangles) and c2 (4,000,000)
+    scenes from device points/faces, against the host builder; the device
+    nodes must equal the host builder's.  Best of 3 builds each."""
+    import numpy as np
+    import torch
+
+    from mambatracer_amd import api
+
+    ctx = api.Context(0)
+    res = {}
+    ok_all = True
+    for name in ("c1", "c2"):
+        if name == "c1":
+            points, _, faces = api.gen_spheres(2, 256)
+        else:
+            tp, tf = api.gen_soup(3, 3_600_000, -0.05, 0.05, 1e-4, 1e-3, first_vertex=0)
+            lp, lf = api.gen_soup(1003, 400_000, -100.0, 100.0, 1.0, 10.0, first_vertex=tp.shape[0])
+            points, faces = np.concatenate([tp, lp]), np.concatenate([tf, lf])
+        t0 = time.perf_counter()
+        hn, hb, hd = api.CNormalBVH.build(points, faces)
+        host_ms = (time.perf_counter() - t0) * 1e3
+        dp, df = ctx.to_device(points), ctx.to_device(faces)
+        api.CNormalBVH.build_device(ctx, dp, df)
+        torch.cuda.synchronize()
+        best = 1e9
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dn, db, dd = api.CNormalBVH.build_device(ctx, dp, df)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        same = bool(np.array_equal(dn.cpu().numpy(), hn)) and dd == hd
+        ok_all &= same
+        res[name] = {"triangles": int(faces.shape[0]), "device_ms": round(best * 1e3, 2),
+                     "host_ms_multithreaded": round(host_ms, 1), "depth": dd, "same_nodes_as_host": same}
+        log(f"{name}: {res[name]}")
+    out = {"metric": "device BVH build time (midpoint split, reference node order)", "value": res["c2"]["device_ms"],
+           "unit": "ms (4M triangles)", "n_gpus": 1, "higher_is_better": False, "dtype": "f32",
+           "data": "synthetic (c1 and c2 scene generators)", "config": {"workload": "mt_bvh_build_device on c1, c2"},
+           "builds": res}
+    print(json.dumps(out), flush=True)
+    return 0 if ok_all else 1
 
 
 def bench_jobs(args) -> int:
