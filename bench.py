@@ -255,7 +255,9 @@ def main() -> int:
                        "note": "mt_render per step (allocates its state per call), HIP events on "
                                "every traversal launch; source of the roofline launch duration"},
         "roofline": {
-            "kernel": "closest-hit traversal (k_trace, packed layout)" if job.mode == 0 else "closest-hit traversal (reference layout)",
+            "kernel": ("closest-hit traversal, packed layout: k_trace_camera (bounce 0, camera rays made in "
+                       "place) and k_trace (bounces 1-7), averaged over the job's launches")
+                      if job.mode == 0 else "closest-hit traversal (reference layout)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

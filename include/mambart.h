@@ -243,6 +243,23 @@ int32_t mt_shade_hits(mt_context* ctx, const mt_scene* scene,
                       const int32_t* d_count, int32_t capacity,
                       float* p, float* n, float* ng, float* uv, int32_t* mat);
 
+/* IntersectPrimitives (NormalBVH.cl:94-233) over the reference's own
+ * records, for a caller that holds its global Ray and Intersection buffers
+ * (the kernel CDeviceRT.java:136 and CDeviceGI.java:172 create with rays,
+ * isects, count, mesh and BVH arguments).  rays: `capacity` Ray records of
+ * 80 bytes (Util.cl:129-138: float4 o, d, inv_d; int4 sign; int2 extra;
+ * float tMin, tMax), read-write; isects: `capacity` Intersection records of
+ * 96 bytes (Util.cl:141-151: float4 p, n, ng, d; float2 uv; int mat, id,
+ * hit; 12 bytes padding).  Both 16-byte aligned.  For slots below *d_count
+ * the closest hit is found in the reference's visiting order using the
+ * record's inv_d and sign; a hit writes every Intersection field (p after
+ * the shadow-terminator offset, w lanes 0; d copied from the ray; hit = 1)
+ * and the hit distance into the ray's tMax.  A miss, and every slot in
+ * [*d_count, capacity), writes only hit = 0 and mat = -1.  mode as for
+ * mt_trace_closest (the packed modes share one kernel here). */
+int32_t mt_intersect_primitives(mt_context* ctx, const mt_scene* scene, void* rays, void* isects,
+                                const int32_t* d_count, int32_t capacity, int32_t mode);
+
 /* Shadow rays toward a point light from each hit (Render.cl:453 with
  * Light.cl:93-97): origin getPoint(ray, t), direction (light - p)/dist,
  * tMin 0.0001, tMax dist - 2*EPS_RAY.  valid[i] = prim[i] >= 0. */

@@ -295,6 +295,43 @@ def shade_hits(ctx: Context, scene: Scene, o_tmin, d_tmax, prim, t, count=None):
     return p, nn, ng, uv, mat
 
 
+RAY_RECORD_WORDS = 20           # Ray, Util.cl:129-138 (80 bytes)
+INTERSECTION_RECORD_WORDS = 24  # Intersection, Util.cl:141-151 (96 bytes)
+
+
+def ray_records(o_tmin, d_tmax):
+    """Reference Ray records (float32 [n, 20] host array) for SoA rays, filled
+    as initGlobalRay does (Util.cl:577-600): o, d (w 0), inv_d = 1/d (w 0),
+    sign = inv_d < 0, extra 0, tMin and tMax from the w lanes."""
+    o_tmin = np.asarray(o_tmin, np.float32)
+    d_tmax = np.asarray(d_tmax, np.float32)
+    n = o_tmin.shape[0]
+    rec = np.zeros((n, RAY_RECORD_WORDS), np.float32)
+    rec[:, 0:3] = o_tmin[:, :3]
+    rec[:, 4:7] = d_tmax[:, :3]
+    with np.errstate(divide="ignore"):
+        inv = (np.float32(1.0) / d_tmax[:, :3]).astype(np.float32)
+    rec[:, 8:11] = inv
+    rec.view(np.int32)[:, 12:15] = (inv < 0).astype(np.int32)
+    rec[:, 18] = o_tmin[:, 3]
+    rec[:, 19] = d_tmax[:, 3]
+    return rec
+
+
+def intersect_records(ctx: Context, scene: Scene, rays, isects, count=None, mode: int = 0):
+    """IntersectPrimitives over reference records (mt_intersect_primitives):
+    rays float32 [n, 20] (tMax updated on hit), isects [n, 24] (float32 or
+    int32 view of the 96-byte Intersection), both device tensors, in place."""
+    n = rays.shape[0]
+    _need(rays, "rays", n, (RAY_RECORD_WORDS,))
+    _need(isects, "isects", n, (INTERSECTION_RECORD_WORDS,), dtype=str(isects.dtype).split(".")[-1]
+          if str(isects.dtype) in ("torch.int32", "torch.float32") else "float32")
+    if count is None:
+        count = ctx.scalar_i32(n)
+    call("mt_intersect_primitives", ctx.handle, scene.handle, _ptr(rays), _ptr(isects), _ptr(count), n, mode)
+    return rays, isects
+
+
 def shadow_rays(ctx: Context, o_tmin, d_tmax, prim, t, light_xyz, count=None):
     torch = _torch()
     n = o_tmin.shape[0]

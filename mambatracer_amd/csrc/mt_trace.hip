@@ -372,37 +372,23 @@ __device__ __forceinline__ void barycentrics(v3 p, v3 p1, v3 p2, v3 p3, float& b
     b2 = (d00 * d21 - d01 * d20) * inv;
 }
 
-__global__ __launch_bounds__(256) void k_shade(const float4* __restrict__ points,
-                                               const float4* __restrict__ normals,
-                                               const float2* __restrict__ uvs,
-                                               const int* __restrict__ faces,
-                                               const float4* __restrict__ o_tmin,
-                                               const float4* __restrict__ d_tmax,
-                                               const int* __restrict__ prim,
-                                               const float* __restrict__ tin,
-                                               const int* __restrict__ d_count, int capacity,
-                                               float4* p_out, float4* n_out, float4* ng_out,
-                                               float2* uv_out, int* mat_out)
+// Hit record of IntersectPrimitives (NormalBVH.cl:124-220) for ray (o, d)
+// whose closest hit is primitive c at distance tt.  Every w lane is 0: the
+// reference derives n and ng from w = 0 inputs, and leaves p.w unset.
+struct HitShading {
+    float4 p, n, ng;
+    float2 uv;
+    int mat;
+};
+
+__device__ __forceinline__ HitShading hit_shading(const float4* __restrict__ points,
+                                                  const float4* __restrict__ normals,
+                                                  const float2* __restrict__ uvs,
+                                                  const int* __restrict__ faces, v3 o, v3 d, int c,
+                                                  float tt)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int count = *d_count;
-    if (count > capacity) count = capacity;
-    if (i >= count) return;
-    int c = prim[i];
-    if (c < 0) {
-        mat_out[i] = -1;
-        float4 z = make_float4(0, 0, 0, 0);
-        p_out[i] = z;
-        n_out[i] = z;
-        ng_out[i] = z;
-        uv_out[i] = make_float2(0, 0);
-        return;
-    }
     const int* F = faces + 10 * c;
-    float4 dd = d_tmax[i], oo = o_tmin[i];
-    v3 d = mk3(dd.x, dd.y, dd.z), o = mk3(oo.x, oo.y, oo.z);
     v3 p1 = xyz(points[F[0]]), p2 = xyz(points[F[1]]), p3 = xyz(points[F[2]]);
-    float tt = tin[i];
     v3 p = mk3(o.x + tt * d.x, o.y + tt * d.y, o.z + tt * d.z); // getPoint, Util.cl:558-565
     float texu = 0.0f, texv = 0.0f;
     bool has_uv = F[3] > -1 && uvs;
@@ -446,11 +432,104 @@ __global__ __launch_bounds__(256) void k_shade(const float4* __restrict__ points
     }
     v3 ng = norm3(cross3(sub3(p2, p1), sub3(p3, p1)));
     if (dot3(ng, d) > 0.00001f) ng = neg3(ng);
-    p_out[i] = make_float4(p.x, p.y, p.z, 0.0f);
-    n_out[i] = make_float4(n.x, n.y, n.z, 0.0f);
-    ng_out[i] = make_float4(ng.x, ng.y, ng.z, 0.0f);
-    uv_out[i] = has_uv ? make_float2(texu, texv) : make_float2(u, v);
-    mat_out[i] = F[9] & 0xFFFF;
+    HitShading H;
+    H.p = make_float4(p.x, p.y, p.z, 0.0f);
+    H.n = make_float4(n.x, n.y, n.z, 0.0f);
+    H.ng = make_float4(ng.x, ng.y, ng.z, 0.0f);
+    H.uv = has_uv ? make_float2(texu, texv) : make_float2(u, v);
+    H.mat = F[9] & 0xFFFF;
+    return H;
+}
+
+__global__ __launch_bounds__(256) void k_shade(const float4* __restrict__ points,
+                                               const float4* __restrict__ normals,
+                                               const float2* __restrict__ uvs,
+                                               const int* __restrict__ faces,
+                                               const float4* __restrict__ o_tmin,
+                                               const float4* __restrict__ d_tmax,
+                                               const int* __restrict__ prim,
+                                               const float* __restrict__ tin,
+                                               const int* __restrict__ d_count, int capacity,
+                                               float4* p_out, float4* n_out, float4* ng_out,
+                                               float2* uv_out, int* mat_out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int count = *d_count;
+    if (count > capacity) count = capacity;
+    if (i >= count) return;
+    int c = prim[i];
+    if (c < 0) {
+        mat_out[i] = -1;
+        float4 z = make_float4(0, 0, 0, 0);
+        p_out[i] = z;
+        n_out[i] = z;
+        ng_out[i] = z;
+        uv_out[i] = make_float2(0, 0);
+        return;
+    }
+    float4 dd = d_tmax[i], oo = o_tmin[i];
+    const HitShading H = hit_shading(points, normals, uvs, faces, mk3(oo.x, oo.y, oo.z), mk3(dd.x, dd.y, dd.z), c,
+                                     tin[i]);
+    p_out[i] = H.p;
+    n_out[i] = H.n;
+    ng_out[i] = H.ng;
+    uv_out[i] = H.uv;
+    mat_out[i] = H.mat;
+}
+
+// IntersectPrimitives over the reference's own records (NormalBVH.cl:94-233).
+// A Ray is 80 bytes (Util.cl:129-138): o, d, inv_d (float4 each), sign
+// (int4), extra (int2), tMin, tMax, so as float4s: o, d, inv_d, sign,
+// (extra.x, extra.y, tMin, tMax).  An Intersection is 96 bytes
+// (Util.cl:141-151): p, n, ng, d (float4 each), uv (float2), mat, id, hit,
+// 12 bytes of padding.  The box tests use the record's inv_d and sign, as
+// the reference does.  A hit writes the whole intersection and the hit
+// distance into the ray's tMax (intersectMesh, NormalBVH.cl:42); a miss,
+// and every slot in [*count, capacity), writes hit = 0 and mat = -1 and
+// leaves the other fields as they were.
+template <int LAYOUT>
+__global__ __launch_bounds__(256) void k_intersect_records(TraceArgs A, const float4* __restrict__ normals,
+                                                           const float2* __restrict__ uvs, float4* rays,
+                                                           float4* isects)
+{
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= A.capacity) return;
+    int* iw = reinterpret_cast<int*>(isects + 6 * (size_t)id);
+    float4* R = rays + 5 * (size_t)id;
+    int child = -1;
+    int h = 0;
+    RayT r;
+    float4 d4 = make_float4(0, 0, 0, 0);
+    if (entry_live(A, id)) {
+        const float4 o4 = R[0], inv4 = R[2], sg = R[3], ex = R[4];
+        d4 = R[1];
+        r.o = mk3(o4.x, o4.y, o4.z);
+        r.d = mk3(d4.x, d4.y, d4.z);
+        r.inv = mk3(inv4.x, inv4.y, inv4.z);
+        r.sx = __float_as_int(sg.x) != 0;
+        r.sy = __float_as_int(sg.y) != 0;
+        r.sz = __float_as_int(sg.z) != 0;
+        r.tmin = ex.z;
+        r.tmax = ex.w;
+        int st[3];
+        h = LAYOUT == 0   ? trace_packed<false, false, true>(A.pn, A.pt, A.root, r, child, st)
+            : LAYOUT == 2 ? trace_packed<false, false, false>(A.pn, A.pt, A.root, r, child, st)
+                          : trace_reference<false, false>(A.nodes, A.bounds, A.faces, A.points, r, child, st);
+    }
+    if (!h) {
+        iw[18] = -1; // mat
+        iw[20] = 0;  // hit = MISS_MARKER
+        return;
+    }
+    reinterpret_cast<float*>(R)[19] = r.tmax;
+    const HitShading H = hit_shading(A.points, normals, uvs, A.faces, r.o, r.d, child, r.tmax);
+    float4* I = isects + 6 * (size_t)id;
+    I[0] = H.p;
+    I[1] = H.n;
+    I[2] = H.ng;
+    I[3] = d4;
+    I[4] = make_float4(H.uv.x, H.uv.y, __int_as_float(H.mat), __int_as_float(child));
+    iw[20] = 1; // hit = HIT_MARKER
 }
 
 __global__ __launch_bounds__(256) void k_shadow_rays(const float4* __restrict__ o_tmin,
@@ -687,6 +766,40 @@ extern "C" int32_t mt_shade_hits(mt_context* ctx, const mt_scene* sc, const floa
         reinterpret_cast<const float4*>(o_tmin), reinterpret_cast<const float4*>(d_tmax), prim, t,
         d_count, capacity, reinterpret_cast<float4*>(p), reinterpret_cast<float4*>(n),
         reinterpret_cast<float4*>(ng), reinterpret_cast<float2*>(uv), mat);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_intersect_primitives(mt_context* ctx, const mt_scene* sc, void* rays, void* isects,
+                                           const int32_t* d_count, int32_t capacity, int32_t mode)
+{
+    MT_ARG_CHECK(ctx && sc && rays && isects && d_count, "mt_intersect_primitives: NULL argument");
+    MT_ARG_CHECK(capacity >= 0, "mt_intersect_primitives: negative capacity");
+    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_REFILL,
+                 "mt_intersect_primitives: unknown mode %d", mode);
+    MT_ARG_CHECK((reinterpret_cast<uintptr_t>(rays) & 15) == 0 && (reinterpret_cast<uintptr_t>(isects) & 15) == 0,
+                 "mt_intersect_primitives: rays and isects must be 16-byte aligned");
+    if (capacity == 0) return MT_OK;
+    TraceArgs A{};
+    A.nodes = sc->desc.nodes;
+    A.bounds = sc->desc.bounds;
+    A.faces = sc->desc.faces;
+    A.points = reinterpret_cast<const float4*>(sc->desc.points);
+    A.pn = sc->pnodes;
+    A.pt = sc->ptris;
+    A.root = sc->root_ref;
+    A.d_count = d_count;
+    A.capacity = capacity;
+    const float4* nrm = reinterpret_cast<const float4*>(sc->desc.normals);
+    const float2* uvs = reinterpret_cast<const float2*>(sc->desc.uvs);
+    float4* R = reinterpret_cast<float4*>(rays);
+    float4* I = reinterpret_cast<float4*>(isects);
+    dim3 grid(mt_blocks(capacity, 256)), block(256);
+    hipStream_t s = ctx->stream;
+    // every packed mode gives the same bits; this writer runs one ray per lane
+    if (mode == MT_TRACE_REFERENCE) k_intersect_records<1><<<grid, block, 0, s>>>(A, nrm, uvs, R, I);
+    else if (mode == MT_TRACE_PACKED_NO_STACK) k_intersect_records<2><<<grid, block, 0, s>>>(A, nrm, uvs, R, I);
+    else k_intersect_records<0><<<grid, block, 0, s>>>(A, nrm, uvs, R, I);
     MT_LAUNCH_CHECK();
     return MT_OK;
 }

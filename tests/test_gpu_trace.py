@@ -156,3 +156,56 @@ def test_autotune_picks_a_packed_mode_with_identical_results(ctx, c0, any_hit):
         np.testing.assert_array_equal(t.cpu().numpy().view(np.int32), ot.view(np.int32))
     with pytest.raises(RuntimeError, match="reps"):
         api.autotune_trace(ctx, sc, do, dd, any_hit=any_hit, reps=0)
+
+
+@pytest.mark.parametrize("mode", [api._lib.MT_TRACE_PACKED, api._lib.MT_TRACE_REFERENCE,
+                                  api._lib.MT_TRACE_PACKED_NO_STACK])
+def test_intersect_records_against_oracle(ctx, mode):
+    """IntersectPrimitives over the reference's 80-byte Ray and 96-byte
+    Intersection records: hits bit-exact against the oracle's closest hit and
+    hit shading (normals on every face, UVs on every other face), tMax
+    written back on hits only, misses and slots past *count writing only
+    hit = 0 and mat = -1."""
+    print(f"[case] intersection records mode={mode}", flush=True)
+    points, normals, faces = api.gen_spheres(2, 4)
+    faces = faces.copy()
+    faces[0::2, 3:6] = faces[0::2, 0:3]
+    uvs = np.random.default_rng(5).random((points.shape[0], 2), dtype=np.float32)
+    nodes, bounds, depth = api.CNormalBVH.build(points, faces)
+    o, d = api.gen_camera_rays([0, 1, -12], [0, 0, 0], [0, 1, 0], 60.0, 128, 128)
+    n = o.shape[0]
+    count = n - 100
+    rec = api.ray_records(o, d)
+    rec[:, 7] = np.float32(0.25)                       # d.w travels into the intersection
+    rec.view(np.int32)[:, 16:18] = np.arange(2 * n, dtype=np.int32).reshape(n, 2)  # extra untouched
+    sentinel = np.full((n, api.INTERSECTION_RECORD_WORDS), 0x7F7F7F7F, np.int32)
+    sc = api.Scene(ctx, points, faces, nodes, bounds, depth, normals=normals, uvs=uvs)
+    drays, disect = ctx.to_device(rec), ctx.to_device(sentinel)
+    api.intersect_records(ctx, sc, drays, disect, count=ctx.scalar_i32(count), mode=mode)
+    torch.cuda.synchronize()
+    got_r, got_i = drays.cpu().numpy(), disect.cpu().numpy()
+    oprim, ot, _, _ = orc.trace_closest(points, faces, nodes, bounds, o[:count], d[:count])
+    op, on, ong, ouv, omat = orc.shade_hits(points, normals, uvs, faces, o[:count], d[:count], oprim, ot)
+    hit = np.zeros(n, bool)
+    hit[:count] = oprim >= 0
+    assert hit.sum() > 1000 and (~hit[:count]).sum() > 100 and (faces[oprim[oprim >= 0], 3] >= 0).any()
+    # rays: only tMax of hit slots changes, to the closest hit's distance
+    want_r = rec.copy()
+    want_r[:count, 19] = np.where(oprim >= 0, ot, rec[:count, 19])
+    np.testing.assert_array_equal(got_r.view(np.int32), want_r.view(np.int32))
+    gi = got_i
+    gf = got_i.view(np.float32)
+    h = np.nonzero(hit)[0]
+    np.testing.assert_array_equal(gi[h, 20], 1)
+    np.testing.assert_array_equal(gi[h, 19], oprim[h])
+    np.testing.assert_array_equal(gi[h, 18], omat[h])
+    for lo, want in [(0, op), (4, on), (8, ong)]:
+        np.testing.assert_array_equal(gf[h, lo:lo + 4].view(np.int32), want[h].view(np.int32))
+    np.testing.assert_array_equal(gf[h, 12:16].view(np.int32), rec[h, 4:8].view(np.int32))
+    np.testing.assert_array_equal(gf[h, 16:18].view(np.int32), ouv[h].view(np.int32))
+    np.testing.assert_array_equal(gi[h, 21:24], 0x7F7F7F7F)
+    m = np.nonzero(~hit)[0]
+    np.testing.assert_array_equal(gi[m, 20], 0)
+    np.testing.assert_array_equal(gi[m, 18], -1)
+    keep = np.r_[0:18, 19, 21:24]
+    np.testing.assert_array_equal(gi[m][:, keep], 0x7F7F7F7F)
