@@ -366,6 +366,35 @@ int32_t mt_env_illuminate(mt_context* ctx, const mt_envmap* env, int32_t n,
 int32_t mt_env_radiance(mt_context* ctx, const mt_envmap* env, int32_t n,
                         const float* dir, float* contrib, float* pdfw, int32_t* texel);
 
+/* CAdaptiveEnvironment.setCameraPosition (CAdaptiveEnvironment.java:61-67,
+ * called per frame by CDeviceGI.java:296): pos = host float[3]; the grid's
+ * cells are spherical bins of the direction camera -> hit point.  Taken by
+ * the adaptive calls below when they are enqueued.  Initially (0, 0, 0). */
+int32_t mt_lightgrid_set_camera(mt_lightgrid* grid, const float* pos);
+/* illuminateEnvironmentLight1 (Light.cl:140-193), the light-grid-guided
+ * environment sample (no live caller in the reference: Render.cl:458-459
+ * are commented out).  For each of n receiving points hit_p (float4): cell
+ * from the camera direction, tile ~ the cell's distribution with (u.x, u.y)
+ * (sampleSubgridContinuous), texel ~ that tile's env-map SAT region with
+ * (u.z, u.w) (u: float4[n]).  Outputs dir (float4, w 0), contrib (env
+ * texel), pdfw (the in-tile solid-angle pdf, as the reference: the tile
+ * pick's pdf is not applied), and, if non-NULL, lum = Luminance(contrib)
+ * and grid_index = the accum entry of (cell, tile).  The env map needs
+ * w >= 16 and h >= 32 (one texel per tile). */
+int32_t mt_env_illuminate_adaptive(mt_context* ctx, const mt_envmap* env, const mt_lightgrid* grid,
+                                   int32_t n, const float* hit_p, const float* u, float* dir,
+                                   float* contrib, float* pdfw, float* lum, int32_t* grid_index);
+/* getRadianceEnvironmentLight1 (Light.cl:229-268) for n escaped rays
+ * (dir float4, origin hit_p float4): tile from the ray direction, pdfa =
+ * getPdfSAT over that tile's region / (2 pi^2 sin(pi y/h)), contrib = env
+ * texel.  accumulate != 0 adds Luminance(contrib) into the grid's accum
+ * entry of (cell, tile) (accumLightGrid, Light.cl:35-38, float atomic add:
+ * per-entry sums are order-dependent in their last bits, as the
+ * reference's).  contrib, pdfa, lum, grid_index may be NULL. */
+int32_t mt_env_radiance_adaptive(mt_context* ctx, const mt_envmap* env, mt_lightgrid* grid, int32_t n,
+                                 const float* dir, const float* hit_p, int32_t accumulate,
+                                 float* contrib, float* pdfa, float* lum, int32_t* grid_index);
+
 /* ------------------------------------------------------------------ */
 /* B. BSDF                                                            */
 /* ------------------------------------------------------------------ */

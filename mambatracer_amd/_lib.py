@@ -178,6 +178,9 @@ SIGNATURES = {
     "mt_sat_sample_region": [P, P, P, I32, I32, I32, I32, I32, I32, I32, P, P, P, P],
     "mt_env_illuminate": [P, P, I32, P, P, P, P, P],
     "mt_env_radiance": [P, P, I32, P, P, P, P],
+    "mt_lightgrid_set_camera": [P, P],
+    "mt_env_illuminate_adaptive": [P, P, P, I32, P, P, P, P, P, P, P],
+    "mt_env_radiance_adaptive": [P, P, P, I32, P, P, I32, P, P, P, P],
     "mt_bsdf_setup": [P, I32, P, P, P, P, P, P, P],
     "mt_bsdf_sample": [P, I32, P, P, P, P, P, P, P, P],
     "mt_bsdf_eval": [P, I32, P, P, P, P, P, P, P],

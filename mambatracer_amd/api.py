@@ -539,6 +539,11 @@ class CAdaptiveEnvironment:
     def buildSAT(self) -> None:
         call("mt_lightgrid_build_sat", self.ctx.handle, self.handle)
 
+    def setCameraPosition(self, pos) -> None:
+        """CAdaptiveEnvironment.setCameraPosition (CAdaptiveEnvironment.java:61-67)."""
+        p = np.ascontiguousarray(np.asarray(pos, dtype=np.float32).reshape(-1)[:3])
+        call("mt_lightgrid_set_camera", self.handle, _ptr(p))
+
     def sample(self, cell, u):
         torch = _torch()
         n = int(cell.shape[0])
@@ -618,6 +623,37 @@ class CEnvironment:
         tx = self.ctx.empty((n,), torch.int32)
         call("mt_env_radiance", self.ctx.handle, self.handle, n, _ptr(dirs), _ptr(c), _ptr(pw), _ptr(tx))
         return c, pw, tx
+
+    def illuminate_adaptive(self, grid: "CAdaptiveEnvironment", hit_p, u):
+        """illuminateEnvironmentLight1 (mt_env_illuminate_adaptive): hit_p, u float4
+        [n] device tensors -> (dir, contrib, pdfw, lum, grid_index)."""
+        torch = _torch()
+        n = int(u.shape[0])
+        _need(hit_p, "hit_p", n, (4,))
+        _need(u, "u", n, (4,))
+        d = self.ctx.empty((n, 4), torch.float32)
+        c = self.ctx.empty((n, 4), torch.float32)
+        pw = self.ctx.empty((n,), torch.float32)
+        lum = self.ctx.empty((n,), torch.float32)
+        gi = self.ctx.empty((n,), torch.int32)
+        call("mt_env_illuminate_adaptive", self.ctx.handle, self.handle, grid.handle, n, _ptr(hit_p), _ptr(u),
+             _ptr(d), _ptr(c), _ptr(pw), _ptr(lum), _ptr(gi))
+        return d, c, pw, lum, gi
+
+    def radiance_adaptive(self, grid: "CAdaptiveEnvironment", dirs, hit_p, accumulate: bool = True):
+        """getRadianceEnvironmentLight1 (mt_env_radiance_adaptive), adding each
+        luminance into the grid's accum when accumulate -> (contrib, pdfa, lum, grid_index)."""
+        torch = _torch()
+        n = int(dirs.shape[0])
+        _need(dirs, "dirs", n, (4,))
+        _need(hit_p, "hit_p", n, (4,))
+        c = self.ctx.empty((n, 4), torch.float32)
+        pa = self.ctx.empty((n,), torch.float32)
+        lum = self.ctx.empty((n,), torch.float32)
+        gi = self.ctx.empty((n,), torch.int32)
+        call("mt_env_radiance_adaptive", self.ctx.handle, self.handle, grid.handle, n, _ptr(dirs), _ptr(hit_p),
+             1 if accumulate else 0, _ptr(c), _ptr(pa), _ptr(lum), _ptr(gi))
+        return c, pa, lum, gi
 
     def close(self):
         if self.handle:

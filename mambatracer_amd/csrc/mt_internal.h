@@ -69,6 +69,7 @@ struct mt_lightgrid {
     float* accum;
     float* func;
     float* sat;
+    float cam[3]; // LightGrid.cameraPosition (LightGrid.cl:26), a launch argument
 };
 
 struct mt_envmap {

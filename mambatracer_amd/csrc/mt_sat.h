@@ -317,3 +317,68 @@ __device__ __forceinline__ float4 env_radiance(const EnvView& E, v3 d, float& pd
     pdfw = p / (((2.0f * MT_PI) * MT_PI) * sinf(v * MT_PI));
     return E.rgba[texel];
 }
+
+// ---------------------------------------------------------------- adaptive environment light
+// subgridIndexFromCamera, LightGrid.cl:68-78: the 100 x 50 cell of the
+// direction from the grid's camera position to the hit point.
+__device__ __forceinline__ int lg_cell_from_camera(v3 cam, v3 hit)
+{
+    return sph_index(LG_CX, LG_CY, norm3(sub3(hit, cam)));
+}
+
+// getSubgridUnitBound (LightGrid.cl:158-167) followed by
+// getSubRegionFromUnitBound (SAT.cl:28-44): the env-map texel rectangle of
+// tile (tx, ty) of a cell, corners truncated from float products.
+__device__ __forceinline__ SatRegion lg_tile_region(int w, int h, int tx, int ty)
+{
+    const float ux0 = (float)tx / (float)LG_RX, uy0 = (float)ty / (float)LG_RY;
+    const float ux1 = (float)(tx + 1) / (float)LG_RX, uy1 = (float)(ty + 1) / (float)LG_RY;
+    const int minx = (int)((float)w * ux0), miny = (int)((float)h * uy0);
+    const int maxx = (int)((float)w * ux1), maxy = (int)((float)h * uy1);
+    return SatRegion{w, h, minx, miny, maxx - minx, maxy - miny};
+}
+
+// illuminateEnvironmentLight1, Light.cl:140-193: tile from the cell's
+// distribution, texel from the tile's SAT region.  The returned pdf is the
+// in-tile pdf only (the tile pick's pdf is not applied, :163,182).
+__device__ __forceinline__ float4 env_illuminate_adaptive(const EnvView& E, const float* __restrict__ lg_func,
+                                                          const float* __restrict__ lg_sat_arr, v3 cam, v3 hit,
+                                                          float4 u, v3& dir, float& pdfw, float& lum, int& gidx)
+{
+    const int s = lg_cell_from_camera(cam, hit);
+    int lx, ly;
+    float luu, lvv, lpdf;
+    lg_sample(lg_func, lg_sat_arr, s, u.x, u.y, lx, ly, luu, lvv, lpdf);
+    const SatRegion R = lg_tile_region(E.w, E.h, lx, ly);
+    float uu, vv, pdf;
+    int ox, oy;
+    r_sample(R, E.sat, E.lum, u.z, u.w, uu, vv, ox, oy, pdf);
+    dir = sph_direction(uu, vv);
+    const float4 c = E.rgba[sph_index(E.w, E.h, dir)];
+    const float sn = sinf(vv * MT_PI);
+    pdfw = pdf / (((2.0f * MT_PI) * MT_PI) * sinf(vv * MT_PI));
+    if (sn == 0.0f) pdfw = 0.0f;
+    lum = luminance3(c.x, c.y, c.z);
+    gidx = lg_global_index(s, lx, ly);
+    return c;
+}
+
+// getRadianceEnvironmentLight1, Light.cl:229-268, without the accumulation
+// (the caller adds lum into accum[gidx], accumLightGrid :35-38).
+__device__ __forceinline__ float4 env_radiance_adaptive(const EnvView& E, v3 cam, v3 d, v3 hit, float& pdfa,
+                                                        float& lum, int& gidx)
+{
+    const int s = lg_cell_from_camera(cam, hit);
+    const int t = sph_index(LG_RX, LG_RY, d); // tileIndexXYFromCamera, LightGrid.cl:80-90
+    const int tx = t % LG_RX, ty = t / LG_RX;
+    const SatRegion R = lg_tile_region(E.w, E.h, tx, ty);
+    int x, y;
+    sph_xy(E.w, E.h, d, x, y);
+    const float p = r_pdf_cond(R, E.sat, E.lum, x, y) * r_pdf_marg(R, E.sat, y); // getPdfSAT, SAT.cl:336-342
+    const float sv = (float)y / (float)E.h;
+    pdfa = p / (((2.0f * MT_PI) * MT_PI) * sinf(sv * MT_PI));
+    const float4 c = E.rgba[sph_index(E.w, E.h, d)];
+    lum = luminance3(c.x, c.y, c.z);
+    gidx = lg_global_index(s, tx, ty);
+    return c;
+}

@@ -289,6 +289,45 @@ __global__ void k_env_radiance(EnvView E, int n, const float4* __restrict__ dir,
     texel[i] = tx;
 }
 
+__global__ void k_env_illuminate_adaptive(EnvView E, const float* __restrict__ lg_func,
+                                          const float* __restrict__ lg_sat_arr, float cx, float cy, float cz, int n,
+                                          const float4* __restrict__ hit, const float4* __restrict__ u,
+                                          float4* __restrict__ dir, float4* __restrict__ contrib,
+                                          float* __restrict__ pdfw, float* __restrict__ lum, int* __restrict__ gidx)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    v3 d;
+    float pw, l;
+    int g;
+    const float4 c = env_illuminate_adaptive(E, lg_func, lg_sat_arr, mk3(cx, cy, cz), xyz(hit[i]), u[i], d, pw, l, g);
+    dir[i] = make_float4(d.x, d.y, d.z, 0.0f);
+    contrib[i] = c;
+    pdfw[i] = pw;
+    if (lum) lum[i] = l;
+    if (gidx) gidx[i] = g;
+}
+
+// accumLightGrid (Light.cl:35-38) adds each luminance with a float atomic, as
+// the reference's compare-and-swap loop (Util.cl:783-795) does: the sum per
+// entry is exact up to the order of the additions.
+__global__ void k_env_radiance_adaptive(EnvView E, float* accum, float cx, float cy, float cz, int n,
+                                        const float4* __restrict__ dir, const float4* __restrict__ hit,
+                                        float4* __restrict__ contrib, float* __restrict__ pdfa,
+                                        float* __restrict__ lum, int* __restrict__ gidx)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float pa, l;
+    int g;
+    const float4 c = env_radiance_adaptive(E, mk3(cx, cy, cz), xyz(dir[i]), xyz(hit[i]), pa, l, g);
+    if (accum) atomicAdd(accum + g, l);
+    if (contrib) contrib[i] = c;
+    if (pdfa) pdfa[i] = pa;
+    if (lum) lum[i] = l;
+    if (gidx) gidx[i] = g;
+}
+
 } // namespace
 
 extern "C" int32_t mt_lightgrid_create(mt_context* ctx, mt_lightgrid** out)
@@ -523,6 +562,60 @@ extern "C" int32_t mt_env_radiance(mt_context* ctx, const mt_envmap* env, int32_
     k_env_radiance<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(E, n, reinterpret_cast<const float4*>(dir),
                                                                reinterpret_cast<float4*>(contrib), pdfw,
                                                                texel);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_lightgrid_set_camera(mt_lightgrid* grid, const float* pos)
+{
+    MT_ARG_CHECK(grid && pos, "mt_lightgrid_set_camera: NULL argument");
+    grid->cam[0] = pos[0];
+    grid->cam[1] = pos[1];
+    grid->cam[2] = pos[2];
+    return MT_OK;
+}
+
+// Tiles of the grid map onto texel rectangles of the environment map; every
+// rectangle must hold a texel (setRegion, SAT.cl:12-24, leaves an empty
+// region undefined), so the map needs at least one texel per tile column/row.
+static int32_t check_adaptive_env(const mt_envmap* env, const char* fn)
+{
+    MT_ARG_CHECK(env->w >= LG_RX && env->h >= LG_RY, "%s: environment map %d x %d smaller than the %d x %d tiles",
+                 fn, env->w, env->h, LG_RX, LG_RY);
+    return MT_OK;
+}
+
+extern "C" int32_t mt_env_illuminate_adaptive(mt_context* ctx, const mt_envmap* env, const mt_lightgrid* grid,
+                                              int32_t n, const float* hit_p, const float* u, float* dir,
+                                              float* contrib, float* pdfw, float* lum, int32_t* grid_index)
+{
+    MT_ARG_CHECK(ctx && env && grid && hit_p && u && dir && contrib && pdfw,
+                 "mt_env_illuminate_adaptive: NULL argument");
+    MT_ARG_CHECK(n >= 0, "mt_env_illuminate_adaptive: negative n");
+    int32_t rc = check_adaptive_env(env, "mt_env_illuminate_adaptive");
+    if (rc) return rc;
+    if (n == 0) return MT_OK;
+    k_env_illuminate_adaptive<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(
+        env_view(env), grid->func, grid->sat, grid->cam[0], grid->cam[1], grid->cam[2], n,
+        reinterpret_cast<const float4*>(hit_p), reinterpret_cast<const float4*>(u), reinterpret_cast<float4*>(dir),
+        reinterpret_cast<float4*>(contrib), pdfw, lum, grid_index);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_env_radiance_adaptive(mt_context* ctx, const mt_envmap* env, mt_lightgrid* grid, int32_t n,
+                                            const float* dir, const float* hit_p, int32_t accumulate,
+                                            float* contrib, float* pdfa, float* lum, int32_t* grid_index)
+{
+    MT_ARG_CHECK(ctx && env && grid && dir && hit_p, "mt_env_radiance_adaptive: NULL argument");
+    MT_ARG_CHECK(n >= 0, "mt_env_radiance_adaptive: negative n");
+    int32_t rc = check_adaptive_env(env, "mt_env_radiance_adaptive");
+    if (rc) return rc;
+    if (n == 0) return MT_OK;
+    k_env_radiance_adaptive<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(
+        env_view(env), accumulate ? grid->accum : nullptr, grid->cam[0], grid->cam[1], grid->cam[2], n,
+        reinterpret_cast<const float4*>(dir), reinterpret_cast<const float4*>(hit_p),
+        reinterpret_cast<float4*>(contrib), pdfa, lum, grid_index);
     MT_LAUNCH_CHECK();
     return MT_OK;
 }

@@ -101,6 +101,13 @@ int orc_env_illuminate(const float* rgba, const float* lum, const float* sat,
 int orc_env_radiance(const float* rgba, const float* lum, const float* sat,
                      int32_t w, int32_t h, int32_t n, const float* dir,
                      float* contrib, float* pdfw, int32_t* texel);
+int orc_env_illuminate_adaptive(const float* rgba, const float* lum, const float* sat, int32_t w, int32_t h,
+                                const float* lg_func, const float* lg_sat_arr, const float* cam, int32_t n,
+                                const float* hit, const float* u, float* dir, float* contrib, float* pdfw,
+                                float* lum_out, int32_t* gidx, int32_t* texel_xy);
+int orc_env_radiance_adaptive(const float* rgba, const float* lum, const float* sat, int32_t w, int32_t h,
+                              float* accum, const float* cam, int32_t n, const float* dir, const float* hit,
+                              float* contrib, float* pdfa, float* lum_out, int32_t* gidx);
 
 /* ---- B: BSDF ---- */
 /* materials: 36 x 4-byte words per material (4 int flags + 8 float4). */

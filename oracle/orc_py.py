@@ -51,6 +51,8 @@ def lib() -> C.CDLL:
             "orc_sat_sample_region": [P, P, I32, I32, I32, I32, I32, I32, I32, P, P, P, P],
             "orc_env_illuminate": [P, P, P, I32, I32, I32, P, P, P, P, P],
             "orc_env_radiance": [P, P, P, I32, I32, I32, P, P, P, P],
+            "orc_env_illuminate_adaptive": [P, P, P, I32, I32, P, P, P, I32, P, P, P, P, P, P, P, P],
+            "orc_env_radiance_adaptive": [P, P, P, I32, I32, P, P, I32, P, P, P, P, P, P],
             "orc_bsdf_setup": [I32, P, P, P, P, P, P, P],
             "orc_bsdf_sample": [I32, P, P, P, P, P, P, P, P],
             "orc_bsdf_eval": [I32, P, P, P, P, P, P, P],
@@ -301,6 +303,40 @@ def env_radiance(rgba, lum, sat, w, h, dirs):
     _ok(lib().orc_env_radiance(_p(rgba), _p(lum), _p(sat), w, h, n, _p(dirs), _p(c), _p(pw), _p(tx)),
         "orc_env_radiance")
     return c, pw, tx
+
+
+def env_illuminate_adaptive(rgba, lum, sat, w, h, lg_func, lg_sat, cam, hit, u):
+    """illuminateEnvironmentLight1: (dir, contrib, pdfw, lum, grid index, texel xy in the map)."""
+    rgba, lum, sat, lg_func, lg_sat = _f32(rgba), _f32(lum), _f32(sat), _f32(lg_func), _f32(lg_sat)
+    cam, hit, u = _f32(cam), _f32(hit), _f32(u)
+    n = u.shape[0]
+    d = np.zeros((n, 4), np.float32)
+    c = np.zeros((n, 4), np.float32)
+    pw = np.zeros(n, np.float32)
+    lo = np.zeros(n, np.float32)
+    gi = np.zeros(n, np.int32)
+    txy = np.zeros((n, 2), np.int32)
+    _ok(lib().orc_env_illuminate_adaptive(_p(rgba), _p(lum), _p(sat), w, h, _p(lg_func), _p(lg_sat), _p(cam), n,
+                                          _p(hit), _p(u), _p(d), _p(c), _p(pw), _p(lo), _p(gi), _p(txy)),
+        "orc_env_illuminate_adaptive")
+    return d, c, pw, lo, gi, txy
+
+
+def env_radiance_adaptive(rgba, lum, sat, w, h, cam, dirs, hit, accum=None):
+    """getRadianceEnvironmentLight1: (contrib, pdfa, lum, grid index); adds lum
+    into `accum` (float32[2 560 000], in place) in index order when given."""
+    rgba, lum, sat, cam, dirs, hit = _f32(rgba), _f32(lum), _f32(sat), _f32(cam), _f32(dirs), _f32(hit)
+    if accum is not None:
+        assert accum.dtype == np.float32 and accum.flags.c_contiguous
+    n = dirs.shape[0]
+    c = np.zeros((n, 4), np.float32)
+    pa = np.zeros(n, np.float32)
+    lo = np.zeros(n, np.float32)
+    gi = np.zeros(n, np.int32)
+    _ok(lib().orc_env_radiance_adaptive(_p(rgba), _p(lum), _p(sat), w, h, _p(accum) if accum is not None else None,
+                                        _p(cam), n, _p(dirs), _p(hit), _p(c), _p(pa), _p(lo), _p(gi)),
+        "orc_env_radiance_adaptive")
+    return c, pa, lo, gi
 
 
 # ---------------------------------------------------------------- B

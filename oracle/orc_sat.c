@@ -31,6 +31,16 @@
  *                        SAT.cl:95-342 and Light.cl:129-137,195-227,270-286;
  *                        pinned against Distribution2D (float64) and the
  *                        closed-form uniform-map case (pdf == 1 everywhere).
+ *   orc_env_illuminate_adaptive, orc_env_radiance_adaptive restate the
+ *                        light-grid-guided environment light (Light.cl:35-38,
+ *                        140-193, 229-268; no live caller in the reference).
+ *                        Pinned by structure: the accum entry names the
+ *                        camera cell and ray tile of an independent float64
+ *                        spherical binning; the tile equals orc_lg_sample's
+ *                        pick; the texel lies in the tile's closed-form
+ *                        rectangle and equals region sampling of it; a
+ *                        constant map gives pdf 1 / (2 pi^2 sin theta) and
+ *                        accum = count * luminance.
  *
  * Deviations (documented): getSphericalGridXY (Util.cl:397-410) yields
  * y == height for d.y == -1 and getFunc/getCummulative index -1 for columns
@@ -150,33 +160,40 @@ static int lg_upper_marg(const float* sat, int s, int first, int last, float v)
     return b;
 }
 
+static void lg_sample_one(const float* func, const float* sat, int s, float u0, float u1, int* ox_out,
+                          int* oy_out, float* uu_out, float* vv_out, float* pdf_out)
+{ /* sampleSubgridContinuous, :310-320 */
+    /* sampleSubgridContinuousMarginal, :279-296 */
+    int ptr = lg_upper_marg(sat, s, 0, LG_RY, u1);
+    int oy = ptr - 1 > 0 ? ptr - 1 : 0;
+    float my0 = lg_marg(sat, s, oy);
+    float du = (u1 - my0) / (lg_marg(sat, s, oy + 1) - my0);
+    float last = lg_sat(sat, s, LG_RX - 1, LG_RY - 1);
+    float pdf_v = (lg_func_int(sat, s, oy) * (float)(LG_RX * LG_RY)) / last;
+    float v = ((float)oy + du) / (float)LG_RY;
+    /* sampleSubgridContinuousConditional, :249-266 */
+    ptr = lg_upper_cond(sat, s, oy, 0, LG_RX, u0);
+    int ox = ptr - 1 > 0 ? ptr - 1 : 0;
+    float cx0 = lg_cond(sat, s, ox, oy);
+    float du2 = (u0 - cx0) / (lg_cond(sat, s, ox + 1, oy) - cx0);
+    float pdf_u = func[lg_global_index(s, ox, oy)] / lg_func_int(sat, s, oy);
+    *ox_out = ox;
+    *oy_out = oy;
+    *uu_out = ((float)ox + du2) / (float)LG_RX;
+    *vv_out = v;
+    *pdf_out = pdf_u * pdf_v;
+}
+
 int orc_lg_sample(const float* func, const float* sat, int32_t n, const int32_t* cell,
                   const float* u, int32_t* tile_xy, float* uv, float* pdf)
-{ /* sampleSubgridContinuous, :310-320 */
+{
     for (int32_t i = 0; i < n; i++) {
         int s = cell[i];
         if (s < 0 || s >= LG_CX * LG_CY) return ORC_ERR_ARG;
-        float u0 = u[2 * i], u1 = u[2 * i + 1];
-        /* sampleSubgridContinuousMarginal, :279-296 */
-        int ptr = lg_upper_marg(sat, s, 0, LG_RY, u1);
-        int oy = ptr - 1 > 0 ? ptr - 1 : 0;
-        float my0 = lg_marg(sat, s, oy);
-        float du = (u1 - my0) / (lg_marg(sat, s, oy + 1) - my0);
-        float last = lg_sat(sat, s, LG_RX - 1, LG_RY - 1);
-        float pdf_v = (lg_func_int(sat, s, oy) * (float)(LG_RX * LG_RY)) / last;
-        float v = ((float)oy + du) / (float)LG_RY;
-        /* sampleSubgridContinuousConditional, :249-266 */
-        ptr = lg_upper_cond(sat, s, oy, 0, LG_RX, u0);
-        int ox = ptr - 1 > 0 ? ptr - 1 : 0;
-        float cx0 = lg_cond(sat, s, ox, oy);
-        float du2 = (u0 - cx0) / (lg_cond(sat, s, ox + 1, oy) - cx0);
-        float pdf_u = func[lg_global_index(s, ox, oy)] / lg_func_int(sat, s, oy);
-        float uu = ((float)ox + du2) / (float)LG_RX;
+        int ox, oy;
+        lg_sample_one(func, sat, s, u[2 * i], u[2 * i + 1], &ox, &oy, &uv[2 * i], &uv[2 * i + 1], &pdf[i]);
         tile_xy[2 * i] = ox;
         tile_xy[2 * i + 1] = oy;
-        uv[2 * i] = uu;
-        uv[2 * i + 1] = v;
-        pdf[i] = pdf_u * pdf_v;
     }
     return ORC_OK;
 }
@@ -383,6 +400,86 @@ int orc_env_radiance(const float* rgba, const float* lum, const float* sat,
         pdfw[i] = p / (((2.0f * ORC_PI) * ORC_PI) * sinf(v * ORC_PI));
         for (int k = 0; k < 4; k++) contrib[4 * i + k] = rgba[4 * (size_t)idx + k];
         texel[i] = idx;
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Adaptive environment light: Light.cl:35-38,140-193,229-268          */
+/* ------------------------------------------------------------------ */
+
+static int cell_from_camera(const float* cam, f4 hit)
+{ /* subgridIndexFromCamera, LightGrid.cl:68-78 */
+    f4 c = f4_make(cam[0], cam[1], cam[2], 0.0f);
+    return spherical_index(LG_CX, LG_CY, normalize3(f4_sub(f4_make(hit.x, hit.y, hit.z, 0.0f), c)));
+}
+
+static oregion tile_region(int w, int h, int tx, int ty)
+{ /* getSubgridUnitBound (LightGrid.cl:158-167), getSubRegionFromUnitBound (SAT.cl:28-44) */
+    float ux0 = (float)tx / (float)LG_RX, uy0 = (float)ty / (float)LG_RY;
+    float ux1 = (float)(tx + 1) / (float)LG_RX, uy1 = (float)(ty + 1) / (float)LG_RY;
+    int minx = (int)((float)w * ux0), miny = (int)((float)h * uy0);
+    int maxx = (int)((float)w * ux1), maxy = (int)((float)h * uy1);
+    oregion R = {w, h, minx, miny, maxx - minx, maxy - miny};
+    return R;
+}
+
+int orc_env_illuminate_adaptive(const float* rgba, const float* lum, const float* sat, int32_t w, int32_t h,
+                                const float* lg_func, const float* lg_sat_arr, const float* cam, int32_t n,
+                                const float* hit, const float* u, float* dir, float* contrib, float* pdfw,
+                                float* lum_out, int32_t* gidx, int32_t* texel_xy)
+{ /* illuminateEnvironmentLight1, Light.cl:140-193 */
+    if (w < LG_RX || h < LG_RY) return ORC_ERR_ARG;
+    for (int32_t i = 0; i < n; i++) {
+        const float* q = u + 4 * (size_t)i;
+        int s = cell_from_camera(cam, f4_load(hit + 4 * (size_t)i));
+        int lx, ly;
+        float luu, lvv, lpdf;
+        lg_sample_one(lg_func, lg_sat_arr, s, q[0], q[1], &lx, &ly, &luu, &lvv, &lpdf);
+        oregion R = tile_region(w, h, lx, ly);
+        float uv[2], pdf;
+        int off[2];
+        r_sample(&R, sat, lum, q[2], q[3], uv, off, &pdf);
+        f4 d = spherical_direction(uv[0], uv[1]);
+        int idx = spherical_index(w, h, d);
+        float sn = sinf(uv[1] * ORC_PI);
+        float pw = pdf / (((2.0f * ORC_PI) * ORC_PI) * sinf(uv[1] * ORC_PI));
+        if (sn == 0.0f) pw = 0.0f;
+        dir[4 * i] = d.x; dir[4 * i + 1] = d.y; dir[4 * i + 2] = d.z; dir[4 * i + 3] = 0.0f;
+        for (int k = 0; k < 4; k++) contrib[4 * i + k] = rgba[4 * (size_t)idx + k];
+        pdfw[i] = pw;
+        lum_out[i] = luminance3(rgba[4 * (size_t)idx], rgba[4 * (size_t)idx + 1], rgba[4 * (size_t)idx + 2]);
+        gidx[i] = lg_global_index(s, lx, ly);
+        if (texel_xy) {
+            texel_xy[2 * i] = off[0];
+            texel_xy[2 * i + 1] = off[1];
+        }
+    }
+    return ORC_OK;
+}
+
+int orc_env_radiance_adaptive(const float* rgba, const float* lum, const float* sat, int32_t w, int32_t h,
+                              float* accum, const float* cam, int32_t n, const float* dir, const float* hit,
+                              float* contrib, float* pdfa, float* lum_out, int32_t* gidx)
+{ /* getRadianceEnvironmentLight1, Light.cl:229-268; accumLightGrid :35-38 in index order */
+    if (w < LG_RX || h < LG_RY) return ORC_ERR_ARG;
+    for (int32_t i = 0; i < n; i++) {
+        f4 d = f4_make(dir[4 * i], dir[4 * i + 1], dir[4 * i + 2], 0.0f);
+        int s = cell_from_camera(cam, f4_load(hit + 4 * (size_t)i));
+        int t = spherical_index(LG_RX, LG_RY, d); /* tileIndexXYFromCamera, LightGrid.cl:80-90 */
+        int tx = t % LG_RX, ty = t / LG_RX;
+        oregion R = tile_region(w, h, tx, ty);
+        int x, y;
+        spherical_xy(w, h, d, &x, &y);
+        float p = r_pdf_cond(&R, sat, lum, x, y) * r_pdf_marg(&R, sat, y); /* getPdfSAT, SAT.cl:336-342 */
+        float sv = (float)y / (float)h;
+        pdfa[i] = p / (((2.0f * ORC_PI) * ORC_PI) * sinf(sv * ORC_PI));
+        int idx = spherical_index(w, h, d);
+        for (int k = 0; k < 4; k++) contrib[4 * i + k] = rgba[4 * (size_t)idx + k];
+        float l = luminance3(rgba[4 * (size_t)idx], rgba[4 * (size_t)idx + 1], rgba[4 * (size_t)idx + 2]);
+        lum_out[i] = l;
+        gidx[i] = lg_global_index(s, tx, ty);
+        if (accum) accum[gidx[i]] += l;
     }
     return ORC_OK;
 }

@@ -142,3 +142,73 @@ def test_region_sampling_bit_exact(ctx):
         np.testing.assert_array_equal(off.cpu().numpy(), ooff)
         np.testing.assert_array_equal(uv.cpu().numpy().view(np.int32), ouv.view(np.int32))
         np.testing.assert_array_equal(pdf.cpu().numpy().view(np.int32), opdf.view(np.int32))
+
+
+@pytest.mark.parametrize("uniform", [False, True])
+def test_adaptive_environment_light(ctx, uniform):
+    """illuminateEnvironmentLight1 / getRadianceEnvironmentLight1 /
+    accumLightGrid against the oracle on a non-zero camera position: accum
+    entries, tile and texel picks equal away from transcendental rounding
+    boundaries (the camera cell and the texel come from acosf/atan2f/sinf),
+    values within 1e-5; accumulation exact on a constant map (equal addends
+    sum the same in any order), within 1e-5 otherwise."""
+    print(f"[case] adaptive env light uniform={uniform}", flush=True)
+    w, h = 1024, 512
+    rng = np.random.default_rng(31)
+    rgba = np.ones((h, w, 4), np.float32) if uniform else _env_map(w, h)
+    env = api.CEnvironment(ctx)
+    env.setEnvironmentMap(rgba)
+    grid = api.CAdaptiveEnvironment(ctx)
+    accum = (rng.random((1600, 1600)) ** 3 + 0.01).astype(np.float32)
+    grid.write_accum(accum)
+    grid.update()
+    cam = np.array([0.5, 1.0, -4.0], np.float32)
+    grid.setCameraPosition(cam)
+    n = 200000
+    hit = np.concatenate([rng.uniform(-5, 5, (n, 3)), np.zeros((n, 1))], 1).astype(np.float32)
+    u = rng.random((n, 4)).astype(np.float32)
+    d, c, pw, lo, gi = env.illuminate_adaptive(grid, ctx.to_device(hit), ctx.to_device(u))
+    lum, sat = orc.env_lum_sat(rgba.reshape(-1, 4), w, h)
+    _, _, gsat = grid.read_arrays()
+    od, oc, opw, olo, ogi, _ = orc.env_illuminate_adaptive(rgba.reshape(-1, 4), lum, sat, w, h, accum.reshape(-1),
+                                                           gsat.reshape(-1), cam, hit, u)
+    gi, d, c, pw, lo = gi.cpu().numpy(), d.cpu().numpy(), c.cpu().numpy(), pw.cpu().numpy(), lo.cpu().numpy()
+    same = gi == ogi
+    assert (~same).sum() <= n * 1e-3, (~same).sum()
+    np.testing.assert_allclose(d[same], od[same], rtol=0, atol=2e-6)
+    tex = same & (c == oc).all(1)
+    assert (~tex).sum() <= n * 2e-3, (~tex).sum()
+    np.testing.assert_array_equal(lo[tex].view(np.int32), olo[tex].view(np.int32))
+    np.testing.assert_allclose(pw[same], opw[same], rtol=1e-5)
+
+    # escaped rays: radiance with accumulation into a cleared accum
+    dirs = rng.normal(size=(n, 3))
+    dirs = np.concatenate([dirs / np.linalg.norm(dirs, axis=1, keepdims=True), np.zeros((n, 1))], 1).astype(np.float32)
+    grid.clearBuffers()
+    c2, pa, lo2, gi2 = env.radiance_adaptive(grid, ctx.to_device(dirs), ctx.to_device(hit))
+    acc_gpu = grid.read_arrays()[0].reshape(-1)
+    oacc = np.zeros(1600 * 1600, np.float32)
+    oc2, opa, olo2, ogi2 = orc.env_radiance_adaptive(rgba.reshape(-1, 4), lum, sat, w, h, cam, dirs, hit, accum=oacc)
+    gi2, pa, lo2 = gi2.cpu().numpy(), pa.cpu().numpy(), lo2.cpu().numpy()
+    same = gi2 == ogi2
+    assert (~same).sum() <= n * 1e-3, (~same).sum()
+    # texel agreement (acosf/atan2f row and column boundaries) from the
+    # full-map radiance call, which reports the texel
+    _, _, tx_gpu = env.radiance(ctx.to_device(dirs))
+    _, _, tx_orc = orc.env_radiance(rgba.reshape(-1, 4), lum, sat, w, h, dirs)
+    both = same & (tx_gpu.cpu().numpy() == tx_orc)
+    assert (~both).sum() <= n * 2e-3, (~both).sum()
+    np.testing.assert_allclose(pa[both], opa[both], rtol=1e-5)
+    # the accum holds exactly the GPU's own (entry, luminance) pairs
+    want = np.zeros(1600 * 1600, np.float32 if uniform else np.float64)
+    np.add.at(want, gi2, lo2.astype(want.dtype))
+    if uniform:
+        np.testing.assert_array_equal(acc_gpu.view(np.int32), want.view(np.int32))
+        # and the oracle's accum wherever no mismatched sample landed
+        clean = np.ones(1600 * 1600, bool)
+        clean[gi2[~same]] = False
+        clean[ogi2[~same]] = False
+        np.testing.assert_array_equal(acc_gpu[clean].view(np.int32), oacc[clean].view(np.int32))
+    else:
+        np.testing.assert_allclose(acc_gpu, want, rtol=1e-5, atol=0)
+    assert (acc_gpu > 0).sum() > 1000

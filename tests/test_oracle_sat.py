@@ -199,3 +199,92 @@ def test_uniform_environment_pdf_closed_form():
     top = (y > 0) & ~wrap
     np.testing.assert_allclose(pw2[top], 1.0 / (2 * math.pi ** 2 * np.sin(math.pi * y[top] / h)), rtol=1e-5)
     assert np.isinf(pw2[(y == 0) & ~wrap]).all()
+
+
+# ---------------------------------------------------------------- adaptive environment light
+def _sph_bin(d, w, h):
+    """getSphericalGridXY (Util.cl:397-410) in float64: the (x, y) bin of direction d."""
+    d = d / np.linalg.norm(d, axis=1, keepdims=True)
+    phi = np.arccos(np.clip(d[:, 1], -1, 1))
+    theta = np.arctan2(d[:, 2], d[:, 0])
+    return ((0.5 - 0.5 * theta / math.pi) * w).astype(np.int64), (phi / math.pi * h).astype(np.int64)
+
+
+def _adaptive_inputs(rng, n, w, h, uniform):
+    rgba = np.ones((w * h, 4), np.float32) if uniform else \
+        np.concatenate([(rng.random((w * h, 3)) ** 2 + 0.01), np.ones((w * h, 1))], 1).astype(np.float32)
+    lum, sat = orc.env_lum_sat(rgba, w, h)
+    accum = (rng.random(1600 * 1600) ** 3 + 0.01).astype(np.float32)
+    lg_sat = orc.sat_tiles(accum, 1600, 1600, 16, 32)
+    cam = np.array([0.3, -0.2, 0.1], np.float32)
+    hit = np.concatenate([rng.uniform(-3, 3, (n, 3)), np.zeros((n, 1))], 1).astype(np.float32)
+    u = rng.random((n, 4)).astype(np.float32)
+    return rgba, lum, sat, accum, lg_sat, cam, hit, u
+
+
+@pytest.mark.parametrize("uniform", [False, True])
+def test_adaptive_illuminate_structure(uniform):
+    """illuminateEnvironmentLight1 pinned by its structure: the accum entry
+    names the camera cell (independent float64 spherical binning) and the
+    tile the cell's distribution picks (lg_sample, pinned above); the texel
+    lies in that tile's rectangle of the map (closed form: tile (tx, ty) of a
+    w x h map covers x in [tx w/16, (tx+1) w/16), y in [ty h/32, (ty+1)
+    h/32)) and equals region sampling of that rectangle; on a constant map
+    the pdf is 1 / (2 pi^2 sin(theta))."""
+    rng = np.random.default_rng(21)
+    n, w, h = 4000, 256, 128
+    rgba, lum, sat, accum, lg_sat, cam, hit, u = _adaptive_inputs(rng, n, w, h, uniform)
+    d, c, pw, lo, gi, txy = orc.env_illuminate_adaptive(rgba, lum, sat, w, h, accum, lg_sat, cam, hit, u)
+    gx, gy = gi % 1600, gi // 1600
+    cx, lx, cy, ly = gx // 16, gx % 16, gy // 32, gy % 32
+    bx, by = _sph_bin(hit[:, :3].astype(np.float64) - cam.astype(np.float64), 100, 50)
+    assert ((bx == cx) & (by == cy)).mean() > 0.999
+    tl, _, _ = orc.lg_sample(accum, lg_sat, (cx + 100 * cy).astype(np.int32), u[:, :2])
+    np.testing.assert_array_equal(tl[:, 0], lx)
+    np.testing.assert_array_equal(tl[:, 1], ly)
+    sx, sy = w // 16, h // 32
+    assert ((txy[:, 0] >= lx * sx) & (txy[:, 0] < (lx + 1) * sx)).all()
+    assert ((txy[:, 1] >= ly * sy) & (txy[:, 1] < (ly + 1) * sy)).all()
+    for k in range(0, n, 97):
+        _, off, pdf = orc.sat_sample_region(lum, sat, w, h, int(lx[k]) * sx, int(ly[k]) * sy, sx, sy, u[k:k + 1, 2:])
+        assert tuple(off[0]) == tuple(txy[k])
+    np.testing.assert_array_equal(lo, np.array([orc_lum(x) for x in c], np.float32))
+    if uniform:
+        D = d[:, :3].astype(np.float64)
+        sin_theta = np.hypot(D[:, 0], D[:, 2])
+        ok = sin_theta > 1e-3
+        np.testing.assert_allclose(pw[ok], 1.0 / (2 * math.pi ** 2 * sin_theta[ok]), rtol=2e-3)
+
+
+def orc_lum(c):
+    """Luminance, RGBSpace.cl:28-32, in float32 left to right."""
+    r, g, b = (np.float32(x) for x in c[:3])
+    return np.float32(np.float32(np.float32(0.2989) * r + np.float32(0.5866) * g) + np.float32(0.1145) * b)
+
+
+def test_adaptive_radiance_structure_and_accumulation():
+    """getRadianceEnvironmentLight1 + accumLightGrid: the entry is (camera
+    cell, tile of the ray direction) by independent float64 binning; on a
+    constant map the in-tile pdf is 1, so pdfa = 1 / (2 pi^2 sin(pi y / h))
+    for texels inside their tile; accum gains count * luminance per entry."""
+    rng = np.random.default_rng(22)
+    n, w, h = 20000, 256, 128
+    rgba, lum, sat, _, _, cam, hit, _ = _adaptive_inputs(rng, n, w, h, True)
+    dirs = rng.normal(size=(n, 3))
+    dirs = np.concatenate([dirs / np.linalg.norm(dirs, axis=1, keepdims=True), np.zeros((n, 1))], 1).astype(np.float32)
+    acc = np.zeros(1600 * 1600, np.float32)
+    c, pa, lo, gi = orc.env_radiance_adaptive(rgba, lum, sat, w, h, cam, dirs, hit, accum=acc)
+    gx, gy = gi % 1600, gi // 1600
+    bx, by = _sph_bin(hit[:, :3].astype(np.float64) - cam.astype(np.float64), 100, 50)
+    tx, ty = _sph_bin(dirs[:, :3].astype(np.float64), 16, 32)
+    same = (bx == gx // 16) & (by == gy // 32) & (tx == gx % 16) & (ty == gy % 32)
+    assert same.mean() > 0.999
+    x, y = _sph_bin(dirs[:, :3].astype(np.float64), w, h)
+    inside = (x // (w // 16) == gx % 16) & (y // (h // 32) == gy % 32) & (y > 0) & (x < w)
+    assert inside.mean() > 0.99
+    np.testing.assert_allclose(pa[inside], 1.0 / (2 * math.pi ** 2 * np.sin(math.pi * y[inside] / h)), rtol=1e-5)
+    L = float(lo[0])
+    assert (lo == lo[0]).all()
+    cnt = np.bincount(gi, minlength=1600 * 1600)
+    np.testing.assert_allclose(acc, cnt * L, rtol=1e-6)
+    assert cnt.max() >= 2
