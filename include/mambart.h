@@ -260,6 +260,13 @@ int32_t mt_shade_hits(mt_context* ctx, const mt_scene* scene,
 int32_t mt_intersect_primitives(mt_context* ctx, const mt_scene* scene, void* rays, void* isects,
                                 const int32_t* d_count, int32_t capacity, int32_t mode);
 
+/* intersectOcclusion (NormalBVH.cl:319-348) over the reference's 80-byte
+ * Ray records (as mt_intersect_primitives): hits[i] = 1 if some triangle
+ * meets the ray within (tMin, tMax) (intersectMesh with bailout), else 0,
+ * for i < *d_count; later slots and the rays are left unchanged. */
+int32_t mt_intersect_occlusion(mt_context* ctx, const mt_scene* scene, const void* rays, int32_t* hits,
+                               const int32_t* d_count, int32_t capacity, int32_t mode);
+
 /* Shadow rays toward a point light from each hit (Render.cl:453 with
  * Light.cl:93-97): origin getPoint(ray, t), direction (light - p)/dist,
  * tMin 0.0001, tMax dist - 2*EPS_RAY.  valid[i] = prim[i] >= 0. */

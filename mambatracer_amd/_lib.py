@@ -157,6 +157,7 @@ SIGNATURES = {
     "mt_trace_any": [P, P, P, P, P, I32, P, P, I32],
     "mt_shade_hits": [P, P, P, P, P, P, P, I32, P, P, P, P, P],
     "mt_intersect_primitives": [P, P, P, P, P, I32, I32],
+    "mt_intersect_occlusion": [P, P, P, P, P, I32, I32],
     "mt_shadow_rays": [P, P, P, P, P, P, P, I32, P, P, P],
     "mt_scan_excl_i32": [P, P, P, P, I32, P],
     "mt_compact": [P, P, I32, P, I32, P, P, C.POINTER(Payload), I32],

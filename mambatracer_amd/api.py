@@ -332,6 +332,18 @@ def intersect_records(ctx: Context, scene: Scene, rays, isects, count=None, mode
     return rays, isects
 
 
+def intersect_occlusion_records(ctx: Context, scene: Scene, rays, hits, count=None, mode: int = 0):
+    """intersectOcclusion over reference Ray records (mt_intersect_occlusion):
+    rays float32 [n, 20] device tensor, hits int32 [n] written below count."""
+    n = rays.shape[0]
+    _need(rays, "rays", n, (RAY_RECORD_WORDS,))
+    _need(hits, "hits", n, (), dtype="int32")
+    if count is None:
+        count = ctx.scalar_i32(n)
+    call("mt_intersect_occlusion", ctx.handle, scene.handle, _ptr(rays), _ptr(hits), _ptr(count), n, mode)
+    return hits
+
+
 def shadow_rays(ctx: Context, o_tmin, d_tmax, prim, t, light_xyz, count=None):
     torch = _torch()
     n = o_tmin.shape[0]

@@ -532,6 +532,32 @@ __global__ __launch_bounds__(256) void k_intersect_records(TraceArgs A, const fl
     iw[20] = 1; // hit = HIT_MARKER
 }
 
+// intersectOcclusion (NormalBVH.cl:319-348) over Ray records: intersectMesh
+// with bailout, i.e. any hit in the reference's visiting order, the ray left
+// unchanged; hits[id] written for id < *count only.
+template <int LAYOUT>
+__global__ __launch_bounds__(256) void k_occlusion_records(TraceArgs A, const float4* __restrict__ rays, int* hits)
+{
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!entry_live(A, id)) return;
+    const float4* R = rays + 5 * (size_t)id;
+    const float4 o4 = R[0], d4 = R[1], inv4 = R[2], sg = R[3], ex = R[4];
+    RayT r;
+    r.o = mk3(o4.x, o4.y, o4.z);
+    r.d = mk3(d4.x, d4.y, d4.z);
+    r.inv = mk3(inv4.x, inv4.y, inv4.z);
+    r.sx = __float_as_int(sg.x) != 0;
+    r.sy = __float_as_int(sg.y) != 0;
+    r.sz = __float_as_int(sg.z) != 0;
+    r.tmin = ex.z;
+    r.tmax = ex.w;
+    int st[3];
+    int child = -1;
+    hits[id] = LAYOUT == 0   ? trace_packed<true, false, true>(A.pn, A.pt, A.root, r, child, st)
+               : LAYOUT == 2 ? trace_packed<true, false, false>(A.pn, A.pt, A.root, r, child, st)
+                             : trace_reference<true, false>(A.nodes, A.bounds, A.faces, A.points, r, child, st);
+}
+
 __global__ __launch_bounds__(256) void k_shadow_rays(const float4* __restrict__ o_tmin,
                                                      const float4* __restrict__ d_tmax,
                                                      const int* __restrict__ prim,
@@ -800,6 +826,35 @@ extern "C" int32_t mt_intersect_primitives(mt_context* ctx, const mt_scene* sc, 
     if (mode == MT_TRACE_REFERENCE) k_intersect_records<1><<<grid, block, 0, s>>>(A, nrm, uvs, R, I);
     else if (mode == MT_TRACE_PACKED_NO_STACK) k_intersect_records<2><<<grid, block, 0, s>>>(A, nrm, uvs, R, I);
     else k_intersect_records<0><<<grid, block, 0, s>>>(A, nrm, uvs, R, I);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_intersect_occlusion(mt_context* ctx, const mt_scene* sc, const void* rays, int32_t* hits,
+                                          const int32_t* d_count, int32_t capacity, int32_t mode)
+{
+    MT_ARG_CHECK(ctx && sc && rays && hits && d_count, "mt_intersect_occlusion: NULL argument");
+    MT_ARG_CHECK(capacity >= 0, "mt_intersect_occlusion: negative capacity");
+    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_REFILL,
+                 "mt_intersect_occlusion: unknown mode %d", mode);
+    MT_ARG_CHECK((reinterpret_cast<uintptr_t>(rays) & 15) == 0, "mt_intersect_occlusion: rays must be 16-byte aligned");
+    if (capacity == 0) return MT_OK;
+    TraceArgs A{};
+    A.nodes = sc->desc.nodes;
+    A.bounds = sc->desc.bounds;
+    A.faces = sc->desc.faces;
+    A.points = reinterpret_cast<const float4*>(sc->desc.points);
+    A.pn = sc->pnodes;
+    A.pt = sc->ptris;
+    A.root = sc->root_ref;
+    A.d_count = d_count;
+    A.capacity = capacity;
+    const float4* R = reinterpret_cast<const float4*>(rays);
+    dim3 grid(mt_blocks(capacity, 256)), block(256);
+    hipStream_t s = ctx->stream;
+    if (mode == MT_TRACE_REFERENCE) k_occlusion_records<1><<<grid, block, 0, s>>>(A, R, hits);
+    else if (mode == MT_TRACE_PACKED_NO_STACK) k_occlusion_records<2><<<grid, block, 0, s>>>(A, R, hits);
+    else k_occlusion_records<0><<<grid, block, 0, s>>>(A, R, hits);
     MT_LAUNCH_CHECK();
     return MT_OK;
 }

@@ -209,3 +209,32 @@ def test_intersect_records_against_oracle(ctx, mode):
     np.testing.assert_array_equal(gi[m, 18], -1)
     keep = np.r_[0:18, 19, 21:24]
     np.testing.assert_array_equal(gi[m][:, keep], 0x7F7F7F7F)
+
+
+@pytest.mark.parametrize("mode", [api._lib.MT_TRACE_PACKED, api._lib.MT_TRACE_REFERENCE,
+                                  api._lib.MT_TRACE_PACKED_NO_STACK])
+def test_occlusion_records_against_oracle(ctx, c0, mode):
+    """intersectOcclusion over 80-byte Ray records: occlusion bit-exact
+    against the oracle's any hit, rays unchanged, slots past *count left as
+    they were."""
+    print(f"[case] occlusion records mode={mode}", flush=True)
+    points, faces, nodes, bounds, _ = c0
+    o, d = c0_rays(256, 256)
+    sc = _scene(ctx, c0)
+    do, dd = ctx.to_device(o), ctx.to_device(d)
+    prim, t, _ = api.intersect_primitives(ctx, sc, do, dd)
+    so, sd, _ = api.shadow_rays(ctx, do, dd, prim, t, np.array([0.3, 0.9, 0.2], np.float32))
+    so, sd = so.cpu().numpy(), sd.cpu().numpy()
+    n = so.shape[0]
+    count = n - 77
+    rec = api.ray_records(so, sd)
+    drays = ctx.to_device(rec)
+    hits = ctx.to_device(np.full(n, -5, np.int32))
+    api.intersect_occlusion_records(ctx, sc, drays, hits, count=ctx.scalar_i32(count), mode=mode)
+    torch.cuda.synchronize()
+    oocc, _, _ = orc.trace_any(points, faces, nodes, bounds, so[:count], sd[:count])
+    got = hits.cpu().numpy()
+    np.testing.assert_array_equal(got[:count], oocc)
+    np.testing.assert_array_equal(got[count:], -5)
+    np.testing.assert_array_equal(drays.cpu().numpy().view(np.int32), rec.view(np.int32))
+    assert 0 < oocc.sum() < count
