@@ -304,6 +304,21 @@ int32_t mt_compact(mt_context* ctx, const int32_t* flags, int32_t flag_mode,
                    const int32_t* d_len, int32_t capacity, int32_t* out_index,
                    int32_t* d_count, const mt_payload* payloads, int32_t npayload);
 
+/* CCompact.execute (CCompact.java:79-89; CompactAtomic Compact.cl:21-40,
+ * InitIntersection / InitIntData Initialize.cl:50-79, TransferIntersection /
+ * TransferPixels Compact.cl:1-18) in place on the reference's own buffers:
+ * of the first *d_len 96-byte Intersection records (Util.cl:141-151) those
+ * with hit != 0 move to the front, each with its pixels[] entry; *d_count =
+ * their number (CCompact's totalCount).  Records [*d_count, *d_len) become
+ * InitIsect records (p, n, ng, d, uv 0; mat = id = -1; hit 0) with pixel 0;
+ * padding bytes are written 0; slots from *d_len on are untouched.
+ * Survivors keep index order (CompactAtomic's order is that of its atomic
+ * increments; the set and count are the same).  temp_isects: caller-owned
+ * scratch of `capacity` records (CCompact's tempIsectBuffer); no temp pixel
+ * buffer is needed.  Records 16-byte aligned. */
+int32_t mt_compact_isects(mt_context* ctx, void* isects, int32_t* pixels, void* temp_isects,
+                          const int32_t* d_len, int32_t capacity, int32_t* d_count);
+
 /* Synchronises the context and reports MT_ERR_STATE if a scan or compaction
  * since the last check gave up waiting on a predecessor tile (a bounded spin
  * that should never trigger); clears the flag.  No counterpart in the

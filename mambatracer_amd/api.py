@@ -472,21 +472,36 @@ def compact(ctx: Context, flags, flag_mode: int = 0, length=None, out_index=True
 class CCompact:
     """CCompact (CCompact.java:30-89): compacts hit slots and their pixel ids.
 
-    init() takes the per-slot hit flags (int32, Intersection.hit), the pixel
-    index queue and the device count; execute() leaves the survivors' pixel
-    ids (and any extra payload streams) in slot order 0..count-1 and the
-    survivor count in totalCount, like the seven reference launches."""
+    init(ctx, isectBuffer, pixels, totalCount), as the reference's init:
+    isectBuffer is either the Intersection records themselves (a [n, 24]
+    float32/int32 device tensor, 96 bytes per record; execute() compacts
+    them in place with mt_compact_isects, tail records re-initialised) or,
+    for SoA callers, the per-slot hit flags (int32 [n]; execute() moves the
+    pixel ids and any `extra` payload streams).  Survivors keep slot order;
+    totalCount receives their number, like the seven reference launches."""
 
-    def init(self, ctx: Context, hit_flags, pixels, totalCount, extra: Sequence = ()):
+    def init(self, ctx: Context, isectBuffer, pixels, totalCount, extra: Sequence = ()):
         self.ctx = ctx
-        self.hit = hit_flags
         self.pixels = pixels
         self.totalCount = totalCount
         self.extra = list(extra)
-        self.length = ctx.scalar_i32(int(hit_flags.shape[0]))
+        n = int(isectBuffer.shape[0])
+        self.records = isectBuffer.dim() == 2 and isectBuffer.shape[1] == INTERSECTION_RECORD_WORDS
+        if self.records:
+            _need(pixels, "pixels", n, (), dtype="int32")
+            self.isects = isectBuffer
+            self.temp = _torch().empty_like(isectBuffer)   # CCompact.tempIsectBuffer
+        else:
+            self.hit = isectBuffer
+        self.length = ctx.scalar_i32(n)
 
     def execute(self) -> None:
         ctx = self.ctx
+        if self.records:
+            n = int(self.isects.shape[0])
+            call("mt_compact_isects", ctx.handle, _ptr(self.isects), _ptr(self.pixels), _ptr(self.temp),
+                 _ptr(self.length), n, _ptr(self.totalCount))
+            return
         streams = [self.pixels, *self.extra]
         _, cnt, dsts = compact(ctx, self.hit, 0, length=self.length, out_index=False, payloads=streams)
         for src, dst in zip(streams, dsts):

@@ -337,6 +337,130 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_compact(const int* __restrict_
     }
 }
 
+// CCompact.execute over the reference's records (Compact.cl:21-40 with the
+// init/transfer kernels, Initialize.cl:50-79): stable compaction of 96-byte
+// Intersection records (Util.cl:141-151: p, n, ng, d float4; uv float2;
+// mat, id, hit int; 12 bytes padding) flagged by hit != 0, carrying the
+// pixel queue along.  Tiles are 1024 records (4 rows) so each lane keeps
+// its records in registers between the flag test and the move: a record is
+// read from HBM once.  Survivors go to the caller's temp buffer with their
+// pixel id in padding word 21; k_isects_finish writes them back in place
+// and fills the tail with InitIsect records (Initialize.cl:13-23).
+constexpr int ISECT_ROWS = 4;
+constexpr int ISECT_TILE = SCAN_THREADS * ISECT_ROWS;
+constexpr int ISECT_F4 = 6; // float4s per Intersection
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_compact_isects(const float4* __restrict__ isects,
+                                                                 const int* __restrict__ pixels,
+                                                                 const int* __restrict__ d_len, int capacity,
+                                                                 float4* __restrict__ tmp,
+                                                                 int* __restrict__ d_count, ScanWork W)
+{
+    __shared__ int s_tile;
+    __shared__ uint32_t s_wave[ISECT_ROWS][SCAN_WAVES];
+    __shared__ uint32_t s_prefix, s_agg;
+    const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const uint32_t epoch = W.ticket[2];
+    int len = *d_len;
+    if (len > capacity) len = capacity;
+    if ((int64_t)blockIdx.x * ISECT_TILE >= len) return;
+    if (tid == 0) s_tile = (int)atomicAdd(&W.ticket[0], 1u);
+    __syncthreads();
+    const int tile = __builtin_amdgcn_readfirstlane(s_tile);
+    const int64_t base = (int64_t)tile * ISECT_TILE;
+    float4 r[ISECT_ROWS][ISECT_F4];
+    uint64_t mask[ISECT_ROWS];
+#pragma unroll
+    for (int k = 0; k < ISECT_ROWS; k++) {
+        const int64_t e = base + k * SCAN_THREADS + tid;
+        bool f = false;
+        if (e < len) {
+            const float4* src = isects + ISECT_F4 * e;
+#pragma unroll
+            for (int q = 0; q < ISECT_F4; q++) r[k][q] = src[q];
+            f = __float_as_int(r[k][5].x) != 0; // isect->hit
+        }
+        mask[k] = __ballot(f);
+        if (lane == 0) s_wave[k][wave] = (uint32_t)__popcll(mask[k]);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < ISECT_ROWS; k++)
+            for (int w = 0; w < SCAN_WAVES; w++) {
+                uint32_t t = s_wave[k][w];
+                s_wave[k][w] = run;
+                run += t;
+            }
+        s_prefix = run;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t agg = s_prefix;
+        uint32_t pre = look_back(W, tile, agg, epoch);
+        if (lane == 0) {
+            s_agg = agg;
+            s_prefix = pre;
+            if ((int64_t)len - 1 < base + ISECT_TILE) *d_count = (int)(pre + agg);
+        }
+    }
+    __syncthreads();
+    const uint32_t pre = s_prefix;
+#pragma unroll
+    for (int k = 0; k < ISECT_ROWS; k++) {
+        if ((mask[k] >> lane) & 1ull) {
+            const int64_t e = base + k * SCAN_THREADS + tid;
+            const uint32_t dst = pre + s_wave[k][wave] + lane_rank(mask[k]);
+            float4* d = tmp + ISECT_F4 * (size_t)dst;
+            float4 w5 = r[k][5];
+            w5.y = __int_as_float(pixels[e]);
+#pragma unroll
+            for (int q = 0; q < 5; q++) d[q] = r[k][q];
+            d[5] = w5;
+        }
+    }
+}
+
+// One lane per float4 of the first *d_len records: survivors back from the
+// temp buffer (pixel id out of the padding, padding cleared), the rest
+// InitIsect (p = n = ng = d = 0, uv = 0, mat = id = -1, hit = 0) with pixel 0
+// (InitIntData, Initialize.cl:72-79).
+__global__ void k_isects_finish(float4* __restrict__ isects, int* __restrict__ pixels,
+                                const float4* __restrict__ tmp, const int* __restrict__ d_len, int capacity,
+                                const int* __restrict__ d_count)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int len = *d_len;
+    if (len > capacity) len = capacity;
+    if (i >= (int64_t)len * ISECT_F4) return;
+    const int64_t j = i / ISECT_F4;
+    const int q = (int)(i - j * ISECT_F4);
+    float4 v;
+    if (j < *d_count) {
+        v = tmp[i];
+        if (q == 5) {
+            pixels[j] = __float_as_int(v.y);
+            v = make_float4(v.x, 0.0f, 0.0f, 0.0f);
+        }
+    } else {
+        v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (q == 4) v = make_float4(0.0f, 0.0f, __int_as_float(-1), __int_as_float(-1));
+        if (q == 5) pixels[j] = 0;
+    }
+    isects[i] = v;
+}
+
+int32_t scan_workspace_tiles(mt_context* ctx, int64_t ntiles, ScanWork& W)
+{
+    size_t bytes = 256 + sizeof(uint64_t) * (size_t)(ntiles > 0 ? ntiles : 1);
+    void* p = nullptr;
+    int32_t rc = mt_ctx_scratch(ctx, bytes, &p);
+    if (rc) return rc;
+    W.ticket = reinterpret_cast<uint32_t*>(p);
+    W.status = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(p) + 256);
+    return MT_OK;
+}
+
 int32_t scan_workspace(mt_context* ctx, int capacity, ScanWork& W)
 {
     int64_t ntiles = ((int64_t)capacity + SCAN_TILE - 1) / SCAN_TILE;
@@ -406,6 +530,31 @@ extern "C" int32_t mt_compact(mt_context* ctx, const int32_t* flags, int32_t fla
     if (capacity == 0) return MT_OK;
     k_compact<<<mt_blocks(capacity, SCAN_TILE), SCAN_THREADS, 0, ctx->stream>>>(
         flags, flag_mode, d_len, capacity, out_index, d_count, P, W);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_compact_isects(mt_context* ctx, void* isects, int32_t* pixels, void* temp_isects,
+                                     const int32_t* d_len, int32_t capacity, int32_t* d_count)
+{
+    MT_ARG_CHECK(ctx && d_len && d_count, "mt_compact_isects: NULL argument");
+    MT_ARG_CHECK(capacity >= 0, "mt_compact_isects: negative capacity");
+    MT_ARG_CHECK(capacity == 0 || (isects && pixels && temp_isects), "mt_compact_isects: NULL buffer");
+    MT_ARG_CHECK(((reinterpret_cast<uintptr_t>(isects) | reinterpret_cast<uintptr_t>(temp_isects)) & 15) == 0,
+                 "mt_compact_isects: records must be 16-byte aligned");
+    ScanWork W;
+    int32_t rc = scan_workspace_tiles(ctx, ((int64_t)capacity + ISECT_TILE - 1) / ISECT_TILE, W);
+    if (rc) return rc;
+    k_scan_reset<<<1, 64, 0, ctx->stream>>>(W, d_len, capacity, d_count);
+    MT_LAUNCH_CHECK();
+    if (capacity == 0) return MT_OK;
+    k_compact_isects<<<mt_blocks(capacity, ISECT_TILE), SCAN_THREADS, 0, ctx->stream>>>(
+        reinterpret_cast<const float4*>(isects), pixels, d_len, capacity, reinterpret_cast<float4*>(temp_isects),
+        d_count, W);
+    MT_LAUNCH_CHECK();
+    k_isects_finish<<<mt_blocks((int64_t)capacity * ISECT_F4, 256), 256, 0, ctx->stream>>>(
+        reinterpret_cast<float4*>(isects), pixels, reinterpret_cast<const float4*>(temp_isects), d_len, capacity,
+        d_count);
     MT_LAUNCH_CHECK();
     return MT_OK;
 }

@@ -94,6 +94,43 @@ def test_ccompact_mirror(ctx):
     np.testing.assert_array_equal(dp.cpu().numpy()[:ocnt], pixels[oidx])
 
 
+@pytest.mark.parametrize("n,length", [(250000, 250000), (100000, 73001), (1024, 1024), (5000, 0)])
+def test_ccompact_records(ctx, n, length):
+    """CCompact.execute on 96-byte Intersection records in place
+    (mt_compact_isects): hit records and their pixel ids packed to the front
+    in slot order (the oracle's stable compaction), the rest of the first
+    `length` records re-initialised (InitIsect, pixel 0), later slots
+    untouched, count set; run twice on the same buffers (temp reuse)."""
+    print(f"[case] ccompact records n={n} length={length}", flush=True)
+    rng = np.random.default_rng(7 + n)
+    rec = rng.integers(-2**31, 2**31 - 1, size=(n, 24), dtype=np.int64).astype(np.int32)
+    rec[:, 20] = np.where(rng.random(n) < 0.55, rng.integers(1, 5, n), 0)   # hit markers, nonzero = hit
+    pixels = rng.permutation(n).astype(np.int32)
+    di, dp = ctx.to_device(rec), ctx.to_device(pixels)
+    cnt = ctx.scalar_i32(-9)
+    cc = api.CCompact()
+    cc.init(ctx, di, dp, cnt)
+    cc.length.fill_(length)
+    for rep in range(2):
+        if rep == 1:   # second pass: the survivors are all hits, the tail misses
+            rec, pixels = di.cpu().numpy().copy(), dp.cpu().numpy().copy()
+        cc.execute()
+        got_r, got_p = di.cpu().numpy(), dp.cpu().numpy()
+        oidx, ocnt = orc.compact_stable((rec[:length, 20] != 0).astype(np.int32))
+        assert int(cnt.item()) == ocnt
+        want = rec[oidx].copy()
+        want[:, 21:24] = 0
+        np.testing.assert_array_equal(got_r[:ocnt], want)
+        np.testing.assert_array_equal(got_p[:ocnt], pixels[oidx])
+        tail = np.zeros((length - ocnt, 24), np.int32)
+        tail[:, 18] = -1
+        tail[:, 19] = -1
+        np.testing.assert_array_equal(got_r[ocnt:length], tail)
+        np.testing.assert_array_equal(got_p[ocnt:length], 0)
+        np.testing.assert_array_equal(got_r[length:], rec[length:])
+        np.testing.assert_array_equal(got_p[length:], pixels[length:])
+
+
 def test_mesh_lights(ctx):
     """identifyMeshLights + prepareLightInfo: emitter faces in index order."""
     rp, rf = api.gen_room(0)
