@@ -238,3 +238,36 @@ def test_occlusion_records_against_oracle(ctx, c0, mode):
     np.testing.assert_array_equal(got[count:], -5)
     np.testing.assert_array_equal(drays.cpu().numpy().view(np.int32), rec.view(np.int32))
     assert 0 < oocc.sum() < count
+
+
+def test_records_edge_cases(ctx):
+    """Record entry points on a zero count (every slot a miss, nothing else
+    written) and on a one-triangle scene (root is a leaf)."""
+    print("[case] record edge cases", flush=True)
+    tri = np.array([[0, 0, 1, 0], [1, 0, 1, 0], [0, 1, 1, 0]], np.float32)
+    faces = np.array([[0, 1, 2, -1, -1, -1, -1, -1, -1, 7]], np.int32)
+    nodes, bounds, depth = api.CNormalBVH.build(tri, faces)
+    sc = api.Scene(ctx, tri, faces, nodes, bounds, depth)
+    o = np.array([[0.2, 0.2, 0, 0.001], [0.9, 0.9, 0, 0.001], [0.2, 0.2, 2, 0.001]], np.float32)
+    d = np.array([[0, 0, 1, np.inf], [0, 0, 1, np.inf], [0, 0, 1, np.inf]], np.float32)
+    rec = api.ray_records(o, d)
+    sentinel = np.full((3, api.INTERSECTION_RECORD_WORDS), 0x55, np.int32)
+    for count in (0, 3):
+        drays, dis = ctx.to_device(rec), ctx.to_device(sentinel)
+        api.intersect_records(ctx, sc, drays, dis, count=ctx.scalar_i32(count))
+        hits = ctx.to_device(np.full(3, -5, np.int32))
+        api.intersect_occlusion_records(ctx, sc, drays, hits, count=ctx.scalar_i32(count))
+        torch.cuda.synchronize()
+        gi, gr = dis.cpu().numpy(), drays.cpu().numpy()
+        if count == 0:
+            np.testing.assert_array_equal(gi[:, 20], 0)
+            np.testing.assert_array_equal(gi[:, 18], -1)
+            np.testing.assert_array_equal(gr.view(np.int32), rec.view(np.int32))
+            np.testing.assert_array_equal(hits.cpu().numpy(), -5)
+        else:
+            np.testing.assert_array_equal(gi[:, 20], [1, 0, 0])   # inside, outside the triangle, behind the origin
+            assert gi[0, 19] == 0 and gi[0, 18] == 7 and gr.view(np.float32)[0, 19] == np.float32(1.0)
+            np.testing.assert_array_equal(gi[1:, 18], -1)
+            np.testing.assert_array_equal(gr[1:].view(np.int32), rec[1:].view(np.int32))
+            # the first ray's tMax is now 1: the occlusion pass sees the hit at t = 1 as outside (tMin, tMax)
+            np.testing.assert_array_equal(hits.cpu().numpy(), [0, 0, 0])
