@@ -540,6 +540,7 @@ extern "C" int32_t mt_compact_isects(mt_context* ctx, void* isects, int32_t* pix
     MT_ARG_CHECK(ctx && d_len && d_count, "mt_compact_isects: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "mt_compact_isects: negative capacity");
     MT_ARG_CHECK(capacity == 0 || (isects && pixels && temp_isects), "mt_compact_isects: NULL buffer");
+    MT_ARG_CHECK(capacity == 0 || isects != temp_isects, "mt_compact_isects: temp_isects must not be isects");
     MT_ARG_CHECK(((reinterpret_cast<uintptr_t>(isects) | reinterpret_cast<uintptr_t>(temp_isects)) & 15) == 0,
                  "mt_compact_isects: records must be 16-byte aligned");
     ScanWork W;
