@@ -340,6 +340,10 @@ int32_t mt_mesh_lights(mt_context* ctx, const mt_scene* scene, const float* mate
  * cells; accum/func/sat are float[2 560 000] each, row-major 1600 wide. */
 int32_t mt_lightgrid_create(mt_context* ctx, mt_lightgrid** out);
 int32_t mt_lightgrid_destroy(mt_lightgrid* grid);
+/* Raw device arrays.  Sampling reads per-cell tables (the sampler's SAT
+ * quotients, computed once per SAT build); asking for func or sat switches
+ * sampling back to the SAT itself until the next clear / set_tile_func /
+ * update / build_sat, so direct writes through these pointers are honoured. */
 int32_t mt_lightgrid_arrays(mt_lightgrid* grid, float** accum, float** func, float** sat);
 /* initLightGrid (LightGrid.cl:404-410; CAdaptiveEnvironment.clearBuffers :123-127) */
 int32_t mt_lightgrid_clear(mt_context* ctx, mt_lightgrid* grid);

@@ -70,6 +70,8 @@ struct mt_lightgrid {
     float* func;
     float* sat;
     float cam[3]; // LightGrid.cameraPosition (LightGrid.cl:26), a launch argument
+    float* tab;    // per-cell sampling tables [LG_CELLS * 640], rebuilt with the SAT
+    bool tab_valid; // false after the raw arrays are handed out, until the next SAT build
 };
 
 struct mt_envmap {
@@ -82,6 +84,10 @@ struct mt_envmap {
     float* tab;
     float *marg, *cond, *fint, *pmarg;
     int *gmarg, *gcond; // search guides over marg / cond rows (same allocation)
+    // per-tile tables of the 16 x 32 light-grid tiles' SAT regions (null when
+    // the map is smaller than the tiling or the tables would be too large)
+    float* ttab;
+    int32_t tsxm, tsym, tstride;
 };
 
 // Guide intervals of the environment sampling tables: the marginal CDF gets

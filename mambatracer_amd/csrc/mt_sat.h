@@ -93,6 +93,64 @@ __device__ __forceinline__ void lg_sample(const float* __restrict__ func, const 
     pdf = pdf_u * pdf_v;
 }
 
+// Per-cell sampling tables: every quantity lg_sample evaluates from SAT
+// corners, each entry the same float expression computed once per SAT build,
+// laid out cell-major so one sample reads a few contiguous cache lines
+// instead of ~60 scattered corners.  Cell s owns LG_TAB floats:
+//   [0, 33)       marg[y]  = getSubgridMarginal(y),        y in [0, 32]
+//   [33, 65)      fint[y]  = getSubgridFuncIntConditional(y)
+//   [65]          last     = getSubgridValueSAT(15, 31)
+//   [80, 80+544)  cond[y][x] = getSubgridConditional(x, y), x in [0, 16], row stride 17
+constexpr int LG_TAB = 640;
+constexpr int LG_TAB_FINT = 33, LG_TAB_LAST = 65, LG_TAB_COND = 80, LG_TAB_CSTRIDE = LG_RX + 1;
+
+__device__ __forceinline__ float lg_table_entry(const float* __restrict__ sat, int s, int i)
+{
+    if (i < LG_TAB_FINT) return lg_marg(sat, s, i);
+    if (i < LG_TAB_LAST) return lg_func_int(sat, s, i - LG_TAB_FINT);
+    if (i == LG_TAB_LAST) return lg_sat(sat, s, LG_RX - 1, LG_RY - 1);
+    if (i < LG_TAB_COND || i >= LG_TAB_COND + LG_RY * LG_TAB_CSTRIDE) return 0.0f;
+    const int k = i - LG_TAB_COND;
+    return lg_cond(sat, s, k % LG_TAB_CSTRIDE, k / LG_TAB_CSTRIDE);
+}
+
+// lg_sample reading the tables: the same bisections over the same values
+// and the same arithmetic on them, so every output has the same bits.
+__device__ __forceinline__ void lg_sample_tab(const float* __restrict__ func, const float* __restrict__ tab,
+                                              int s, float u0, float u1, int& ox, int& oy, float& uu, float& vv,
+                                              float& pdf)
+{
+    const float* T = tab + (size_t)s * LG_TAB;
+    int b = 0, e = LG_RY;
+    while (b != e) {
+        int mid = b + (e - b) / 2;
+        bool right = !(u1 < T[mid]);
+        b = right ? mid + 1 : b;
+        e = right ? e : mid;
+    }
+    oy = b - 1 > 0 ? b - 1 : 0;
+    const float m0 = T[oy];
+    const float du = (u1 - m0) / (T[oy + 1] - m0);
+    const float fi = T[LG_TAB_FINT + oy];
+    const float pdf_v = (fi * (float)(LG_RX * LG_RY)) / T[LG_TAB_LAST];
+    vv = ((float)oy + du) / (float)LG_RY;
+    const float* C = T + LG_TAB_COND + oy * LG_TAB_CSTRIDE;
+    b = 0;
+    e = LG_RX;
+    while (b != e) {
+        int mid = b + (e - b) / 2;
+        bool right = !(u0 < C[mid]);
+        b = right ? mid + 1 : b;
+        e = right ? e : mid;
+    }
+    ox = b - 1 > 0 ? b - 1 : 0;
+    const float c0 = C[ox];
+    const float du2 = (u0 - c0) / (C[ox + 1] - c0);
+    const float pdf_u = func[lg_global_index(s, ox, oy)] / fi;
+    uu = ((float)ox + du2) / (float)LG_RX;
+    pdf = pdf_u * pdf_v;
+}
+
 // ---------------------------------------------------------------- spherical maps
 __device__ __forceinline__ void sph_xy(int w, int h, v3 d, int& x, int& y)
 { // getSphericalGridXY, Util.cl:397-410
@@ -224,13 +282,18 @@ struct EnvView {
     const float* pmarg;
     const int* gmarg;   // search guides (null: plain binary search)
     const int* gcond;
+    const float* ttab;  // light-grid tile-region tables (null: evaluate from the SAT)
+    int tsxm, tsym, tstride;
 };
 
 inline EnvView env_view(const mt_envmap* env)
 {
-    if (!env) return EnvView{nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (!env)
+        return EnvView{nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, 0, 0, 0};
     return EnvView{reinterpret_cast<const float4*>(env->rgba), env->lum, env->sat, env->w, env->h,
-                   env->marg, env->cond, env->fint, env->pmarg, env->gmarg, env->gcond};
+                   env->marg, env->cond, env->fint, env->pmarg, env->gmarg, env->gcond,
+                   env->ttab, env->tsxm, env->tsym, env->tstride};
 }
 
 // upperBoundMarginal / upperBoundConditional (SAT.cl:206-242): the number of
@@ -338,21 +401,102 @@ __device__ __forceinline__ SatRegion lg_tile_region(int w, int h, int tx, int ty
     return SatRegion{w, h, minx, miny, maxx - minx, maxy - miny};
 }
 
+// Tile-region tables.  For tile (tx, ty) with region R = lg_tile_region(...),
+// every quantity r_sample / getPdfSAT evaluates from SAT corners inside
+// the rows and columns the sampler can reach (the bisections stay in the
+// region; the offsets max(0, ptr - 1) reach one row / column before it):
+//   marg[j] = r_marg(R, miny - 1 + j),                j in [0, sym + 2)
+//   fint[j] = r_func_int(R, miny - 1 + j),            j in [0, sym + 1)
+//   last    = the region's total
+//   cond[j][i] = r_cond(R, minx - 1 + i, miny - 1 + j), i in [0, sxm + 2)
+// (entries for negative rows / columns, and past the region, are 0 and
+// never read).  sxm, sym bound every tile's width and height.
+__host__ __device__ inline int env_tile_stride(int sxm, int sym)
+{
+    const int t = (sym + 2) + (sym + 1) + 1 + (sym + 1) * (sxm + 2);
+    return (t + 31) & ~31;
+}
+
+__device__ __forceinline__ float env_tile_entry(const EnvView& E, int k, int i)
+{
+    const SatRegion R = lg_tile_region(E.w, E.h, k % LG_RX, k / LG_RX);
+    const int nm = E.tsym + 2, nf = E.tsym + 1, cs = E.tsxm + 2;
+    if (i < nm) {
+        const int y = R.miny - 1 + i;
+        return (y >= 0 && y <= R.miny + R.sy) ? r_marg(R, E.sat, y) : 0.0f;
+    }
+    i -= nm;
+    if (i < nf) {
+        const int y = R.miny - 1 + i;
+        return (y >= 0 && y < R.miny + R.sy) ? r_func_int(R, E.sat, y) : 0.0f;
+    }
+    i -= nf;
+    if (i == 0) return r_range(R, E.sat, R.minx, R.miny, R.minx + R.sx - 1, R.miny + R.sy - 1);
+    i -= 1;
+    if (i < nf * cs) {
+        const int y = R.miny - 1 + i / cs, x = R.minx - 1 + i % cs;
+        return (y >= 0 && x >= 0 && y < R.miny + R.sy && x <= R.minx + R.sx) ? r_cond(R, E.sat, x, y) : 0.0f;
+    }
+    return 0.0f;
+}
+
+// r_sample on tile region R reading its table T: the same bisections over
+// the same values, the same arithmetic on them.
+__device__ __forceinline__ void r_sample_tile(const EnvView& E, const SatRegion& R, const float* __restrict__ T,
+                                              float u0, float u1, float& uu, float& vv, int& ox, int& oy, float& pdf)
+{
+    const int nm = E.tsym + 2, nf = E.tsym + 1, cs = E.tsxm + 2;
+    const float* M = T - (R.miny - 1);        // M[y] = marg of row y
+    const float* FI = T + nm - (R.miny - 1);  // FI[y]
+    const float last = T[nm + nf];
+    int b = R.miny, e = R.miny + R.sy;
+    while (b != e) {
+        int mid = b + (e - b) / 2;
+        bool right = !(u1 < M[mid]);
+        b = right ? mid + 1 : b;
+        e = right ? e : mid;
+    }
+    oy = b - 1 > 0 ? b - 1 : 0;
+    const float m0 = M[oy];
+    const float du = (u1 - m0) / (M[oy + 1] - m0);
+    const float fi = FI[oy];
+    const float pdf_v = (fi * (float)(R.sx * R.sy)) / last;
+    vv = ((float)oy + du) / (float)R.nv;
+    const float* C = T + nm + nf + 1 + (oy - (R.miny - 1)) * cs - (R.minx - 1); // C[x] = cond(x, oy)
+    b = R.minx;
+    e = R.minx + R.sx;
+    while (b != e) {
+        int mid = b + (e - b) / 2;
+        bool right = !(u0 < C[mid]);
+        b = right ? mid + 1 : b;
+        e = right ? e : mid;
+    }
+    ox = b - 1 > 0 ? b - 1 : 0;
+    const float c0 = C[ox];
+    const float du2 = (u0 - c0) / (C[ox + 1] - c0);
+    const float pdf_u = r_func(R, E.lum, ox, oy) / fi;
+    uu = ((float)ox + du2) / (float)R.nu;
+    pdf = pdf_u * pdf_v;
+}
+
 // illuminateEnvironmentLight1, Light.cl:140-193: tile from the cell's
 // distribution, texel from the tile's SAT region.  The returned pdf is the
 // in-tile pdf only (the tile pick's pdf is not applied, :163,182).
 __device__ __forceinline__ float4 env_illuminate_adaptive(const EnvView& E, const float* __restrict__ lg_func,
-                                                          const float* __restrict__ lg_sat_arr, v3 cam, v3 hit,
+                                                          const float* __restrict__ lg_sat_arr,
+                                                          const float* __restrict__ lg_tab, v3 cam, v3 hit,
                                                           float4 u, v3& dir, float& pdfw, float& lum, int& gidx)
 {
     const int s = lg_cell_from_camera(cam, hit);
     int lx, ly;
     float luu, lvv, lpdf;
-    lg_sample(lg_func, lg_sat_arr, s, u.x, u.y, lx, ly, luu, lvv, lpdf);
+    if (lg_tab) lg_sample_tab(lg_func, lg_tab, s, u.x, u.y, lx, ly, luu, lvv, lpdf);
+    else lg_sample(lg_func, lg_sat_arr, s, u.x, u.y, lx, ly, luu, lvv, lpdf);
     const SatRegion R = lg_tile_region(E.w, E.h, lx, ly);
     float uu, vv, pdf;
     int ox, oy;
-    r_sample(R, E.sat, E.lum, u.z, u.w, uu, vv, ox, oy, pdf);
+    if (E.ttab) r_sample_tile(E, R, E.ttab + (size_t)(ly * LG_RX + lx) * E.tstride, u.z, u.w, uu, vv, ox, oy, pdf);
+    else r_sample(R, E.sat, E.lum, u.z, u.w, uu, vv, ox, oy, pdf);
     dir = sph_direction(uu, vv);
     const float4 c = E.rgba[sph_index(E.w, E.h, dir)];
     const float sn = sinf(vv * MT_PI);
@@ -374,7 +518,15 @@ __device__ __forceinline__ float4 env_radiance_adaptive(const EnvView& E, v3 cam
     const SatRegion R = lg_tile_region(E.w, E.h, tx, ty);
     int x, y;
     sph_xy(E.w, E.h, d, x, y);
-    const float p = r_pdf_cond(R, E.sat, E.lum, x, y) * r_pdf_marg(R, E.sat, y); // getPdfSAT, SAT.cl:336-342
+    float p;
+    if (E.ttab && y >= R.miny - 1 && y >= 0 && y < R.miny + R.sy) { // the table's rows: same values
+        const float* T = E.ttab + (size_t)t * E.tstride;
+        const float fi = T[E.tsym + 2 + (y - (R.miny - 1))];
+        const float last = T[(E.tsym + 2) + (E.tsym + 1)];
+        p = (r_func(R, E.lum, x, y) / fi) * ((fi * (float)(R.sx * R.sy)) / last); // getPdfSAT, SAT.cl:336-342
+    } else {
+        p = r_pdf_cond(R, E.sat, E.lum, x, y) * r_pdf_marg(R, E.sat, y);
+    }
     const float sv = (float)y / (float)E.h;
     pdfa = p / (((2.0f * MT_PI) * MT_PI) * sinf(sv * MT_PI));
     const float4 c = E.rgba[sph_index(E.w, E.h, d)];

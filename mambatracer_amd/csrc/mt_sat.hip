@@ -109,9 +109,25 @@ __global__ void k_sat_cols(float* __restrict__ sat, int64_t nseg, int nu, int nv
     for (int l = 0; l < ry; l++) sat[xi + (yi0 + l) * nu] = a[l];
 }
 
-__global__ void k_lg_sample(const float* __restrict__ func, const float* __restrict__ sat, int n,
-                            const int* __restrict__ cell, const float2* __restrict__ u,
-                            int2* __restrict__ tile_xy, float2* __restrict__ uv, float* __restrict__ pdf)
+__global__ void k_env_tile_tables(EnvView E, float* __restrict__ ttab)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)LG_RX * LG_RY * E.tstride) return;
+    const int k = (int)(i / E.tstride);
+    ttab[i] = env_tile_entry(E, k, (int)(i - (int64_t)k * E.tstride));
+}
+
+// One work-group per cell writes its sampling tables.
+__global__ __launch_bounds__(256) void k_lg_tables(const float* __restrict__ sat, float* __restrict__ tab)
+{
+    const int s = blockIdx.x;
+    for (int i = threadIdx.x; i < LG_TAB; i += blockDim.x) tab[(size_t)s * LG_TAB + i] = lg_table_entry(sat, s, i);
+}
+
+__global__ void k_lg_sample(const float* __restrict__ func, const float* __restrict__ sat,
+                            const float* __restrict__ tab, int n, const int* __restrict__ cell,
+                            const float2* __restrict__ u, int2* __restrict__ tile_xy, float2* __restrict__ uv,
+                            float* __restrict__ pdf)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -125,7 +141,8 @@ __global__ void k_lg_sample(const float* __restrict__ func, const float* __restr
     float2 uu = u[i];
     int ox, oy;
     float a, b, p;
-    lg_sample(func, sat, s, uu.x, uu.y, ox, oy, a, b, p);
+    if (tab) lg_sample_tab(func, tab, s, uu.x, uu.y, ox, oy, a, b, p);
+    else lg_sample(func, sat, s, uu.x, uu.y, ox, oy, a, b, p);
     tile_xy[i] = make_int2(ox, oy);
     uv[i] = make_float2(a, b);
     pdf[i] = p;
@@ -290,7 +307,8 @@ __global__ void k_env_radiance(EnvView E, int n, const float4* __restrict__ dir,
 }
 
 __global__ void k_env_illuminate_adaptive(EnvView E, const float* __restrict__ lg_func,
-                                          const float* __restrict__ lg_sat_arr, float cx, float cy, float cz, int n,
+                                          const float* __restrict__ lg_sat_arr, const float* __restrict__ lg_tab,
+                                          float cx, float cy, float cz, int n,
                                           const float4* __restrict__ hit, const float4* __restrict__ u,
                                           float4* __restrict__ dir, float4* __restrict__ contrib,
                                           float* __restrict__ pdfw, float* __restrict__ lum, int* __restrict__ gidx)
@@ -300,7 +318,8 @@ __global__ void k_env_illuminate_adaptive(EnvView E, const float* __restrict__ l
     v3 d;
     float pw, l;
     int g;
-    const float4 c = env_illuminate_adaptive(E, lg_func, lg_sat_arr, mk3(cx, cy, cz), xyz(hit[i]), u[i], d, pw, l, g);
+    const float4 c =
+        env_illuminate_adaptive(E, lg_func, lg_sat_arr, lg_tab, mk3(cx, cy, cz), xyz(hit[i]), u[i], d, pw, l, g);
     dir[i] = make_float4(d.x, d.y, d.z, 0.0f);
     contrib[i] = c;
     pdfw[i] = pw;
@@ -345,6 +364,13 @@ extern "C" int32_t mt_lightgrid_create(mt_context* ctx, mt_lightgrid** out)
     g->accum = p;
     g->func = p + LG_SIZE;
     g->sat = p + 2 * (size_t)LG_SIZE;
+    if (hipMalloc(&g->tab, sizeof(float) * LG_TAB * (size_t)LG_CELLS) != hipSuccess) {
+        hipFree(p);
+        delete g;
+        mt_set_error("mt_lightgrid_create: device allocation failed");
+        return MT_ERR_ALLOC;
+    }
+    g->tab_valid = false;
     *out = g;
     return mt_lightgrid_clear(ctx, g);
 }
@@ -354,6 +380,7 @@ extern "C" int32_t mt_lightgrid_destroy(mt_lightgrid* g)
     if (!g) return MT_OK;
     hipStreamSynchronize(g->ctx->stream);
     hipFree(g->accum);
+    hipFree(g->tab);
     delete g;
     return MT_OK;
 }
@@ -361,9 +388,21 @@ extern "C" int32_t mt_lightgrid_destroy(mt_lightgrid* g)
 extern "C" int32_t mt_lightgrid_arrays(mt_lightgrid* g, float** accum, float** func, float** sat)
 {
     MT_ARG_CHECK(g != nullptr, "mt_lightgrid_arrays: NULL grid");
+    // the caller may now write func/sat directly: sample from the SAT itself
+    // until the next SAT build refreshes the tables
+    if (func || sat) g->tab_valid = false;
     if (accum) *accum = g->accum;
     if (func) *func = g->func;
     if (sat) *sat = g->sat;
+    return MT_OK;
+}
+
+// Refreshes the sampling tables after the SAT changed (stream-ordered).
+static int32_t lg_refresh_tables(mt_context* ctx, mt_lightgrid* g)
+{
+    k_lg_tables<<<LG_CELLS, 256, 0, ctx->stream>>>(g->sat, g->tab);
+    MT_LAUNCH_CHECK();
+    g->tab_valid = true;
     return MT_OK;
 }
 
@@ -372,7 +411,7 @@ extern "C" int32_t mt_lightgrid_clear(mt_context* ctx, mt_lightgrid* g)
     MT_ARG_CHECK(ctx && g, "mt_lightgrid_clear: NULL argument");
     k_lg_clear<<<mt_blocks(LG_SIZE, 256), 256, 0, ctx->stream>>>(g->accum, g->func, g->sat);
     MT_LAUNCH_CHECK();
-    return MT_OK;
+    return lg_refresh_tables(ctx, g);
 }
 
 extern "C" int32_t mt_lightgrid_set_tile_func(mt_context* ctx, mt_lightgrid* g, const float* tile)
@@ -390,6 +429,11 @@ extern "C" int32_t mt_lightgrid_set_tile_func(mt_context* ctx, mt_lightgrid* g, 
         k_lg_build<<<LG_CELLS / 4, 256, 0, ctx->stream>>>(g->func, g->func, g->sat, 0);
         e = hipGetLastError();
     }
+    if (e == hipSuccess) {
+        k_lg_tables<<<LG_CELLS, 256, 0, ctx->stream>>>(g->sat, g->tab);
+        e = hipGetLastError();
+        g->tab_valid = e == hipSuccess;
+    }
     hipError_t e2 = hipStreamSynchronize(ctx->stream);
     hipFree(dtile);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -404,7 +448,7 @@ extern "C" int32_t mt_lightgrid_update(mt_context* ctx, mt_lightgrid* g)
     MT_ARG_CHECK(ctx && g, "mt_lightgrid_update: NULL argument");
     k_lg_build<<<LG_CELLS / 4, 256, 0, ctx->stream>>>(g->accum, g->func, g->sat, 1);
     MT_LAUNCH_CHECK();
-    return MT_OK;
+    return lg_refresh_tables(ctx, g);
 }
 
 extern "C" int32_t mt_lightgrid_build_sat(mt_context* ctx, mt_lightgrid* g)
@@ -412,7 +456,7 @@ extern "C" int32_t mt_lightgrid_build_sat(mt_context* ctx, mt_lightgrid* g)
     MT_ARG_CHECK(ctx && g, "mt_lightgrid_build_sat: NULL argument");
     k_lg_build<<<LG_CELLS / 4, 256, 0, ctx->stream>>>(g->func, g->func, g->sat, 0);
     MT_LAUNCH_CHECK();
-    return MT_OK;
+    return lg_refresh_tables(ctx, g);
 }
 
 extern "C" int32_t mt_lightgrid_sample(mt_context* ctx, const mt_lightgrid* g, int32_t n,
@@ -422,7 +466,7 @@ extern "C" int32_t mt_lightgrid_sample(mt_context* ctx, const mt_lightgrid* g, i
     MT_ARG_CHECK(ctx && g && cell && u && tile_xy && uv && pdf, "mt_lightgrid_sample: NULL argument");
     MT_ARG_CHECK(n >= 0, "mt_lightgrid_sample: negative n");
     if (n == 0) return MT_OK;
-    k_lg_sample<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(g->func, g->sat, n, cell,
+    k_lg_sample<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(g->func, g->sat, g->tab_valid ? g->tab : nullptr, n, cell,
                                                             reinterpret_cast<const float2*>(u),
                                                             reinterpret_cast<int2*>(tile_xy),
                                                             reinterpret_cast<float2*>(uv), pdf);
@@ -502,9 +546,25 @@ extern "C" int32_t mt_env_create(mt_context* ctx, const float* rgba, const float
     k_env_tables<<<mt_blocks(ncond + h + 1, 256), 256, 0, ctx->stream>>>(sat, w, h, e->marg, e->cond, e->fint,
                                                                          e->pmarg);
     k_env_guides<<<mt_blocks(ngm + ngc, 256), 256, 0, ctx->stream>>>(e->marg, e->cond, w, h, e->gmarg, e->gcond);
+    // tile-region tables for the light-grid guided sampler (16 x 32 tiles)
+    e->ttab = nullptr;
+    if (w >= LG_RX && h >= LG_RY) {
+        e->tsxm = w / LG_RX + 2;
+        e->tsym = h / LG_RY + 2;
+        e->tstride = env_tile_stride(e->tsxm, e->tsym);
+        const int64_t nt = (int64_t)LG_RX * LG_RY * e->tstride;
+        if (nt <= (int64_t)1 << 26 && hipMalloc(&e->ttab, sizeof(float) * (size_t)nt) == hipSuccess) {
+            EnvView E = env_view(e);
+            E.ttab = nullptr; // entries are evaluated from the SAT
+            k_env_tile_tables<<<mt_blocks(nt, 256), 256, 0, ctx->stream>>>(E, e->ttab);
+        } else {
+            e->ttab = nullptr;
+        }
+    }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         hipFree(e->tab);
+        if (e->ttab) hipFree(e->ttab);
         delete e;
         mt_set_error("mt_env_create: %s", hipGetErrorString(err));
         return MT_ERR_HIP;
@@ -518,6 +578,7 @@ extern "C" int32_t mt_env_destroy(mt_envmap* e)
     if (!e) return MT_OK;
     hipStreamSynchronize(e->ctx->stream);
     hipFree(e->tab);
+    if (e->ttab) hipFree(e->ttab);
     delete e;
     return MT_OK;
 }
@@ -596,7 +657,8 @@ extern "C" int32_t mt_env_illuminate_adaptive(mt_context* ctx, const mt_envmap* 
     if (rc) return rc;
     if (n == 0) return MT_OK;
     k_env_illuminate_adaptive<<<mt_blocks(n, 256), 256, 0, ctx->stream>>>(
-        env_view(env), grid->func, grid->sat, grid->cam[0], grid->cam[1], grid->cam[2], n,
+        env_view(env), grid->func, grid->sat, grid->tab_valid ? grid->tab : nullptr, grid->cam[0], grid->cam[1],
+        grid->cam[2], n,
         reinterpret_cast<const float4*>(hit_p), reinterpret_cast<const float4*>(u), reinterpret_cast<float4*>(dir),
         reinterpret_cast<float4*>(contrib), pdfw, lum, grid_index);
     MT_LAUNCH_CHECK();

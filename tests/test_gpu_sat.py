@@ -63,6 +63,14 @@ def test_lightgrid_sampling_bit_exact(ctx):
     np.testing.assert_array_equal(txy.cpu().numpy(), otxy)
     np.testing.assert_array_equal(uv.cpu().numpy().view(np.int32), ouv.view(np.int32))
     np.testing.assert_array_equal(pdf.cpu().numpy().view(np.int32), opdf.view(np.int32))
+    # the per-cell tables (used above) and the SAT corners give the same bits:
+    # handing out the raw arrays switches sampling to the SAT until the next build
+    f = api.C.c_void_p()
+    api.call("mt_lightgrid_arrays", lg.handle, None, api.C.byref(f), None)
+    txy2, uv2, pdf2 = lg.sample(ctx.to_device(cell), ctx.to_device(u))
+    np.testing.assert_array_equal(txy2.cpu().numpy(), otxy)
+    np.testing.assert_array_equal(uv2.cpu().numpy().view(np.int32), ouv.view(np.int32))
+    np.testing.assert_array_equal(pdf2.cpu().numpy().view(np.int32), opdf.view(np.int32))
 
 
 def test_lightgrid_cells(ctx):
