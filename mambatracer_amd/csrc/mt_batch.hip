@@ -522,17 +522,17 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
         return MT_ERR_ALLOC;
     }
     if (hipMalloc(&b->arena, b->arena_bytes) != hipSuccess) {
-        hipFree(b->mem);
+        (void)hipFree(b->mem);
         delete b;
         mt_set_error("mt_batch_create: cannot allocate an arena of %.2f GB", arena_bytes / 1e9);
         return MT_ERR_ALLOC;
     }
     if (hipHostMalloc(&b->h_active, 4 * J) != hipSuccess || hipHostMalloc(&b->h_done, 4 * J) != hipSuccess ||
         hipHostMalloc(&b->h_tot, 16) != hipSuccess) {
-        if (b->h_active) hipHostFree(b->h_active);
-        if (b->h_done) hipHostFree(b->h_done);
-        hipFree(b->arena);
-        hipFree(b->mem);
+        if (b->h_active) (void)hipHostFree(b->h_active);
+        if (b->h_done) (void)hipHostFree(b->h_done);
+        (void)hipFree(b->arena);
+        (void)hipFree(b->mem);
         delete b;
         mt_set_error("mt_batch_create: pinned host allocation failed");
         return MT_ERR_ALLOC;
@@ -583,14 +583,15 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
 extern "C" int32_t mt_batch_destroy(mt_batch* b)
 {
     if (!b) return MT_OK;
-    hipStreamSynchronize(b->ctx->stream);
-    hipHostFree(b->h_active);
-    hipHostFree(b->h_done);
-    hipHostFree(b->h_tot);
-    hipFree(b->arena);
-    hipFree(b->mem);
+    MtFirstError fe;
+    fe(hipStreamSynchronize(b->ctx->stream));
+    fe(hipHostFree(b->h_active));
+    fe(hipHostFree(b->h_done));
+    fe(hipHostFree(b->h_tot));
+    fe(hipFree(b->arena));
+    fe(hipFree(b->mem));
     delete b;
-    return MT_OK;
+    return fe.status("mt_batch_destroy");
 }
 
 extern "C" int32_t mt_batch_submit(mt_batch* b, const mt_job_desc* job)

@@ -76,13 +76,14 @@ extern "C" int32_t mt_ctx_create(int32_t device, void* stream, int32_t flags, mt
 extern "C" int32_t mt_ctx_destroy(mt_context* ctx)
 {
     if (!ctx) return MT_OK;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    if (ctx->scratch) hipFree(ctx->scratch);
-    if (ctx->temp) hipFree(ctx->temp);
-    if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+    MtFirstError fe;
+    fe(hipSetDevice(ctx->device));
+    fe(hipStreamSynchronize(ctx->stream));
+    if (ctx->scratch) fe(hipFree(ctx->scratch));
+    if (ctx->temp) fe(hipFree(ctx->temp));
+    if (ctx->own_stream) fe(hipStreamDestroy(ctx->stream));
     delete ctx;
-    return MT_OK;
+    return fe.status("mt_ctx_destroy");
 }
 
 extern "C" int32_t mt_ctx_sync(mt_context* ctx)

@@ -31,6 +31,22 @@ void mt_set_error(const char* fmt, ...);
 
 #define MT_LAUNCH_CHECK() MT_HIP_CHECK(hipGetLastError())
 
+// Teardown: every release is attempted; the first failure is kept so that a
+// destroy call can still report it.
+struct MtFirstError {
+    hipError_t e = hipSuccess;
+    void operator()(hipError_t x)
+    {
+        if (e == hipSuccess) e = x;
+    }
+    int32_t status(const char* what) const
+    {
+        if (e == hipSuccess) return MT_OK;
+        mt_set_error("%s: %s", what, hipGetErrorString(e));
+        return MT_ERR_HIP;
+    }
+};
+
 // ---------------------------------------------------------------- objects
 struct mt_context {
     int device;

@@ -321,9 +321,9 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
 
 void free_render(RenderPlan& R)
 {
-    for (int k = 0; k < R.nev; k++) hipEventDestroy(R.ev[k]);
+    for (int k = 0; k < R.nev; k++) (void)hipEventDestroy(R.ev[k]);
     R.nev = 0;
-    if (R.base) hipFree(R.base);
+    if (R.base) (void)hipFree(R.base);
     R.base = nullptr;
 }
 
@@ -380,6 +380,12 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
     int* ilp = R.want_ilp ? I(R.oIlp) : nullptr;
     hipEvent_t* ev = R.ev;
     const bool profile = R.profile;
+    auto mark = [&](hipEvent_t e) -> int32_t {
+        const hipError_t x = hipEventRecord(e, st);
+        if (x == hipSuccess) return MT_OK;
+        mt_set_error("mt_render: hipEventRecord failed: %s", hipGetErrorString(x));
+        return MT_ERR_HIP;
+    };
 
     k_zero_i32<<<1, 64, 0, st>>>(counts, 64);
     k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(shc, 64 * SHADOW_SEGS);
@@ -398,7 +404,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
     for (int b = 0; b < nb && rc == MT_OK; b++) {
         int* dcount = counts + b;
         const bool first = fused && b == 0;
-        if (profile) hipEventRecord(ev[4 * b], st);
+        if (profile && (rc = mark(ev[4 * b]))) break;
         if (first) {
             k_trace_camera<<<grid, RT, 0, st>>>(R.C, np, seed_ptr(R), scene->pnodes, scene->ptris, scene->root_ref,
                                                 I(R.oPrim), reinterpret_cast<float*>(base + R.oT_), counts);
@@ -408,7 +414,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
                                   dcount, np, I(R.oPrim), reinterpret_cast<float*>(base + R.oT_), ilp, desc->mode);
         }
         if (rc) break;
-        if (profile) hipEventRecord(ev[4 * b + 1], st);
+        if (profile && (rc = mark(ev[4 * b + 1]))) break;
         if (ilp) k_sum_ilp<<<grid, RT, 0, st>>>(ilp, dcount, nullptr, 0, tot);
         ShadeArgs A;
         A.V = R.V;
@@ -434,18 +440,18 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         A.seed = seed_ptr(R);
         k_shade<<<gshade, RT, 0, st>>>(A);
         if ((rc = chk())) break;
-        if (profile) hipEventRecord(ev[4 * b + 2], st);
+        if (profile && (rc = mark(ev[4 * b + 2]))) break;
         if (fused) {
             k_trace_shadow<<<gshadow, RT, 0, st>>>(scene->pnodes, scene->ptris, scene->root_ref, seg_counts, seg_cap,
                                                     F4(R.oSo), F4(R.oSd), F4(R.oSc), first ? nullptr : q[cur], S.L);
             if ((rc = chk())) break;
-            if (profile) hipEventRecord(ev[4 * b + 3], st);
+            if (profile && (rc = mark(ev[4 * b + 3]))) break;
         } else {
             // any hit over the segmented shadow queue (segment lengths on the device)
             rc = mt_trace_any_seg(ctx, scene, F4(R.oSo), F4(R.oSd), MtSegQueue{seg_counts, SHADOW_SEGS, seg_cap},
                                   I(R.oOcc), ilp, desc->mode);
             if (rc) break;
-            if (profile) hipEventRecord(ev[4 * b + 3], st);
+            if (profile && (rc = mark(ev[4 * b + 3]))) break;
             if (ilp) k_sum_ilp<<<gshadow, RT, 0, st>>>(ilp, nullptr, seg_counts, seg_cap, tot + 3);
             k_nee_accum<<<gshadow, RT, 0, st>>>(q[cur], seg_counts, seg_cap, F4(R.oSc), I(R.oOcc), S);
             if ((rc = chk())) break;
@@ -537,8 +543,12 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
     k_set_seed<<<1, 64, 0, ctx->stream>>>(seed_ptr(R), desc->seed);
     rc = enqueue_render(ctx, R);
     if (rc == MT_OK && stats) rc = read_stats(ctx, R, stats);
-    hipStreamSynchronize(ctx->stream);
+    const hipError_t se = hipStreamSynchronize(ctx->stream);
     free_render(R);
+    if (rc == MT_OK && se != hipSuccess) {
+        mt_set_error("mt_render: %s", hipGetErrorString(se));
+        rc = MT_ERR_HIP;
+    }
     if (rc == MT_OK) rc = mt_scan_check(ctx);
     return rc;
 }
@@ -556,13 +566,14 @@ struct mt_render_graph {
 extern "C" int32_t mt_render_graph_destroy(mt_render_graph* g)
 {
     if (!g) return MT_OK;
-    hipStreamSynchronize(g->ctx->stream);
-    if (g->exec) hipGraphExecDestroy(g->exec);
-    if (g->graph) hipGraphDestroy(g->graph);
+    MtFirstError fe;
+    fe(hipStreamSynchronize(g->ctx->stream));
+    if (g->exec) fe(hipGraphExecDestroy(g->exec));
+    if (g->graph) fe(hipGraphDestroy(g->graph));
     free_render(g->R);
-    if (g->scratch) hipFree(g->scratch);
+    if (g->scratch) fe(hipFree(g->scratch));
     delete g;
-    return MT_OK;
+    return fe.status("mt_render_graph_destroy");
 }
 
 extern "C" int32_t mt_render_graph_create(mt_context* ctx, const mt_scene* scene, const float* materials,
@@ -627,7 +638,7 @@ extern "C" int32_t mt_render_graph_create(mt_context* ctx, const mt_scene* scene
     ctx->stream = ctx_stream;
     ctx->scratch = ctx_scratch;
     ctx->scratch_bytes = ctx_scratch_bytes;
-    if (cs) hipStreamDestroy(cs);
+    if (cs) (void)hipStreamDestroy(cs);
     if (rc == MT_OK) {
         hipError_t e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
         if (e != hipSuccess) {

@@ -365,7 +365,7 @@ extern "C" int32_t mt_lightgrid_create(mt_context* ctx, mt_lightgrid** out)
     g->func = p + LG_SIZE;
     g->sat = p + 2 * (size_t)LG_SIZE;
     if (hipMalloc(&g->tab, sizeof(float) * LG_TAB * (size_t)LG_CELLS) != hipSuccess) {
-        hipFree(p);
+        (void)hipFree(p);
         delete g;
         mt_set_error("mt_lightgrid_create: device allocation failed");
         return MT_ERR_ALLOC;
@@ -378,11 +378,12 @@ extern "C" int32_t mt_lightgrid_create(mt_context* ctx, mt_lightgrid** out)
 extern "C" int32_t mt_lightgrid_destroy(mt_lightgrid* g)
 {
     if (!g) return MT_OK;
-    hipStreamSynchronize(g->ctx->stream);
-    hipFree(g->accum);
-    hipFree(g->tab);
+    MtFirstError fe;
+    fe(hipStreamSynchronize(g->ctx->stream));
+    fe(hipFree(g->accum));
+    fe(hipFree(g->tab));
     delete g;
-    return MT_OK;
+    return fe.status("mt_lightgrid_destroy");
 }
 
 extern "C" int32_t mt_lightgrid_arrays(mt_lightgrid* g, float** accum, float** func, float** sat)
@@ -435,7 +436,8 @@ extern "C" int32_t mt_lightgrid_set_tile_func(mt_context* ctx, mt_lightgrid* g, 
         g->tab_valid = e == hipSuccess;
     }
     hipError_t e2 = hipStreamSynchronize(ctx->stream);
-    hipFree(dtile);
+    if (e2 == hipSuccess) e2 = hipFree(dtile);
+    else (void)hipFree(dtile);
     if (e != hipSuccess || e2 != hipSuccess) {
         mt_set_error("mt_lightgrid_set_tile_func: %s", hipGetErrorString(e != hipSuccess ? e : e2));
         return MT_ERR_HIP;
@@ -563,8 +565,8 @@ extern "C" int32_t mt_env_create(mt_context* ctx, const float* rgba, const float
     }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
-        hipFree(e->tab);
-        if (e->ttab) hipFree(e->ttab);
+        (void)hipFree(e->tab);
+        if (e->ttab) (void)hipFree(e->ttab);
         delete e;
         mt_set_error("mt_env_create: %s", hipGetErrorString(err));
         return MT_ERR_HIP;
@@ -576,11 +578,12 @@ extern "C" int32_t mt_env_create(mt_context* ctx, const float* rgba, const float
 extern "C" int32_t mt_env_destroy(mt_envmap* e)
 {
     if (!e) return MT_OK;
-    hipStreamSynchronize(e->ctx->stream);
-    hipFree(e->tab);
-    if (e->ttab) hipFree(e->ttab);
+    MtFirstError fe;
+    fe(hipStreamSynchronize(e->ctx->stream));
+    fe(hipFree(e->tab));
+    if (e->ttab) fe(hipFree(e->ttab));
     delete e;
-    return MT_OK;
+    return fe.status("mt_env_destroy");
 }
 
 extern "C" int32_t mt_sat_sample_region(mt_context* ctx, const float* func, const float* sat, int32_t nu,

@@ -603,7 +603,8 @@ extern "C" int32_t mt_mesh_lights(mt_context* ctx, const mt_scene* scene, const 
     int32_t rc = mt_buf_fill_i32(ctx, tmp + n, n, 1);
     if (!rc) rc = mt_compact(ctx, tmp, 0, tmp + n, n, light_faces, d_count, nullptr, 0);
     hipError_t e = hipStreamSynchronize(ctx->stream);
-    hipFree(tmp);
+    if (e == hipSuccess) e = hipFree(tmp);
+    else (void)hipFree(tmp);
     if (rc) return rc;
     if (e != hipSuccess) {
         mt_set_error("mt_mesh_lights: %s", hipGetErrorString(e));

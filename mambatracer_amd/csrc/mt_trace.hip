@@ -608,9 +608,9 @@ extern "C" int32_t mt_scene_create(mt_context* ctx, const mt_scene_desc* desc, m
     sc->ptris = nullptr;
     int* work = nullptr; // [nn] flags, [nn] ranks, [2] err/total
     auto fail = [&](int32_t code) {
-        if (work) hipFree(work);
-        if (sc->pnodes) hipFree(sc->pnodes);
-        if (sc->ptris) hipFree(sc->ptris);
+        if (work) (void)hipFree(work);
+        if (sc->pnodes) (void)hipFree(sc->pnodes);
+        if (sc->ptris) (void)hipFree(sc->ptris);
         delete sc;
         return code;
     };
@@ -664,7 +664,7 @@ extern "C" int32_t mt_scene_create(mt_context* ctx, const mt_scene_desc* desc, m
         mt_set_error("mt_scene_create: packing failed");
         return fail(MT_ERR_HIP);
     }
-    hipFree(work);
+    (void)hipFree(work);
     sc->root_ref = desc->num_tris == 1 ? ~0 : 0;
     *out = sc;
     return MT_OK;
@@ -673,11 +673,12 @@ extern "C" int32_t mt_scene_create(mt_context* ctx, const mt_scene_desc* desc, m
 extern "C" int32_t mt_scene_destroy(mt_scene* sc)
 {
     if (!sc) return MT_OK;
-    hipStreamSynchronize(sc->ctx->stream);
-    if (sc->pnodes) hipFree(sc->pnodes);
-    if (sc->ptris) hipFree(sc->ptris);
+    MtFirstError fe;
+    fe(hipStreamSynchronize(sc->ctx->stream));
+    if (sc->pnodes) fe(hipFree(sc->pnodes));
+    if (sc->ptris) fe(hipFree(sc->ptris));
     delete sc;
-    return MT_OK;
+    return fe.status("mt_scene_destroy");
 }
 
 extern "C" int32_t mt_trace_closest(mt_context* ctx, const mt_scene* scene, const float* o_tmin,
@@ -708,7 +709,7 @@ extern "C" int32_t mt_trace_autotune(mt_context* ctx, const mt_scene* scene, con
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MT_HIP_CHECK(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
-        hipEventDestroy(e0);
+        (void)hipEventDestroy(e0);
         mt_set_error("mt_trace_autotune: hipEventCreate failed");
         return MT_ERR_HIP;
     }
@@ -742,8 +743,8 @@ extern "C" int32_t mt_trace_autotune(mt_context* ctx, const mt_scene* scene, con
             pick = mode;
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc) return rc;
     *best_mode = pick;
     *best_ms = best;
