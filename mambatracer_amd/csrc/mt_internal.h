@@ -143,6 +143,8 @@ int32_t mt_trace_jobs(mt_context* ctx, const MtSceneRef* refs, const int32_t* ta
 // Any hit over a segmented queue for one scene (traversal mode as
 // mt_trace_any; the per-lane refill mode traverses like the short-stack mode
 // here); stats as mt_trace_any, indexed by queue entry.
+int32_t mt_trace_closest_seg(mt_context* ctx, const mt_scene* scene, const float4* o_tmin, const float4* d_tmax,
+                             MtSegQueue seg, int32_t* prim, float* t, int32_t mode);
 int32_t mt_trace_any_seg(mt_context* ctx, const mt_scene* scene, const float4* o_tmin, const float4* d_tmax,
                          MtSegQueue seg, int32_t* occluded, int32_t* stats, int32_t mode);
 

@@ -121,7 +121,12 @@ struct ShadeArgs {
     float4* d_tmax;
     const int* prim;
     const float* tin;
-    int* alive;           // out: 1 if the slot's path continues
+    int* alive;           // out: 1 if the slot's path continues (unfused loop)
+    const int* in_seg;    // fused loop, bounces >= 1: segment lengths of this bounce's queue
+    int* next_seg;        // fused loop: segment lengths of the next queue, which
+    int* q_next;          //   continuing paths append to (path id, next ray)
+    float4* o_next;
+    float4* d_next;
     float4* s_o_tmin;     // out: segmented queue of shadow rays
     float4* s_d_tmax;
     float4* s_contrib;    // out: unoccluded contribution, w = the ray's slot (int bits)
@@ -139,8 +144,9 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, uint32_t seed, in
 {
     PathOut P;
     P.has_shadow = false;
+    P.alive = false;
+    int pid = -1;
     if (live) {
-        int pid;
         float4 ro, rd, T4;
         if (A.first) { // what k_camera would have stored: InitPathData, Initialize.cl:25-48
             pid = slot;
@@ -156,8 +162,10 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, uint32_t seed, in
         shade_path(A.V, seed, A.rr_start, A.flags, pid, A.bounce, ro, rd, T4, A.prim[slot], A.tin[slot], P);
         if (P.alive) {
             A.S.T[pid] = P.T;
-            A.o_tmin[slot] = P.ro;
-            A.d_tmax[slot] = P.rd;
+            if (!A.next_seg) {
+                A.o_tmin[slot] = P.ro;
+                A.d_tmax[slot] = P.rd;
+            }
         }
         if (A.first) { // radiance starts at 0 (the same sums as k_camera's zero plus Ladd)
             A.S.L[pid] = make_float4(0.0f + P.Ladd.x, 0.0f + P.Ladd.y, 0.0f + P.Ladd.z, 0.0f);
@@ -165,9 +173,20 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, uint32_t seed, in
             float4 L = A.S.L[pid];
             A.S.L[pid] = make_float4(L.x + P.Ladd.x, L.y + P.Ladd.y, L.z + P.Ladd.z, L.w);
         }
-        A.alive[slot] = P.alive ? 1 : 0;
+        if (!A.next_seg) A.alive[slot] = P.alive ? 1 : 0;
     }
     const int seg = blockIdx.x % SHADOW_SEGS;
+    if (A.next_seg) {
+        // the next queue, segmented like the shadow queue: no result depends
+        // on its order (everything is indexed by path id), so continuing
+        // paths are appended here instead of compacted in a separate pass
+        const int kn = seg * A.seg_cap + wave_append(P.alive, A.next_seg + seg);
+        if (P.alive) {
+            A.q_next[kn] = pid;
+            A.o_next[kn] = P.ro;
+            A.d_next[kn] = P.rd;
+        }
+    }
     const int k = seg * A.seg_cap + wave_append(P.has_shadow, A.shadow_count + seg);
     if (P.has_shadow) {
         A.s_o_tmin[k] = P.so;
@@ -180,10 +199,20 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, uint32_t seed, in
 // assume exactly that); work-groups past *d_count leave at once.
 __global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
 {
-    const int count = *A.d_count;
-    if ((int64_t)blockIdx.x * blockDim.x >= count) return;
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    shade_slot(A, *A.seed, slot, slot < count);
+    const int base = blockIdx.x * blockDim.x;
+    const int slot = base + threadIdx.x;
+    bool live;
+    if (A.in_seg) { // a work-group lies inside one segment (seg_cap is a multiple of RT)
+        const int p = base / A.seg_cap;
+        const int c = A.in_seg[p];
+        if (base - p * A.seg_cap >= c) return;
+        live = slot - p * A.seg_cap < c;
+    } else {
+        const int count = *A.d_count;
+        if (base >= count) return;
+        live = slot < count;
+    }
+    shade_slot(A, *A.seed, slot, live);
 }
 
 // Unoccluded shadow-queue entries add their contribution to their path.
@@ -252,8 +281,9 @@ struct RenderPlan {
     bool want_ilp, profile;
     char* base;
     size_t bytes;
-    size_t oT, oL, oA_o, oA_d, oB_o, oB_d, oQA, oQB, oPrim, oT_, oAlive, oOcc, oSo, oSd, oSc, oCnt, oShc, oIlp,
-        oTot, oSeed;
+    size_t oT, oL, oA_o, oA_d, oB_o, oB_d, oQA, oQB, oPrim, oT_, oAlive, oOcc, oSo, oSd, oSc, oCnt, oShc, oQsc,
+        oIlp, oTot, oSeed;
+    bool fused; // camera rays traced in place, shadow adds fused, next queue appended by the shading stage
     hipEvent_t ev[4 * 64];
     int nev;
 };
@@ -290,19 +320,22 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
     const size_t n = (size_t)R.np;
     size_t bytes = 0;
     auto take = [&](size_t b) { size_t off = bytes; bytes += (b + 255) & ~(size_t)255; return off; };
-    R.oT = take(16 * n);
-    R.oL = take(16 * n);
-    R.oA_o = take(16 * n);
-    R.oA_d = take(16 * n);
-    R.oB_o = take(16 * n);
-    R.oB_d = take(16 * n);
-    R.oQA = take(4 * n);
-    R.oQB = take(4 * n);
-    // shadow rays: a segmented queue filled by k_shade's work-groups
+    // shadow rays, and the path queues of bounces >= 1 in the fused loop:
+    // segmented queues filled by k_shade's work-groups (segment = group mod
+    // SHADOW_SEGS, capacity = the slots of the groups that map to it)
     R.seg_cap = shadow_seg_cap(R.np, RT);
     R.sq = (size_t)SHADOW_SEGS * R.seg_cap;
-    R.oPrim = take(4 * n);
-    R.oT_ = take(4 * n);
+    const size_t nq = R.sq > n ? R.sq : n;
+    R.oT = take(16 * n);
+    R.oL = take(16 * n);
+    R.oA_o = take(16 * nq);
+    R.oA_d = take(16 * nq);
+    R.oB_o = take(16 * nq);
+    R.oB_d = take(16 * nq);
+    R.oQA = take(4 * nq);
+    R.oQB = take(4 * nq);
+    R.oPrim = take(4 * nq);
+    R.oT_ = take(4 * nq);
     R.oAlive = take(4 * n);
     R.oOcc = take(4 * R.sq);
     R.oSo = take(16 * R.sq);
@@ -310,7 +343,9 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
     R.oSc = take(16 * R.sq);
     R.oCnt = take(4 * 64);
     R.oShc = take(4 * 64 * (size_t)SHADOW_SEGS);
+    R.oQsc = take(4 * 64 * (size_t)SHADOW_SEGS);
     R.want_ilp = (desc->flags & MT_RENDER_TRAVERSAL_STATS) != 0;
+    R.fused = (desc->mode == MT_TRACE_PACKED || desc->mode == MT_TRACE_PACKED_STACK) && !R.want_ilp;
     R.profile = (desc->flags & MT_RENDER_PROFILE) != 0;
     R.oIlp = R.want_ilp ? take(12 * (R.sq > n ? R.sq : n)) : 0;
     R.oTot = take(8 * 8 * ILP_SLOTS);
@@ -365,6 +400,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
     int* q[2] = {I(R.oQA), I(R.oQB)};
     int* counts = I(R.oCnt);
     int* shc = I(R.oShc);
+    int* qsc = I(R.oQsc); // row b: segment lengths of bounce b's queue (fused loop, b >= 1)
     auto chk = [&](void) -> int32_t {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
@@ -389,12 +425,15 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
 
     k_zero_i32<<<1, 64, 0, st>>>(counts, 64);
     k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(shc, 64 * SHADOW_SEGS);
+    k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(qsc, 64 * SHADOW_SEGS);
     k_zero_i32<<<mt_blocks(16 * ILP_SLOTS, RT), RT, 0, st>>>(reinterpret_cast<int*>(tot), 16 * ILP_SLOTS);
     // Fused loop (default traversal, no traversal counters): bounce 0 traces
     // camera rays generated in place and shades with slot = pid; shadow rays
-    // add their light inside the any-hit kernel.  Otherwise camera rays are
-    // stored first and every stage is a separate launch.
-    const bool fused = (desc->mode == MT_TRACE_PACKED || desc->mode == MT_TRACE_PACKED_STACK) && !ilp;
+    // add their light inside the any-hit kernel; continuing paths append
+    // themselves to the next bounce's segmented queue.  Otherwise camera rays
+    // are stored first, every stage is a separate launch and each bounce's
+    // survivors are compacted in slot order.
+    const bool fused = R.fused;
     int32_t rc = MT_OK;
     if (!fused) {
         k_camera<<<grid, RT, 0, st>>>(R.C, np, seed_ptr(R), ro[0], rd[0], q[0], S, counts);
@@ -409,6 +448,10 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
             k_trace_camera<<<grid, RT, 0, st>>>(R.C, np, seed_ptr(R), scene->pnodes, scene->ptris, scene->root_ref,
                                                 I(R.oPrim), reinterpret_cast<float*>(base + R.oT_), counts);
             rc = chk();
+        } else if (fused) {
+            rc = mt_trace_closest_seg(ctx, scene, ro[cur], rd[cur], MtSegQueue{qsc + (size_t)b * SHADOW_SEGS,
+                                                                               SHADOW_SEGS, seg_cap},
+                                      I(R.oPrim), reinterpret_cast<float*>(base + R.oT_), desc->mode);
         } else {
             rc = mt_trace_closest(ctx, scene, reinterpret_cast<float*>(ro[cur]), reinterpret_cast<float*>(rd[cur]),
                                   dcount, np, I(R.oPrim), reinterpret_cast<float*>(base + R.oT_), ilp, desc->mode);
@@ -428,6 +471,11 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         A.prim = I(R.oPrim);
         A.tin = reinterpret_cast<const float*>(base + R.oT_);
         A.alive = I(R.oAlive);
+        A.in_seg = fused && !first ? qsc + (size_t)b * SHADOW_SEGS : nullptr;
+        A.next_seg = fused ? qsc + (size_t)(b + 1) * SHADOW_SEGS : nullptr;
+        A.q_next = q[1 - cur];
+        A.o_next = ro[1 - cur];
+        A.d_next = rd[1 - cur];
         A.s_o_tmin = F4(R.oSo);
         A.s_d_tmax = F4(R.oSd);
         A.s_contrib = F4(R.oSc);
@@ -438,7 +486,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         A.rr_start = desc->rr_start;
         A.flags = desc->flags;
         A.seed = seed_ptr(R);
-        k_shade<<<gshade, RT, 0, st>>>(A);
+        k_shade<<<A.in_seg ? mt_blocks((int64_t)R.sq, RT) : gshade, RT, 0, st>>>(A);
         if ((rc = chk())) break;
         if (profile && (rc = mark(ev[4 * b + 2]))) break;
         if (fused) {
@@ -456,7 +504,9 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
             k_nee_accum<<<gshadow, RT, 0, st>>>(q[cur], seg_counts, seg_cap, F4(R.oSc), I(R.oOcc), S);
             if ((rc = chk())) break;
         }
-        if (b + 1 < nb) {
+        if (fused) {
+            cur = 1 - cur;
+        } else if (b + 1 < nb) {
             if (first) {
                 // the survivors' slots are their path ids: the compaction's
                 // source indices become the next queue
@@ -500,6 +550,20 @@ int32_t read_stats(mt_context* ctx, RenderPlan& R, mt_render_stats* stats)
     stats->extension_rays = 0;
     stats->shadow_rays = 0;
     stats->paths = R.np;
+    if (R.fused) { // bounces >= 1: the segment lengths of the appended queues
+        std::vector<int> hq((size_t)nb * SHADOW_SEGS);
+        e = hipMemcpyAsync(hq.data(), R.base + R.oQsc, hq.size() * sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            mt_set_error("mt_render: stats readback failed: %s", hipGetErrorString(e));
+            return MT_ERR_HIP;
+        }
+        for (int b = 1; b < nb; b++) {
+            int64_t c = 0;
+            for (int p = 0; p < SHADOW_SEGS; p++) c += hq[(size_t)b * SHADOW_SEGS + p];
+            hc[b] = (int)c;
+        }
+    }
     for (int b = 0; b < 64; b++) stats->bounce_counts[b] = b < nb ? hc[b] : 0;
     for (int b = 0; b < nb; b++) stats->extension_rays += hc[b];
     for (int v : hs) stats->shadow_rays += v;

@@ -751,6 +751,15 @@ extern "C" int32_t mt_trace_autotune(mt_context* ctx, const mt_scene* scene, con
     return MT_OK;
 }
 
+int32_t mt_trace_closest_seg(mt_context* ctx, const mt_scene* scene, const float4* o_tmin, const float4* d_tmax,
+                             MtSegQueue seg, int32_t* prim, float* t, int32_t mode)
+{
+    MT_ARG_CHECK(seg.counts && seg.nseg > 0 && seg.seg_cap > 0, "trace_closest_seg: bad segment table");
+    return launch_trace<false>(ctx, scene, reinterpret_cast<const float*>(o_tmin),
+                               reinterpret_cast<const float*>(d_tmax), nullptr, 0, prim, t, nullptr, nullptr, mode,
+                               seg);
+}
+
 int32_t mt_trace_any_seg(mt_context* ctx, const mt_scene* scene, const float4* o_tmin, const float4* d_tmax,
                          MtSegQueue seg, int32_t* occluded, int32_t* stats, int32_t mode)
 {
