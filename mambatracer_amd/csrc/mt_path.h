@@ -392,6 +392,16 @@ inline int shadow_seg_cap(int64_t slots, int group)
     return (int)((groups + SHADOW_SEGS - 1) / SHADOW_SEGS) * group;
 }
 
+// Entry of this lane in a segmented queue walked chunk-major: work-group b
+// takes chunk b / SHADOW_SEGS of segment b % SHADOW_SEGS (seg_cap is a
+// multiple of the group size), so the live chunks at the front of every
+// segment are dispatched first.
+__device__ __forceinline__ int seg_entry(int seg_cap)
+{
+    const int p = blockIdx.x % SHADOW_SEGS, c = blockIdx.x / SHADOW_SEGS;
+    return p * seg_cap + c * blockDim.x + threadIdx.x;
+}
+
 // Whether entry `id` of a segmented queue holds an item.
 __device__ __forceinline__ bool seg_live(const int* __restrict__ counts, int seg_cap, int id)
 {

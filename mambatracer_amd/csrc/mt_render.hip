@@ -96,7 +96,7 @@ __global__ __launch_bounds__(RT) void k_trace_shadow(const float4* __restrict__ 
                                                      const float4* __restrict__ contrib, const int* __restrict__ queue,
                                                      float4* __restrict__ L)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = seg_entry(seg_cap);
     if (!seg_live(seg_counts, seg_cap, k)) return;
     RayT r;
     ray_init(r, s_o[k], s_d[k]);
@@ -199,17 +199,19 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, uint32_t seed, in
 // assume exactly that); work-groups past *d_count leave at once.
 __global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
 {
-    const int base = blockIdx.x * blockDim.x;
-    const int slot = base + threadIdx.x;
+    int slot;
     bool live;
-    if (A.in_seg) { // a work-group lies inside one segment (seg_cap is a multiple of RT)
-        const int p = base / A.seg_cap;
+    if (A.in_seg) { // chunk-major over the segmented queue, as seg_entry
+        const int p = blockIdx.x % SHADOW_SEGS, off = (blockIdx.x / SHADOW_SEGS) * blockDim.x;
         const int c = A.in_seg[p];
-        if (base - p * A.seg_cap >= c) return;
-        live = slot - p * A.seg_cap < c;
+        if (off >= c) return;
+        slot = p * A.seg_cap + off + threadIdx.x;
+        live = off + (int)threadIdx.x < c;
     } else {
+        const int base = blockIdx.x * blockDim.x;
         const int count = *A.d_count;
         if (base >= count) return;
+        slot = base + threadIdx.x;
         live = slot < count;
     }
     shade_slot(A, *A.seed, slot, live);

@@ -61,6 +61,19 @@ __device__ __forceinline__ bool entry_live(const TraceArgs& A, int id)
     return id < count;
 }
 
+// Queue entry of this lane.  A segmented queue is walked chunk-major:
+// work-group b takes chunk b / nseg of segment b % nseg, so the live chunks
+// at the front of every segment are dispatched first, and neighbouring
+// groups hold entries that neighbouring groups of the producer appended.
+__device__ __forceinline__ int queue_entry(const TraceArgs& A)
+{
+    if (A.seg.counts) {
+        const int p = blockIdx.x % A.seg.nseg, c = blockIdx.x / A.seg.nseg;
+        return p * A.seg.seg_cap + c * blockDim.x + threadIdx.x;
+    }
+    return blockIdx.x * blockDim.x + threadIdx.x;
+}
+
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
 { // set bits of `mask` below this lane (v_mbcnt_lo/hi)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
@@ -166,7 +179,7 @@ __global__ __launch_bounds__(256) void k_trace_refill(TraceArgs A)
 template <bool ANY, bool STATS, int LAYOUT>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
 {
-    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    const int id = queue_entry(A);
     if (!entry_live(A, id)) return;
     RayT r;
     ray_init(r, A.o_tmin[id], A.d_tmax[id]);
@@ -218,6 +231,7 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
     if (ANY) MT_ARG_CHECK(occluded != nullptr, "trace_any: occluded is NULL");
     else MT_ARG_CHECK(prim && t, "trace_closest: prim/t NULL");
     if (seg.counts) {
+        MT_ARG_CHECK(seg.seg_cap % 256 == 0, "trace: segment capacity %d not a multiple of 256", seg.seg_cap);
         capacity = seg.nseg * seg.seg_cap;
         if (mode == MT_TRACE_PACKED_REFILL) mode = MT_TRACE_PACKED_STACK; // the refill kernel reads *d_count
     }
