@@ -4,8 +4,8 @@
 Workload (one "step" = one render job, inputs resident in HBM): 256 UV
 spheres x 4096 triangles + ground quad (1,048,578 triangles), GGX materials,
 1024x512 HDR environment sampled through its summed-area table, 1920x1080 at
-16 spp, up to 8 bounces with Russian roulette after bounce 3 and stable
-compaction after every bounce.  value = (closest-hit + shadow rays) / s over
+16 spp, up to 8 bounces with Russian roulette after bounce 3; after every bounce the
+surviving paths form the next bounce's queue.  value = (closest-hit + shadow rays) / s over
 the timed steps, each one replay of the job's captured render
 (mt_render_graph, as a render thread re-runs a frame) plus its ray-count
 readback.  The same job through uncaptured mt_render is timed after it and
@@ -242,7 +242,8 @@ def main() -> int:
         "config": {
             "workload": "c1 wavefront path tracer: 256 UV spheres x 4096 tris + ground (1,048,578 tris), "
                         f"{job.width}x{job.height} x {job.spp} spp, <=8 bounces, RR from bounce 3, "
-                        "env 1024x512 sampled via full-map SAT, stable compaction every bounce",
+                        "env 1024x512 sampled via full-map SAT, surviving paths compacted every bounce (appended by the "
+                        "shading stage to segmented queues)",
             "global_batch": npaths,
             "traversal_layout": "packed" if job.mode == 0 else "reference",
             "parallelism": "single GPU",

@@ -384,7 +384,7 @@ __device__ __forceinline__ int wave_append(bool flag, int* counter)
 // Shadow-ray queues are segmented (see MtSegQueue): work-group b of a
 // launch appends to segment b % SHADOW_SEGS, whose capacity is the slots of
 // the work-groups that map to it.
-constexpr int SHADOW_SEGS = 256;
+constexpr int SHADOW_SEGS = 1024;
 
 inline int shadow_seg_cap(int64_t slots, int group)
 {
