@@ -49,7 +49,10 @@ constexpr int BOUNCE_BITS = 6; // max_bounces <= 63
 // Rows of done counters: a job's ended paths are counted by many waves of
 // one launch, and one counter per job would serialise their atomics.
 constexpr int DONE_SEGS = 64;
-static_assert(SHADOW_SEGS % 64 == 0, "k_refill clears the segment counters one wave-width at a time");
+// Shadow-queue segments of the multi-job engine (its iterations are smaller
+// than a whole render's bounces, so fewer, fuller segments).
+constexpr int JOB_SEGS = 256;
+static_assert(JOB_SEGS % 64 == 0, "k_refill clears the segment counters one wave-width at a time");
 
 struct JobRec {
     SceneView V;
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(64) void k_refill(const int* __restrict__ active, i
                                                int* __restrict__ s_seg, unsigned long long* __restrict__ tot)
 {
     const int lane = threadIdx.x;
-    for (int p = lane; p < SHADOW_SEGS; p += 64) s_seg[p] = 0; // shadow queue of this iteration
+    for (int p = lane; p < JOB_SEGS; p += 64) s_seg[p] = 0; // shadow queue of this iteration
     const int live = cnt[0];
     const int64_t avail = (int64_t)capacity - live;
     int64_t carry = 0;
@@ -184,7 +187,7 @@ struct ShadeJobsArgs {
     int* alive;
     float4 *s_o, *s_d, *s_c; // segmented shadow queue: rays and contributions
     int2* s_tag;             // the entry's slot tag (job word, path id)
-    int* s_seg;              // SHADOW_SEGS segment lengths
+    int* s_seg;              // JOB_SEGS segment lengths
     int seg_cap;
 };
 
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(BT) void k_shade_jobs(ShadeJobsArgs A)
         sc = P.contrib;
         stag = make_int2(tg.x, pid);
     }
-    const int seg = blockIdx.x % SHADOW_SEGS;
+    const int seg = blockIdx.x % JOB_SEGS;
     const int k = seg * A.seg_cap + wave_append(has_shadow, A.s_seg + seg);
     if (has_shadow) {
         A.s_o[k] = so;
@@ -260,11 +263,11 @@ __global__ __launch_bounds__(BT) void k_shadow_jobs(const JobRec* __restrict__ j
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0) {
         int c = 0;
-        for (int p = threadIdx.x; p < SHADOW_SEGS; p += BT) c += s_seg[p];
+        for (int p = threadIdx.x; p < JOB_SEGS; p += BT) c += s_seg[p];
         for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
         if (__lane_id() == 0 && c) atomicAdd(&tot[1], (unsigned long long)c);
     }
-    if (!seg_live(s_seg, seg_cap, k)) return;
+    if (!seg_live<JOB_SEGS>(s_seg, seg_cap, k)) return;
     const int2 tg = s_tag[k];
     const int job = tg.x >> BOUNCE_BITS;
     const MtSceneRef R = refs[job];
@@ -459,7 +462,7 @@ int32_t iterate(mt_batch* b)
     A.seg_cap = b->seg_cap;
     k_shade_jobs<<<grid, BT, 0, s>>>(A);
     MT_LAUNCH_CHECK();
-    k_shadow_jobs<<<mt_blocks((int64_t)SHADOW_SEGS * b->seg_cap, BT), BT, 0, s>>>(
+    k_shadow_jobs<<<mt_blocks((int64_t)JOB_SEGS * b->seg_cap, BT), BT, 0, s>>>(
         b->jobs, b->refs, b->s_seg, b->seg_cap, b->s_tag, b->s_o, b->s_d, b->s_c, b->tot);
     MT_LAUNCH_CHECK();
     mt_payload P[4] = {{q.tag, nq.tag, 8}, {q.ro, nq.ro, 16}, {q.rd, nq.rd, 16}, {q.T, nq.T, 16}};
@@ -506,9 +509,9 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     size_t oJobs = take(sizeof(JobRec) * J), oRefs = take(sizeof(MtSceneRef) * J);
     size_t oNp = take(4 * J), oIss = take(4 * J), oDone = take(4 * J * DONE_SEGS), oDtot = take(4 * J);
     size_t oAct = take(4 * J), oGs = take(4 * J), oGp = take(4 * J), oGj = take(4 * J);
-    size_t oCnt = take(16), oTot = take(16), oSeg = take(4 * SHADOW_SEGS);
-    b->seg_cap = shadow_seg_cap(queue_capacity, BT);
-    const size_t sq = (size_t)SHADOW_SEGS * b->seg_cap;
+    size_t oCnt = take(16), oTot = take(16), oSeg = take(4 * JOB_SEGS);
+    b->seg_cap = shadow_seg_cap<JOB_SEGS>(queue_capacity, BT);
+    const size_t sq = (size_t)JOB_SEGS * b->seg_cap;
     size_t oQ[2][4];
     for (int k = 0; k < 2; k++) {
         oQ[k][0] = take(8 * n);

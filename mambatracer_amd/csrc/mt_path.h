@@ -381,15 +381,18 @@ __device__ __forceinline__ int wave_append(bool flag, int* counter)
     return base + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Shadow-ray queues are segmented (see MtSegQueue): work-group b of a
-// launch appends to segment b % SHADOW_SEGS, whose capacity is the slots of
-// the work-groups that map to it.
+// Shadow-ray and path queues are segmented (see MtSegQueue): work-group b of
+// a launch appends to segment b % NSEG, whose capacity is the slots of the
+// work-groups that map to it.  The render loop uses SHADOW_SEGS segments
+// (about one resident producer group per segment); the multi-job engine,
+// whose per-iteration batches are smaller, its own count.
 constexpr int SHADOW_SEGS = 1024;
 
+template <int NSEG = SHADOW_SEGS>
 inline int shadow_seg_cap(int64_t slots, int group)
 {
     const int64_t groups = (slots + group - 1) / group;
-    return (int)((groups + SHADOW_SEGS - 1) / SHADOW_SEGS) * group;
+    return (int)((groups + NSEG - 1) / NSEG) * group;
 }
 
 // Entry of this lane in a segmented queue walked chunk-major: work-group b
@@ -402,11 +405,12 @@ __device__ __forceinline__ int seg_entry(int seg_cap)
     return p * seg_cap + c * blockDim.x + threadIdx.x;
 }
 
-// Whether entry `id` of a segmented queue holds an item.
+// Whether entry `id` of a segmented queue of NSEG segments holds an item.
+template <int NSEG = SHADOW_SEGS>
 __device__ __forceinline__ bool seg_live(const int* __restrict__ counts, int seg_cap, int id)
 {
     const int p = id / seg_cap;
-    return p < SHADOW_SEGS && id - p * seg_cap < counts[p];
+    return p < NSEG && id - p * seg_cap < counts[p];
 }
 
 // Per-pixel mean over samples in sample order (averageAccum, Image.cl:29-45,
