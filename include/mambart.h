@@ -487,8 +487,11 @@ typedef struct {
  * camera rays, then per bounce IntersectPrimitives, SetupBSDFPath,
  * LightHitPass (environment miss radiance with MIS, emitter hits),
  * DirectLight (environment sample through the full-map SAT, BSDF
- * evaluation, any-hit shadow ray), SampleBSDFRayDirection and a stable
- * compaction of live paths.  materials is a device array of 36-word records;
+ * evaluation, any-hit shadow ray), SampleBSDFRayDirection, and the live
+ * paths gathered into the next bounce's queue (the default traversal
+ * appends them while shading; with MT_RENDER_TRAVERSAL_STATS or the
+ * reference layout a stable compaction does it; the image is the same
+ * either way).  materials is a device array of 36-word records;
  * env may be NULL (no environment light); light_faces/num_lights (device,
  * from mt_mesh_lights) may be NULL/0.  image receives float4[w*h]: the mean
  * over spp of each pixel's radiance.  RNG is keyed by path id, so a path's
