@@ -496,7 +496,10 @@ typedef struct {
  * from mt_mesh_lights) may be NULL/0.  image receives float4[w*h]: the mean
  * over spp of each pixel's radiance.  RNG is keyed by path id, so a path's
  * result does not depend on which other paths share the wavefront.
- * stats (host) may be NULL; filling it synchronises the context. */
+ * The wavefront state (about 100 B per path) lives in the context's
+ * grow-only workspace, so repeated renders allocate nothing; it is held
+ * until mt_ctx_destroy.  Synchronises the context before returning.
+ * stats (host) may be NULL. */
 int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float* materials,
                   const mt_envmap* env, const int32_t* light_faces,
                   const int32_t* d_num_lights, const mt_render_desc* desc,

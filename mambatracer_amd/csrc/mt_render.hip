@@ -286,6 +286,7 @@ struct RenderPlan {
     size_t oT, oL, oA_o, oA_d, oB_o, oB_d, oQA, oQB, oPrim, oT_, oAlive, oOcc, oSo, oSd, oSc, oCnt, oShc, oQsc,
         oIlp, oTot, oSeed;
     bool fused; // camera rays traced in place, shadow adds fused, next queue appended by the shading stage
+    bool borrowed; // base is the context's reusable workspace (mt_render), not owned
     hipEvent_t ev[4 * 64];
     int nev;
 };
@@ -308,6 +309,7 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
     R.scene = scene;
     R.image = image;
     R.base = nullptr;
+    R.borrowed = false;
     R.nev = 0;
     int32_t rc = camera_from_desc(desc, R.C);
     if (rc) return rc;
@@ -360,13 +362,25 @@ void free_render(RenderPlan& R)
 {
     for (int k = 0; k < R.nev; k++) (void)hipEventDestroy(R.ev[k]);
     R.nev = 0;
-    if (R.base) (void)hipFree(R.base);
+    if (R.base && !R.borrowed) (void)hipFree(R.base);
     R.base = nullptr;
 }
 
-int32_t alloc_render(RenderPlan& R)
+// ctx != nullptr: the state lives in the context's grow-only workspace
+// (calls on a context are serialised and mt_render synchronises before it
+// returns), so repeated renders allocate nothing; otherwise it is owned.
+int32_t alloc_render(RenderPlan& R, mt_context* ctx)
 {
-    if (hipMalloc(&R.base, R.bytes) != hipSuccess) {
+    R.borrowed = ctx != nullptr;
+    if (ctx) {
+        void* p = nullptr;
+        int32_t rc = mt_ctx_temp(ctx, R.bytes, &p);
+        if (rc) {
+            R.base = nullptr;
+            return rc;
+        }
+        R.base = reinterpret_cast<char*>(p);
+    } else if (hipMalloc(&R.base, R.bytes) != hipSuccess) {
         R.base = nullptr;
         mt_set_error("mt_render: cannot allocate %.2f GB of wavefront state", R.bytes / 1e9);
         return MT_ERR_ALLOC;
@@ -604,7 +618,7 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
     RenderPlan R;
     int32_t rc = plan_render(scene, materials, env, light_faces, d_num_lights, desc, image, R);
     if (rc) return rc;
-    rc = alloc_render(R);
+    rc = alloc_render(R, ctx);
     if (rc) return rc;
     k_set_seed<<<1, 64, 0, ctx->stream>>>(seed_ptr(R), desc->seed);
     rc = enqueue_render(ctx, R);
@@ -660,7 +674,7 @@ extern "C" int32_t mt_render_graph_create(mt_context* ctx, const mt_scene* scene
     g->R.base = nullptr;
     g->R.nev = 0;
     int32_t rc = plan_render(scene, materials, env, light_faces, d_num_lights, desc, image, g->R);
-    if (rc == MT_OK) rc = alloc_render(g->R);
+    if (rc == MT_OK) rc = alloc_render(g->R, nullptr);
     if (rc) {
         delete g;
         return rc;
