@@ -255,7 +255,7 @@ def main() -> int:
         "timed_path": "captured render (mt_render_graph) replayed once per step",
         "uncaptured": {"Mrays_per_s": round(e_rays / e_elapsed / 1e6, 3),
                        "ms_per_job": round(e_elapsed * 1e3 / args.steps, 3),
-                       "note": "mt_render per step (allocates its state per call), HIP events on "
+                       "note": "mt_render per step (state in the context's reused workspace), HIP events on "
                                "every traversal launch; source of the roofline launch duration"},
         "roofline": {
             "kernel": ("closest-hit traversal, packed layout: k_trace_camera (bounce 0, camera rays made in "
