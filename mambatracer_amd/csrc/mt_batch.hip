@@ -260,7 +260,7 @@ __global__ __launch_bounds__(BT) void k_shadow_jobs(const JobRec* __restrict__ j
                                                     const float4* __restrict__ contrib,
                                                     unsigned long long* __restrict__ tot)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = seg_entry<JOB_SEGS>(seg_cap); // chunk-major: live chunks dispatch first
     if (blockIdx.x == 0) {
         int c = 0;
         for (int p = threadIdx.x; p < JOB_SEGS; p += BT) c += s_seg[p];

@@ -399,9 +399,10 @@ inline int shadow_seg_cap(int64_t slots, int group)
 // takes chunk b / SHADOW_SEGS of segment b % SHADOW_SEGS (seg_cap is a
 // multiple of the group size), so the live chunks at the front of every
 // segment are dispatched first.
+template <int NSEG = SHADOW_SEGS>
 __device__ __forceinline__ int seg_entry(int seg_cap)
 {
-    const int p = blockIdx.x % SHADOW_SEGS, c = blockIdx.x / SHADOW_SEGS;
+    const int p = blockIdx.x % NSEG, c = blockIdx.x / NSEG;
     return p * seg_cap + c * blockDim.x + threadIdx.x;
 }
 
