@@ -47,3 +47,16 @@ def test_library_has_gfx950_code():
     assert ".hip_fatbin" in out
     data = open(_lib.LIB_PATH, "rb").read()
     assert b"gfx950" in data
+
+
+def test_binding_argument_counts_match_header():
+    """Each ctypes signature has as many parameters as the header's declaration
+    (a drifted binding would pass arguments into the wrong slots)."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    decls = re.findall(r"^\s*(?:int32_t|const char\*)\s+(mt_\w+)\s*\(([^;]*?)\)\s*;", text, flags=re.M | re.S)
+    assert len(decls) > 40
+    for name, params in decls:
+        params = params.strip()
+        n = 0 if params in ("", "void") else params.count(",") + 1
+        assert name in _lib.SIGNATURES, name
+        assert len(_lib.SIGNATURES[name]) == n, (name, n, len(_lib.SIGNATURES[name]))
