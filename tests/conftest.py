@@ -6,6 +6,14 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def golden(name):
+    """A known-answer fixture from tests/golden (written by make_kats.py)."""
+    import json
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 

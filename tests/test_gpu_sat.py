@@ -8,27 +8,26 @@ import torch
 
 from mambatracer_amd import api
 from oracle import orc_py as orc
+from tests.conftest import golden
 
 pytestmark = pytest.mark.gpu
 
 
 def test_simplegrid_impulse_known_answer(ctx):
     """test/lightgrid/TestSimpleGrid.java:39-67: 16x16 grid, 0.01 at row 11 col 10."""
-    f = np.zeros((16, 16), np.float32)
-    f[11, 10] = 0.01
-    sat = api.sat_tiles(ctx, ctx.to_device(f.reshape(-1)), 16, 16, 16, 16).cpu().numpy().reshape(16, 16)
-    exp = np.zeros((16, 16), np.float32)
-    exp[11:, 10:] = np.float32(0.01)
-    np.testing.assert_array_equal(sat, exp)
+    k = golden("simplegrid_impulse.json")
+    f = np.array(k["func_rows"], np.float32)
+    sat = api.sat_tiles(ctx, ctx.to_device(f.reshape(-1)), k["nu"], k["nv"], k["rx"], k["ry"]).cpu().numpy()
+    np.testing.assert_array_equal(sat.reshape(16, 16), np.array(k["sat_rows"], np.float32))
 
 
 def test_lightgrid_all_ones_known_answer(ctx):
     """test/lightgrid/TestLightGrid.java:41-62: all-ones tile -> sat = (x+1)(y+1) per tile."""
+    k = golden("lightgrid_all_ones.json")
     lg = api.CAdaptiveEnvironment(ctx)
-    lg.setHDRLuminance(np.ones(512, np.float32))
+    lg.setHDRLuminance(np.full(512, k["tile_value"], np.float32))
     _, func, sat = lg.read_arrays()
-    y, x = np.mgrid[0:1600, 0:1600]
-    exp = ((x % 16 + 1) * (y % 32 + 1)).astype(np.float32)
+    exp = np.tile(np.array(k["tile_sat_rows"], np.float32), (50, 100))
     np.testing.assert_array_equal(func, np.ones_like(func))
     np.testing.assert_array_equal(sat, exp)
 

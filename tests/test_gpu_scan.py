@@ -7,6 +7,7 @@ import torch
 
 from mambatracer_amd import api
 from oracle import orc_py as orc
+from tests.conftest import golden
 
 pytestmark = pytest.mark.gpu
 
@@ -30,11 +31,11 @@ def test_scan_random(ctx, n):
 
 def test_scan_all_ones_known_answer(ctx):
     """PrefixSumIntegerTest: 200 ones -> out[i] = i, total = 200."""
-    x = np.ones(200, np.int32)
-    ps = api.PrefixSumInteger(ctx, ctx.to_device(x))
+    k = golden("prefix_sum_all_ones.json")
+    ps = api.PrefixSumInteger(ctx, ctx.to_device(np.array(k["input"], np.int32)))
     ps.execute()
-    assert ps.getCount() == 200
-    np.testing.assert_array_equal(ps.getPrefixSum().cpu().numpy(), np.arange(200))
+    assert ps.getCount() == k["total"]
+    np.testing.assert_array_equal(ps.getPrefixSum().cpu().numpy(), k["exclusive_scan"])
 
 
 def test_scan_device_length_shorter_than_capacity(ctx):

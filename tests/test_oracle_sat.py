@@ -10,24 +10,23 @@ import numpy as np
 import pytest
 
 from oracle import orc_py as orc
+from tests.conftest import golden
 
 
 # ---------------------------------------------------------------- SAT build
 def test_simplegrid_impulse_known_answer():
-    f = np.zeros((16, 16), np.float32)
-    f[11, 10] = 0.01
-    sat = orc.sat_tiles(f.reshape(-1), 16, 16, 16, 16).reshape(16, 16)
-    exp = np.zeros((16, 16), np.float32)
-    exp[11:, 10:] = np.float32(0.01)
-    np.testing.assert_array_equal(sat, exp)
+    k = golden("simplegrid_impulse.json")
+    f = np.array(k["func_rows"], np.float32)
+    sat = orc.sat_tiles(f.reshape(-1), k["nu"], k["nv"], k["rx"], k["ry"]).reshape(16, 16)
+    np.testing.assert_array_equal(sat, np.array(k["sat_rows"], np.float32))
 
 
 def test_lightgrid_all_ones_known_answer():
-    func = orc.lg_set_tile_func(np.ones(512, np.float32))
+    k = golden("lightgrid_all_ones.json")
+    func = orc.lg_set_tile_func(np.full(512, k["tile_value"], np.float32))
     assert (func == 1).all()
-    sat = orc.sat_tiles(func, 1600, 1600, 16, 32).reshape(1600, 1600)
-    y, x = np.mgrid[0:1600, 0:1600]
-    np.testing.assert_array_equal(sat, ((x % 16 + 1) * (y % 32 + 1)).astype(np.float32))
+    sat = orc.sat_tiles(func, k["nu"], k["nv"], k["rx"], k["ry"]).reshape(1600, 1600)
+    np.testing.assert_array_equal(sat, np.tile(np.array(k["tile_sat_rows"], np.float32), (50, 100)))
 
 
 def test_tile_sat_matches_float64():
@@ -135,7 +134,15 @@ def test_region_sampling_against_distribution2d(region):
 
 def test_sampling_1d_reference_case():
     """test/sampling/Test.java:17: {0,1,0,1,1,1,0,0}; zero-valued cells are never picked."""
-    f = np.array([0, 1, 0, 1, 1, 1, 0, 0], np.float32)
+    k = golden("distribution1d_case.json")
+    f = np.array(k["func"], np.float32)
+    # the fixture's closed-form samples
+    us = np.array([[c["u"], 0.5] for c in k["samples"]], np.float32)
+    kuv, koff, kpdf = orc.sat_sample_region(*orc.env_lum_sat(np.repeat(f.reshape(-1, 1), 4, 1), 8, 1), 8, 1, 0, 0,
+                                            8, 1, us)
+    for j, c in enumerate(k["samples"]):
+        assert koff[j, 0] == c["offset"]
+        assert abs(kuv[j, 0] - c["u_out"]) < 1e-6 and abs(kpdf[j] - c["pdf"]) < 1e-5
     lum, sat = orc.env_lum_sat(np.repeat(f.reshape(-1, 1), 4, 1), 8, 1)
     u = np.stack([np.linspace(0, 0.999, 400), np.full(400, 0.5)], 1).astype(np.float32)
     uv, off, pdf = orc.sat_sample_region(lum, sat, 8, 1, 0, 0, 8, 1, u)

@@ -4,12 +4,14 @@ import numpy as np
 import pytest
 
 from oracle import orc_py as orc
+from tests.conftest import golden
 
 
 def test_prefix_sum_all_ones_known_answer():
-    out, total = orc.scan_excl(np.ones(200, np.int32))
-    np.testing.assert_array_equal(out, np.arange(200))
-    assert total == 200
+    k = golden("prefix_sum_all_ones.json")
+    out, total = orc.scan_excl(np.array(k["input"], np.int32))
+    np.testing.assert_array_equal(out, k["exclusive_scan"])
+    assert total == k["total"]
 
 
 @pytest.mark.parametrize("n", [1, 2, 127, 128, 129, 10000])
