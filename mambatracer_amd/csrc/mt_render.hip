@@ -522,16 +522,9 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         }
         if (fused) {
             cur = 1 - cur;
-        } else if (b + 1 < nb) {
-            if (first) {
-                // the survivors' slots are their path ids: the compaction's
-                // source indices become the next queue
-                mt_payload P[2] = {{ro[cur], ro[1 - cur], 16}, {rd[cur], rd[1 - cur], 16}};
-                rc = mt_compact(ctx, I(R.oAlive), 0, dcount, np, q[1 - cur], counts + b + 1, P, 2);
-            } else {
-                mt_payload P[3] = {{q[cur], q[1 - cur], 4}, {ro[cur], ro[1 - cur], 16}, {rd[cur], rd[1 - cur], 16}};
-                rc = mt_compact(ctx, I(R.oAlive), 0, dcount, np, nullptr, counts + b + 1, P, 3);
-            }
+        } else if (b + 1 < nb) { // unfused loop: stable compaction of the survivors
+            mt_payload P[3] = {{q[cur], q[1 - cur], 4}, {ro[cur], ro[1 - cur], 16}, {rd[cur], rd[1 - cur], 16}};
+            rc = mt_compact(ctx, I(R.oAlive), 0, dcount, np, nullptr, counts + b + 1, P, 3);
             if (rc) break;
             cur = 1 - cur;
         }
