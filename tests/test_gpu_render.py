@@ -160,3 +160,25 @@ def test_fused_and_staged_loops_agree(ctx, c1_small, mesh_lights):
         assert int(st.extension_rays) == int(ref_st.extension_rays)
         assert int(st.shadow_rays) == int(ref_st.shadow_rays)
         assert list(st.bounce_counts) == list(ref_st.bounce_counts)
+
+
+@pytest.mark.parametrize("w,h,spp", [(1, 1, 1), (7, 3, 5), (17, 9, 1)])
+def test_tiny_jobs_fused_and_staged_agree(ctx, c1_small, w, h, spp):
+    """Jobs far smaller than one work-group or one queue segment: the fused
+    loop (appended, segmented path queues) and the staged loop (compaction)
+    give the same image bits and ray counts, and the counts add up."""
+    print(f"[case] tiny job {w}x{h}x{spp}", flush=True)
+    points, normals, faces, nodes, bounds, depth = c1_small
+    sc = api.Scene(ctx, points, faces, nodes, bounds, depth, normals=normals)
+    dm = ctx.to_device(W.c1_materials())
+    env = api.CEnvironment(ctx)
+    env.setEnvironmentMap(W.env_map())
+    base = dict(width=w, height=h, spp=spp, pos=(0.0, 0.5, -8.0), look=(0.0, -1.0, 0.0), seed=9)
+    (img_f, st_f), (img_s, st_s) = [api.render(ctx, sc, dm, env, W.RenderJob(**base, **extra))
+                                    for extra in ({}, {"flags": _lib.MT_RENDER_TRAVERSAL_STATS})]
+    np.testing.assert_array_equal(img_f.cpu().numpy().view(np.int32), img_s.cpu().numpy().view(np.int32))
+    assert list(st_f.bounce_counts) == list(st_s.bounce_counts)
+    assert int(st_f.shadow_rays) == int(st_s.shadow_rays)
+    assert st_f.bounce_counts[0] == w * h * spp == int(st_f.paths)
+    assert int(st_f.extension_rays) == sum(st_f.bounce_counts)
+    assert np.isfinite(img_f.cpu().numpy()).all()
