@@ -58,6 +58,33 @@ def _need(x, name: str, n: int, tail=(), dtype="float32"):
         raise ValueError(f"{name}: need a contiguous {dtype} device tensor of shape {(n, *tail)}, got {got}")
 
 
+def _need_rows(x, name: str, tail=(), dtype="float32"):
+    """Raises unless x is a contiguous device tensor of shape (m, *tail), m >= 1."""
+    torch = _torch()
+    want = getattr(torch, dtype)
+    ok = isinstance(x, torch.Tensor) and x.dtype == want and x.is_contiguous() and x.device.type == "cuda" \
+        and x.dim() == 1 + len(tail) and x.shape[0] >= 1 and tuple(x.shape[1:]) == tuple(tail)
+    if not ok:
+        got = f"{type(x).__name__} {getattr(x, 'dtype', '')} {tuple(getattr(x, 'shape', ()))}"
+        raise ValueError(f"{name}: need a contiguous {dtype} device tensor of shape (m, *{tuple(tail)}), got {got}")
+
+
+def _check_job_inputs(job, materials, image=None, light_faces=None, num_lights=None):
+    """What the bounce loop reads and writes through raw pointers: the 36-word
+    material records (float32), the light list (int32, with its device count)
+    and, if given, the float4 image of w*h pixels it writes."""
+    _need_rows(materials, "materials", (36,))
+    if (light_faces is None) != (num_lights is None):
+        raise ValueError("light_faces and num_lights go together")
+    if light_faces is not None:
+        _need_rows(light_faces, "light_faces", (), "int32")
+        if not (isinstance(num_lights, _torch().Tensor) and num_lights.numel() == 1):
+            raise ValueError("num_lights: need a one-element int32 device tensor")
+        _need(num_lights.reshape(1), "num_lights", 1, (), "int32")
+    if image is not None:
+        _need(image, "image", int(job.width) * int(job.height), (4,))
+
+
 # ---------------------------------------------------------------------- context
 class Context:
     """One device, one in-order stream (torch's current stream by default)."""
@@ -748,6 +775,7 @@ def render(ctx: Context, scene: Scene, materials, env: Optional["CEnvironment"],
            num_lights=None, stats: bool = True):
     """CDeviceGI.loop for one job (see mt_render); returns (image float32[h*w,4], stats or None)."""
     torch = _torch()
+    _check_job_inputs(job, materials, None, light_faces, num_lights)
     desc = render_desc(job)
     image = ctx.empty((job.width * job.height, 4), torch.float32)
     st = _lib.RenderStats() if stats else None
@@ -765,6 +793,7 @@ class RenderGraph:
     def __init__(self, ctx: Context, scene: Scene, materials, env: Optional["CEnvironment"], job,
                  light_faces=None, num_lights=None):
         torch = _torch()
+        _check_job_inputs(job, materials, None, light_faces, num_lights)
         self.ctx = ctx
         self.image = ctx.empty((job.width * job.height, 4), torch.float32)
         # the graph keeps the addresses of everything it reads
@@ -837,7 +866,11 @@ class Batch:
 
     def submit(self, tag: int, scene: Scene, materials, env: Optional["CEnvironment"], job, image,
                light_faces=None, num_lights=None) -> None:
-        """Queues a job; `image` (device float32[h*w,4]) is written when it completes."""
+        """Queues a job; `image` (device float32[h*w,4]) is written when it completes.
+        A tag may not be reused while its job is still in flight."""
+        if int(tag) in self._keep:
+            raise ValueError(f"tag {tag} is still in flight")
+        _check_job_inputs(job, materials, image, light_faces, num_lights)
         d = _lib.JobDesc()
         d.scene = scene.handle
         d.materials = _ptr(materials)

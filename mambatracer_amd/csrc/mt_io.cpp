@@ -291,7 +291,9 @@ extern "C" int32_t mt_hdr_decode(const uint8_t* data, uint64_t len, int32_t* wid
             while (x < w) {
                 if (end - p < 4) goto truncated;
                 if (p[0] == 1 && p[1] == 1 && p[2] == 1) {
-                    if (x == 0) goto corrupt;
+                    // a run count takes at most 3 consecutive records (24 bits);
+                    // a longer chain cannot describe a line of < 2^31 pixels
+                    if (x == 0 || shift > 16) goto corrupt;
                     const long n = (long)p[3] << shift;
                     if (n > w - x) goto corrupt;
                     for (long k = 0; k < n; k++) std::memcpy(&line[4 * (size_t)(x + k)], &line[4 * (size_t)(x - 1)], 4);

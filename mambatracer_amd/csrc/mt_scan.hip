@@ -540,7 +540,15 @@ extern "C" int32_t mt_compact_isects(mt_context* ctx, void* isects, int32_t* pix
     MT_ARG_CHECK(ctx && d_len && d_count, "mt_compact_isects: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "mt_compact_isects: negative capacity");
     MT_ARG_CHECK(capacity == 0 || (isects && pixels && temp_isects), "mt_compact_isects: NULL buffer");
-    MT_ARG_CHECK(capacity == 0 || isects != temp_isects, "mt_compact_isects: temp_isects must not be isects");
+    {
+        // the three buffers are read and written by different passes: no two may share a byte
+        const uintptr_t rb = (uintptr_t)capacity * (ISECT_F4 * 16), pb = (uintptr_t)capacity * 4;
+        const uintptr_t a = reinterpret_cast<uintptr_t>(isects), t = reinterpret_cast<uintptr_t>(temp_isects),
+                        q = reinterpret_cast<uintptr_t>(pixels);
+        auto apart = [](uintptr_t x, uintptr_t xn, uintptr_t y, uintptr_t yn) { return x + xn <= y || y + yn <= x; };
+        MT_ARG_CHECK(capacity == 0 || (apart(a, rb, t, rb) && apart(q, pb, t, rb) && apart(q, pb, a, rb)),
+                     "mt_compact_isects: isects, pixels and temp_isects overlap");
+    }
     MT_ARG_CHECK(((reinterpret_cast<uintptr_t>(isects) | reinterpret_cast<uintptr_t>(temp_isects)) & 15) == 0,
                  "mt_compact_isects: records must be 16-byte aligned");
     ScanWork W;
