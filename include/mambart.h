@@ -304,6 +304,22 @@ int32_t mt_compact(mt_context* ctx, const int32_t* flags, int32_t flag_mode,
                    const int32_t* d_len, int32_t capacity, int32_t* out_index,
                    int32_t* d_count, const mt_payload* payloads, int32_t npayload);
 
+/* mt_compact without allocation.  No counterpart in the reference: this is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as mt_compact, but the look-back
+ * workspace is a fixed per-device array inside the library (256 KiB, held
+ * by each device until the process ends) instead of the context's grown
+ * workspace, so the call allocates nothing.  capacity must not exceed
+ * MT_NOALLOC_MAX_ELEMS (MT_ERR_ARG otherwise).  All calls on one device
+ * share that array: calls that overlap in time on one device (two
+ * contexts, or two streams) race and give invalid results; calls ordered
+ * on one stream are fine.  mt_scan_check reports its spin-limit flag. */
+#define MT_NOALLOC_MAX_ELEMS (1 << 28)
+int32_t mt_compact_noalloc(mt_context* ctx, const int32_t* flags, int32_t flag_mode,
+                           const int32_t* d_len, int32_t capacity, int32_t* out_index,
+                           int32_t* d_count, const mt_payload* payloads, int32_t npayload);
+
 /* CCompact.execute (CCompact.java:79-89; CompactAtomic Compact.cl:21-40,
  * InitIntersection / InitIntData Initialize.cl:50-79, TransferIntersection /
  * TransferPixels Compact.cl:1-18) in place on the reference's own buffers:

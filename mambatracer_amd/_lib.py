@@ -161,6 +161,7 @@ SIGNATURES = {
     "mt_shadow_rays": [P, P, P, P, P, P, P, I32, P, P, P],
     "mt_scan_excl_i32": [P, P, P, P, I32, P],
     "mt_compact": [P, P, I32, P, I32, P, P, C.POINTER(Payload), I32],
+    "mt_compact_noalloc": [P, P, I32, P, I32, P, P, C.POINTER(Payload), I32],
     "mt_scan_check": [P],
     "mt_compact_isects": [P, P, P, P, P, I32, P],
     "mt_mesh_lights": [P, P, P, P, P],

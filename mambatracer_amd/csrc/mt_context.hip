@@ -47,6 +47,8 @@ extern "C" int32_t mt_ctx_create(int32_t device, void* stream, int32_t flags, mt
     MT_HIP_CHECK(hipGetDeviceCount(&ndev));
     MT_ARG_CHECK(device >= 0 && device < ndev, "mt_ctx_create: device %d not in [0,%d)", device, ndev);
     MT_HIP_CHECK(hipSetDevice(device));
+    int32_t rc = mt_noalloc_setup(device); // once per device, with the device current
+    if (rc) return rc;
     mt_context* c = new (std::nothrow) mt_context();
     if (!c) {
         mt_set_error("mt_ctx_create: out of host memory");

@@ -114,6 +114,10 @@ constexpr int ENV_GUIDE_C = 256;
 int32_t mt_ctx_scratch(mt_context* ctx, size_t bytes, void** out);
 int32_t mt_ctx_temp(mt_context* ctx, size_t bytes, void** out);
 int32_t mt_scan_reserve(mt_context* ctx, int32_t capacity);
+// No-alloc compaction workspace (mt_scan.hip): resolve once per device at
+// context creation; look up later.
+int32_t mt_noalloc_setup(int device);
+void* mt_noalloc_workspace(int device);
 
 // One scene's packed traversal layout, as a row of a per-job table.
 struct MtSceneRef {

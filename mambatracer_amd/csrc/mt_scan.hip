@@ -32,6 +32,8 @@
 // compaction is stable (survivors in index order), matching prepareLightInfo.
 #include "mt_internal.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int SCAN_THREADS = 256;
@@ -501,30 +503,37 @@ extern "C" int32_t mt_scan_excl_i32(mt_context* ctx, const int32_t* in, int32_t*
     return MT_OK;
 }
 
-extern "C" int32_t mt_compact(mt_context* ctx, const int32_t* flags, int32_t flag_mode,
-                              const int32_t* d_len, int32_t capacity, int32_t* out_index,
-                              int32_t* d_count, const mt_payload* payloads, int32_t npayload)
+namespace {
+
+// Argument checks and payload table shared by mt_compact and
+// mt_compact_noalloc (`who` names the entry point in errors).
+int32_t compact_args(const char* who, mt_context* ctx, const int32_t* flags, int32_t flag_mode,
+                     const int32_t* d_len, int32_t capacity, int32_t* d_count, const mt_payload* payloads,
+                     int32_t npayload, PayloadSet& P)
 {
-    MT_ARG_CHECK(ctx && d_len && d_count, "mt_compact: NULL argument");
-    MT_ARG_CHECK(capacity == 0 || flags, "mt_compact: NULL flags");
-    MT_ARG_CHECK(capacity >= 0, "mt_compact: negative capacity");
-    MT_ARG_CHECK(flag_mode == 0 || flag_mode == 1, "mt_compact: flag_mode must be 0 or 1");
-    MT_ARG_CHECK(npayload >= 0 && npayload <= 8, "mt_compact: npayload %d not in [0,8]", npayload);
-    MT_ARG_CHECK(npayload == 0 || payloads, "mt_compact: payloads is NULL");
-    PayloadSet P;
+    MT_ARG_CHECK(ctx && d_len && d_count, "%s: NULL argument", who);
+    MT_ARG_CHECK(capacity == 0 || flags, "%s: NULL flags", who);
+    MT_ARG_CHECK(capacity >= 0, "%s: negative capacity", who);
+    MT_ARG_CHECK(flag_mode == 0 || flag_mode == 1, "%s: flag_mode must be 0 or 1", who);
+    MT_ARG_CHECK(npayload >= 0 && npayload <= 8, "%s: npayload %d not in [0,8]", who, npayload);
+    MT_ARG_CHECK(npayload == 0 || payloads, "%s: payloads is NULL", who);
     P.n = npayload;
     for (int j = 0; j < npayload; j++) {
-        MT_ARG_CHECK(capacity == 0 || (payloads[j].src && payloads[j].dst),
-                     "mt_compact: payload %d has a NULL pointer", j);
+        MT_ARG_CHECK(capacity == 0 || (payloads[j].src && payloads[j].dst), "%s: payload %d has a NULL pointer",
+                     who, j);
         int b = payloads[j].elem_bytes;
-        MT_ARG_CHECK(b == 4 || b == 8 || b == 16, "mt_compact: payload %d elem_bytes %d not 4/8/16", j, b);
+        MT_ARG_CHECK(b == 4 || b == 8 || b == 16, "%s: payload %d elem_bytes %d not 4/8/16", who, j, b);
         P.src[j] = payloads[j].src;
         P.dst[j] = payloads[j].dst;
         P.bytes[j] = b;
     }
-    ScanWork W;
-    int32_t rc = scan_workspace(ctx, capacity, W);
-    if (rc) return rc;
+    return MT_OK;
+}
+
+// The two launches of a compaction on look-back workspace W.
+int32_t compact_launch(mt_context* ctx, const int32_t* flags, int32_t flag_mode, const int32_t* d_len,
+                       int32_t capacity, int32_t* out_index, int32_t* d_count, const PayloadSet& P, ScanWork W)
+{
     k_scan_reset<<<1, 64, 0, ctx->stream>>>(W, d_len, capacity, d_count);
     MT_LAUNCH_CHECK();
     if (capacity == 0) return MT_OK;
@@ -532,6 +541,89 @@ extern "C" int32_t mt_compact(mt_context* ctx, const int32_t* flags, int32_t fla
         flags, flag_mode, d_len, capacity, out_index, d_count, P, W);
     MT_LAUNCH_CHECK();
     return MT_OK;
+}
+
+// Workspace of mt_compact_noalloc: 256 B of ticket / error flag / epoch
+// words, then one 8-byte look-back status word per 8192-element tile, for
+// capacities up to MT_NOALLOC_MAX_ELEMS (256 KiB in all).  Each device
+// that loads the library holds its own copy until the process ends.  The
+// array is zero when the code object loads: status words of epoch 0 are
+// never current, the ticket is reset by k_scan_reset at every call, and the
+// error flag is read and cleared by mt_scan_check.
+constexpr int64_t NOALLOC_TILES = ((int64_t)MT_NOALLOC_MAX_ELEMS + SCAN_TILE - 1) / SCAN_TILE;
+constexpr size_t NOALLOC_BYTES = 256 + sizeof(uint64_t) * (size_t)NOALLOC_TILES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// Device address of `scratch` on each device, resolved once per device by
+// mt_noalloc_setup (host code never passes the array's own name, which is a
+// host-side stand-in, to a kernel or copy).
+constexpr int NOALLOC_MAX_DEVICES = 64;
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+std::mutex g_noalloc_mu;
+
+ScanWork noalloc_work(void* p)
+{
+    ScanWork W;
+    W.ticket = reinterpret_cast<uint32_t*>(p);
+    W.status = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(p) + 256);
+    return W;
+}
+
+} // namespace
+
+// Resolves this device's address of the no-alloc workspace; called by
+// mt_ctx_create with `device` current.  Idempotent per device.
+int32_t mt_noalloc_setup(int device)
+{
+    MT_ARG_CHECK(device >= 0 && device < NOALLOC_MAX_DEVICES, "device %d beyond the %d supported", device,
+                 NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return MT_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        mt_set_error("hipGetSymbolAddress(no-alloc workspace) failed on device %d: %s", device,
+                     hipGetErrorString(e));
+        return MT_ERR_HIP;
+    }
+    g_noalloc_ptr[device] = p;
+    return MT_OK;
+}
+
+// This device's no-alloc workspace, or nullptr before mt_noalloc_setup.
+void* mt_noalloc_workspace(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    return g_noalloc_ptr[device];
+}
+
+extern "C" int32_t mt_compact(mt_context* ctx, const int32_t* flags, int32_t flag_mode,
+                              const int32_t* d_len, int32_t capacity, int32_t* out_index,
+                              int32_t* d_count, const mt_payload* payloads, int32_t npayload)
+{
+    PayloadSet P;
+    int32_t rc = compact_args("mt_compact", ctx, flags, flag_mode, d_len, capacity, d_count, payloads, npayload, P);
+    if (rc) return rc;
+    ScanWork W;
+    rc = scan_workspace(ctx, capacity, W);
+    if (rc) return rc;
+    return compact_launch(ctx, flags, flag_mode, d_len, capacity, out_index, d_count, P, W);
+}
+
+extern "C" int32_t mt_compact_noalloc(mt_context* ctx, const int32_t* flags, int32_t flag_mode,
+                                      const int32_t* d_len, int32_t capacity, int32_t* out_index,
+                                      int32_t* d_count, const mt_payload* payloads, int32_t npayload)
+{
+    PayloadSet P;
+    int32_t rc = compact_args("mt_compact_noalloc", ctx, flags, flag_mode, d_len, capacity, d_count, payloads,
+                              npayload, P);
+    if (rc) return rc;
+    MT_ARG_CHECK(capacity <= MT_NOALLOC_MAX_ELEMS, "mt_compact_noalloc: capacity %d exceeds the maximum %d",
+                 capacity, MT_NOALLOC_MAX_ELEMS);
+    void* p = mt_noalloc_workspace(ctx->device);
+    MT_ARG_CHECK(p != nullptr, "mt_compact_noalloc: no workspace on device %d (context not set up)", ctx->device);
+    return compact_launch(ctx, flags, flag_mode, d_len, capacity, out_index, d_count, P, noalloc_work(p));
 }
 
 extern "C" int32_t mt_compact_isects(mt_context* ctx, void* isects, int32_t* pixels, void* temp_isects,
@@ -573,16 +665,20 @@ extern "C" int32_t mt_compact_isects(mt_context* ctx, void* isects, int32_t* pix
 extern "C" int32_t mt_scan_check(mt_context* ctx)
 {
     MT_ARG_CHECK(ctx != nullptr, "mt_scan_check: ctx is NULL");
-    if (!ctx->scratch) return MT_OK;
-    uint32_t flag = 0;
-    uint32_t* tk = reinterpret_cast<uint32_t*>(ctx->scratch);
-    MT_HIP_CHECK(hipMemcpyAsync(&flag, tk + 1, sizeof(flag), hipMemcpyDeviceToHost, ctx->stream));
-    MT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    if (flag) {
-        int32_t rc = mt_buf_fill_i32(ctx, reinterpret_cast<int32_t*>(tk + 1), 0, 1);
-        if (rc) return rc;
-        mt_set_error("scan/compaction look-back exceeded its spin limit; results invalid");
-        return MT_ERR_STATE;
+    // the context's workspace, then the device's no-alloc workspace
+    void* spaces[2] = {ctx->scratch, mt_noalloc_workspace(ctx->device)};
+    for (void* sp : spaces) {
+        if (!sp) continue;
+        uint32_t flag = 0;
+        uint32_t* tk = reinterpret_cast<uint32_t*>(sp);
+        MT_HIP_CHECK(hipMemcpyAsync(&flag, tk + 1, sizeof(flag), hipMemcpyDeviceToHost, ctx->stream));
+        MT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (flag) {
+            int32_t rc = mt_buf_fill_i32(ctx, reinterpret_cast<int32_t*>(tk + 1), 0, 1);
+            if (rc) return rc;
+            mt_set_error("scan/compaction look-back exceeded its spin limit; results invalid");
+            return MT_ERR_STATE;
+        }
     }
     return MT_OK;
 }
