@@ -52,7 +52,7 @@ def traffic_from_profiles(workload: str = "c1"):
         return None
 
 
-def cpu_baseline(W, scene_arrays, mats, rgba, job, npaths, seconds, threads):
+def cpu_baseline(W, scene_arrays, mats, rgba, job, npaths, seconds, threads, grid=None, grid_mode=0):
     import numpy as np
     from oracle import orc_py as orc
     points, normals, faces, nodes, bounds, _ = scene_arrays
@@ -62,15 +62,16 @@ def cpu_baseline(W, scene_arrays, mats, rgba, job, npaths, seconds, threads):
     rng = np.random.default_rng(12345)
     # untimed warm-up (threads, page faults), then a timed probe sizes the sample
     warm = np.sort(rng.choice(npaths, 20000, replace=False)).astype(np.int32)
-    orc.render_paths(points, normals, faces, nodes, bounds, mats, env, job, warm, nthreads=threads)
+    kw = dict(nthreads=threads, grid=grid, grid_mode=grid_mode)
+    orc.render_paths(points, normals, faces, nodes, bounds, mats, env, job, warm, **kw)
     probe = np.sort(rng.choice(npaths, 400000, replace=False)).astype(np.int32)
     t0 = time.perf_counter()
-    orc.render_paths(points, normals, faces, nodes, bounds, mats, env, job, probe, nthreads=threads)
+    orc.render_paths(points, normals, faces, nodes, bounds, mats, env, job, probe, **kw)
     dt = time.perf_counter() - t0
     n = int(min(npaths, max(400000, 400000 * seconds / max(dt, 1e-3))))
     sample = np.sort(rng.choice(npaths, n, replace=False)).astype(np.int32)
     t0 = time.perf_counter()
-    _, rays = orc.render_paths(points, normals, faces, nodes, bounds, mats, env, job, sample, nthreads=threads)
+    _, rays = orc.render_paths(points, normals, faces, nodes, bounds, mats, env, job, sample, **kw)
     dt = time.perf_counter() - t0
     return {
         "value": round(float(rays.sum()) / dt / 1e6, 3),
@@ -92,6 +93,13 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--mode", type=int, default=0, help="0 packed traversal layout, 1 reference layout")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--lighting", choices=["guided", "guided-tile-pdf", "full-map"], default="guided",
+                    help="c1 environment lighting: the adaptive light grid with the reference's pdf (default), "
+                         "the same with the tile pick's pdf applied, or the full-map SAT")
+    ap.add_argument("--qps-seconds", type=float, default=3.0,
+                    help="c1: seconds of Poisson arrivals at 70%% of the measured capacity for p50/p99 latency")
+    ap.add_argument("--other-steps", type=int, default=5,
+                    help="c1: timed replays of each other lighting path, reported beside the headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "compact", "bvh", "stages"], default="c1",
                     help="c1: one 1080p render per step (default line); c2: 64M random rays through 4M "
@@ -107,6 +115,9 @@ def main() -> int:
     ap.add_argument("--policy", type=int, default=0, help="c3/c4: 0 oldest job first, 1 smallest job first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB device to device and exit (PMC byte-counter calibration)")
+    ap.add_argument("--gather-calibration", action="store_true",
+                    help="gather 1 GiB of 64-byte, 64-byte-aligned records in a random order and exit (PMC "
+                         "byte-counter calibration for the traversal's dependent 64-byte node reads)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by pixels or jobs "
@@ -130,6 +141,19 @@ def main() -> int:
         torch.cuda.synchronize()
         print(f"copied {a.numel() * 4} bytes", flush=True)
         return 0
+    if args.gather_calibration:
+        # 16M records of 64 B (a float32 [n, 16] tensor, rows 64-B aligned),
+        # each read exactly once in a random order (index_select's gather
+        # kernel), written out in order
+        n = 1 << 24
+        src = torch.rand((n, 16), dtype=torch.float32, device="cuda")
+        perm = torch.randperm(n, device="cuda")
+        torch.cuda.synchronize()
+        out = torch.index_select(src, 0, perm)
+        torch.cuda.synchronize()
+        print(f"gathered {n} records of 64 bytes ({n * 64} bytes) in a random order; "
+              f"checksum {float(out[:, 0].sum()):.3f}", flush=True)
+        return 0
     if args.workload == "c2":
         return bench_c2(args)
     if args.workload == "compact":
@@ -140,6 +164,111 @@ def main() -> int:
         return bench_stages(args)
     if args.workload != "c1":
         return bench_jobs(args)
+    return bench_c1(args)
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the CPUs this process may run on
+    (its affinity mask), capped by OMP_NUM_THREADS when that is set (the
+    share of a machine's cores a GPU job is given)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+LIGHTING = {
+    "guided": ("env 1024x512 sampled via the adaptive light grid (illuminateEnvironmentLight1 / "
+               "getRadianceEnvironmentLight1 + accumLightGrid, the reference's in-tile pdf) with its tile SATs"),
+    "guided-tile-pdf": ("env 1024x512 sampled via the adaptive light grid with its tile SATs, tile-pick pdf "
+                        "applied"),
+    "full-map": "env 1024x512 sampled via its full-map SAT",
+}
+
+
+def lighting_flags(lighting: str) -> int:
+    from mambatracer_amd import _lib
+    if lighting == "full-map":
+        return 0
+    f = _lib.MT_RENDER_LIGHTGRID_ENV
+    return f | (_lib.MT_RENDER_LIGHTGRID_TILE_PDF if lighting == "guided-tile-pdf" else 0)
+
+
+def prepare_grid(api, ctx, env, rgba, job, render_frame, frames: int = 5):
+    """The light grid as the reference's render thread leaves it after its
+    adaptive phase: seeded from the environment (setHDRLuminance), camera
+    position set (CDeviceGI.java:296), then frames 0..frames-1 each followed
+    by mt_lightgrid_update (CDeviceGI.java:263-264).  Returns the grid."""
+    import numpy as np
+    w, h = rgba.shape[1], rgba.shape[0]
+    lum = np.asarray(env.lum.cpu().numpy(), np.float32)
+    grid = api.CAdaptiveEnvironment(ctx)
+    grid.setHDRLuminance(api.lightgrid_seed_tile(lum, w, h))
+    grid.setCameraPosition(job.pos)
+    env.attachLightGrid(grid)
+    for f in range(frames):
+        render_frame(1000 + f)
+        grid.update()
+    return grid
+
+
+def latency_at_qps(graph, seed, rate, duration, rng_seed=7):
+    """p50/p99 of render-job latency (arrival -> image complete) for jobs of
+    this view arriving as a Poisson process of `rate` per second for
+    `duration` seconds, served in arrival order on the render stream: each
+    job is launched at its arrival time (queued behind earlier jobs when the
+    GPU is busy) and completes at its end event."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(rng_seed)
+    t, arrivals = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        arrivals.append(t)
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ends = [torch.cuda.Event(enable_timing=True) for _ in arrivals]
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    h0 = time.perf_counter()
+    for i, a in enumerate(arrivals):
+        wait = a - (time.perf_counter() - h0)
+        if wait > 0:
+            time.sleep(wait)
+        graph.launch(seed + i)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    lat = np.array([ev0.elapsed_time(e) for e in ends]) - np.array(arrivals) * 1e3
+    lat = np.maximum(lat, 0.0)
+    return {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+            "qps": round(rate, 2), "jobs": len(arrivals), "duration_s": duration,
+            "mean": round(float(lat.mean()), 3)}
+
+
+def bench_c1(args) -> int:
+    import numpy as np
+    import torch
+
+    from mambatracer_amd import _lib, api
+    from mambatracer_amd import workloads as W
+
     t0 = time.perf_counter()
     scene_arrays = W.c1_scene()
     points, normals, faces, nodes, bounds, depth = scene_arrays
@@ -154,9 +283,19 @@ def main() -> int:
     env.setEnvironmentMap(rgba)
     job = W.RenderJob(width=args.width, height=args.height, spp=args.spp, mode=args.mode)
     npaths = job.width * job.height * job.spp
+    lflags = lighting_flags(args.lighting)
+
+    grid = None
+    if lflags:
+        def frame(seed):
+            job.seed, job.flags = seed, lflags
+            api.render(ctx, scene, dmats, env, job, stats=False)
+        grid = prepare_grid(api, ctx, env, rgba, job, frame)
+        job.seed = W.RenderJob().seed
+        log("light grid prepared: 5 frames, each followed by an update")
 
     # untimed: counting traversal for the algorithmic bytes
-    job.flags = _lib.MT_RENDER_TRAVERSAL_STATS
+    job.flags = _lib.MT_RENDER_TRAVERSAL_STATS | lflags
     _, st = api.render(ctx, scene, dmats, env, job)
     ilp = [int(x) for x in st.closest_ilp]
     ext_rays = int(st.extension_rays)
@@ -167,7 +306,7 @@ def main() -> int:
     # The job's launch sequence captured once (mt_render_graph: state laid
     # out once, one graph launch per frame); each step replays it with the
     # job's seed and reads its ray counts.
-    job.flags = 0
+    job.flags = lflags
     graph = api.RenderGraph(ctx, scene, dmats, env, job)
     for i in range(args.warmup):
         graph.launch(job.seed)
@@ -186,13 +325,13 @@ def main() -> int:
         log(f"step {i + 1}/{args.steps}: {lat[-1]:.1f} ms, {int(st.extension_rays)} + {int(st.shadow_rays)} rays")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
-    img = graph.image
+    img = graph.image.clone()
 
     # The same job through mt_render (uncaptured), with every traversal
     # launch bracketed by HIP events on the context stream: the closest-hit
     # kernel's average launch duration for the roofline, and the uncaptured
     # throughput for comparison.
-    job.flags = _lib.MT_RENDER_PROFILE
+    job.flags = _lib.MT_RENDER_PROFILE | lflags
     api.render(ctx, scene, dmats, env, job)
     e_rays, closest_ms, launches = 0, 0.0, 0
     torch.cuda.synchronize()
@@ -204,7 +343,9 @@ def main() -> int:
         launches += int(est.closest_launches)
     torch.cuda.synchronize()
     e_elapsed = time.perf_counter() - e_start
-    if not torch.equal(eimg, img):
+    if not lflags and not torch.equal(eimg, img):
+        # (with the light grid, each frame's sums change the grid the next
+        # frame samples, so consecutive frames differ by design)
         log("captured and uncaptured renders differ")
         return 1
     log(f"uncaptured: {e_elapsed * 1e3 / args.steps:.2f} ms per job")
@@ -216,14 +357,61 @@ def main() -> int:
     value = rays / elapsed / 1e6
     per_launch_bytes = alg_bytes / max(est.closest_launches, 1)
     avg_launch_s = closest_ms / max(launches, 1) / 1e3
-    achieved = per_launch_bytes / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
-    traffic = traffic_from_profiles()
+    alg_achieved = per_launch_bytes / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    traffic = traffic_from_profiles("c1")
+    achieved = traffic / avg_launch_s / 1e9 if (traffic and avg_launch_s > 0) else None
+
+    # render-job latency at a fixed arrival rate: 70% of the measured capacity
+    qps = None
+    if args.qps_seconds > 0:
+        rate = 0.7 * args.steps / elapsed
+        job.flags = lflags
+        qps = latency_at_qps(graph, job.seed, rate, args.qps_seconds)
+        log(f"latency at {rate:.1f} jobs/s: {qps}")
+
+    # the other lighting paths, beside the headline (captured, same view)
+    others = {}
+    for other in LIGHTING:
+        if other == args.lighting or args.other_steps <= 0:
+            continue
+        of = lighting_flags(other)
+        if of and grid is None:
+            def frame(seed, of=of):
+                job.seed, job.flags = seed, of
+                api.render(ctx, scene, dmats, env, job, stats=False)
+            grid = prepare_grid(api, ctx, env, rgba, job, frame)
+            job.seed = W.RenderJob().seed
+        job.flags = of
+        g2 = api.RenderGraph(ctx, scene, dmats, env, job)
+        g2.launch(job.seed)
+        g2.stats()
+        torch.cuda.synchronize()
+        r2, s2 = 0, time.perf_counter()
+        for i in range(args.other_steps):
+            g2.launch(job.seed)
+            st2 = g2.stats()
+            r2 += int(st2.extension_rays) + int(st2.shadow_rays)
+        e2 = time.perf_counter() - s2
+        g2.close()
+        others[other] = {"Mrays_per_s": round(r2 / e2 / 1e6, 3), "ms_per_job": round(e2 * 1e3 / args.other_steps, 3),
+                         "steps": args.other_steps, "lighting": LIGHTING[other]}
+        log(f"{other}: {others[other]}")
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on {threads} threads ...")
-        cpu = cpu_baseline(W, scene_arrays, mats, rgba, job, npaths, args.cpu_seconds, threads)
+        threads = cpu_threads()
+        log(f"cpu baseline on {threads} threads ({cpu_model()}) ...")
+        og = None
+        if lflags:
+            from oracle import orc_py as orc
+            acc, fn, st_ = grid.read_arrays()
+            og = orc.LightGrid(cam=job.pos)
+            og.accum, og.func = acc.reshape(-1).copy(), fn.reshape(-1).copy()
+            og.sat = orc.sat_tiles(og.func, 1600, 1600, 16, 32)
+        job.flags = lflags
+        cpu = cpu_baseline(W, scene_arrays, mats, rgba, job, npaths, args.cpu_seconds, threads, og,
+                           2 if args.lighting == "guided-tile-pdf" else (1 if lflags else 0))
+        cpu["cpu_model"] = cpu_model()
 
     lat_sorted = sorted(lat)
     out = {
@@ -242,32 +430,45 @@ def main() -> int:
         "config": {
             "workload": "c1 wavefront path tracer: 256 UV spheres x 4096 tris + ground (1,048,578 tris), "
                         f"{job.width}x{job.height} x {job.spp} spp, <=8 bounces, RR from bounce 3, "
-                        "env 1024x512 sampled via full-map SAT, surviving paths compacted every bounce (appended by the "
+                        f"{LIGHTING[args.lighting]}, surviving paths compacted every bounce (appended by the "
                         "shading stage to segmented queues)",
             "global_batch": npaths,
+            "lighting": args.lighting,
             "traversal_layout": "packed" if job.mode == 0 else "reference",
             "parallelism": "single GPU",
         },
         "samples_per_s": round(npaths * args.steps / elapsed, 1),
-        "job_latency_ms": {"p50": round(lat_sorted[len(lat) // 2], 3),
-                           "p99": round(lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
-                           "qps": "sequential (one job in flight)"},
-        "timed_path": "captured render (mt_render_graph) replayed once per step",
+        "job_latency_ms": qps if qps is not None else {
+            "p50": round(lat_sorted[len(lat) // 2], 3),
+            "p99": round(lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
+            "qps": "sequential (one job in flight)"},
+        "step_latency_ms": {"p50": round(lat_sorted[len(lat) // 2], 3),
+                            "p99": round(lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))], 3)},
+        "timed_path": "captured render (mt_render_graph) replayed once per step"
+                      + ("; the light grid as left by the reference's adaptive phase (frames 0-4 each followed "
+                         "by an update), timed frames accumulate into it as later frames do" if lflags else ""),
         "uncaptured": {"Mrays_per_s": round(e_rays / e_elapsed / 1e6, 3),
                        "ms_per_job": round(e_elapsed * 1e3 / args.steps, 3),
                        "note": "mt_render per step (state in the context's reused workspace), HIP events on "
                                "every traversal launch; source of the roofline launch duration"},
+        "other_lighting": others,
         "roofline": {
             "kernel": ("closest-hit traversal, packed layout: k_trace_camera (bounce 0, camera rays made in "
                        "place) and k_trace (bounces 1-7), averaged over the job's launches")
                       if job.mode == 0 else "closest-hit traversal (reference layout)",
             "bound": "hbm",
-            "achieved": round(achieved, 1),
+            "achieved": round(achieved, 1) if achieved is not None else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved is not None else None,
             "traffic": traffic,
+            "achieved_basis": "measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                              "gather-calibrated, profiles/) / live average launch duration",
+            "alg_achieved": round(alg_achieved, 1),
+            "alg_frac": round(alg_achieved / HBM_PEAK_GBS, 4),
             "algorithmic_bytes_per_launch": round(per_launch_bytes),
+            "alg_basis": "reference-layout bytes touched, 40 + 148 I + 116 L + 28 P per ray, no cache credit "
+                         "(upper BVH levels are L2/MALL-resident, so this exceeds HBM traffic)",
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
         },
         "cpu_baseline": cpu,

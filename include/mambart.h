@@ -436,6 +436,10 @@ int32_t mt_env_illuminate_adaptive(mt_context* ctx, const mt_envmap* env, const 
 int32_t mt_env_radiance_adaptive(mt_context* ctx, const mt_envmap* env, mt_lightgrid* grid, int32_t n,
                                  const float* dir, const float* hit_p, int32_t accumulate,
                                  float* contrib, float* pdfa, float* lum, int32_t* grid_index);
+/* The environment's light grid (EnvironmentLight.lightGrid, Light.cl:15-33;
+ * set by getEnvironmentLight, :129-137), used by renders with
+ * MT_RENDER_LIGHTGRID_ENV; NULL detaches.  The grid must outlive its use. */
+int32_t mt_env_attach_lightgrid(mt_envmap* env, mt_lightgrid* grid);
 
 /* ------------------------------------------------------------------ */
 /* B. BSDF                                                            */
@@ -484,6 +488,33 @@ typedef struct {
 #define MT_RENDER_PROFILE 2
 /* run the counting traversal variant and total its per-ray counters */
 #define MT_RENDER_TRAVERSAL_STATS 4
+/* Environment light guided by the light grid attached to env
+ * (mt_env_attach_lightgrid): DirectLight samples it with
+ * illuminateEnvironmentLight1 (Light.cl:140-193; the grid cell of the
+ * receiving point, a tile from the cell's distribution, a texel from the
+ * tile's SAT region) and escaped rays take getRadianceEnvironmentLight1
+ * (:229-268) at the path's last hit point ((0,0,0) for camera rays,
+ * InitPathData, Initialize.cl:46), the pairing Render.cl:458-459 leaves
+ * commented out.  Every escaped ray, and every unoccluded environment light
+ * sample (Render.cl:458-459), adds the texel's luminance to the grid entry
+ * of (cell, tile) (accumLightGrid, Light.cl:35-38): in fixed point (2^-20)
+ * with integer atomics during the render, so the sums do not depend on the
+ * order of the additions, then added once per entry to the grid's float
+ * accum at the end of the render.  The grid's camera position and sampling
+ * state are read when the render is enqueued (for a captured render: at
+ * mt_render_graph_create; the grid arrays keep their addresses, so updates
+ * between replays are seen).  Callers update the grid between frames as
+ * CDeviceGI.loop does for frames 0-4 (mt_lightgrid_update,
+ * CDeviceGI.java:263-264). */
+#define MT_RENDER_LIGHTGRID_ENV 8
+/* With MT_RENDER_LIGHTGRID_ENV: the solid-angle pdf of a guided sample, and
+ * of an escaped ray's direction, includes the probability density of the
+ * tile pick (the reference computes it as pdfLL and leaves it unused,
+ * Light.cl:163,182, so its estimator is biased wherever the cell's tile
+ * distribution is not uniform).  Without this flag the reference's in-tile
+ * pdf is used.  Exact for maps whose width is a multiple of 16 and height a
+ * multiple of 32 (texel rectangles of equal size). */
+#define MT_RENDER_LIGHTGRID_TILE_PDF 16
 
 typedef struct {
     int64_t extension_rays; /* closest-hit rays traced (all bounces) */

@@ -92,6 +92,8 @@ class RenderStats(C.Structure):
 MT_RENDER_PICKED_LIGHT = 1
 MT_RENDER_PROFILE = 2
 MT_RENDER_TRAVERSAL_STATS = 4
+MT_RENDER_LIGHTGRID_ENV = 8
+MT_RENDER_LIGHTGRID_TILE_PDF = 16
 
 
 class JobDesc(C.Structure):
@@ -184,6 +186,7 @@ SIGNATURES = {
     "mt_lightgrid_set_camera": [P, P],
     "mt_env_illuminate_adaptive": [P, P, P, I32, P, P, P, P, P, P, P],
     "mt_env_radiance_adaptive": [P, P, P, I32, P, P, I32, P, P, P, P],
+    "mt_env_attach_lightgrid": [P, P],
     "mt_bsdf_setup": [P, I32, P, P, P, P, P, P, P],
     "mt_bsdf_sample": [P, I32, P, P, P, P, P, P, P, P],
     "mt_bsdf_eval": [P, I32, P, P, P, P, P, P, P],

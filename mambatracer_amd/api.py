@@ -629,6 +629,23 @@ def lightgrid_cells(ctx: Context, dirs):
     return cell, txy
 
 
+def lightgrid_seed_tile(lum: np.ndarray, w: int, h: int, samples: int = 500) -> np.ndarray:
+    """The 16 x 32 tile function CAdaptiveEnvironment.setHDRLuminance seeds every
+    cell with (CAdaptiveEnvironment.java:73-77: SAT.sampleToGrid(500, 16, 32),
+    un-vendored): here the expected histogram of `samples` draws from the
+    environment's luminance distribution, i.e. samples * (luminance summed over
+    each tile's texel rectangle) / (total), entry i = tile (i % 16, i / 16).
+    Rectangles as getSubRegionFromUnitBound (SAT.cl:28-44)."""
+    lum = np.asarray(lum, np.float64).reshape(h, w)
+    tile = np.zeros(512, np.float64)
+    for ty in range(32):
+        y0, y1 = int(np.float32(h) * np.float32(ty / np.float32(32))), int(np.float32(h) * np.float32((ty + 1) / np.float32(32)))
+        for tx in range(16):
+            x0, x1 = int(np.float32(w) * np.float32(tx / np.float32(16))), int(np.float32(w) * np.float32((tx + 1) / np.float32(16)))
+            tile[ty * 16 + tx] = lum[y0:y1, x0:x1].sum()
+    return (samples * tile / tile.sum()).astype(np.float32)
+
+
 def sat_tiles(ctx: Context, func, nu: int, nv: int, rx: int, ry: int):
     sat = _torch().empty_like(func)
     call("mt_sat_tiles", ctx.handle, _ptr(func), _ptr(sat), nu, nv, rx, ry)
@@ -677,6 +694,12 @@ class CEnvironment:
         tx = self.ctx.empty((n,), torch.int32)
         call("mt_env_radiance", self.ctx.handle, self.handle, n, _ptr(dirs), _ptr(c), _ptr(pw), _ptr(tx))
         return c, pw, tx
+
+    def attachLightGrid(self, grid: Optional["CAdaptiveEnvironment"]) -> None:
+        """EnvironmentLight.lightGrid (Light.cl:15-33): the grid renders with
+        MT_RENDER_LIGHTGRID_ENV sample and accumulate into (None detaches)."""
+        call("mt_env_attach_lightgrid", self.handle, grid.handle if grid is not None else None)
+        self._grid = grid  # the environment reads the grid's arrays while attached
 
     def illuminate_adaptive(self, grid: "CAdaptiveEnvironment", hit_p, u):
         """illuminateEnvironmentLight1 (mt_env_illuminate_adaptive): hit_p, u float4

@@ -34,6 +34,7 @@ SOURCES = [
     "mt_bvh.hip",
     "mt_image.hip",
     "mt_io.cpp",
+    "mt_error.cpp",
 ]
 
 HIPCC_FLAGS = [

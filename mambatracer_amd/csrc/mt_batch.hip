@@ -396,6 +396,7 @@ int32_t admit(mt_batch* b)
         rec.V.light_faces = d.light_faces;
         rec.V.num_lights = d.d_num_lights;
         rec.V.env = env_view(d.env);
+        rec.V.G = LgView{nullptr, nullptr, nullptr, 0.0f, 0.0f, 0.0f, nullptr, 0}; // full-map environment sampling
         rec.L = reinterpret_cast<float4*>(b->arena + off);
         rec.npaths = J.npaths;
         rec.max_bounces = d.render.max_bounces;

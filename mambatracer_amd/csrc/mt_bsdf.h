@@ -123,8 +123,8 @@ __device__ __forceinline__ v3 ggx_sample_vndf(v3 v, float ax, float ay, float r1
     v3 t2v = cross3(vh, t1v);
     float r = sqrtf(r1);
     float phi = (float)((2.0 * (double)MT_PI) * (double)r2);
-    float t1 = r * cosf(phi);
-    float t2 = r * sinf(phi);
+    float t1 = r * mt_xm_cosf(phi);
+    float t2 = r * mt_xm_sinf(phi);
     float s = (float)(0.5 * (1.0 + (double)vh.z));
     t2 = (float)((1.0 - (double)s) * sqrt(1.0 - (double)(t1 * t1)) + (double)s * (double)t2);
     float k = sqrtf(fmaxf(0.0f, (1.0f - t1 * t1) - t2 * t2));
@@ -169,7 +169,7 @@ __device__ __forceinline__ MtBsdf mt_bsdf_setup_one(v3 ray_d, v3 n, v3 ng, int m
 
 __device__ __forceinline__ v3 mt_schlick(float c, float4 f0)
 { // SchlickColor, Bsdf.cl:185-189
-    float e = powf(1.0f - c, 5.0f);
+    float e = mt_xm_pow5f(1.0f - c);
     return mk3(f0.x + (1.0f - f0.x) * e, f0.y + (1.0f - f0.y) * e, f0.z + (1.0f - f0.z) * e);
 }
 
@@ -242,7 +242,7 @@ __device__ __forceinline__ v3 mt_bsdf_sample_one(const MtBsdf& B, const MtMateri
         if (B.fix.z >= MT_EPS_COSINE) {
             float term1 = (2.0f * MT_PI) * u0;
             float term2 = sqrtf(1.0f - u1);
-            gen = mk3(cosf(term1) * term2, sinf(term1) * term2, sqrtf(u1));
+            gen = mk3(mt_xm_cosf(term1) * term2, mt_xm_sinf(term1) * term2, sqrtf(u1));
             pdf += ((gen.z * 1.0f) / MT_PI) * B.diff_prob;
             f = mt_lambert(m);
         }

@@ -20,25 +20,6 @@ __global__ void k_fill_i32(int32_t* dst, int32_t v, uint64_t n)
     for (; i < n; i += stride) dst[i] = v;
 }
 
-static thread_local char g_err[512] = "";
-
-void mt_set_error(const char* fmt, ...)
-{
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_err, sizeof(g_err), fmt, ap);
-    va_end(ap);
-}
-
-extern "C" int32_t mt_last_error(char* buf, int32_t len)
-{
-    if (!buf || len <= 0) return MT_ERR_ARG;
-    snprintf(buf, (size_t)len, "%s", g_err);
-    return MT_OK;
-}
-
-extern "C" const char* mt_version(void) { return "mambart 0.1 gfx950"; }
-
 extern "C" int32_t mt_ctx_create(int32_t device, void* stream, int32_t flags, mt_context** out)
 {
     MT_ARG_CHECK(out != nullptr, "mt_ctx_create: out is NULL");

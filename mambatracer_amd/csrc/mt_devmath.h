@@ -14,6 +14,10 @@
 
 #include <hip/hip_runtime.h>
 
+// sin / cos / acos / atan2 / x^5 shared with the host (one definition, same bits)
+#define MT_XM_FN __host__ __device__ static inline
+#include "mt_xmath.h"
+
 #define MT_PI 3.14159265359f   // Util.cl:7 (M_PI redefined as this float)
 #define MT_EPS_COSINE 1e-6f    // Util.cl:5
 #define MT_EPS_RAY 1e-3f       // Util.cl:6

@@ -88,6 +88,11 @@ struct mt_lightgrid {
     float cam[3]; // LightGrid.cameraPosition (LightGrid.cl:26), a launch argument
     float* tab;    // per-cell sampling tables [LG_CELLS * 640], rebuilt with the SAT
     bool tab_valid; // false after the raw arrays are handed out, until the next SAT build
+    // Accumulation of the render loop's guided environment mode, in fixed
+    // point (units of 2^-20 luminance, integer atomics: the per-entry sums do
+    // not depend on the order of the additions); folded into accum, and
+    // cleared, at the end of every render.
+    unsigned long long* pend;
 };
 
 struct mt_envmap {
@@ -104,6 +109,9 @@ struct mt_envmap {
     // the map is smaller than the tiling or the tables would be too large)
     float* ttab;
     int32_t tsxm, tsym, tstride;
+    // light grid of the guided environment mode (mt_env_attach_lightgrid;
+    // EnvironmentLight.lightGrid, Light.cl:15-33), or null
+    mt_lightgrid* grid;
 };
 
 // Guide intervals of the environment sampling tables: the marginal CDF gets
@@ -117,6 +125,7 @@ int32_t mt_scan_reserve(mt_context* ctx, int32_t capacity);
 // No-alloc compaction workspace (mt_scan.hip): resolve once per device at
 // context creation; look up later.
 int32_t mt_noalloc_setup(int device);
+int32_t mt_lightgrid_fold(hipStream_t st, mt_lightgrid* g);
 void* mt_noalloc_workspace(int device);
 
 // One scene's packed traversal layout, as a row of a per-job table.

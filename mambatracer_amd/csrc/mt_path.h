@@ -69,6 +69,7 @@ struct SceneView {
     const int* light_faces;
     const int* num_lights;
     EnvView env;
+    LgView G; // light-grid guided environment light (G.mode 0: off)
 };
 
 // RNG stream of one path at one bounce: generate_seed's form
@@ -185,6 +186,11 @@ struct PathOut {
     v3 Ladd;             // radiance gathered at this vertex (emitter / environment hit)
     bool has_shadow;     // a shadow ray so/sd carries the unoccluded contribution
     float4 so, sd, contrib;
+    // guided environment mode: light-grid accumulation of this vertex
+    int acc_gidx;                 // escaped ray: entry of (cell, tile), -1 none
+    unsigned long long acc_q;     //   and its fixed-point luminance
+    int shadow_gidx;              // environment light sample: entry the shadow ray adds
+    float shadow_lum;             //   this luminance to when unoccluded (-1 none)
 };
 
 // One bounce of one path: the ray (ro, rd) of path `pid` at depth `bounce`
@@ -204,12 +210,31 @@ __device__ __forceinline__ void shade_path(const SceneView& V, uint32_t seed, in
     out.so = make_float4(0, 0, 0, INFINITY);
     out.sd = make_float4(1, 0, 0, -1.0f);
     out.contrib = make_float4(0, 0, 0, 0);
+    out.acc_gidx = -1;
+    out.acc_q = 0ull;
+    out.shadow_gidx = -1;
+    out.shadow_lum = 0.0f;
     if (c < 0) {
         // LightHitPass, environment branch (Render.cl:106-123)
         if (V.env.w > 0) {
             float pdfw;
-            int tx;
-            float4 cv = env_radiance(V.env, d, pdfw, tx);
+            float4 cv;
+            if (V.G.mode) {
+                // getRadianceEnvironmentLight1 (Light.cl:229-268) at the path's
+                // last hit point: InitPathData's (0,0,0) for a camera ray
+                // (Initialize.cl:46), else the ray's origin (SampleBSDFRayDirection
+                // stores isect->p as hitpoint, Render.cl:278-279)
+                const v3 hp = bounce == 0 ? mk3(0.0f, 0.0f, 0.0f) : xyz(ro);
+                float lum;
+                int gidx, cell, tile;
+                cv = env_radiance_adaptive(V.env, mk3(V.G.cx, V.G.cy, V.G.cz), d, hp, pdfw, lum, gidx, cell, tile);
+                if (V.G.mode == 2) pdfw = pdfw * lg_tile_pdf(V.G, cell, tile % LG_RX, tile / LG_RX);
+                out.acc_gidx = gidx; // accumLightGrid, Light.cl:264
+                out.acc_q = lg_fixed(lum);
+            } else {
+                int tx;
+                cv = env_radiance(V.env, d, pdfw, tx);
+            }
             v3 contrib = xyz(cv);
             if (!(contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f)) {
                 float w = last_specular ? 1.0f : mis2(last_pdf, pdfw);
@@ -281,10 +306,12 @@ __device__ __forceinline__ void shade_path(const SceneView& V, uint32_t seed, in
                 // DirectLight (Render.cl:377-466) via illuminateLight (:317-375)
                 int32_t sx, sy;
                 seed_for(seed, pid, bounce, 1, sx, sy);
+                // random_float4 (Sampling.cl:80-86): xy pick a light point or,
+                // in the guided mode, the grid tile; zw the texel inside it
                 float s0 = mt_get_random(sx, sy), s1 = mt_get_random(sx, sy);
                 float s2 = mt_get_random(sx, sy), s3 = mt_get_random(sx, sy);
-                (void)s2;
-                (void)s3;
+                int env_gidx = -1;
+                float env_lum = 0.0f;
                 float pick = 1.0f;
                 bool use_env = V.env.w > 0, use_mesh = false;
                 if (use_env && nlights > 0) {
@@ -298,7 +325,16 @@ __device__ __forceinline__ void shade_path(const SceneView& V, uint32_t seed, in
                 v3 ldir = mk3(0, 0, 0);
                 float dist = 0.0f, lpdf = 0.0f;
                 v3 rad = mk3(0, 0, 0);
-                if (use_env) { // illuminateEnvironmentLight, Light.cl:195-227
+                if (use_env && V.G.mode) { // illuminateEnvironmentLight1, Light.cl:140-193
+                    float tpdf;
+                    float4 cv = env_illuminate_adaptive(V.env, V.G.func, V.G.sat, V.G.tab,
+                                                        mk3(V.G.cx, V.G.cy, V.G.cz), p,
+                                                        make_float4(s0, s1, s2, s3), ldir, lpdf, env_lum, env_gidx,
+                                                        tpdf);
+                    if (V.G.mode == 2) lpdf = (tpdf > 0.0f && tpdf < INFINITY) ? lpdf * tpdf : 0.0f;
+                    rad = xyz(cv);
+                    dist = MT_FLOATMAX;
+                } else if (use_env) { // illuminateEnvironmentLight, Light.cl:195-227
                     int tx;
                     float4 cv = env_illuminate(V.env, s0, s1, ldir, lpdf, tx);
                     rad = xyz(cv);
@@ -334,6 +370,10 @@ __device__ __forceinline__ void shade_path(const SceneView& V, uint32_t seed, in
                             out.sd = make_float4(ldir.x, ldir.y, ldir.z, dist - 2 * MT_EPS_RAY);
                             out.contrib = make_float4(cv.x, cv.y, cv.z, 1.0f);
                             out.has_shadow = true;
+                            // the commented-out accumLightGrid after an
+                            // unoccluded test (Render.cl:458-459)
+                            out.shadow_gidx = env_gidx;
+                            out.shadow_lum = env_lum;
                         }
                     }
                 }

@@ -90,24 +90,39 @@ __global__ __launch_bounds__(RT) void k_trace_camera(Camera C, int npaths, const
 // Any hit over the segmented shadow queue with the direct-light add fused
 // in: an unoccluded entry adds its contribution to its path's radiance (what
 // k_nee_accum does after mt_trace_any_seg).  queue == nullptr: slot = pid.
+// With the light-grid guided environment (pend != nullptr), an unoccluded
+// environment sample also adds its luminance to its grid entry (s_lg: entry,
+// luminance bits; the commented-out accumLightGrid, Render.cl:458-459).
 __global__ __launch_bounds__(RT) void k_trace_shadow(const float4* __restrict__ pn, const float4* __restrict__ pt,
                                                      int root, const int* __restrict__ seg_counts, int seg_cap,
                                                      const float4* __restrict__ s_o, const float4* __restrict__ s_d,
                                                      const float4* __restrict__ contrib, const int* __restrict__ queue,
-                                                     float4* __restrict__ L)
+                                                     float4* __restrict__ L, const int2* __restrict__ s_lg,
+                                                     unsigned long long* __restrict__ pend)
 {
     const int k = seg_entry(seg_cap);
-    if (!seg_live(seg_counts, seg_cap, k)) return;
-    RayT r;
-    ray_init(r, s_o[k], s_d[k]);
-    int st[3] = {0, 0, 0};
-    int child = -1;
-    if (trace_packed<true, false, true>(pn, pt, root, r, child, st)) return;
-    const float4 c = contrib[k];
-    const int slot = __float_as_int(c.w);
-    const int pid = queue ? queue[slot] : slot;
-    const float4 v = L[pid];
-    L[pid] = make_float4(v.x + c.x, v.y + c.y, v.z + c.z, v.w);
+    const bool live = seg_live(seg_counts, seg_cap, k);
+    if (__ballot(live) == 0ull) return; // wave-uniform exit
+    bool vis = false;
+    if (live) {
+        RayT r;
+        ray_init(r, s_o[k], s_d[k]);
+        int st[3] = {0, 0, 0};
+        int child = -1;
+        vis = !trace_packed<true, false, true>(pn, pt, root, r, child, st);
+        if (vis) {
+            const float4 c = contrib[k];
+            const int slot = __float_as_int(c.w);
+            const int pid = queue ? queue[slot] : slot;
+            const float4 v = L[pid];
+            L[pid] = make_float4(v.x + c.x, v.y + c.y, v.z + c.z, v.w);
+        }
+    }
+    if (pend) { // the whole wave is active here
+        int2 e = make_int2(-1, 0);
+        if (vis) e = s_lg[k];
+        lg_wave_accum(e.x >= 0, e.x, lg_fixed(__int_as_float(e.y)), pend);
+    }
 }
 
 struct ShadeArgs {
@@ -130,6 +145,7 @@ struct ShadeArgs {
     float4* s_o_tmin;     // out: segmented queue of shadow rays
     float4* s_d_tmax;
     float4* s_contrib;    // out: unoccluded contribution, w = the ray's slot (int bits)
+    int2* s_lg;           // out (guided environment): grid entry, luminance bits of the light sample
     int* shadow_count;    // SHADOW_SEGS segment lengths
     int seg_cap;
     int bounce;
@@ -145,6 +161,8 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, uint32_t seed, in
     PathOut P;
     P.has_shadow = false;
     P.alive = false;
+    P.acc_gidx = -1;
+    P.acc_q = 0ull;
     int pid = -1;
     if (live) {
         float4 ro, rd, T4;
@@ -175,6 +193,10 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, uint32_t seed, in
         }
         if (!A.next_seg) A.alive[slot] = P.alive ? 1 : 0;
     }
+    // escaped rays of the guided environment mode add to the light grid
+    // (accumLightGrid in getRadianceEnvironmentLight1, Light.cl:264); the
+    // whole wave is active here
+    if (A.V.G.mode) lg_wave_accum(P.acc_gidx >= 0, P.acc_gidx, P.acc_q, A.V.G.pend);
     const int seg = blockIdx.x % SHADOW_SEGS;
     if (A.next_seg) {
         // the next queue, segmented like the shadow queue: no result depends
@@ -192,6 +214,7 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, uint32_t seed, in
         A.s_o_tmin[k] = P.so;
         A.s_d_tmax[k] = P.sd;
         A.s_contrib[k] = make_float4(P.contrib.x, P.contrib.y, P.contrib.z, __int_as_float(slot));
+        if (A.s_lg) A.s_lg[k] = make_int2(P.shadow_gidx, __float_as_int(P.shadow_lum));
     }
 }
 
@@ -218,16 +241,26 @@ __global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
 }
 
 // Unoccluded shadow-queue entries add their contribution to their path.
+// (pend != nullptr: an unoccluded environment sample of the guided mode also
+// adds its luminance to its grid entry, as in k_trace_shadow.)
 __global__ __launch_bounds__(RT) void k_nee_accum(const int* __restrict__ queue, const int* __restrict__ seg_counts,
                                                   int seg_cap, const float4* __restrict__ contrib,
-                                                  const int* __restrict__ occluded, PathState S)
+                                                  const int* __restrict__ occluded, PathState S,
+                                                  const int2* __restrict__ s_lg, unsigned long long* __restrict__ pend)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!seg_live(seg_counts, seg_cap, k) || occluded[k]) return;
-    const float4 c = contrib[k];
-    const int pid = queue[__float_as_int(c.w)];
-    float4 L = S.L[pid];
-    S.L[pid] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
+    const bool vis = seg_live(seg_counts, seg_cap, k) && !occluded[k];
+    if (vis) {
+        const float4 c = contrib[k];
+        const int pid = queue[__float_as_int(c.w)];
+        float4 L = S.L[pid];
+        S.L[pid] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
+    }
+    if (pend) {
+        int2 e = make_int2(-1, 0);
+        if (vis) e = s_lg[k];
+        lg_wave_accum(e.x >= 0, e.x, lg_fixed(__int_as_float(e.y)), pend);
+    }
 }
 
 __global__ void k_zero_i32(int* p, int n)
@@ -284,7 +317,8 @@ struct RenderPlan {
     char* base;
     size_t bytes;
     size_t oT, oL, oA_o, oA_d, oB_o, oB_d, oQA, oQB, oPrim, oT_, oAlive, oOcc, oSo, oSd, oSc, oCnt, oShc, oQsc,
-        oIlp, oTot, oSeed;
+        oIlp, oTot, oSeed, oSlg;
+    mt_lightgrid* grid; // guided environment mode: the grid the render samples and accumulates into
     bool fused; // camera rays traced in place, shadow adds fused, next queue appended by the shading stage
     bool borrowed; // base is the context's reusable workspace (mt_render), not owned
     hipEvent_t ev[4 * 64];
@@ -320,6 +354,19 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
     R.V.light_faces = light_faces;
     R.V.num_lights = d_num_lights;
     R.V.env = env_view(env);
+    R.grid = nullptr;
+    R.V.G = LgView{nullptr, nullptr, nullptr, 0.0f, 0.0f, 0.0f, nullptr, 0};
+    const bool guided = (desc->flags & MT_RENDER_LIGHTGRID_ENV) != 0;
+    MT_ARG_CHECK(guided || !(desc->flags & MT_RENDER_LIGHTGRID_TILE_PDF),
+                 "mt_render: MT_RENDER_LIGHTGRID_TILE_PDF needs MT_RENDER_LIGHTGRID_ENV");
+    if (guided) {
+        MT_ARG_CHECK(env && env->grid, "mt_render: MT_RENDER_LIGHTGRID_ENV needs an environment with a light grid "
+                                       "(mt_env_attach_lightgrid)");
+        mt_lightgrid* g = env->grid;
+        R.grid = g;
+        R.V.G = LgView{g->func, g->sat, g->tab_valid ? g->tab : nullptr, g->cam[0], g->cam[1], g->cam[2], g->pend,
+                       (desc->flags & MT_RENDER_LIGHTGRID_TILE_PDF) ? 2 : 1};
+    }
 
     const size_t n = (size_t)R.np;
     size_t bytes = 0;
@@ -354,6 +401,7 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
     R.oIlp = R.want_ilp ? take(12 * (R.sq > n ? R.sq : n)) : 0;
     R.oTot = take(8 * 8 * ILP_SLOTS);
     R.oSeed = take(4);
+    R.oSlg = guided ? take(8 * R.sq) : 0;
     R.bytes = bytes;
     return MT_OK;
 }
@@ -495,6 +543,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         A.s_o_tmin = F4(R.oSo);
         A.s_d_tmax = F4(R.oSd);
         A.s_contrib = F4(R.oSc);
+        A.s_lg = R.grid ? reinterpret_cast<int2*>(base + R.oSlg) : nullptr;
         int* seg_counts = shc + (size_t)b * SHADOW_SEGS;
         A.shadow_count = seg_counts;
         A.seg_cap = seg_cap;
@@ -507,7 +556,8 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         if (profile && (rc = mark(ev[4 * b + 2]))) break;
         if (fused) {
             k_trace_shadow<<<gshadow, RT, 0, st>>>(scene->pnodes, scene->ptris, scene->root_ref, seg_counts, seg_cap,
-                                                    F4(R.oSo), F4(R.oSd), F4(R.oSc), first ? nullptr : q[cur], S.L);
+                                                    F4(R.oSo), F4(R.oSd), F4(R.oSc), first ? nullptr : q[cur], S.L,
+                                                    A.s_lg, R.grid ? R.grid->pend : nullptr);
             if ((rc = chk())) break;
             if (profile && (rc = mark(ev[4 * b + 3]))) break;
         } else {
@@ -517,7 +567,8 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
             if (rc) break;
             if (profile && (rc = mark(ev[4 * b + 3]))) break;
             if (ilp) k_sum_ilp<<<gshadow, RT, 0, st>>>(ilp, nullptr, seg_counts, seg_cap, tot + 3);
-            k_nee_accum<<<gshadow, RT, 0, st>>>(q[cur], seg_counts, seg_cap, F4(R.oSc), I(R.oOcc), S);
+            k_nee_accum<<<gshadow, RT, 0, st>>>(q[cur], seg_counts, seg_cap, F4(R.oSc), I(R.oOcc), S, A.s_lg,
+                                                R.grid ? R.grid->pend : nullptr);
             if ((rc = chk())) break;
         }
         if (fused) {
@@ -534,6 +585,8 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
                                                          reinterpret_cast<float4*>(R.image));
         rc = chk();
     }
+    // the frame's light-grid sums join the grid's float accumulation
+    if (rc == MT_OK && R.grid) rc = mt_lightgrid_fold(st, R.grid);
     return rc;
 }
 

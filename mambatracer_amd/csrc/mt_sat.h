@@ -154,8 +154,8 @@ __device__ __forceinline__ void lg_sample_tab(const float* __restrict__ func, co
 // ---------------------------------------------------------------- spherical maps
 __device__ __forceinline__ void sph_xy(int w, int h, v3 d, int& x, int& y)
 { // getSphericalGridXY, Util.cl:397-410
-    float phi = acosf(d.y);
-    float theta = atan2f(d.z, d.x);
+    float phi = mt_xm_acosf(d.y);
+    float theta = mt_xm_atan2f(d.z, d.x);
     x = (int)((0.5f - (0.5f * theta) / MT_PI) * (float)w);
     y = (int)((phi / MT_PI) * (float)h);
 }
@@ -173,7 +173,7 @@ __device__ __forceinline__ v3 sph_direction(float u, float v)
 { // getSphericalDirection, Util.cl:423-439
     float phi = MT_PI * v;
     float theta = (u * 2.0f) * MT_PI;
-    return mk3(-sinf(phi) * cosf(theta), cosf(phi), sinf(phi) * sinf(theta));
+    return mk3(-mt_xm_sinf(phi) * mt_xm_cosf(theta), mt_xm_cosf(phi), mt_xm_sinf(phi) * mt_xm_sinf(theta));
 }
 
 // ---------------------------------------------------------------- region SAT
@@ -358,8 +358,8 @@ __device__ __forceinline__ float4 env_illuminate(const EnvView& E, float u0, flo
     else r_sample(R, E.sat, E.lum, u0, u1, uu, vv, ox, oy, pdf);
     dir = sph_direction(uu, vv);
     texel = sph_index(E.w, E.h, dir);
-    float s = sinf(vv * MT_PI);
-    pdfw = pdf / (((2.0f * MT_PI) * MT_PI) * sinf(vv * MT_PI));
+    float s = mt_xm_sinf(vv * MT_PI);
+    pdfw = pdf / (((2.0f * MT_PI) * MT_PI) * mt_xm_sinf(vv * MT_PI));
     if (s == 0.0f) pdfw = 0.0f;
     return E.rgba[texel];
 }
@@ -377,7 +377,7 @@ __device__ __forceinline__ float4 env_radiance(const EnvView& E, v3 d, float& pd
     else
         p = r_pdf_cond(R, E.sat, E.lum, x, y) * r_pdf_marg(R, E.sat, y);
     float v = (float)y / (float)E.h;
-    pdfw = p / (((2.0f * MT_PI) * MT_PI) * sinf(v * MT_PI));
+    pdfw = p / (((2.0f * MT_PI) * MT_PI) * mt_xm_sinf(v * MT_PI));
     return E.rgba[texel];
 }
 
@@ -481,11 +481,14 @@ __device__ __forceinline__ void r_sample_tile(const EnvView& E, const SatRegion&
 
 // illuminateEnvironmentLight1, Light.cl:140-193: tile from the cell's
 // distribution, texel from the tile's SAT region.  The returned pdf is the
-// in-tile pdf only (the tile pick's pdf is not applied, :163,182).
+// in-tile pdf only (the tile pick's pdf is not applied, :163,182); tile_pdf
+// is that pick's continuous pdf over the cell's 16 x 32 tiles (pdfLL, which
+// the reference computes and leaves unused).
 __device__ __forceinline__ float4 env_illuminate_adaptive(const EnvView& E, const float* __restrict__ lg_func,
                                                           const float* __restrict__ lg_sat_arr,
                                                           const float* __restrict__ lg_tab, v3 cam, v3 hit,
-                                                          float4 u, v3& dir, float& pdfw, float& lum, int& gidx)
+                                                          float4 u, v3& dir, float& pdfw, float& lum, int& gidx,
+                                                          float& tile_pdf)
 {
     const int s = lg_cell_from_camera(cam, hit);
     int lx, ly;
@@ -499,18 +502,21 @@ __device__ __forceinline__ float4 env_illuminate_adaptive(const EnvView& E, cons
     else r_sample(R, E.sat, E.lum, u.z, u.w, uu, vv, ox, oy, pdf);
     dir = sph_direction(uu, vv);
     const float4 c = E.rgba[sph_index(E.w, E.h, dir)];
-    const float sn = sinf(vv * MT_PI);
-    pdfw = pdf / (((2.0f * MT_PI) * MT_PI) * sinf(vv * MT_PI));
+    const float sn = mt_xm_sinf(vv * MT_PI);
+    pdfw = pdf / (((2.0f * MT_PI) * MT_PI) * mt_xm_sinf(vv * MT_PI));
     if (sn == 0.0f) pdfw = 0.0f;
     lum = luminance3(c.x, c.y, c.z);
     gidx = lg_global_index(s, lx, ly);
+    tile_pdf = lpdf;
     return c;
 }
 
 // getRadianceEnvironmentLight1, Light.cl:229-268, without the accumulation
-// (the caller adds lum into accum[gidx], accumLightGrid :35-38).
+// (the caller adds lum into accum[gidx], accumLightGrid :35-38); cell and
+// tile are the grid cell of the hit point and the tile (ty * 16 + tx) of
+// the direction.
 __device__ __forceinline__ float4 env_radiance_adaptive(const EnvView& E, v3 cam, v3 d, v3 hit, float& pdfa,
-                                                        float& lum, int& gidx)
+                                                        float& lum, int& gidx, int& cell, int& tile)
 {
     const int s = lg_cell_from_camera(cam, hit);
     const int t = sph_index(LG_RX, LG_RY, d); // tileIndexXYFromCamera, LightGrid.cl:80-90
@@ -528,9 +534,78 @@ __device__ __forceinline__ float4 env_radiance_adaptive(const EnvView& E, v3 cam
         p = r_pdf_cond(R, E.sat, E.lum, x, y) * r_pdf_marg(R, E.sat, y);
     }
     const float sv = (float)y / (float)E.h;
-    pdfa = p / (((2.0f * MT_PI) * MT_PI) * sinf(sv * MT_PI));
+    pdfa = p / (((2.0f * MT_PI) * MT_PI) * mt_xm_sinf(sv * MT_PI));
     const float4 c = E.rgba[sph_index(E.w, E.h, d)];
     lum = luminance3(c.x, c.y, c.z);
     gidx = lg_global_index(s, tx, ty);
+    cell = s;
+    tile = t;
     return c;
+}
+
+// ---------------------------------------------------------------- light grid in the bounce loop
+// What the bounce loop's guided environment mode reads: the grid's func,
+// SAT and sampling tables (tab null: evaluate from the SAT), the grid's
+// camera position, and the fixed-point accumulation it adds into.
+// mode 0: full-map environment sampling (illuminateEnvironmentLight /
+// getRadianceEnvironmentLight); 1: the guided pair with the reference's pdf
+// (in-tile only); 2: the guided pair with the tile pick's pdf applied.
+struct LgView {
+    const float* func;
+    const float* sat;
+    const float* tab;
+    float cx, cy, cz;
+    unsigned long long* pend;
+    int mode;
+};
+
+// Continuous pdf of tile (tx, ty) in cell s: the value sampleSubgridContinuous
+// returns when it picks that tile, (func / rowInt) * (rowInt * 512 / total);
+// 0 for a tile the cell cannot pick (row or total 0, or not finite).
+__device__ __forceinline__ float lg_tile_pdf(const LgView& G, int s, int tx, int ty)
+{
+    float fi, last;
+    if (G.tab) {
+        const float* T = G.tab + (size_t)s * LG_TAB;
+        fi = T[LG_TAB_FINT + ty];
+        last = T[LG_TAB_LAST];
+    } else {
+        fi = lg_func_int(G.sat, s, ty);
+        last = lg_sat(G.sat, s, LG_RX - 1, LG_RY - 1);
+    }
+    const float f = G.func[lg_global_index(s, tx, ty)];
+    const float p = (f / fi) * ((fi * (float)(LG_RX * LG_RY)) / last);
+    return (fi > 0.0f && last > 0.0f && p > 0.0f && p < INFINITY) ? p : 0.0f;
+}
+
+// Fixed-point addend of a luminance (units of 2^-20, round half up); 0 for
+// anything not finite and non-negative.
+__host__ __device__ inline unsigned long long lg_fixed(float lum)
+{
+    const double v = (double)lum * 1048576.0;
+    return (v >= 0.0 && v < 9.0e15) ? (unsigned long long)floor(v + 0.5) : 0ull;
+}
+
+// accumLightGrid (Light.cl:35-38) for the lanes with `add`: entry gidx of
+// the fixed-point accumulation gains q.  Call with the whole wave active.
+// Lanes that share an entry with the wave's first pending lane are summed
+// and added with one atomic (camera-coherent misses share a few entries);
+// after four such rounds the rest add their own.  Integer sums: the result
+// does not depend on how the additions are grouped.
+__device__ __forceinline__ void lg_wave_accum(bool add, int gidx, unsigned long long q,
+                                              unsigned long long* __restrict__ pend)
+{
+    add = add && q != 0ull;
+    uint64_t m = __ballot(add);
+    for (int round = 0; round < 4 && m; round++) {
+        const int leader = __builtin_ctzll(m);
+        const int g = __shfl(gidx, leader);
+        const bool mine = add && gidx == g;
+        unsigned long long v = mine ? q : 0ull;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if ((int)__lane_id() == leader) atomicAdd(pend + g, v);
+        if (mine) add = false;
+        m = __ballot(add);
+    }
+    if (add) atomicAdd(pend + gidx, q);
 }

@@ -66,13 +66,28 @@ __global__ __launch_bounds__(256) void k_lg_build(const float* __restrict__ src,
     }
 }
 
-__global__ void k_lg_clear(float* __restrict__ a, float* __restrict__ b, float* __restrict__ c)
+__global__ void k_lg_clear(float* __restrict__ a, float* __restrict__ b, float* __restrict__ c,
+                           unsigned long long* __restrict__ pend)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < LG_SIZE) {
         if (a) a[i] = 0.0f;
         b[i] = 0.0f;
         c[i] = 0.0f;
+        if (pend) pend[i] = 0ull;
+    }
+}
+
+// End of a render in the guided environment mode: the frame's fixed-point
+// sums join the float accumulation, one rounding per entry, and are cleared.
+__global__ void k_lg_fold(unsigned long long* __restrict__ pend, float* __restrict__ accum)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= LG_SIZE) return;
+    const unsigned long long q = pend[i];
+    if (q) {
+        accum[i] = accum[i] + (float)((double)q * (1.0 / 1048576.0));
+        pend[i] = 0ull;
     }
 }
 
@@ -318,8 +333,9 @@ __global__ void k_env_illuminate_adaptive(EnvView E, const float* __restrict__ l
     v3 d;
     float pw, l;
     int g;
+    float tp;
     const float4 c =
-        env_illuminate_adaptive(E, lg_func, lg_sat_arr, lg_tab, mk3(cx, cy, cz), xyz(hit[i]), u[i], d, pw, l, g);
+        env_illuminate_adaptive(E, lg_func, lg_sat_arr, lg_tab, mk3(cx, cy, cz), xyz(hit[i]), u[i], d, pw, l, g, tp);
     dir[i] = make_float4(d.x, d.y, d.z, 0.0f);
     contrib[i] = c;
     pdfw[i] = pw;
@@ -338,8 +354,8 @@ __global__ void k_env_radiance_adaptive(EnvView E, float* accum, float cx, float
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float pa, l;
-    int g;
-    const float4 c = env_radiance_adaptive(E, mk3(cx, cy, cz), xyz(dir[i]), xyz(hit[i]), pa, l, g);
+    int g, s, t;
+    const float4 c = env_radiance_adaptive(E, mk3(cx, cy, cz), xyz(dir[i]), xyz(hit[i]), pa, l, g, s, t);
     if (accum) atomicAdd(accum + g, l);
     if (contrib) contrib[i] = c;
     if (pdfa) pdfa[i] = pa;
@@ -371,6 +387,13 @@ extern "C" int32_t mt_lightgrid_create(mt_context* ctx, mt_lightgrid** out)
         return MT_ERR_ALLOC;
     }
     g->tab_valid = false;
+    if (hipMalloc(&g->pend, sizeof(unsigned long long) * (size_t)LG_SIZE) != hipSuccess) {
+        (void)hipFree(p);
+        (void)hipFree(g->tab);
+        delete g;
+        mt_set_error("mt_lightgrid_create: device allocation failed");
+        return MT_ERR_ALLOC;
+    }
     *out = g;
     return mt_lightgrid_clear(ctx, g);
 }
@@ -382,6 +405,7 @@ extern "C" int32_t mt_lightgrid_destroy(mt_lightgrid* g)
     fe(hipStreamSynchronize(g->ctx->stream));
     fe(hipFree(g->accum));
     fe(hipFree(g->tab));
+    fe(hipFree(g->pend));
     delete g;
     return fe.status("mt_lightgrid_destroy");
 }
@@ -410,7 +434,7 @@ static int32_t lg_refresh_tables(mt_context* ctx, mt_lightgrid* g)
 extern "C" int32_t mt_lightgrid_clear(mt_context* ctx, mt_lightgrid* g)
 {
     MT_ARG_CHECK(ctx && g, "mt_lightgrid_clear: NULL argument");
-    k_lg_clear<<<mt_blocks(LG_SIZE, 256), 256, 0, ctx->stream>>>(g->accum, g->func, g->sat);
+    k_lg_clear<<<mt_blocks(LG_SIZE, 256), 256, 0, ctx->stream>>>(g->accum, g->func, g->sat, g->pend);
     MT_LAUNCH_CHECK();
     return lg_refresh_tables(ctx, g);
 }
@@ -681,6 +705,26 @@ extern "C" int32_t mt_env_radiance_adaptive(mt_context* ctx, const mt_envmap* en
         env_view(env), accumulate ? grid->accum : nullptr, grid->cam[0], grid->cam[1], grid->cam[2], n,
         reinterpret_cast<const float4*>(dir), reinterpret_cast<const float4*>(hit_p),
         reinterpret_cast<float4*>(contrib), pdfa, lum, grid_index);
+    MT_LAUNCH_CHECK();
+    return MT_OK;
+}
+
+extern "C" int32_t mt_env_attach_lightgrid(mt_envmap* env, mt_lightgrid* grid)
+{
+    MT_ARG_CHECK(env != nullptr, "mt_env_attach_lightgrid: NULL environment");
+    if (grid) {
+        int32_t rc = check_adaptive_env(env, "mt_env_attach_lightgrid");
+        if (rc) return rc;
+    }
+    env->grid = grid;
+    return MT_OK;
+}
+
+// Folds the render loop's fixed-point accumulation into the grid's accum
+// (stream-ordered; no allocation, so a captured render may include it).
+int32_t mt_lightgrid_fold(hipStream_t st, mt_lightgrid* g)
+{
+    k_lg_fold<<<mt_blocks(LG_SIZE, 256), 256, 0, st>>>(g->pend, g->accum);
     MT_LAUNCH_CHECK();
     return MT_OK;
 }

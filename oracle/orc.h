@@ -82,6 +82,8 @@ int32_t orc_image_init_argb(void);
 int32_t orc_argb(float r, float g, float b, float a);
 
 /* ---- C: light grid, SATs, environment light ---- */
+/* shared transcendentals (mt_xmath.h): kind 0 sin, 1 cos, 2 acos, 3 atan2(a, b), 4 a^5 */
+int orc_xm_eval(int32_t kind, int64_t n, const float* a, const float* b, float* out);
 int orc_sat_tiles(const float* func, float* sat, int32_t nu, int32_t nv,
                   int32_t rx, int32_t ry);
 int orc_lg_set_tile_func(float* func, const float* tile);
@@ -108,6 +110,19 @@ int orc_env_illuminate_adaptive(const float* rgba, const float* lum, const float
 int orc_env_radiance_adaptive(const float* rgba, const float* lum, const float* sat, int32_t w, int32_t h,
                               float* accum, const float* cam, int32_t n, const float* dir, const float* hit,
                               float* contrib, float* pdfa, float* lum_out, int32_t* gidx);
+
+/* one-sample forms used by the bounce loop, the tile-pick pdf and the
+ * fixed-point grid accumulation of the guided environment mode */
+void orc_env_illum_adaptive_one(const float* rgba, const float* lum, const float* sat, int32_t w, int32_t h,
+                                const float* lg_func, const float* lg_sat_arr, const float* cam, const float* hit,
+                                const float* q, float* dir, float* contrib, float* pdfw, float* lum_out,
+                                int32_t* gidx, int32_t* texel_xy, float* tile_pdf);
+void orc_env_rad_adaptive_one(const float* rgba, const float* lum, const float* sat, int32_t w, int32_t h,
+                              const float* cam, const float* dir, const float* hit, float* contrib, float* pdfa,
+                              float* lum_out, int32_t* gidx, int32_t* cell, int32_t* tile);
+float orc_lg_tile_pdf(const float* lg_func, const float* lg_sat_arr, int32_t s, int32_t tx, int32_t ty);
+uint64_t orc_lg_fixed(float lum);
+int orc_lg_fold(uint64_t* pend, float* accum);
 
 /* ---- B: BSDF ---- */
 /* materials: 36 x 4-byte words per material (4 int flags + 8 float4). */
@@ -139,6 +154,14 @@ int orc_render_paths(const float* points, const float* normals, const int32_t* f
                      int32_t h, int32_t spp, int32_t max_bounces, int32_t rr_start, uint32_t seed,
                      int32_t flags, int64_t npaths, const int32_t* pids, float* L, int64_t* rays,
                      int32_t nthreads);
+int orc_render_paths_lg(const float* points, const float* normals, const int32_t* faces,
+                        const int32_t* nodes, const float* bounds, const float* materials,
+                        const float* rgba, const float* lum, const float* sat, int32_t ew, int32_t eh,
+                        const int32_t* light_faces, int32_t nlights, const float* cam, int32_t w,
+                        int32_t h, int32_t spp, int32_t max_bounces, int32_t rr_start, uint32_t seed,
+                        int32_t flags, int64_t npaths, const int32_t* pids, float* L, int64_t* rays,
+                        int32_t nthreads, const float* lg_func, const float* lg_sat, const float* lg_cam,
+                        uint64_t* pend, int32_t lg_mode);
 
 /* ---- RNG (Sampling.cl:15-59) ---- */
 uint32_t orc_wang_hash(int32_t seed);

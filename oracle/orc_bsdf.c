@@ -122,8 +122,8 @@ static f4 ggx_sample_vndf(f4 v, float ax, float ay, float r1, float r2)
     f4 t2v = cross3(vh, t1v);
     float r = sqrtf(r1);
     float phi = (float)((2.0 * (double)ORC_PI) * (double)r2);
-    float t1 = r * cosf(phi);
-    float t2 = r * sinf(phi);
+    float t1 = r * mt_xm_cosf(phi);
+    float t2 = r * mt_xm_sinf(phi);
     float s = (float)(0.5 * (1.0 + (double)vh.z));
     t2 = (float)((1.0 - (double)s) * sqrt(1.0 - (double)(t1 * t1)) + (double)s * (double)t2);
     float k = sqrtf(fmaxf(0.0f, (1.0f - t1 * t1) - t2 * t2));
@@ -221,7 +221,7 @@ int orc_bsdf_setup(int32_t n, const float* ray_d, const float* isect_n,
 
 static f4 schlick(float c, f4 f0)
 { /* SchlickColor, Bsdf.cl:185-189 */
-    float e = powf(1.0f - c, 5.0f);
+    float e = mt_xm_pow5f(1.0f - c);
     return f4_make(f0.x + (1.0f - f0.x) * e, f0.y + (1.0f - f0.y) * e, f0.z + (1.0f - f0.z) * e, 0.0f);
 }
 
@@ -262,7 +262,7 @@ static f4 sample_diffuse(const obsdf* B, float s0, float s1, f4* gen, float* pdf
     if (B->fix.z < EPS_COSINE) return f4_make(0, 0, 0, 0);
     float term1 = (2.0f * ORC_PI) * s0;
     float term2 = sqrtf(1.0f - s1);
-    *gen = f4_make(cosf(term1) * term2, sinf(term1) * term2, sqrtf(s1), 0.0f);
+    *gen = f4_make(mt_xm_cosf(term1) * term2, mt_xm_sinf(term1) * term2, sqrtf(s1), 0.0f);
     float updf = (gen->z * 1.0f) / ORC_PI;
     *pdf += updf * B->diff_prob;
     f4 c = mat_f4(B->m, M_DIFF_COLOR);

@@ -20,6 +20,12 @@
 #include <stddef.h>
 #include <stdint.h>
 
+/* sin / cos / acos / atan2 / x^5: the one definition the GPU kernels also
+ * compile (mambatracer_amd/csrc/mt_xmath.h), so both round identically;
+ * pinned on its own against float64 numpy in tests/test_xmath.py. */
+#define MT_XM_FN static inline
+#include "mt_xmath.h"
+
 typedef struct { float x, y, z, w; } f4;
 
 static inline f4 f4_make(float x, float y, float z, float w) { f4 r; r.x = x; r.y = y; r.z = z; r.w = w; return r; }

@@ -35,6 +35,8 @@ def lib() -> C.CDLL:
         P, I32, I64, F = C.c_void_p, C.c_int32, C.c_int64, C.c_float
         sig = {
             "orc_bvh_build": [I32, P, P, P, P, P],
+            "orc_xm_eval": [I32, I64, P, P, P],
+            "orc_lg_fold": [P, P],
             "orc_trace_closest": [P, P, P, P, I64, P, P, P, P, P, P, I32],
             "orc_trace_any": [P, P, P, P, I64, P, P, P, P, P, I32],
             "orc_trace_brute": [P, P, I32, I64, P, P, P, P],
@@ -416,15 +418,46 @@ def camera_basis(pos, look, up, vfov_deg, w, h):
     return cam
 
 
+class LightGrid:
+    """The oracle's light grid (LightGrid.cl:9-37; CAdaptiveEnvironment.java):
+    accum / func / sat float32[1600 * 1600], the camera position, and `pend`,
+    the fixed-point accumulation (uint64, units of 2^-20) of the guided
+    render mode, which fold() adds into accum at the end of a frame."""
+
+    def __init__(self, tile=None, cam=(0.0, 0.0, 0.0)):
+        self.accum = np.zeros(1600 * 1600, np.float32)
+        self.func = np.zeros(1600 * 1600, np.float32)
+        self.sat = np.zeros(1600 * 1600, np.float32)
+        self.pend = np.zeros(1600 * 1600, np.uint64)
+        self.cam = np.asarray(cam, np.float32).copy()
+        if tile is not None:
+            self.set_tile_func(tile)
+
+    def set_tile_func(self, tile):
+        """setToAllFunc + the row/column SATs (CAdaptiveEnvironment.setHDRLuminance :69-88)."""
+        self.func = lg_set_tile_func(tile)
+        self.sat = sat_tiles(self.func, 1600, 1600, 16, 32)
+
+    def update(self):
+        """CAdaptiveEnvironment.update (:135-145): func <- accum, then the SATs."""
+        self.func = self.accum.copy()
+        self.sat = sat_tiles(self.func, 1600, 1600, 16, 32)
+
+    def fold(self):
+        _ok(lib().orc_lg_fold(_p(self.pend), _p(self.accum)), "orc_lg_fold")
+
+
 def render_paths(points, normals, faces, nodes, bounds, materials, env, job, pids, light_faces=None,
-                 nthreads=8):
+                 nthreads=8, grid=None, grid_mode=0):
     """Traces the listed path ids one at a time; returns (L float32[max_pid+1,4], rays[2]).
-    env = (rgba[h*w,4], lum, sat, w, h) or None."""
+    env = (rgba[h*w,4], lum, sat, w, h) or None.  grid (a LightGrid) with
+    grid_mode 1 or 2: the light-grid guided environment (reference pdf /
+    tile pick's pdf applied); its pend gains the paths' accumulation."""
     L = lib()
     P, I32, I64, U32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32
-    L.orc_render_paths.argtypes = [P, P, P, P, P, P, P, P, P, I32, I32, P, I32, P, I32, I32, I32, I32, I32,
-                                   U32, I32, I64, P, P, P, I32]
-    L.orc_render_paths.restype = C.c_int32
+    L.orc_render_paths_lg.argtypes = [P, P, P, P, P, P, P, P, P, I32, I32, P, I32, P, I32, I32, I32, I32, I32,
+                                      U32, I32, I64, P, P, P, I32, P, P, P, P, I32]
+    L.orc_render_paths_lg.restype = C.c_int32
     cam = camera_basis(job.pos, job.look, job.up, job.vfov_deg, job.width, job.height)
     pids = _i32(pids)
     out = np.zeros((int(pids.max()) + 1 if pids.size else 0, 4), np.float32)
@@ -438,11 +471,14 @@ def render_paths(points, normals, faces, nodes, bounds, materials, env, job, pid
     normals = None if normals is None else _f32(normals)
     lf = None if light_faces is None else _i32(light_faces)
     points, faces, nodes, bounds, materials = _f32(points), _i32(faces), _i32(nodes), _f32(bounds), _f32(materials)
-    _ok(L.orc_render_paths(_p(points), _p(normals), _p(faces), _p(nodes), _p(bounds),
-                           _p(materials), _p(rgba), _p(lum), _p(sat), ew, eh, _p(lf),
-                           0 if lf is None else lf.size, _p(cam), job.width, job.height, job.spp,
-                           job.max_bounces, job.rr_start, job.seed, job.flags, pids.size, _p(pids), _p(out),
-                           _p(rays), nthreads), "orc_render_paths")
+    g = grid if grid_mode else None
+    _ok(L.orc_render_paths_lg(_p(points), _p(normals), _p(faces), _p(nodes), _p(bounds),
+                              _p(materials), _p(rgba), _p(lum), _p(sat), ew, eh, _p(lf),
+                              0 if lf is None else lf.size, _p(cam), job.width, job.height, job.spp,
+                              job.max_bounces, job.rr_start, job.seed, job.flags, pids.size, _p(pids), _p(out),
+                              _p(rays), nthreads, _p(g.func) if g else None, _p(g.sat) if g else None,
+                              _p(g.cam) if g else None, _p(g.pend) if g else None, int(grid_mode) if g else 0),
+        "orc_render_paths_lg")
     return out, rays
 
 
@@ -477,3 +513,15 @@ def resolve(L_paths, w, h, spp):
     for s in range(spp):
         acc = (acc + L_paths[first + s, :3]).astype(np.float32)
     return (acc * np.float32(1.0 / spp)).astype(np.float32)
+
+
+XM_SIN, XM_COS, XM_ACOS, XM_ATAN2, XM_POW5 = range(5)
+
+
+def xm_eval(kind, a, b=None):
+    """The shared transcendental functions (mt_xmath.h) on float32 arrays."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(a if b is None else b, np.float32)
+    out = np.empty_like(a)
+    _ok(lib().orc_xm_eval(kind, a.size, _p(a), _p(b), _p(out)), "orc_xm_eval")
+    return out

@@ -9,14 +9,21 @@
  * by path id, early end of zero-throughput paths, optional Russian
  * roulette, pinhole camera with in-pixel jitter).
  *
- * Parity status: per-path results follow the product's arithmetic except
- * where device sin/cos/acos/atan2/pow round differently from the host libm;
- * a path whose discrete choice (texel, roulette, lobe) flips on such a
- * rounding difference diverges, so images are compared by relative RMSE.
- * Pinned additionally by closed forms in tests/test_oracle_render.py
- * (white furnace: a closed diffuse-free sphere of constant environment
- * radiance 1 renders exactly 1 at every pixel seen without a hit; a
- * Lambertian plane under a constant sky matches albedo * sky integrals).
+ * The light-grid guided environment mode (lg_mode 1, 2) restates
+ * illuminateEnvironmentLight1 / getRadianceEnvironmentLight1 with
+ * accumLightGrid (Light.cl:35-38,140-268; the pairing Render.cl:458-459
+ * leaves commented out) on the oracle's own grid arrays; grid sums are kept
+ * in fixed point (2^-20 units) so they do not depend on thread order.
+ *
+ * Parity status: per-path results follow the product's arithmetic exactly
+ * (the transcendental functions are the shared mt_xmath.h definitions), so
+ * images compare bit for bit.  The loop itself is pinned by closed forms in
+ * tests/test_oracle_render_pins.py: furnace (a Lambertian plane under a
+ * constant environment, two segments, mean = the quadrature of the MIS
+ * estimator, within 1% of albedo * radiance) for the full-map light and
+ * both guided variants, the reference's guided pdf landing on its
+ * closed-form bias; a triangle emitter's Lambert polygon irradiance for
+ * illuminateMeshLight, pdfAtoW and the emitter MIS.
  */
 #include "orc.h"
 #include "orc_math.h"
@@ -59,7 +66,20 @@ typedef struct {
     uint32_t seed;
     float* L;          /* float4 per path */
     int64_t ext, shadow;
+    /* light-grid guided environment (lg_mode 1: reference pdf, 2: with the
+     * tile pick's pdf; 0: full-map sampling) */
+    const float* lg_func;
+    const float* lg_sat;
+    const float* lg_cam;
+    uint64_t* pend;    /* fixed-point accumulation, atomic adds */
+    int32_t lg_mode;
 } orc_job;
+
+static void lg_add(const orc_job* J, int32_t gidx, float lum)
+{ /* accumLightGrid, Light.cl:35-38, in fixed point */
+    uint64_t q = orc_lg_fixed(lum);
+    if (q) __atomic_fetch_add(&J->pend[gidx], q, __ATOMIC_RELAXED);
+}
 
 static int emitter(const float* m)
 {
@@ -145,11 +165,23 @@ static void trace_path(orc_job* J, int32_t pid)
         float add[3] = {0, 0, 0};
         int alive = 0, has_shadow = 0;
         float so[4], sd[4], sc[3];
+        int32_t sh_gidx = -1;
+        float sh_lum = 0.0f;
         if (prim < 0) {
             if (J->ew > 0) {
                 float dd[4] = {d[0], d[1], d[2], 0}, c4[4], pw;
-                int32_t tx;
-                orc_env_radiance(J->rgba, J->lum, J->sat, J->ew, J->eh, 1, dd, c4, &pw, &tx);
+                if (J->lg_mode) { /* getRadianceEnvironmentLight1 at the path's last hit point */
+                    float hp[4] = {0, 0, 0, 0}, l;
+                    if (bnc > 0) { hp[0] = o[0]; hp[1] = o[1]; hp[2] = o[2]; }
+                    int32_t gidx, cell, tile;
+                    orc_env_rad_adaptive_one(J->rgba, J->lum, J->sat, J->ew, J->eh, J->lg_cam, dd, hp, c4, &pw, &l,
+                                             &gidx, &cell, &tile);
+                    if (J->lg_mode == 2) pw = pw * orc_lg_tile_pdf(J->lg_func, J->lg_sat, cell, tile % 16, tile / 16);
+                    lg_add(J, gidx, l);
+                } else {
+                    int32_t tx;
+                    orc_env_radiance(J->rgba, J->lum, J->sat, J->ew, J->eh, 1, dd, c4, &pw, &tx);
+                }
                 if (!(c4[0] == 0.0f && c4[1] == 0.0f && c4[2] == 0.0f)) {
                     float w = last_spec ? 1.0f : mis2(last_pdf, pw);
                     float v[3] = {T[0] * c4[0] * w, T[1] * c4[1] * w, T[2] * c4[2] * w};
@@ -185,8 +217,9 @@ static void trace_path(orc_job* J, int32_t pid)
                     int32_t ls[2];
                     seed_for(J->seed, pid, bnc, 1, ls);
                     float s0 = orc_get_random(ls), s1 = orc_get_random(ls);
-                    orc_get_random(ls);
-                    orc_get_random(ls);
+                    float s2 = orc_get_random(ls), s3 = orc_get_random(ls);
+                    int32_t env_gidx = -1;
+                    float env_lum = 0.0f;
                     float pick = 1.0f;
                     int use_env = J->ew > 0, use_mesh = 0;
                     if (use_env && J->nlights > 0) {
@@ -198,7 +231,14 @@ static void trace_path(orc_job* J, int32_t pid)
                         use_mesh = 1;
                     }
                     float ldir[4] = {0, 0, 0, 0}, dist = 0.0f, lpdf = 0.0f, rad[4] = {0, 0, 0, 0};
-                    if (use_env) {
+                    if (use_env && J->lg_mode) { /* illuminateEnvironmentLight1 */
+                        float q4[4] = {s0, s1, s2, s3}, tpdf, p4[4] = {p[0], p[1], p[2], 0};
+                        orc_env_illum_adaptive_one(J->rgba, J->lum, J->sat, J->ew, J->eh, J->lg_func, J->lg_sat,
+                                                   J->lg_cam, p4, q4, ldir, rad, &lpdf, &env_lum, &env_gidx, NULL,
+                                                   &tpdf);
+                        if (J->lg_mode == 2) lpdf = (tpdf > 0.0f && tpdf < INFINITY) ? lpdf * tpdf : 0.0f;
+                        dist = 3.402823e+38f;
+                    } else if (use_env) {
                         float u2[2] = {s0, s1};
                         int32_t tx;
                         orc_env_illuminate(J->rgba, J->lum, J->sat, J->ew, J->eh, 1, u2, ldir, rad, &lpdf, &tx);
@@ -236,6 +276,8 @@ static void trace_path(orc_job* J, int32_t pid)
                                 sd[0] = ldir[0]; sd[1] = ldir[1]; sd[2] = ldir[2]; sd[3] = dist - 2 * 1e-3f;
                                 memcpy(sc, cv, sizeof(sc));
                                 has_shadow = 1;
+                                sh_gidx = env_gidx;
+                                sh_lum = env_lum;
                             }
                         }
                     }
@@ -271,7 +313,10 @@ static void trace_path(orc_job* J, int32_t pid)
             int32_t occ;
             J->shadow++;
             orc_trace_any(J->points, J->faces, J->nodes, J->bounds, 1, so, sd, &occ, NULL, NULL, 1);
-            if (!occ) { L[0] = L[0] + sc[0]; L[1] = L[1] + sc[1]; L[2] = L[2] + sc[2]; }
+            if (!occ) {
+                L[0] = L[0] + sc[0]; L[1] = L[1] + sc[1]; L[2] = L[2] + sc[2];
+                if (J->lg_mode && sh_gidx >= 0) lg_add(J, sh_gidx, sh_lum); /* Render.cl:458-459 */
+            }
         }
         if (!alive) break;
     }
@@ -327,8 +372,28 @@ int orc_render_paths(const float* points, const float* normals, const int32_t* f
                      int32_t flags, int64_t npaths, const int32_t* pids, float* L, int64_t* rays,
                      int32_t nthreads)
 {
+    return orc_render_paths_lg(points, normals, faces, nodes, bounds, materials, rgba, lum, sat, ew, eh,
+                               light_faces, nlights, cam, w, h, spp, max_bounces, rr_start, seed, flags, npaths,
+                               pids, L, rays, nthreads, NULL, NULL, NULL, NULL, 0);
+}
+
+/* orc_render_paths with the light-grid guided environment: lg_func / lg_sat
+ * the grid's arrays (1600 x 1600), lg_cam its camera position, pend the
+ * fixed-point accumulation (uint64[1600 * 1600], added to atomically),
+ * lg_mode 1 (the reference's in-tile pdf) or 2 (tile pick's pdf applied). */
+int orc_render_paths_lg(const float* points, const float* normals, const int32_t* faces,
+                        const int32_t* nodes, const float* bounds, const float* materials,
+                        const float* rgba, const float* lum, const float* sat, int32_t ew, int32_t eh,
+                        const int32_t* light_faces, int32_t nlights, const float* cam, int32_t w,
+                        int32_t h, int32_t spp, int32_t max_bounces, int32_t rr_start, uint32_t seed,
+                        int32_t flags, int64_t npaths, const int32_t* pids, float* L, int64_t* rays,
+                        int32_t nthreads, const float* lg_func, const float* lg_sat, const float* lg_cam,
+                        uint64_t* pend, int32_t lg_mode)
+{
     orc_job J;
     memset(&J, 0, sizeof(J));
+    if (lg_mode && !(lg_func && lg_sat && lg_cam && pend && rgba)) return ORC_ERR_ARG;
+    J.lg_func = lg_func; J.lg_sat = lg_sat; J.lg_cam = lg_cam; J.pend = pend; J.lg_mode = lg_mode;
     J.points = points; J.normals = normals; J.faces = faces; J.nodes = nodes; J.bounds = bounds;
     J.materials = materials; J.rgba = rgba; J.lum = lum; J.sat = sat; J.ew = rgba ? ew : 0; J.eh = eh;
     J.light_faces = light_faces; J.nlights = light_faces ? nlights : 0; J.cam = cam;

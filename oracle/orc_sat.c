@@ -208,8 +208,8 @@ int orc_lg_set_tile_func(float* func, const float* tile)
 
 static void spherical_xy(int w, int h, f4 d, int* x, int* y)
 { /* getSphericalGridXY, Util.cl:397-410 */
-    float phi = acosf(d.y);
-    float theta = atan2f(d.z, d.x);
+    float phi = mt_xm_acosf(d.y);
+    float theta = mt_xm_atan2f(d.z, d.x);
     *x = (int)((0.5f - (0.5f * theta) / ORC_PI) * (float)w);
     *y = (int)((phi / ORC_PI) * (float)h);
 }
@@ -360,7 +360,7 @@ static f4 spherical_direction(float u, float v)
 { /* getSphericalDirection, Util.cl:423-439 */
     float phi = ORC_PI * v;
     float theta = (u * 2.0f) * ORC_PI;
-    return f4_make(-sinf(phi) * cosf(theta), cosf(phi), sinf(phi) * sinf(theta), 0.0f);
+    return f4_make(-mt_xm_sinf(phi) * mt_xm_cosf(theta), mt_xm_cosf(phi), mt_xm_sinf(phi) * mt_xm_sinf(theta), 0.0f);
 }
 
 int orc_env_illuminate(const float* rgba, const float* lum, const float* sat,
@@ -374,8 +374,8 @@ int orc_env_illuminate(const float* rgba, const float* lum, const float* sat,
         r_sample(&R, sat, lum, u[2 * i], u[2 * i + 1], uv, off, &pdf);
         f4 d = spherical_direction(uv[0], uv[1]);
         int idx = spherical_index(w, h, d);
-        float s = sinf(uv[1] * ORC_PI);
-        float pw = pdf / (((2.0f * ORC_PI) * ORC_PI) * sinf(uv[1] * ORC_PI));
+        float s = mt_xm_sinf(uv[1] * ORC_PI);
+        float pw = pdf / (((2.0f * ORC_PI) * ORC_PI) * mt_xm_sinf(uv[1] * ORC_PI));
         if (s == 0.0f) pw = 0.0f;
         dir[4 * i] = d.x; dir[4 * i + 1] = d.y; dir[4 * i + 2] = d.z; dir[4 * i + 3] = 0.0f;
         for (int k = 0; k < 4; k++) contrib[4 * i + k] = rgba[4 * (size_t)idx + k];
@@ -397,7 +397,7 @@ int orc_env_radiance(const float* rgba, const float* lum, const float* sat,
         int idx = spherical_index(w, h, d);
         float p = r_pdf_cond(&R, sat, lum, x, y) * r_pdf_marg(&R, sat, y); /* getPdfSAT, SAT.cl:336-342 */
         float v = (float)y / (float)h;
-        pdfw[i] = p / (((2.0f * ORC_PI) * ORC_PI) * sinf(v * ORC_PI));
+        pdfw[i] = p / (((2.0f * ORC_PI) * ORC_PI) * mt_xm_sinf(v * ORC_PI));
         for (int k = 0; k < 4; k++) contrib[4 * i + k] = rgba[4 * (size_t)idx + k];
         texel[i] = idx;
     }
@@ -424,38 +424,74 @@ static oregion tile_region(int w, int h, int tx, int ty)
     return R;
 }
 
+/* illuminateEnvironmentLight1, Light.cl:140-193, one sample; *tile_pdf =
+ * the tile pick's continuous pdf (pdfLL, unused by the reference). */
+void orc_env_illum_adaptive_one(const float* rgba, const float* lum, const float* sat, int32_t w, int32_t h,
+                                const float* lg_func, const float* lg_sat_arr, const float* cam, const float* hit,
+                                const float* q, float* dir, float* contrib, float* pdfw, float* lum_out,
+                                int32_t* gidx, int32_t* texel_xy, float* tile_pdf)
+{
+    int s = cell_from_camera(cam, f4_make(hit[0], hit[1], hit[2], 0.0f));
+    int lx, ly;
+    float luu, lvv, lpdf;
+    lg_sample_one(lg_func, lg_sat_arr, s, q[0], q[1], &lx, &ly, &luu, &lvv, &lpdf);
+    oregion R = tile_region(w, h, lx, ly);
+    float uv[2], pdf;
+    int off[2];
+    r_sample(&R, sat, lum, q[2], q[3], uv, off, &pdf);
+    f4 d = spherical_direction(uv[0], uv[1]);
+    int idx = spherical_index(w, h, d);
+    float sn = mt_xm_sinf(uv[1] * ORC_PI);
+    float pw = pdf / (((2.0f * ORC_PI) * ORC_PI) * mt_xm_sinf(uv[1] * ORC_PI));
+    if (sn == 0.0f) pw = 0.0f;
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z; dir[3] = 0.0f;
+    for (int k = 0; k < 4; k++) contrib[k] = rgba[4 * (size_t)idx + k];
+    *pdfw = pw;
+    *lum_out = luminance3(rgba[4 * (size_t)idx], rgba[4 * (size_t)idx + 1], rgba[4 * (size_t)idx + 2]);
+    *gidx = lg_global_index(s, lx, ly);
+    if (texel_xy) {
+        texel_xy[0] = off[0];
+        texel_xy[1] = off[1];
+    }
+    if (tile_pdf) *tile_pdf = lpdf;
+}
+
 int orc_env_illuminate_adaptive(const float* rgba, const float* lum, const float* sat, int32_t w, int32_t h,
                                 const float* lg_func, const float* lg_sat_arr, const float* cam, int32_t n,
                                 const float* hit, const float* u, float* dir, float* contrib, float* pdfw,
                                 float* lum_out, int32_t* gidx, int32_t* texel_xy)
 { /* illuminateEnvironmentLight1, Light.cl:140-193 */
     if (w < LG_RX || h < LG_RY) return ORC_ERR_ARG;
-    for (int32_t i = 0; i < n; i++) {
-        const float* q = u + 4 * (size_t)i;
-        int s = cell_from_camera(cam, f4_load(hit + 4 * (size_t)i));
-        int lx, ly;
-        float luu, lvv, lpdf;
-        lg_sample_one(lg_func, lg_sat_arr, s, q[0], q[1], &lx, &ly, &luu, &lvv, &lpdf);
-        oregion R = tile_region(w, h, lx, ly);
-        float uv[2], pdf;
-        int off[2];
-        r_sample(&R, sat, lum, q[2], q[3], uv, off, &pdf);
-        f4 d = spherical_direction(uv[0], uv[1]);
-        int idx = spherical_index(w, h, d);
-        float sn = sinf(uv[1] * ORC_PI);
-        float pw = pdf / (((2.0f * ORC_PI) * ORC_PI) * sinf(uv[1] * ORC_PI));
-        if (sn == 0.0f) pw = 0.0f;
-        dir[4 * i] = d.x; dir[4 * i + 1] = d.y; dir[4 * i + 2] = d.z; dir[4 * i + 3] = 0.0f;
-        for (int k = 0; k < 4; k++) contrib[4 * i + k] = rgba[4 * (size_t)idx + k];
-        pdfw[i] = pw;
-        lum_out[i] = luminance3(rgba[4 * (size_t)idx], rgba[4 * (size_t)idx + 1], rgba[4 * (size_t)idx + 2]);
-        gidx[i] = lg_global_index(s, lx, ly);
-        if (texel_xy) {
-            texel_xy[2 * i] = off[0];
-            texel_xy[2 * i + 1] = off[1];
-        }
-    }
+    for (int32_t i = 0; i < n; i++)
+        orc_env_illum_adaptive_one(rgba, lum, sat, w, h, lg_func, lg_sat_arr, cam, hit + 4 * (size_t)i,
+                                   u + 4 * (size_t)i, dir + 4 * (size_t)i, contrib + 4 * (size_t)i, pdfw + i,
+                                   lum_out + i, gidx + i, texel_xy ? texel_xy + 2 * (size_t)i : NULL, NULL);
     return ORC_OK;
+}
+
+/* getRadianceEnvironmentLight1, Light.cl:229-268, one escaped ray, without
+ * the accumulation; *cell / *tile = the grid cell of the hit point and the
+ * tile (ty * 16 + tx) of the direction. */
+void orc_env_rad_adaptive_one(const float* rgba, const float* lum, const float* sat, int32_t w, int32_t h,
+                              const float* cam, const float* dir, const float* hit, float* contrib, float* pdfa,
+                              float* lum_out, int32_t* gidx, int32_t* cell, int32_t* tile)
+{
+    f4 d = f4_make(dir[0], dir[1], dir[2], 0.0f);
+    int s = cell_from_camera(cam, f4_make(hit[0], hit[1], hit[2], 0.0f));
+    int t = spherical_index(LG_RX, LG_RY, d); /* tileIndexXYFromCamera, LightGrid.cl:80-90 */
+    int tx = t % LG_RX, ty = t / LG_RX;
+    oregion R = tile_region(w, h, tx, ty);
+    int x, y;
+    spherical_xy(w, h, d, &x, &y);
+    float p = r_pdf_cond(&R, sat, lum, x, y) * r_pdf_marg(&R, sat, y); /* getPdfSAT, SAT.cl:336-342 */
+    float sv = (float)y / (float)h;
+    *pdfa = p / (((2.0f * ORC_PI) * ORC_PI) * mt_xm_sinf(sv * ORC_PI));
+    int idx = spherical_index(w, h, d);
+    for (int k = 0; k < 4; k++) contrib[k] = rgba[4 * (size_t)idx + k];
+    *lum_out = luminance3(rgba[4 * (size_t)idx], rgba[4 * (size_t)idx + 1], rgba[4 * (size_t)idx + 2]);
+    *gidx = lg_global_index(s, tx, ty);
+    if (cell) *cell = s;
+    if (tile) *tile = t;
 }
 
 int orc_env_radiance_adaptive(const float* rgba, const float* lum, const float* sat, int32_t w, int32_t h,
@@ -464,22 +500,39 @@ int orc_env_radiance_adaptive(const float* rgba, const float* lum, const float* 
 { /* getRadianceEnvironmentLight1, Light.cl:229-268; accumLightGrid :35-38 in index order */
     if (w < LG_RX || h < LG_RY) return ORC_ERR_ARG;
     for (int32_t i = 0; i < n; i++) {
-        f4 d = f4_make(dir[4 * i], dir[4 * i + 1], dir[4 * i + 2], 0.0f);
-        int s = cell_from_camera(cam, f4_load(hit + 4 * (size_t)i));
-        int t = spherical_index(LG_RX, LG_RY, d); /* tileIndexXYFromCamera, LightGrid.cl:80-90 */
-        int tx = t % LG_RX, ty = t / LG_RX;
-        oregion R = tile_region(w, h, tx, ty);
-        int x, y;
-        spherical_xy(w, h, d, &x, &y);
-        float p = r_pdf_cond(&R, sat, lum, x, y) * r_pdf_marg(&R, sat, y); /* getPdfSAT, SAT.cl:336-342 */
-        float sv = (float)y / (float)h;
-        pdfa[i] = p / (((2.0f * ORC_PI) * ORC_PI) * sinf(sv * ORC_PI));
-        int idx = spherical_index(w, h, d);
-        for (int k = 0; k < 4; k++) contrib[4 * i + k] = rgba[4 * (size_t)idx + k];
-        float l = luminance3(rgba[4 * (size_t)idx], rgba[4 * (size_t)idx + 1], rgba[4 * (size_t)idx + 2]);
-        lum_out[i] = l;
-        gidx[i] = lg_global_index(s, tx, ty);
-        if (accum) accum[gidx[i]] += l;
+        orc_env_rad_adaptive_one(rgba, lum, sat, w, h, cam, dir + 4 * (size_t)i, hit + 4 * (size_t)i,
+                                 contrib + 4 * (size_t)i, pdfa + i, lum_out + i, gidx + i, NULL, NULL);
+        if (accum) accum[gidx[i]] += lum_out[i];
     }
+    return ORC_OK;
+}
+
+/* Continuous pdf of tile (tx, ty) in cell s, the value sampleSubgridContinuous
+ * returns when it picks the tile: (func / rowInt) * (rowInt * 512 / total);
+ * 0 for a tile the cell cannot pick. */
+float orc_lg_tile_pdf(const float* lg_func, const float* lg_sat_arr, int32_t s, int32_t tx, int32_t ty)
+{
+    float fi = lg_func_int(lg_sat_arr, s, ty);
+    float last = lg_sat(lg_sat_arr, s, LG_RX - 1, LG_RY - 1);
+    float f = lg_func[lg_global_index(s, tx, ty)];
+    float p = (f / fi) * ((fi * (float)(LG_RX * LG_RY)) / last);
+    return (fi > 0.0f && last > 0.0f && p > 0.0f && p < INFINITY) ? p : 0.0f;
+}
+
+/* accumLightGrid in fixed point (units of 2^-20, round half up). */
+uint64_t orc_lg_fixed(float lum)
+{
+    double v = (double)lum * 1048576.0;
+    return (v >= 0.0 && v < 9.0e15) ? (uint64_t)floor(v + 0.5) : 0u;
+}
+
+/* End of a frame: pend folds into accum (one rounding per entry), cleared. */
+int orc_lg_fold(uint64_t* pend, float* accum)
+{
+    for (int64_t i = 0; i < (int64_t)LG_NU * LG_NV; i++)
+        if (pend[i]) {
+            accum[i] = accum[i] + (float)((double)pend[i] * (1.0 / 1048576.0));
+            pend[i] = 0;
+        }
     return ORC_OK;
 }

@@ -5,8 +5,11 @@ GPU and check (a) a seeded random sample of it against the oracle at the
 usual bars, and (b) size-independent properties of the whole output:
 counts, compaction order, agreement between traversal modes.
 
-Bars: config-1 pixels as the render tests (>= 95% within 1e-4 relative,
-relative RMSE <= 1e-3); config-2 primitive ids, t and occlusion bit-exact."""
+Bars: bit-exact everywhere: config-1 pixels (both lighting paths, the
+light-grid guided one across frames with grid updates, its grid sums too);
+config-2 primitive ids, t and occlusion."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -45,10 +48,76 @@ def test_c1_full_render_sampled_pixels(ctx):
         acc = (acc + L[first + s, :3]).astype(np.float32)
     oimg = (acc * np.float32(1.0 / spp)).astype(np.float32)
     got = img[px]
-    close = np.all(np.abs(got - oimg) <= 1e-4 * np.abs(oimg) + 1e-7, axis=1).mean()
-    rmse = np.sqrt(np.mean((got - oimg) ** 2)) / np.sqrt(np.mean(oimg ** 2))
-    print(f"pixels within 1e-4 relative {close:.4f}, relative RMSE {rmse:.2e}")
-    assert close >= 0.95 and rmse <= 1e-3
+    bad = np.any(got.view(np.int32) != oimg.view(np.int32), axis=1)
+    print(f"pixels differing from the oracle: {int(bad.sum())} of {len(px)} {px[bad][:6].tolist()}")
+    assert not bad.any()
+
+
+def _threads():
+    return max(1, min(32, os.cpu_count() or 1))
+
+
+@pytest.mark.parametrize("tile_pdf", [False, True])
+def test_c1_full_guided_frames(ctx, tile_pdf):
+    """Config 1 with the environment sampled through the light grid:
+    frames 0, 1, 2 of one view with the grid updated after frames 0 and 1
+    (CDeviceGI.java:263-264).  The oracle renders frames 0 and 1 whole (their
+    grid sums feed the updates) and 300 seeded pixels of frame 2; the grid's
+    accum after each frame and frame 2's pixels must be bit-exact."""
+    print(f"[case] c1 full guided frames tile_pdf={tile_pdf}", flush=True)
+    points, normals, faces, nodes, bounds, depth = W.c1_scene()
+    mats = W.c1_materials()
+    rgba = W.env_map()
+    lum, sat = orc.env_lum_sat(rgba.reshape(-1, 4), 1024, 512)
+    oenv = (rgba.reshape(-1, 4), lum, sat, 1024, 512)
+    sc = api.Scene(ctx, points, faces, nodes, bounds, depth, normals=normals)
+    env = api.CEnvironment(ctx)
+    env.setEnvironmentMap(rgba)
+    flags = _lib.MT_RENDER_LIGHTGRID_ENV | (_lib.MT_RENDER_LIGHTGRID_TILE_PDF if tile_pdf else 0)
+    job = W.RenderJob(flags=flags)
+    tile = api.lightgrid_seed_tile(lum, 1024, 512)
+    grid = api.CAdaptiveEnvironment(ctx)
+    grid.setHDRLuminance(tile)
+    grid.setCameraPosition(job.pos)
+    env.attachLightGrid(grid)
+    og = orc.LightGrid(tile, cam=job.pos)
+    dm = ctx.to_device(mats)
+    w, h, spp = job.width, job.height, job.spp
+    for frame in range(3):
+        job.seed = 2 + frame
+        img, st = api.render(ctx, sc, dm, env, job)
+        img = img.cpu().numpy()[:, :3]
+        assert np.isfinite(img).all() and img.mean() > 0
+        if frame < 2:
+            L, rays = orc.render_paths(points, normals, faces, nodes, bounds, mats, oenv, job,
+                                       np.arange(w * h * spp, dtype=np.int32), nthreads=_threads(), grid=og,
+                                       grid_mode=2 if tile_pdf else 1)
+            og.fold()
+            oimg = orc.resolve(L, w, h, spp)
+            del L
+            acc = grid.read_arrays()[0].reshape(-1)
+            nbad_acc = int((acc.view(np.int32) != og.accum.view(np.int32)).sum())
+            bad = np.any(img.view(np.int32) != oimg.view(np.int32), axis=1)
+            print(f"frame {frame}: rays gpu {st.extension_rays}+{st.shadow_rays} oracle {rays[0]}+{rays[1]}; "
+                  f"pixels differing {int(bad.sum())}; accum entries differing {nbad_acc}", flush=True)
+            assert (int(st.extension_rays), int(st.shadow_rays)) == (int(rays[0]), int(rays[1]))
+            assert not bad.any() and nbad_acc == 0
+            grid.update()
+            og.update()
+        else:
+            rng = np.random.default_rng(12)
+            px = np.sort(rng.choice(w * h, 300, replace=False))
+            first = orc.pixel_first_paths(w, h, spp)[px]
+            pids = (first[:, None] + np.arange(spp)[None, :]).reshape(-1).astype(np.int32)
+            L, _ = orc.render_paths(points, normals, faces, nodes, bounds, mats, oenv, job, np.sort(pids),
+                                    nthreads=_threads(), grid=og, grid_mode=2 if tile_pdf else 1)
+            acc3 = np.zeros((len(px), 3), np.float32)
+            for s in range(spp):
+                acc3 = (acc3 + L[first + s, :3]).astype(np.float32)
+            oimg = (acc3 * np.float32(1.0 / spp)).astype(np.float32)
+            bad = np.any(img[px].view(np.int32) != oimg.view(np.int32), axis=1)
+            print(f"frame 2: sampled pixels differing {int(bad.sum())} of {len(px)}", flush=True)
+            assert not bad.any()
 
 
 def test_c2_full_batch_sampled_rays_and_whole_batch_properties(ctx):

@@ -84,7 +84,14 @@ def test_random_rays_brute_force():
     prim, t, _, _ = orc.trace_closest(p, f, nodes, bounds, o, d)
     bp, bt = orc.trace_brute(p, f, o, d)
     np.testing.assert_array_equal(bt.view(np.int32), t.view(np.int32))
-    assert (bp == prim).mean() > 0.999
+    # every id difference must be an exact t-tie: the traversal's triangle
+    # reaches the brute-force distance too (listed)
+    diff = np.flatnonzero(bp != prim)
+    for k in diff:
+        _, t_other = orc.trace_brute(p, f[[prim[k]]], o[k:k + 1], d[k:k + 1])
+        print(f"ray {k}: brute {bp[k]} vs traversal {prim[k]}, t {bt[k]!r} both")
+        assert t_other[0] == bt[k] and prim[k] >= 0
+    assert len(diff) <= 2
 
 
 def test_any_hit_equals_closest_within_tmax(c0):
