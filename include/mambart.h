@@ -639,7 +639,7 @@ typedef struct {
     int64_t iterations;           /* wavefront iterations run */
     int64_t extension_rays;       /* closest-hit rays traced */
     int64_t shadow_rays;          /* any-hit rays with a light sample */
-    int64_t jobs_submitted, jobs_admitted, jobs_completed;
+    int64_t jobs_submitted, jobs_admitted, jobs_completed, jobs_cancelled;
     int32_t active_jobs, pending_jobs;
     uint64_t arena_used, arena_peak; /* bytes of per-job buffers held */
 } mt_batch_stats;
@@ -654,10 +654,11 @@ int32_t mt_batch_destroy(mt_batch* batch);
 int32_t mt_batch_job_bytes(const mt_render_desc* desc, uint64_t* bytes);
 /* Queues a job (validated now; MT_ERR_CAPACITY if it can never fit). */
 int32_t mt_batch_submit(mt_batch* batch, const mt_job_desc* job);
-/* Admits what fits, runs one wavefront iteration and synchronises.  Writes
- * the tags of jobs whose image is complete (at most max_done; the rest
- * follow on later calls) to done_tags, their number to *n_done, and
- * *busy = 1 while any job is pending or in flight. */
+/* Admits what fits, runs the wavefront iterations of one step (1 by
+ * default, mt_batch_set_iterations) and synchronises.  Writes the tags of
+ * jobs whose image is complete, or whose cancellation has drained, (at most
+ * max_done; the rest follow on later calls) to done_tags, their number to
+ * *n_done, and *busy = 1 while any job is pending or in flight. */
 int32_t mt_batch_step(mt_batch* batch, int32_t max_done, int64_t* done_tags, int32_t* n_done, int32_t* busy);
 int32_t mt_batch_get_stats(mt_batch* batch, mt_batch_stats* stats);
 /* Which admitted jobs get free queue slots first: the oldest (default), or
@@ -666,6 +667,20 @@ int32_t mt_batch_get_stats(mt_batch* batch, mt_batch_stats* stats);
 #define MT_BATCH_OLDEST_FIRST 0
 #define MT_BATCH_SMALLEST_FIRST 1
 int32_t mt_batch_set_policy(mt_batch* batch, int32_t policy);
+/* Wavefront iterations per mt_batch_step (1 .. 1024, default 1): more
+ * iterations between host round trips; a job that finishes early in a step
+ * is resolved and reported at the end of the step. */
+int32_t mt_batch_set_iterations(mt_batch* batch, int32_t iterations);
+/* Cancels the job with this tag (CDeviceGI.stop, CDeviceGI.java:438-439,
+ * ends the reference's one running job; here any one job of the mix).
+ * A job not yet admitted is dropped at once; an admitted job is issued no
+ * more paths and its paths in the queues end at their next shading, so it
+ * drains within one iteration.  Either way its tag is reported by a later
+ * mt_batch_step like a completed job's, its image is not written, and its
+ * slot and arena space are then free.  Other jobs' images are unaffected
+ * (bit-identical to their plain renders).  *found = 0 when no pending or
+ * running job has the tag. */
+int32_t mt_batch_cancel(mt_batch* batch, int64_t tag, int32_t* found);
 
 #ifdef __cplusplus
 }

@@ -117,6 +117,7 @@ class BatchStats(C.Structure):
         ("jobs_submitted", C.c_int64),
         ("jobs_admitted", C.c_int64),
         ("jobs_completed", C.c_int64),
+        ("jobs_cancelled", C.c_int64),
         ("active_jobs", C.c_int32),
         ("pending_jobs", C.c_int32),
         ("arena_used", C.c_uint64),
@@ -205,6 +206,8 @@ SIGNATURES = {
     "mt_batch_submit": [P, C.POINTER(JobDesc)],
     "mt_batch_step": [P, I32, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "mt_batch_get_stats": [P, C.POINTER(BatchStats)],
+    "mt_batch_cancel": [P, C.c_int64, P],
+    "mt_batch_set_iterations": [P, I32],
     "mt_batch_set_policy": [P, I32],
 }
 MT_BATCH_OLDEST_FIRST = 0

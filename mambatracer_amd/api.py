@@ -930,6 +930,18 @@ class Batch:
         call("mt_batch_get_stats", self.handle, C.byref(s))
         return s
 
+    def cancel(self, tag: int) -> bool:
+        """Cancels a pending or running job (mt_batch_cancel); its tag is
+        reported by a later step once its paths have drained, its image is
+        not written.  Returns False when no pending or running job has the tag."""
+        found = C.c_int32(0)
+        call("mt_batch_cancel", self.handle, int(tag), C.byref(found))
+        return bool(found.value)
+
+    def set_iterations(self, iterations: int) -> None:
+        """Wavefront iterations per step (mt_batch_set_iterations)."""
+        call("mt_batch_set_iterations", self.handle, int(iterations))
+
     def close(self) -> None:
         if self.handle:
             _lib.load().mt_batch_destroy(self.handle)

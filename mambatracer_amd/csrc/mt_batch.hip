@@ -69,7 +69,7 @@ struct Queue {
 
 // One wave of DONE_SEGS lanes.
 __global__ void k_set_job(JobRec* jobs, MtSceneRef* refs, int* npaths, int* issued, int* done, int* done_tot,
-                          int max_jobs, int slot, JobRec rec, MtSceneRef ref)
+                          int* cancel, int max_jobs, int slot, JobRec rec, MtSceneRef ref)
 {
     if (threadIdx.x < DONE_SEGS) done[(size_t)threadIdx.x * max_jobs + slot] = 0;
     if (threadIdx.x == 0) {
@@ -78,13 +78,27 @@ __global__ void k_set_job(JobRec* jobs, MtSceneRef* refs, int* npaths, int* issu
         npaths[slot] = rec.npaths;
         issued[slot] = 0;
         done_tot[slot] = 0;
+        cancel[slot] = 0;
     }
 }
 
-// done_tot[j] = sum of job j's DONE_SEGS rows, for the active jobs.
+// Cancels the job in `slot`: it is issued no more paths (its path count
+// drops to what it has been issued) and its paths in the queue end at their
+// next shading (k_shade_jobs), so it finishes once they have drained.
+__global__ void k_cancel_job(int* npaths, const int* issued, int* cancel, int slot)
+{
+    if (threadIdx.x == 0) {
+        npaths[slot] = issued[slot];
+        cancel[slot] = 1;
+    }
+}
+
+// done_tot[j] = sum of job j's DONE_SEGS rows, for the active jobs, and
+// fin[j] = 1 once every path the job will ever be issued has ended.
 __global__ __launch_bounds__(BT) void k_done_sum(const int* __restrict__ active, int n_active,
                                                  const int* __restrict__ done, int max_jobs,
-                                                 int* __restrict__ done_tot)
+                                                 const int* __restrict__ npaths, int* __restrict__ done_tot,
+                                                 int* __restrict__ fin)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_active) return;
@@ -92,6 +106,7 @@ __global__ __launch_bounds__(BT) void k_done_sum(const int* __restrict__ active,
     int s = 0;
     for (int r = 0; r < DONE_SEGS; r++) s += done[(size_t)r * max_jobs + j];
     done_tot[j] = s;
+    fin[j] = s >= npaths[j] ? 1 : 0;
 }
 
 __global__ void k_zero_counts(int* cnt, unsigned long long* tot)
@@ -178,6 +193,7 @@ __global__ __launch_bounds__(BT) void k_generate(const JobRec* __restrict__ jobs
 
 struct ShadeJobsArgs {
     const JobRec* jobs;
+    const int* cancel;       // per job slot: 1 = cancelled, its paths end unshaded
     int* done;               // [DONE_SEGS][max_jobs]
     int max_jobs;
     const int* cnt;
@@ -207,8 +223,12 @@ __global__ __launch_bounds__(BT) void k_shade_jobs(ShadeJobsArgs A)
         const int bounce = tg.x & ((1 << BOUNCE_BITS) - 1), pid = tg.y;
         const JobRec& J = A.jobs[job];
         PathOut P;
-        shade_path(J.V, J.seed, J.rr_start, J.flags, pid, bounce, A.Q.ro[slot], A.Q.rd[slot], A.Q.T[slot],
-                   A.prim[slot], A.tin[slot], P);
+        P.alive = false;
+        P.has_shadow = false;
+        P.Ladd = mk3(0.0f, 0.0f, 0.0f);
+        if (!A.cancel[job])
+            shade_path(J.V, J.seed, J.rr_start, J.flags, pid, bounce, A.Q.ro[slot], A.Q.rd[slot], A.Q.T[slot],
+                       A.prim[slot], A.tin[slot], P);
         const bool alive = P.alive && bounce + 1 < J.max_bounces;
         if (alive) {
             A.Q.T[slot] = P.T;
@@ -292,7 +312,7 @@ struct mt_batch {
     char* arena; // per-job radiance buffers
     JobRec* jobs;
     MtSceneRef* refs;
-    int *npaths, *issued, *done, *done_tot;
+    int *npaths, *issued, *done, *done_tot, *cancel, *fin;
     int *active, *gen_start, *gen_pid0, *gen_job;
     int* cnt;                 // [0] live after compaction, [1] live + new
     int* s_seg;               // shadow queue segment lengths
@@ -305,13 +325,14 @@ struct mt_batch {
     float4 *s_o, *s_d, *s_c;
     int2* s_tag;
     int* h_active;            // pinned
-    int* h_done;              // pinned
+    int* h_done;              // pinned: fin flags of the job slots
     unsigned long long* h_tot; // pinned
     struct Job {
         mt_job_desc d;
         int slot;
         int npaths;
         uint64_t off, len;
+        bool cancelled;
     };
     std::deque<Job> pending;
     std::vector<Job> active_jobs; // admission order
@@ -320,6 +341,7 @@ struct mt_batch {
     std::deque<int64_t> finished;             // tags not yet reported
     bool active_dirty;
     int policy;
+    int iters_per_step;       // wavefront iterations between host round trips
     mt_batch_stats st;
 };
 
@@ -405,7 +427,7 @@ int32_t admit(mt_batch* b)
         rec.seed = d.render.seed;
         MtSceneRef ref{d.scene->pnodes, d.scene->ptris, d.scene->root_ref, 0};
         k_set_job<<<1, DONE_SEGS, 0, b->ctx->stream>>>(b->jobs, b->refs, b->npaths, b->issued, b->done, b->done_tot,
-                                                       b->max_jobs, J.slot, rec, ref);
+                                                       b->cancel, b->max_jobs, J.slot, rec, ref);
         MT_LAUNCH_CHECK();
         b->st.arena_used += J.len;
         if (b->st.arena_used > b->st.arena_peak) b->st.arena_peak = b->st.arena_used;
@@ -448,6 +470,7 @@ int32_t iterate(mt_batch* b)
     if (rc) return rc;
     ShadeJobsArgs A;
     A.jobs = b->jobs;
+    A.cancel = b->cancel;
     A.done = b->done;
     A.max_jobs = b->max_jobs;
     A.cnt = b->cnt;
@@ -469,7 +492,8 @@ int32_t iterate(mt_batch* b)
     mt_payload P[4] = {{q.tag, nq.tag, 8}, {q.ro, nq.ro, 16}, {q.rd, nq.rd, 16}, {q.T, nq.T, 16}};
     rc = mt_compact(b->ctx, b->alive, 0, b->cnt + 1, Qc, nullptr, b->cnt, P, 4);
     if (rc) return rc;
-    k_done_sum<<<mt_blocks(n_active, BT), BT, 0, s>>>(b->active, n_active, b->done, b->max_jobs, b->done_tot);
+    k_done_sum<<<mt_blocks(n_active, BT), BT, 0, s>>>(b->active, n_active, b->done, b->max_jobs, b->npaths,
+                                                      b->done_tot, b->fin);
     MT_LAUNCH_CHECK();
     b->cur = 1 - b->cur;
     b->st.iterations++;
@@ -509,6 +533,7 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     auto take = [&](size_t sz) { size_t off = bytes; bytes += (sz + 255) & ~(size_t)255; return off; };
     size_t oJobs = take(sizeof(JobRec) * J), oRefs = take(sizeof(MtSceneRef) * J);
     size_t oNp = take(4 * J), oIss = take(4 * J), oDone = take(4 * J * DONE_SEGS), oDtot = take(4 * J);
+    size_t oCan = take(4 * J), oFin = take(4 * J);
     size_t oAct = take(4 * J), oGs = take(4 * J), oGp = take(4 * J), oGj = take(4 * J);
     size_t oCnt = take(16), oTot = take(16), oSeg = take(4 * JOB_SEGS);
     b->seg_cap = shadow_seg_cap<JOB_SEGS>(queue_capacity, BT);
@@ -548,6 +573,8 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     b->issued = reinterpret_cast<int*>(m + oIss);
     b->done = reinterpret_cast<int*>(m + oDone);
     b->done_tot = reinterpret_cast<int*>(m + oDtot);
+    b->cancel = reinterpret_cast<int*>(m + oCan);
+    b->fin = reinterpret_cast<int*>(m + oFin);
     b->s_seg = reinterpret_cast<int*>(m + oSeg);
     b->active = reinterpret_cast<int*>(m + oAct);
     b->gen_start = reinterpret_cast<int*>(m + oGs);
@@ -572,6 +599,7 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     b->active_dirty = true;
     b->st = mt_batch_stats{};
     b->policy = MT_BATCH_OLDEST_FIRST;
+    b->iters_per_step = 1;
     for (int k = max_jobs - 1; k >= 0; k--) b->free_slots.push_back(k);
     b->free_blocks[0] = b->arena_bytes;
     k_zero_counts<<<1, 64, 0, ctx->stream>>>(b->cnt, b->tot);
@@ -609,6 +637,7 @@ extern "C" int32_t mt_batch_submit(mt_batch* b, const mt_job_desc* job)
     J.npaths = job->render.width * job->render.height * job->render.spp;
     J.len = job_bytes(job->render);
     J.off = 0;
+    J.cancelled = false;
     if (J.len > b->arena_bytes) {
         mt_set_error("mt_batch_submit: job needs %llu arena bytes, the arena holds %llu",
                      (unsigned long long)J.len, (unsigned long long)b->arena_bytes);
@@ -627,9 +656,13 @@ extern "C" int32_t mt_batch_step(mt_batch* b, int32_t max_done, int64_t* done_ta
     int32_t rc = admit(b);
     if (rc) return rc;
     if (!b->active_jobs.empty()) {
-        rc = iterate(b);
-        if (rc) return rc;
-        MT_HIP_CHECK(hipMemcpyAsync(b->h_done, b->done_tot, 4 * (size_t)b->max_jobs, hipMemcpyDeviceToHost, s));
+        // several iterations back to back: finished jobs are detected on the
+        // device (k_done_sum's flags) and read back once per step
+        for (int it = 0; it < b->iters_per_step; it++) {
+            rc = iterate(b);
+            if (rc) return rc;
+        }
+        MT_HIP_CHECK(hipMemcpyAsync(b->h_done, b->fin, 4 * (size_t)b->max_jobs, hipMemcpyDeviceToHost, s));
         MT_HIP_CHECK(hipMemcpyAsync(b->h_tot, b->tot, 16, hipMemcpyDeviceToHost, s));
         MT_HIP_CHECK(hipStreamSynchronize(s));
         b->st.extension_rays = (int64_t)b->h_tot[0];
@@ -637,23 +670,25 @@ extern "C" int32_t mt_batch_step(mt_batch* b, int32_t max_done, int64_t* done_ta
         bool any = false;
         for (size_t i = 0; i < b->active_jobs.size();) {
             mt_batch::Job& J = b->active_jobs[i];
-            if (b->h_done[J.slot] < J.npaths) {
+            if (!b->h_done[J.slot]) {
                 i++;
                 continue;
             }
             const mt_render_desc& r = J.d.render;
             const int npix = r.width * r.height;
-            k_resolve<<<mt_blocks(npix, resolve_pixels_per_group(r.spp)), BT, 0, s>>>(reinterpret_cast<const float4*>(b->arena + J.off),
-                                                         r.width, r.height, r.spp,
-                                                         reinterpret_cast<float4*>(J.d.image));
-            MT_LAUNCH_CHECK();
+            if (!J.cancelled) {
+                k_resolve<<<mt_blocks(npix, resolve_pixels_per_group(r.spp)), BT, 0, s>>>(
+                    reinterpret_cast<const float4*>(b->arena + J.off), r.width, r.height, r.spp,
+                    reinterpret_cast<float4*>(J.d.image));
+                MT_LAUNCH_CHECK();
+                b->st.jobs_completed++;
+            }
             // stream order keeps the freed slot and arena range untouched
             // until this resolve has read them
             arena_give(b, J.off, J.len);
             b->st.arena_used -= J.len;
             b->free_slots.push_back(J.slot);
             b->finished.push_back(J.d.tag);
-            b->st.jobs_completed++;
             b->active_jobs.erase(b->active_jobs.begin() + (ptrdiff_t)i);
             b->active_dirty = true;
             any = true;
@@ -669,6 +704,38 @@ extern "C" int32_t mt_batch_step(mt_batch* b, int32_t max_done, int64_t* done_ta
     }
     *n_done = nd;
     *busy = (!b->active_jobs.empty() || !b->pending.empty() || !b->finished.empty()) ? 1 : 0;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_batch_cancel(mt_batch* b, int64_t tag, int32_t* found)
+{
+    MT_ARG_CHECK(b && found, "mt_batch_cancel: NULL argument");
+    *found = 0;
+    for (auto it = b->pending.begin(); it != b->pending.end(); ++it) {
+        if (it->d.tag != tag) continue;
+        b->pending.erase(it);
+        b->finished.push_back(tag); // never admitted: nothing to drain
+        b->st.jobs_cancelled++;
+        *found = 1;
+        return MT_OK;
+    }
+    for (mt_batch::Job& J : b->active_jobs) {
+        if (J.d.tag != tag || J.cancelled) continue;
+        k_cancel_job<<<1, 64, 0, b->ctx->stream>>>(b->npaths, b->issued, b->cancel, J.slot);
+        MT_LAUNCH_CHECK();
+        J.cancelled = true;
+        b->st.jobs_cancelled++;
+        *found = 1;
+        return MT_OK;
+    }
+    return MT_OK;
+}
+
+extern "C" int32_t mt_batch_set_iterations(mt_batch* b, int32_t iterations)
+{
+    MT_ARG_CHECK(b, "mt_batch_set_iterations: NULL batch");
+    MT_ARG_CHECK(iterations >= 1 && iterations <= 1024, "mt_batch_set_iterations: %d not in [1,1024]", iterations);
+    b->iters_per_step = iterations;
     return MT_OK;
 }
 

@@ -260,6 +260,10 @@ extern "C" int32_t mt_hdr_decode(const uint8_t* data, uint64_t len, int32_t* wid
         mt_set_error("mt_hdr_decode: unsupported resolution line '%s' (need -Y h +X w)", s.c_str());
         return MT_ERR_ARG;
     }
+    if ((int64_t)w * h > ((int64_t)1 << 30)) { // float4 pixels: 16 GiB, beyond any environment map
+        mt_set_error("mt_hdr_decode: %d x %d pixels exceed the 2^30 limit", w, h);
+        return MT_ERR_ARG;
+    }
     *width = w;
     *height = h;
     if (!rgba) return MT_OK;

@@ -14,6 +14,55 @@
 #include "mt_devmath.h"
 #include "mt_internal.h"
 
+// ---------------------------------------------------------------- CDF search
+// upperBound (LightGrid.cl:212-247, SAT.cl:206-242) by bisection: the number
+// of leading entries of a[b, e) with !(u < a[i]) when a is non-decreasing,
+// and for any a the index the reference's bisection returns.  The next four
+// bisection steps read at most 15 entries whose positions are fixed by the
+// current range (a node's children are the halves of its range), so those
+// entries are loaded together and the four steps then run on the loaded
+// values: a 64-entry range costs two rounds of memory latency instead of
+// six dependent loads, and every comparison is the bisection's own.
+__device__ __forceinline__ int cdf_search4(const float* __restrict__ a, int b, int e, float u)
+{
+    while (b != e) {
+        int pos[15];
+        float v[15];
+        int lo[15], hi[15];
+        lo[0] = b;
+        hi[0] = e;
+#pragma unroll
+        for (int i = 0; i < 15; i++) { // nodes of the next four levels, breadth first
+            const bool live = lo[i] < hi[i];
+            const int mid = lo[i] + (hi[i] - lo[i]) / 2;
+            pos[i] = live ? mid : -1;
+            if (2 * i + 2 < 15) {
+                lo[2 * i + 1] = lo[i];
+                hi[2 * i + 1] = live ? mid : lo[i];
+                lo[2 * i + 2] = live ? mid + 1 : lo[i];
+                hi[2 * i + 2] = live ? hi[i] : lo[i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 15; i++) v[i] = pos[i] >= 0 ? a[pos[i]] : 0.0f;
+        int node = 0;
+#pragma unroll
+        for (int level = 0; level < 4; level++) {
+            if (b == e) break;
+            const int first = (1 << level) - 1; // this level's nodes: [first, 2 * first]
+            float x = v[first];
+#pragma unroll
+            for (int i = first + 1; i <= 2 * first; i++) x = node == i ? v[i] : x;
+            const int mid = b + (e - b) / 2;
+            const bool right = !(u < x);
+            b = right ? mid + 1 : b;
+            e = right ? e : mid;
+            node = right ? 2 * node + 2 : 2 * node + 1;
+        }
+    }
+    return b;
+}
+
 // ---------------------------------------------------------------- light grid
 __device__ __forceinline__ int lg_global_index(int s, int lx, int ly)
 { // globalIndexInSubgrid, LightGrid.cl:104-139
@@ -121,13 +170,7 @@ __device__ __forceinline__ void lg_sample_tab(const float* __restrict__ func, co
                                               float& pdf)
 {
     const float* T = tab + (size_t)s * LG_TAB;
-    int b = 0, e = LG_RY;
-    while (b != e) {
-        int mid = b + (e - b) / 2;
-        bool right = !(u1 < T[mid]);
-        b = right ? mid + 1 : b;
-        e = right ? e : mid;
-    }
+    int b = cdf_search4(T, 0, LG_RY, u1);
     oy = b - 1 > 0 ? b - 1 : 0;
     const float m0 = T[oy];
     const float du = (u1 - m0) / (T[oy + 1] - m0);
@@ -135,14 +178,7 @@ __device__ __forceinline__ void lg_sample_tab(const float* __restrict__ func, co
     const float pdf_v = (fi * (float)(LG_RX * LG_RY)) / T[LG_TAB_LAST];
     vv = ((float)oy + du) / (float)LG_RY;
     const float* C = T + LG_TAB_COND + oy * LG_TAB_CSTRIDE;
-    b = 0;
-    e = LG_RX;
-    while (b != e) {
-        int mid = b + (e - b) / 2;
-        bool right = !(u0 < C[mid]);
-        b = right ? mid + 1 : b;
-        e = right ? e : mid;
-    }
+    b = cdf_search4(C, 0, LG_RX, u0);
     ox = b - 1 > 0 ? b - 1 : 0;
     const float c0 = C[ox];
     const float du2 = (u0 - c0) / (C[ox + 1] - c0);
@@ -152,22 +188,31 @@ __device__ __forceinline__ void lg_sample_tab(const float* __restrict__ func, co
 }
 
 // ---------------------------------------------------------------- spherical maps
-__device__ __forceinline__ void sph_xy(int w, int h, v3 d, int& x, int& y)
+// The two angles getSphericalGridXY bins (Util.cl:399-400); callers that bin
+// one direction on several grids compute them once (same values, same bits).
+struct SphAngles {
+    float phi, theta;
+};
+__device__ __forceinline__ SphAngles sph_angles(v3 d) { return SphAngles{mt_xm_acosf(d.y), mt_xm_atan2f(d.z, d.x)}; }
+
+__device__ __forceinline__ void sph_xy_of(const SphAngles& a, int w, int h, int& x, int& y)
 { // getSphericalGridXY, Util.cl:397-410
-    float phi = mt_xm_acosf(d.y);
-    float theta = mt_xm_atan2f(d.z, d.x);
-    x = (int)((0.5f - (0.5f * theta) / MT_PI) * (float)w);
-    y = (int)((phi / MT_PI) * (float)h);
+    x = (int)((0.5f - (0.5f * a.theta) / MT_PI) * (float)w);
+    y = (int)((a.phi / MT_PI) * (float)h);
 }
 
-__device__ __forceinline__ int sph_index(int w, int h, v3 d)
+__device__ __forceinline__ int sph_index_of(const SphAngles& a, int w, int h)
 { // getSphericalGridIndex, Util.cl:413-417 (clamped to the map)
     int x, y;
-    sph_xy(w, h, d, x, y);
+    sph_xy_of(a, w, h, x, y);
     int idx = (int)((float)x + (float)y * (float)w);
     idx = idx < 0 ? 0 : idx;
     return idx > w * h - 1 ? w * h - 1 : idx;
 }
+
+__device__ __forceinline__ void sph_xy(int w, int h, v3 d, int& x, int& y) { sph_xy_of(sph_angles(d), w, h, x, y); }
+
+__device__ __forceinline__ int sph_index(int w, int h, v3 d) { return sph_index_of(sph_angles(d), w, h); }
 
 __device__ __forceinline__ v3 sph_direction(float u, float v)
 { // getSphericalDirection, Util.cl:423-439
@@ -368,9 +413,10 @@ __device__ __forceinline__ float4 env_illuminate(const EnvView& E, float u0, flo
 __device__ __forceinline__ float4 env_radiance(const EnvView& E, v3 d, float& pdfw, int& texel)
 {
     SatRegion R{E.w, E.h, 0, 0, E.w, E.h};
+    const SphAngles a = sph_angles(d);
     int x, y;
-    sph_xy(E.w, E.h, d, x, y);
-    texel = sph_index(E.w, E.h, d);
+    sph_xy_of(a, E.w, E.h, x, y);
+    texel = sph_index_of(a, E.w, E.h);
     float p;
     if (E.marg && x >= 0 && y >= 0 && x <= E.w && y <= E.h) // getFunc outside the map reads 0
         p = (r_func(R, E.lum, x, y) / E.fint[y]) * E.pmarg[y];
@@ -449,13 +495,7 @@ __device__ __forceinline__ void r_sample_tile(const EnvView& E, const SatRegion&
     const float* M = T - (R.miny - 1);        // M[y] = marg of row y
     const float* FI = T + nm - (R.miny - 1);  // FI[y]
     const float last = T[nm + nf];
-    int b = R.miny, e = R.miny + R.sy;
-    while (b != e) {
-        int mid = b + (e - b) / 2;
-        bool right = !(u1 < M[mid]);
-        b = right ? mid + 1 : b;
-        e = right ? e : mid;
-    }
+    int b = cdf_search4(M, R.miny, R.miny + R.sy, u1);
     oy = b - 1 > 0 ? b - 1 : 0;
     const float m0 = M[oy];
     const float du = (u1 - m0) / (M[oy + 1] - m0);
@@ -463,14 +503,7 @@ __device__ __forceinline__ void r_sample_tile(const EnvView& E, const SatRegion&
     const float pdf_v = (fi * (float)(R.sx * R.sy)) / last;
     vv = ((float)oy + du) / (float)R.nv;
     const float* C = T + nm + nf + 1 + (oy - (R.miny - 1)) * cs - (R.minx - 1); // C[x] = cond(x, oy)
-    b = R.minx;
-    e = R.minx + R.sx;
-    while (b != e) {
-        int mid = b + (e - b) / 2;
-        bool right = !(u0 < C[mid]);
-        b = right ? mid + 1 : b;
-        e = right ? e : mid;
-    }
+    b = cdf_search4(C, R.minx, R.minx + R.sx, u0);
     ox = b - 1 > 0 ? b - 1 : 0;
     const float c0 = C[ox];
     const float du2 = (u0 - c0) / (C[ox + 1] - c0);
@@ -519,11 +552,12 @@ __device__ __forceinline__ float4 env_radiance_adaptive(const EnvView& E, v3 cam
                                                         float& lum, int& gidx, int& cell, int& tile)
 {
     const int s = lg_cell_from_camera(cam, hit);
-    const int t = sph_index(LG_RX, LG_RY, d); // tileIndexXYFromCamera, LightGrid.cl:80-90
+    const SphAngles a = sph_angles(d);
+    const int t = sph_index_of(a, LG_RX, LG_RY); // tileIndexXYFromCamera, LightGrid.cl:80-90
     const int tx = t % LG_RX, ty = t / LG_RX;
     const SatRegion R = lg_tile_region(E.w, E.h, tx, ty);
     int x, y;
-    sph_xy(E.w, E.h, d, x, y);
+    sph_xy_of(a, E.w, E.h, x, y);
     float p;
     if (E.ttab && y >= R.miny - 1 && y >= 0 && y < R.miny + R.sy) { // the table's rows: same values
         const float* T = E.ttab + (size_t)t * E.tstride;
@@ -535,7 +569,7 @@ __device__ __forceinline__ float4 env_radiance_adaptive(const EnvView& E, v3 cam
     }
     const float sv = (float)y / (float)E.h;
     pdfa = p / (((2.0f * MT_PI) * MT_PI) * mt_xm_sinf(sv * MT_PI));
-    const float4 c = E.rgba[sph_index(E.w, E.h, d)];
+    const float4 c = E.rgba[sph_index_of(a, E.w, E.h)];
     lum = luminance3(c.x, c.y, c.z);
     gidx = lg_global_index(s, tx, ty);
     cell = s;

@@ -3,22 +3,25 @@
  * the GPU kernels and the host: sin, cos, acos, atan2 and the fifth power of
  * SchlickColor (pow(1 - cos, 5), Bsdf.cl:185-189).  The reference calls the
  * OpenCL builtins, whose last bits depend on the vendor library; here every
- * result is a fixed sequence of IEEE double operations (+, -, *, /, sqrt,
- * floor; no fused multiply-add) rounded once to float, so device and host
- * agree bit for bit and the results are within one float ulp of the exact
- * values (nearly always correctly rounded).
+ * result is a fixed sequence of IEEE single-precision operations (+, -, *,
+ * /, sqrt, floor; no fused multiply-add), so device and host agree bit for
+ * bit; results are within 2 ulp of the exact values (most are correctly
+ * rounded).
  *
- * Methods (textbook; coefficients are exact Taylor coefficients 1/n!
- * and 1/n rounded to double):
- *   sin, cos   Cody-Waite reduction by pi/2 with a two-part constant (pi/2
- *              = HI + LO, HI holding 33 significant bits so k * HI is exact
- *              for |k| < 2^20), then the Taylor series on [-pi/4, pi/4] to
- *              degree 15 (sin) and 16 (cos); valid for |x| < 2^20.
+ * Methods (textbook; coefficients are the Taylor coefficients 1/n! and 1/n
+ * rounded to float; each series is cut where its remainder is below 1e-8
+ * relative):
+ *   sin, cos   Cody-Waite reduction by pi/2 with a three-part constant
+ *              (pi/2 = C1 + C2 + C3, C1 and C2 holding 12 significant bits,
+ *              so k * C1 and k * C2 are exact for |k| < 2^12), then the
+ *              Taylor series on [-pi/4, pi/4] to degree 9 (sin) and 10
+ *              (cos); valid for |x| < 6000.
  *   atan2      ratio min/max in [0, 1]; above 2 - sqrt(3) the argument is
  *              moved by the addition formula atan z = pi/6 +
  *              atan((z sqrt3 - 1)/(z + sqrt3)); the Taylor series of atan to
- *              degree 27 on |z| <= 2 - sqrt(3); quadrants and signed zeros
- *              as C99 atan2.
+ *              degree 13 on |z| <= 2 - sqrt(3); quadrants and signed zeros
+ *              as C99 atan2; pi/2 and pi are applied as a float head plus a
+ *              tail.
  *   acos       atan2(sqrt((1 - y)(1 + y)), y).
  *
  * Usable from C and from HIP: the includer defines MT_XM_FN, the function
@@ -31,120 +34,112 @@
 #error "define MT_XM_FN (function qualifiers) before including mt_xmath.h"
 #endif
 
-#define MT_XM_PI 3.141592653589793
-#define MT_XM_PIO2 1.5707963267948966
-#define MT_XM_PIO2_HI 1.5707963267341256 /* pi/2 truncated to 33 bits */
-#define MT_XM_PIO2_LO 6.077100506506192e-11 /* pi/2 - HI */
-#define MT_XM_PI6 0.5235987755982989
-#define MT_XM_SQRT3 1.7320508075688772
-#define MT_XM_TAN_PI12 0.2679491924311228 /* 2 - sqrt(3) */
-#define MT_XM_2OPI 0.6366197723675814
+#define MT_XM_C1 1.5703125f               /* pi/2 = C1 + C2 + C3 */
+#define MT_XM_C2 0.0004837512969970703f
+#define MT_XM_C3 7.549790126404332e-08f
+#define MT_XM_2OPI 0.6366197466850281f     /* 2/pi */
+#define MT_XM_PIO2_HI 1.5707963705062866f  /* float(pi/2) */
+#define MT_XM_PIO2_LO -4.371139000186241e-08f /* pi/2 - float(pi/2) */
+#define MT_XM_PI_HI 3.1415927410125732f    /* float(pi) */
+#define MT_XM_PI_LO -8.742278000372482e-08f /* pi - float(pi) */
+#define MT_XM_PI6 0.5235987901687622f      /* float(pi/6) */
+#define MT_XM_PI4 0.7853981852531433f      /* float(pi/4) */
+#define MT_XM_3PI4 2.356194496154785f      /* float(3pi/4) */
+#define MT_XM_SQRT3 1.7320507764816284f
+#define MT_XM_TAN_PI12 0.2679491937160492f /* float(2 - sqrt(3)) */
 
 /* sin(r), cos(r) for |r| <= pi/4: Taylor series, Horner in r^2 */
-MT_XM_FN double mt_xm_sin_poly(double r)
+MT_XM_FN float mt_xm_sin_poly(float r)
 {
-    const double z = r * r;
-    double p = 7.647163731819816e-13;             /* 1/15! */
-    p = p * z - 1.6059043836821613e-10;           /* 1/13! */
-    p = p * z + 2.505210838544172e-08;            /* 1/11! */
-    p = p * z - 2.7557319223985893e-06;           /* 1/9! */
-    p = p * z + 0.0001984126984126984;            /* 1/7! */
-    p = p * z - 0.008333333333333333;             /* 1/5! */
-    p = p * z + 0.16666666666666666;              /* 1/3! */
-    return r - (r * z) * p;
+    const float z = r * r;
+    float p = 2.7557318844628753e-06f;     /* 1/9! */
+    p = p * z - 0.00019841270113829523f;   /* 1/7! */
+    p = p * z + 0.008333333767950535f;     /* 1/5! */
+    p = p * z - 0.1666666716337204f;       /* 1/3! */
+    return r + (r * z) * p;
 }
 
-MT_XM_FN double mt_xm_cos_poly(double r)
+MT_XM_FN float mt_xm_cos_poly(float r)
 {
-    const double z = r * r;
-    double p = 4.779477332387385e-14;             /* 1/16! */
-    p = p * z - 1.1470745597729725e-11;           /* 1/14! */
-    p = p * z + 2.08767569878681e-09;             /* 1/12! */
-    p = p * z - 2.755731922398589e-07;            /* 1/10! */
-    p = p * z + 2.48015873015873e-05;             /* 1/8! */
-    p = p * z - 0.001388888888888889;             /* 1/6! */
-    p = p * z + 0.041666666666666664;             /* 1/4! */
-    return (1.0 - 0.5 * z) + (z * z) * p;
+    const float z = r * r;
+    float p = -2.755731998149713e-07f;     /* 1/10! */
+    p = p * z + 2.4801587642286904e-05f;   /* 1/8! */
+    p = p * z - 0.0013888889225199819f;    /* 1/6! */
+    p = p * z + 0.0416666679084301f;       /* 1/4! */
+    return (1.0f - 0.5f * z) + (z * z) * p;
 }
 
 /* quadrant q = k mod 4 and reduced argument r = x - k pi/2 */
-MT_XM_FN double mt_xm_reduce(double x, int* q)
+MT_XM_FN float mt_xm_reduce(float x, int* q)
 {
-    const double k = floor(x * MT_XM_2OPI + 0.5);
-    *q = (int)(k - 4.0 * floor(k * 0.25));
-    return (x - k * MT_XM_PIO2_HI) - k * MT_XM_PIO2_LO;
+    const float k = floorf(x * MT_XM_2OPI + 0.5f);
+    *q = (int)(k - 4.0f * floorf(k * 0.25f));
+    return ((x - k * MT_XM_C1) - k * MT_XM_C2) - k * MT_XM_C3;
 }
 
-MT_XM_FN float mt_xm_sinf(float xf)
-{
-    int q;
-    const double r = mt_xm_reduce((double)xf, &q);
-    const double v = (q & 1) ? mt_xm_cos_poly(r) : mt_xm_sin_poly(r);
-    return (float)((q & 2) ? -v : v);
-}
-
-MT_XM_FN float mt_xm_cosf(float xf)
+MT_XM_FN float mt_xm_sinf(float x)
 {
     int q;
-    const double r = mt_xm_reduce((double)xf, &q);
-    const double v = (q & 1) ? mt_xm_sin_poly(r) : mt_xm_cos_poly(r);
-    return (float)(((q + 1) & 2) ? -v : v);
+    const float r = mt_xm_reduce(x, &q);
+    const float v = (q & 1) ? mt_xm_cos_poly(r) : mt_xm_sin_poly(r);
+    return (q & 2) ? -v : v;
 }
 
-/* atan(z) for |z| <= 2 - sqrt(3): Taylor series to z^27 */
-MT_XM_FN double mt_xm_atan_poly(double t)
+MT_XM_FN float mt_xm_cosf(float x)
 {
-    const double z = t * t;
-    double p = 1.0 / 27.0;
-    p = p * z - 1.0 / 25.0;
-    p = p * z + 1.0 / 23.0;
-    p = p * z - 1.0 / 21.0;
-    p = p * z + 1.0 / 19.0;
-    p = p * z - 1.0 / 17.0;
-    p = p * z + 1.0 / 15.0;
-    p = p * z - 1.0 / 13.0;
-    p = p * z + 1.0 / 11.0;
-    p = p * z - 1.0 / 9.0;
-    p = p * z + 1.0 / 7.0;
-    p = p * z - 1.0 / 5.0;
-    p = p * z + 1.0 / 3.0;
-    return t - (t * z) * p;
+    int q;
+    const float r = mt_xm_reduce(x, &q);
+    const float v = (q & 1) ? mt_xm_sin_poly(r) : mt_xm_cos_poly(r);
+    return ((q + 1) & 2) ? -v : v;
 }
 
-MT_XM_FN double mt_xm_atan2(double y, double x)
+/* atan(t) for |t| <= 2 - sqrt(3): Taylor series to t^13 */
+MT_XM_FN float mt_xm_atan_poly(float t)
+{
+    const float z = t * t;
+    float p = 0.07692307978868484f;        /* 1/13 */
+    p = p * z - 0.09090909361839294f;      /* 1/11 */
+    p = p * z + 0.1111111119389534f;       /* 1/9 */
+    p = p * z - 0.1428571492433548f;       /* 1/7 */
+    p = p * z + 0.20000000298023224f;      /* 1/5 */
+    p = p * z - 0.3333333432674408f;       /* 1/3 */
+    return t + (t * z) * p;
+}
+
+/* atan(z) for z in [0, 1] */
+MT_XM_FN float mt_xm_atan01(float z)
+{
+    if (z > MT_XM_TAN_PI12) return MT_XM_PI6 + mt_xm_atan_poly((z * MT_XM_SQRT3 - 1.0f) / (z + MT_XM_SQRT3));
+    return mt_xm_atan_poly(z);
+}
+
+MT_XM_FN float mt_xm_atan2f(float y, float x)
 {
     if (x != x || y != y) return x + y; /* NaN */
-    const double ax = fabs(x), ay = fabs(y);
+    const float ax = fabsf(x), ay = fabsf(y);
     const int xneg = signbit(x) != 0, yneg = signbit(y) != 0;
-    double a;
-    if (ay == 0.0 && ax == 0.0) {
-        a = xneg ? MT_XM_PI : 0.0;
+    float a;
+    if (ay == 0.0f && ax == 0.0f) {
+        a = xneg ? MT_XM_PI_HI : 0.0f;
+    } else if (ax == ay) { /* also both infinite */
+        a = xneg ? MT_XM_3PI4 : MT_XM_PI4;
+    } else if (ay > ax) { /* angle from the y axis */
+        const float t = mt_xm_atan01(ax / ay);
+        a = xneg ? (MT_XM_PIO2_HI + t) + MT_XM_PIO2_LO : (MT_XM_PIO2_HI - t) + MT_XM_PIO2_LO;
     } else {
-        const int swap = ay > ax;
-        double z;
-        if (ax == ay) z = 1.0; /* also both infinite */
-        else z = swap ? ax / ay : ay / ax;
-        if (z > MT_XM_TAN_PI12) a = MT_XM_PI6 + mt_xm_atan_poly((z * MT_XM_SQRT3 - 1.0) / (z + MT_XM_SQRT3));
-        else a = mt_xm_atan_poly(z);
-        if (swap) a = MT_XM_PIO2 - a;
-        if (xneg) a = MT_XM_PI - a;
+        const float t = mt_xm_atan01(ay / ax);
+        a = xneg ? (MT_XM_PI_HI - t) + MT_XM_PI_LO : t;
     }
     return yneg ? -a : a;
 }
 
-MT_XM_FN float mt_xm_atan2f(float y, float x) { return (float)mt_xm_atan2((double)y, (double)x); }
-
-MT_XM_FN float mt_xm_acosf(float yf)
-{
-    const double y = (double)yf;
-    return (float)mt_xm_atan2(sqrt((1.0 - y) * (1.0 + y)), y);
-}
+MT_XM_FN float mt_xm_acosf(float y) { return mt_xm_atan2f(sqrtf((1.0f - y) * (1.0f + y)), y); }
 
 /* x^5 (pow(x, 5.f) of SchlickColor) */
-MT_XM_FN float mt_xm_pow5f(float xf)
+MT_XM_FN float mt_xm_pow5f(float x)
 {
-    const double x = (double)xf, x2 = x * x;
-    return (float)((x2 * x2) * x);
+    const float x2 = x * x;
+    return (x2 * x2) * x;
 }
 
 #endif

@@ -8,6 +8,9 @@ renderThread -> execute -> loop, src/cl/device/CDeviceGI.java:190-204,
 order), keeps stepping the shared queues and timestamps each job when its
 image is complete.  `serve_sequential` runs the same arrivals through plain
 `api.render` calls, first come first served or shortest job first.
+`serve_two_lane` runs plain renders on two streams of the device, one host
+thread each: jobs that fill the GPU by themselves go to one lane, the rest
+to the other, so a small job never waits behind a large one's whole render.
 
 A request is a `Request`: arrival time (seconds from the start of the run),
 the resident scene, materials, environment and lights it renders with, and
@@ -15,6 +18,7 @@ the RenderJob.  Both loops return one `Done` record per request.
 """
 from __future__ import annotations
 
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Any, List, Optional
@@ -125,6 +129,76 @@ def serve_sequential(ctx: api.Context, requests: List[Request], keep_images: boo
         rays += int(st.extension_rays) + int(st.shadow_rays)
         out.append(Done(r.tag, r.arrival, time.perf_counter() - t0, img if keep_images else None))
     return out, rays
+
+
+def serve_two_lane(ctx: api.Context, requests: List[Request], keep_images: bool = False,
+                   big_paths: int = 1 << 22, small_order: str = "sjf"):
+    """Two render lanes, each a context on its own stream of ctx's device
+    driven by its own host thread: requests of at least `big_paths` paths
+    (a 1080p x 16 spp job, a 1024^2 x 64 spp soup) go to the large lane,
+    first come first served; the rest to the small lane, shortest job first
+    (or "fifo").  The lanes' kernels share the GPU, so small jobs are not
+    queued behind a large job's whole render.  Every job is a plain
+    mt_render, so its image is the one it has alone.  Returns (Done records
+    in completion order, rays traced)."""
+    if small_order not in ("fifo", "sjf"):
+        raise ValueError(f"unknown order {small_order!r}")
+    torch = api._torch()
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(device=ctx.torch_device), torch.cuda.Stream(device=ctx.torch_device)]
+    lanes_ctx = [api.Context(ctx.device, stream=s) for s in streams]
+    reqs = sorted(requests, key=lambda r: r.arrival)
+    waiting: List[List[Request]] = [[], []]
+    cond = threading.Condition()
+    closed = [False]
+    out: List[Done] = []
+    rays = [0]
+    errors: List[BaseException] = []
+    t0 = time.perf_counter()
+
+    def lane(k: int) -> None:
+        try:
+            while True:
+                with cond:
+                    while not waiting[k] and not closed[0]:
+                        cond.wait()
+                    if not waiting[k]:
+                        return
+                    q = waiting[k]
+                    i = 0
+                    if k == 0 and small_order == "sjf":
+                        i = min(range(len(q)), key=lambda j: (_paths(q[j]), q[j].arrival))
+                    r = q.pop(i)
+                img, st = api.render(lanes_ctx[k], r.scene, r.materials, r.env, r.job, light_faces=r.light_faces,
+                                     num_lights=r.num_lights, stats=True)   # synchronises this lane
+                t = time.perf_counter() - t0
+                with cond:
+                    rays[0] += int(st.extension_rays) + int(st.shadow_rays)
+                    out.append(Done(r.tag, r.arrival, t, img if keep_images else None))
+        except BaseException as e:  # reported by the caller's thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=lane, args=(k,), daemon=True) for k in (0, 1)]
+    for th in threads:
+        th.start()
+    try:
+        for r in reqs:
+            _wait_until(t0, r.arrival)
+            with cond:
+                waiting[1 if _paths(r) >= big_paths else 0].append(r)
+                cond.notify_all()
+    finally:
+        with cond:
+            closed[0] = True
+            cond.notify_all()
+        for th in threads:
+            th.join()
+        for c in lanes_ctx:
+            c.close()
+    if errors:
+        raise errors[0]
+    out.sort(key=lambda d: d.complete)
+    return out, rays[0]
 
 
 def summary(done: List[Done], rays: int) -> dict:

@@ -189,3 +189,64 @@ def test_serving_loops_agree(ctx, world):
     assert first3 == sorted([0, 1, 2], key=lambda t: (sizes[t], t))
     with pytest.raises(ValueError):
         serve.serve_sequential(ctx, reqs[:1], order="lifo")
+
+
+def test_cancel_pending_and_running_jobs(ctx, world):
+    """mt_batch_cancel: a job cancelled before admission and one cancelled
+    while its paths are in the queues are both reported once drained, with
+    their images untouched; every other job's image stays bit-identical to
+    its plain render, and the freed slot and arena serve a later job."""
+    print("[case] cancel", flush=True)
+    scenes, dm, env, (lf, nl) = world
+    jobs = list(enumerate(_jobs()))[:5]
+    b = api.Batch(ctx, queue_capacity=4096, arena_bytes=64 << 20, max_jobs=2)
+    imgs = {}
+    for tag, (sc, job) in jobs:
+        imgs[tag] = torch.full((job.width * job.height, 4), -7.0, dtype=torch.float32, device="cuda")
+        lights = (lf, nl) if sc == 3 else (None, None)
+        b.submit(tag, scenes[sc], dm, env, job, imgs[tag], light_faces=lights[0], num_lights=lights[1])
+    assert b.cancel(4)                     # still pending (2 slots): dropped at once
+    tags, busy = b.step()                  # admits jobs 0 and 1, one iteration
+    assert b.cancel(1)                     # running: drains
+    assert not b.cancel(99)
+    order = list(tags)
+    while busy:
+        t, busy = b.step()
+        order.extend(t)
+    # a job submitted after the cancellations reuses the freed slot / arena
+    sc, late = jobs[2][1]
+    late = W.soup_job(W.JobSpec(9, sc, late.width, late.spp, 9100))
+    limg = ctx.empty((late.width * late.height, 4), torch.float32)
+    b.submit(9, scenes[sc], dm, env, late, limg)
+    order.extend(b.run())
+    st = b.stats()
+    b.close()
+    torch.cuda.synchronize()
+    assert sorted(order) == [0, 1, 2, 3, 4, 9]
+    assert st.jobs_cancelled == 2 and st.jobs_completed == 4 and st.arena_used == 0
+    for tag in (1, 4):
+        assert (imgs[tag] == -7.0).all(), f"cancelled job {tag} wrote its image"
+    for tag in (0, 2, 3):
+        sc, job = jobs[tag][1]
+        np.testing.assert_array_equal(imgs[tag].cpu().numpy(), _plain(ctx, world, sc, job), err_msg=f"job {tag}")
+    np.testing.assert_array_equal(limg.cpu().numpy(), _plain(ctx, world, sc, late))
+
+
+@pytest.mark.parametrize("iters", [3, 16])
+def test_iterations_per_step(ctx, world, iters):
+    """Several wavefront iterations per host round trip: same images."""
+    print(f"[case] {iters} iterations per step", flush=True)
+    scenes, dm, env, (lf, nl) = world
+    jobs = list(enumerate(_jobs()))
+    b = api.Batch(ctx, queue_capacity=1 << 14, arena_bytes=256 << 20, max_jobs=64)
+    b.set_iterations(iters)
+    imgs = {}
+    for tag, (sc, job) in jobs:
+        imgs[tag] = ctx.empty((job.width * job.height, 4), torch.float32)
+        lights = (lf, nl) if sc == 3 else (None, None)
+        b.submit(tag, scenes[sc], dm, env, job, imgs[tag], light_faces=lights[0], num_lights=lights[1])
+    order = b.run()
+    b.close()
+    assert sorted(order) == list(range(len(jobs)))
+    for tag, (sc, job) in jobs:
+        np.testing.assert_array_equal(imgs[tag].cpu().numpy(), _plain(ctx, world, sc, job), err_msg=f"job {tag}")
