@@ -199,6 +199,8 @@ int32_t mt_scene_destroy(mt_scene* scene);
 #define MT_TRACE_PACKED_NO_STACK 2 /* packed, one ray per lane, parent-link backtracking */
 #define MT_TRACE_PACKED_STACK 3    /* packed, one ray per lane, short stack */
 #define MT_TRACE_PACKED_REFILL 4   /* packed, short stack, persistent waves with refill */
+#define MT_TRACE_PACKED_SORTED 5   /* packed, short stack, each work-group's rays taken in
+                                      direction-bin order (same results per ray) */
 
 /* Closest hit (intersectMesh with bailout = false, NormalBVH.cl:14-92, as
  * run by IntersectPrimitives :94-123; launched at CDeviceGI.java:230 and

@@ -25,6 +25,7 @@ MT_TRACE_REFERENCE = 1
 MT_TRACE_PACKED_NO_STACK = 2
 MT_TRACE_PACKED_STACK = 3
 MT_TRACE_PACKED_REFILL = 4
+MT_TRACE_PACKED_SORTED = 5
 MT_BSDF_WORDS = 20
 
 

@@ -91,7 +91,8 @@ struct mt_lightgrid {
     // Accumulation of the render loop's guided environment mode, in fixed
     // point (units of 2^-20 luminance, integer atomics: the per-entry sums do
     // not depend on the order of the additions); folded into accum, and
-    // cleared, at the end of every render.
+    // cleared, at the end of every render.  LG_PEND_COPIES copies of
+    // LG_SIZE entries.
     unsigned long long* pend;
 };
 
@@ -170,6 +171,13 @@ constexpr int LG_CX = LG_NU / LG_RX;  // 100
 constexpr int LG_CY = LG_NV / LG_RY;  // 50
 constexpr int LG_CELLS = LG_CX * LG_CY;
 constexpr int LG_SIZE = LG_NU * LG_NV;  // 2 560 000
+// Copies of the light grid's fixed-point accumulation.  Work-group b adds
+// into copy b % LG_PEND_COPIES; work-groups are dispatched round-robin over
+// the device's 8 XCDs, so each copy takes one XCD's atomics, and additions to
+// one entry from different XCDs (every camera ray that misses the scene adds
+// to the same few entries) do not queue on one address.  The fold adds the
+// copies: integer sums, the same whichever copy an addition went to.
+constexpr int LG_PEND_COPIES = 8;
 
 inline int mt_blocks(int64_t n, int threads)
 {

@@ -396,7 +396,8 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
     R.oShc = take(4 * 64 * (size_t)SHADOW_SEGS);
     R.oQsc = take(4 * 64 * (size_t)SHADOW_SEGS);
     R.want_ilp = (desc->flags & MT_RENDER_TRAVERSAL_STATS) != 0;
-    R.fused = (desc->mode == MT_TRACE_PACKED || desc->mode == MT_TRACE_PACKED_STACK) && !R.want_ilp;
+    R.fused = (desc->mode == MT_TRACE_PACKED || desc->mode == MT_TRACE_PACKED_STACK ||
+               desc->mode == MT_TRACE_PACKED_SORTED) && !R.want_ilp;
     R.profile = (desc->flags & MT_RENDER_PROFILE) != 0;
     R.oIlp = R.want_ilp ? take(12 * (R.sq > n ? R.sq : n)) : 0;
     R.oTot = take(8 * 8 * ILP_SLOTS);

@@ -621,7 +621,8 @@ __host__ __device__ inline unsigned long long lg_fixed(float lum)
 }
 
 // accumLightGrid (Light.cl:35-38) for the lanes with `add`: entry gidx of
-// the fixed-point accumulation gains q.  Call with the whole wave active.
+// the fixed-point accumulation gains q (in the work-group's copy of it).
+// Call with the whole wave active.
 // Lanes that share an entry with the wave's first pending lane are summed
 // and added with one atomic (camera-coherent misses share a few entries);
 // after four such rounds the rest add their own.  Integer sums: the result
@@ -629,6 +630,7 @@ __host__ __device__ inline unsigned long long lg_fixed(float lum)
 __device__ __forceinline__ void lg_wave_accum(bool add, int gidx, unsigned long long q,
                                               unsigned long long* __restrict__ pend)
 {
+    pend += (size_t)(blockIdx.x % LG_PEND_COPIES) * LG_SIZE; // this work-group's copy
     add = add && q != 0ull;
     uint64_t m = __ballot(add);
     for (int round = 0; round < 4 && m; round++) {

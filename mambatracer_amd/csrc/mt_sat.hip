@@ -74,7 +74,9 @@ __global__ void k_lg_clear(float* __restrict__ a, float* __restrict__ b, float* 
         if (a) a[i] = 0.0f;
         b[i] = 0.0f;
         c[i] = 0.0f;
-        if (pend) pend[i] = 0ull;
+        if (pend) {
+            for (int k = 0; k < LG_PEND_COPIES; k++) pend[(size_t)k * LG_SIZE + i] = 0ull;
+        }
     }
 }
 
@@ -84,11 +86,15 @@ __global__ void k_lg_fold(unsigned long long* __restrict__ pend, float* __restri
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= LG_SIZE) return;
-    const unsigned long long q = pend[i];
-    if (q) {
-        accum[i] = accum[i] + (float)((double)q * (1.0 / 1048576.0));
-        pend[i] = 0ull;
+    unsigned long long q = 0ull;
+    for (int k = 0; k < LG_PEND_COPIES; k++) {
+        const unsigned long long v = pend[(size_t)k * LG_SIZE + i];
+        if (v) {
+            q += v;
+            pend[(size_t)k * LG_SIZE + i] = 0ull;
+        }
     }
+    if (q) accum[i] = accum[i] + (float)((double)q * (1.0 / 1048576.0));
 }
 
 __global__ void k_lg_set_tile(float* __restrict__ func, const float* __restrict__ tile)
@@ -387,7 +393,7 @@ extern "C" int32_t mt_lightgrid_create(mt_context* ctx, mt_lightgrid** out)
         return MT_ERR_ALLOC;
     }
     g->tab_valid = false;
-    if (hipMalloc(&g->pend, sizeof(unsigned long long) * (size_t)LG_SIZE) != hipSuccess) {
+    if (hipMalloc(&g->pend, sizeof(unsigned long long) * (size_t)LG_PEND_COPIES * LG_SIZE) != hipSuccess) {
         (void)hipFree(p);
         (void)hipFree(g->tab);
         delete g;

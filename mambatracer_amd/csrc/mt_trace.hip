@@ -201,6 +201,76 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
     }
 }
 
+// One ray per lane as k_trace LAYOUT 0, after the work-group's rays are
+// ordered by direction: a ray's bin is its direction octant and dominant
+// axis (24 bins, dead entries last), and lanes take the group's rays in bin
+// order (stable within a bin).  A wave then holds rays heading the same way
+// from the producer's neighbouring paths, which visit similar nodes, so
+// fewer lanes idle while the wave's longest ray finishes.  Each ray is
+// traversed as in k_trace and its result stored at its own entry: the
+// output is identical, only the lane that computes it changes.
+constexpr int SORT_BINS = 32;
+
+__device__ __forceinline__ int dir_bin(float4 d)
+{
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    const int major = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    const int oct = (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0);
+    return oct * 3 + major;
+}
+
+template <bool ANY>
+__global__ __launch_bounds__(256) void k_trace_sorted(TraceArgs A)
+{
+    __shared__ int s_cnt[4][SORT_BINS];
+    __shared__ int s_off[4][SORT_BINS];
+    __shared__ int s_ids[256];
+    const int id = queue_entry(A);
+    const bool live = entry_live(A, id);
+    if (__syncthreads_or(live) == 0) return; // no ray in this group
+    const int wave = threadIdx.x >> 6, lane = __lane_id();
+    const int bin = live ? dir_bin(A.d_tmax[id]) : SORT_BINS - 1;
+    int rank = 0;
+#pragma unroll
+    for (int b = 0; b < SORT_BINS; b++) {
+        const uint64_t m = __ballot(bin == b);
+        if (bin == b) rank = (int)lane_rank(m);
+        if (lane == 0) s_cnt[wave][b] = (int)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < SORT_BINS) { // one lane per bin: exclusive scan of the bin totals
+        const int b = threadIdx.x;
+        const int c0 = s_cnt[0][b], c1 = s_cnt[1][b], c2 = s_cnt[2][b], c3 = s_cnt[3][b];
+        const int tot = c0 + c1 + c2 + c3;
+        int incl = tot;
+        for (int o = 1; o < SORT_BINS; o <<= 1) {
+            const int y = __shfl_up(incl, o, SORT_BINS);
+            if (b >= o) incl += y;
+        }
+        const int ex = incl - tot;
+        s_off[0][b] = ex;
+        s_off[1][b] = ex + c0;
+        s_off[2][b] = ex + c0 + c1;
+        s_off[3][b] = ex + c0 + c1 + c2;
+    }
+    __syncthreads();
+    s_ids[s_off[wave][bin] + rank] = live ? id : -1;
+    __syncthreads();
+    const int sid = s_ids[threadIdx.x];
+    if (sid < 0) return;
+    RayT r;
+    ray_init(r, A.o_tmin[sid], A.d_tmax[sid]);
+    int st[3] = {0, 0, 0};
+    int child = -1;
+    const int h = trace_packed<ANY, false, true>(A.pn, A.pt, A.root, r, child, st);
+    if (ANY) {
+        A.occluded[sid] = h;
+    } else {
+        A.prim[sid] = h ? child : -1;
+        A.tout[sid] = r.tmax;
+    }
+}
+
 // Rays of many jobs in one queue: ray id's scene is refs[tag[2*id] >> shift]
 // (the job word of a wavefront slot).  Same per-ray traversal as k_trace
 // LAYOUT 0, so a ray's result does not depend on which queue carries it.
@@ -227,7 +297,7 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
 {
     MT_ARG_CHECK(ctx && sc && o_tmin && d_tmax && (d_count || seg.counts), "trace: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "trace: negative capacity");
-    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_REFILL, "trace: unknown mode %d", mode);
+    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_SORTED, "trace: unknown mode %d", mode);
     if (ANY) MT_ARG_CHECK(occluded != nullptr, "trace_any: occluded is NULL");
     else MT_ARG_CHECK(prim && t, "trace_closest: prim/t NULL");
     if (seg.counts) {
@@ -260,7 +330,10 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
     // 4M-triangle soup: 18.3 vs 21.3 ms); on a path tracer's bounce batches it
     // loses (1080p x 16 spp render: 25.8 vs 23.5 ms), so it is opt-in.
     if (mode == MT_TRACE_PACKED) mode = MT_TRACE_PACKED_STACK;
-    if (mode == MT_TRACE_PACKED_REFILL) {
+    if (mode == MT_TRACE_PACKED_SORTED && stats) mode = MT_TRACE_PACKED_STACK; // counting: per-ray order only
+    if (mode == MT_TRACE_PACKED_SORTED) {
+        k_trace_sorted<ANY><<<grid, block, 0, s>>>(A);
+    } else if (mode == MT_TRACE_PACKED_REFILL) {
         void* scr = nullptr;
         int32_t rc = mt_ctx_scratch(ctx, 256, &scr);
         if (rc) return rc;
@@ -825,7 +898,7 @@ extern "C" int32_t mt_intersect_primitives(mt_context* ctx, const mt_scene* sc, 
 {
     MT_ARG_CHECK(ctx && sc && rays && isects && d_count, "mt_intersect_primitives: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "mt_intersect_primitives: negative capacity");
-    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_REFILL,
+    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_SORTED,
                  "mt_intersect_primitives: unknown mode %d", mode);
     MT_ARG_CHECK((reinterpret_cast<uintptr_t>(rays) & 15) == 0 && (reinterpret_cast<uintptr_t>(isects) & 15) == 0,
                  "mt_intersect_primitives: rays and isects must be 16-byte aligned");
@@ -859,7 +932,7 @@ extern "C" int32_t mt_intersect_occlusion(mt_context* ctx, const mt_scene* sc, c
 {
     MT_ARG_CHECK(ctx && sc && rays && hits && d_count, "mt_intersect_occlusion: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "mt_intersect_occlusion: negative capacity");
-    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_REFILL,
+    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_SORTED,
                  "mt_intersect_occlusion: unknown mode %d", mode);
     MT_ARG_CHECK((reinterpret_cast<uintptr_t>(rays) & 15) == 0, "mt_intersect_occlusion: rays must be 16-byte aligned");
     if (capacity == 0) return MT_OK;

@@ -214,10 +214,10 @@ def test_cancel_pending_and_running_jobs(ctx, world):
         t, busy = b.step()
         order.extend(t)
     # a job submitted after the cancellations reuses the freed slot / arena
-    sc, late = jobs[2][1]
-    late = W.soup_job(W.JobSpec(9, sc, late.width, late.spp, 9100))
+    late_sc, j2 = jobs[2][1]
+    late = W.soup_job(W.JobSpec(9, late_sc, j2.width, j2.spp, 9100))
     limg = ctx.empty((late.width * late.height, 4), torch.float32)
-    b.submit(9, scenes[sc], dm, env, late, limg)
+    b.submit(9, scenes[late_sc], dm, env, late, limg)
     order.extend(b.run())
     st = b.stats()
     b.close()
@@ -229,7 +229,7 @@ def test_cancel_pending_and_running_jobs(ctx, world):
     for tag in (0, 2, 3):
         sc, job = jobs[tag][1]
         np.testing.assert_array_equal(imgs[tag].cpu().numpy(), _plain(ctx, world, sc, job), err_msg=f"job {tag}")
-    np.testing.assert_array_equal(limg.cpu().numpy(), _plain(ctx, world, sc, late))
+    np.testing.assert_array_equal(limg.cpu().numpy(), _plain(ctx, world, late_sc, late))
 
 
 @pytest.mark.parametrize("iters", [3, 16])

@@ -16,7 +16,7 @@ from tests.conftest import c0_rays
 pytestmark = pytest.mark.gpu
 
 MODES = [api._lib.MT_TRACE_PACKED, api._lib.MT_TRACE_REFERENCE, api._lib.MT_TRACE_PACKED_NO_STACK,
-         api._lib.MT_TRACE_PACKED_STACK, api._lib.MT_TRACE_PACKED_REFILL]
+         api._lib.MT_TRACE_PACKED_STACK, api._lib.MT_TRACE_PACKED_REFILL, api._lib.MT_TRACE_PACKED_SORTED]
 
 
 def _scene(ctx, arrs, normals=None):
@@ -271,3 +271,28 @@ def test_records_edge_cases(ctx):
             np.testing.assert_array_equal(gr[1:].view(np.int32), rec[1:].view(np.int32))
             # the first ray's tMax is now 1: the occlusion pass sees the hit at t = 1 as outside (tMin, tMax)
             np.testing.assert_array_equal(hits.cpu().numpy(), [0, 0, 0])
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_direction_sorted_lanes_same_results(ctx, c0, any_hit):
+    """MT_TRACE_PACKED_SORTED (each work-group's rays taken in direction-bin
+    order) returns exactly the per-ray results of one ray per lane, on
+    camera rays and on random rays with a live length that ends mid-group."""
+    print(f"[case] sorted lanes any={any_hit}", flush=True)
+    points, faces, nodes, bounds, _ = c0
+    sc = _scene(ctx, c0)
+    o1, d1 = c0_rays(128, 128)
+    o2, d2 = api.gen_random_rays(77, 0, 50_000, [-1.0] * 3, [1.0] * 3)
+    for o, d in ((o1, d1), (o2, d2)):
+        do, dd = ctx.to_device(o), ctx.to_device(d)
+        count = ctx.scalar_i32(len(o) - 37)
+        fn = api.intersect_occlusion if any_hit else api.intersect_primitives
+        a = fn(ctx, sc, do, dd, count=count, mode=api._lib.MT_TRACE_PACKED_STACK)
+        b = fn(ctx, sc, do, dd, count=count, mode=api._lib.MT_TRACE_PACKED_SORTED)
+        torch.cuda.synchronize()
+        n = len(o) - 37
+        for x, y in zip(a, b):
+            if x is None:
+                continue
+            xa, ya = x.cpu().numpy()[:n], y.cpu().numpy()[:n]
+            assert np.array_equal(xa.view(np.int32), ya.view(np.int32))
