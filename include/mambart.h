@@ -201,6 +201,8 @@ int32_t mt_scene_destroy(mt_scene* scene);
 #define MT_TRACE_PACKED_REFILL 4   /* packed, short stack, persistent waves with refill */
 #define MT_TRACE_PACKED_SORTED 5   /* packed, short stack, each work-group's rays taken in
                                       direction-bin order (same results per ray) */
+#define MT_TRACE_PACKED_POOL 6     /* packed, short stack, each wave refills finished lanes
+                                      from its own run of 256 rays (same results per ray) */
 
 /* Closest hit (intersectMesh with bailout = false, NormalBVH.cl:14-92, as
  * run by IntersectPrimitives :94-123; launched at CDeviceGI.java:230 and

@@ -26,6 +26,7 @@ MT_TRACE_PACKED_NO_STACK = 2
 MT_TRACE_PACKED_STACK = 3
 MT_TRACE_PACKED_REFILL = 4
 MT_TRACE_PACKED_SORTED = 5
+MT_TRACE_PACKED_POOL = 6
 MT_BSDF_WORDS = 20
 
 

@@ -67,17 +67,42 @@ __global__ __launch_bounds__(RT) void k_camera(Camera C, int npaths, const uint3
 
 // Bounce 0 of the fused loop: path pid = slot traces its camera ray
 // (generated here, never stored) to its closest hit; same per-ray traversal
-// as mt_trace_closest's default mode, so prim/t are the same bits.
+// as mt_trace_closest's default mode, so prim/t are the same bits.  POOL 0:
+// one path per lane; POOL > 0: each wave traces the run of 64 * POOL
+// consecutive paths (4 runs per work-group), refilling finished lanes
+// (pool_trace).
+template <int POOL>
 __global__ __launch_bounds__(RT) void k_trace_camera(Camera C, int npaths, const uint32_t* __restrict__ d_seed,
                                                      const float4* __restrict__ pn, const float4* __restrict__ pt,
                                                      int root, int* __restrict__ prim, float* __restrict__ tout,
                                                      int* __restrict__ count0)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count0 = npaths;
+    const uint32_t seed = *d_seed;
+    if (POOL > 0) {
+        const int next = (int)(blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 * POOL);
+        const int end = npaths < next + 64 * POOL ? npaths : next + 64 * POOL;
+        if (next >= end) return;
+        const Camera Cr = C; // a register copy: a reference into the kernel argument spills it to scratch
+        pool_trace<false>(
+            pn, pt, root, next, end,
+            [&](int pid, RayT& r) {
+                float4 o, d;
+                camera_ray(Cr, pid, seed, o, d);
+                ray_init(r, o, d);
+            },
+            [&](bool fin, int pid, const RayT& r, const TravState& S) {
+                if (fin) {
+                    prim[pid] = S.hit ? S.child : -1;
+                    tout[pid] = r.tmax;
+                }
+            });
+        return;
+    }
     const int pid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pid == 0) *count0 = npaths;
     if (pid >= npaths) return;
     float4 o, d;
-    camera_ray(C, pid, *d_seed, o, d);
+    camera_ray(C, pid, seed, o, d);
     RayT r;
     ray_init(r, o, d);
     int st[3] = {0, 0, 0};
@@ -93,6 +118,9 @@ __global__ __launch_bounds__(RT) void k_trace_camera(Camera C, int npaths, const
 // With the light-grid guided environment (pend != nullptr), an unoccluded
 // environment sample also adds its luminance to its grid entry (s_lg: entry,
 // luminance bits; the commented-out accumLightGrid, Render.cl:458-459).
+// POOL 0: one entry per lane; POOL > 0: each wave traces a run of 64 * POOL
+// entries of one segment, refilling finished lanes (pool_trace).
+template <int POOL>
 __global__ __launch_bounds__(RT) void k_trace_shadow(const float4* __restrict__ pn, const float4* __restrict__ pt,
                                                      int root, const int* __restrict__ seg_counts, int seg_cap,
                                                      const float4* __restrict__ s_o, const float4* __restrict__ s_d,
@@ -100,6 +128,29 @@ __global__ __launch_bounds__(RT) void k_trace_shadow(const float4* __restrict__ 
                                                      float4* __restrict__ L, const int2* __restrict__ s_lg,
                                                      unsigned long long* __restrict__ pend)
 {
+    if (POOL > 0) {
+        int next, end;
+        pool_seg_run(seg_counts, SHADOW_SEGS, seg_cap, 64 * POOL, next, end);
+        if (next >= end) return; // wave-uniform
+        pool_trace<true>(
+            pn, pt, root, next, end, [&](int k, RayT& r) { ray_init(r, s_o[k], s_d[k]); },
+            [&](bool fin, int k, const RayT&, const TravState& S) {
+                const bool vis = fin && !S.hit;
+                if (vis) {
+                    const float4 c = contrib[k];
+                    const int slot = __float_as_int(c.w);
+                    const int pid = queue ? queue[slot] : slot;
+                    const float4 v = L[pid];
+                    L[pid] = make_float4(v.x + c.x, v.y + c.y, v.z + c.z, v.w);
+                }
+                if (pend) { // the whole wave is active here
+                    int2 e = make_int2(-1, 0);
+                    if (vis) e = s_lg[k];
+                    lg_wave_accum(e.x >= 0, e.x, lg_fixed(__int_as_float(e.y)), pend);
+                }
+            });
+        return;
+    }
     const int k = seg_entry(seg_cap);
     const bool live = seg_live(seg_counts, seg_cap, k);
     if (__ballot(live) == 0ull) return; // wave-uniform exit
@@ -320,6 +371,7 @@ struct RenderPlan {
         oIlp, oTot, oSeed, oSlg;
     mt_lightgrid* grid; // guided environment mode: the grid the render samples and accumulates into
     bool fused; // camera rays traced in place, shadow adds fused, next queue appended by the shading stage
+    bool pool;  // fused kernels trace wave runs with lane refill (pool_trace)
     bool borrowed; // base is the context's reusable workspace (mt_render), not owned
     hipEvent_t ev[4 * 64];
     int nev;
@@ -397,7 +449,8 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
     R.oQsc = take(4 * 64 * (size_t)SHADOW_SEGS);
     R.want_ilp = (desc->flags & MT_RENDER_TRAVERSAL_STATS) != 0;
     R.fused = (desc->mode == MT_TRACE_PACKED || desc->mode == MT_TRACE_PACKED_STACK ||
-               desc->mode == MT_TRACE_PACKED_SORTED) && !R.want_ilp;
+               desc->mode == MT_TRACE_PACKED_SORTED || desc->mode == MT_TRACE_PACKED_POOL) && !R.want_ilp;
+    R.pool = desc->mode == MT_TRACE_PACKED_POOL;
     R.profile = (desc->flags & MT_RENDER_PROFILE) != 0;
     R.oIlp = R.want_ilp ? take(12 * (R.sq > n ? R.sq : n)) : 0;
     R.oTot = take(8 * 8 * ILP_SLOTS);
@@ -477,6 +530,10 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
     const int grid = mt_blocks(np, RT);
     const int gshade = grid;              // one slot per lane; empty work-groups retire at once
     const int gshadow = mt_blocks((int64_t)R.sq, RT);
+    // wave-pool kernels: runs of 64 * POOL_RAYS entries, 4 per work-group
+    const int64_t pool_group = 4 * 64 * (int64_t)POOL_RAYS;
+    const int gcam_pool = mt_blocks(np, (int)pool_group);
+    const int gshadow_pool = SHADOW_SEGS * (int)((seg_cap + pool_group - 1) / pool_group);
     unsigned long long* tot = reinterpret_cast<unsigned long long*>(base + R.oTot);
     int* ilp = R.want_ilp ? I(R.oIlp) : nullptr;
     hipEvent_t* ev = R.ev;
@@ -510,8 +567,14 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         const bool first = fused && b == 0;
         if (profile && (rc = mark(ev[4 * b]))) break;
         if (first) {
-            k_trace_camera<<<grid, RT, 0, st>>>(R.C, np, seed_ptr(R), scene->pnodes, scene->ptris, scene->root_ref,
-                                                I(R.oPrim), reinterpret_cast<float*>(base + R.oT_), counts);
+            if (R.pool)
+                k_trace_camera<POOL_RAYS><<<gcam_pool, RT, 0, st>>>(R.C, np, seed_ptr(R), scene->pnodes, scene->ptris,
+                                                                    scene->root_ref, I(R.oPrim),
+                                                                    reinterpret_cast<float*>(base + R.oT_), counts);
+            else
+                k_trace_camera<0><<<grid, RT, 0, st>>>(R.C, np, seed_ptr(R), scene->pnodes, scene->ptris,
+                                                       scene->root_ref, I(R.oPrim),
+                                                       reinterpret_cast<float*>(base + R.oT_), counts);
             rc = chk();
         } else if (fused) {
             rc = mt_trace_closest_seg(ctx, scene, ro[cur], rd[cur], MtSegQueue{qsc + (size_t)b * SHADOW_SEGS,
@@ -556,9 +619,14 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         if ((rc = chk())) break;
         if (profile && (rc = mark(ev[4 * b + 2]))) break;
         if (fused) {
-            k_trace_shadow<<<gshadow, RT, 0, st>>>(scene->pnodes, scene->ptris, scene->root_ref, seg_counts, seg_cap,
-                                                    F4(R.oSo), F4(R.oSd), F4(R.oSc), first ? nullptr : q[cur], S.L,
-                                                    A.s_lg, R.grid ? R.grid->pend : nullptr);
+            if (R.pool)
+                k_trace_shadow<POOL_RAYS><<<gshadow_pool, RT, 0, st>>>(
+                    scene->pnodes, scene->ptris, scene->root_ref, seg_counts, seg_cap, F4(R.oSo), F4(R.oSd),
+                    F4(R.oSc), first ? nullptr : q[cur], S.L, A.s_lg, R.grid ? R.grid->pend : nullptr);
+            else
+                k_trace_shadow<0><<<gshadow, RT, 0, st>>>(
+                    scene->pnodes, scene->ptris, scene->root_ref, seg_counts, seg_cap, F4(R.oSo), F4(R.oSd),
+                    F4(R.oSc), first ? nullptr : q[cur], S.L, A.s_lg, R.grid ? R.grid->pend : nullptr);
             if ((rc = chk())) break;
             if (profile && (rc = mark(ev[4 * b + 3]))) break;
         } else {

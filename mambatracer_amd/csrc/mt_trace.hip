@@ -74,11 +74,6 @@ __device__ __forceinline__ int queue_entry(const TraceArgs& A)
     return blockIdx.x * blockDim.x + threadIdx.x;
 }
 
-__device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
-{ // set bits of `mask` below this lane (v_mbcnt_lo/hi)
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
 template <bool ANY, bool STATS>
 __device__ __forceinline__ void store_result(const TraceArgs& A, int id, const RayT& r, const TravState& S)
 {
@@ -201,6 +196,31 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
     }
 }
 
+// Wave-pool traversal (pool_trace): each wave owns a run of 64 * POOL
+// consecutive queue entries (segmented queue: work-group b takes chunk
+// b / nseg of segment b % nseg, 256 * POOL entries, one run per wave) and
+// refills its finished lanes from it.
+template <bool ANY, int POOL>
+__global__ __launch_bounds__(256) void k_trace_pool(TraceArgs A)
+{
+    constexpr int RUN = 64 * POOL;
+    int next, end;
+    if (A.seg.counts) {
+        pool_seg_run(A.seg.counts, A.seg.nseg, A.seg.seg_cap, RUN, next, end);
+    } else {
+        int count = *A.d_count;
+        count = count < A.capacity ? count : A.capacity;
+        next = (int)(blockIdx.x * 4 + (threadIdx.x >> 6)) * RUN;
+        end = count < next + RUN ? count : next + RUN;
+    }
+    if (next >= end) return;
+    pool_trace<ANY>(
+        A.pn, A.pt, A.root, next, end, [&](int id, RayT& r) { ray_init(r, A.o_tmin[id], A.d_tmax[id]); },
+        [&](bool fin, int id, const RayT& r, const TravState& S) {
+            if (fin) store_result<ANY, false>(A, id, r, S);
+        });
+}
+
 // One ray per lane as k_trace LAYOUT 0, after the work-group's rays are
 // ordered by direction: a ray's bin is its direction octant and dominant
 // axis (24 bins, dead entries last), and lanes take the group's rays in bin
@@ -297,7 +317,7 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
 {
     MT_ARG_CHECK(ctx && sc && o_tmin && d_tmax && (d_count || seg.counts), "trace: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "trace: negative capacity");
-    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_SORTED, "trace: unknown mode %d", mode);
+    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_POOL, "trace: unknown mode %d", mode);
     if (ANY) MT_ARG_CHECK(occluded != nullptr, "trace_any: occluded is NULL");
     else MT_ARG_CHECK(prim && t, "trace_closest: prim/t NULL");
     if (seg.counts) {
@@ -329,9 +349,20 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
     // wins only on long incoherent batches (16M random any-hit rays through a
     // 4M-triangle soup: 18.3 vs 21.3 ms); on a path tracer's bounce batches it
     // loses (1080p x 16 spp render: 25.8 vs 23.5 ms), so it is opt-in.
-    if (mode == MT_TRACE_PACKED) mode = MT_TRACE_PACKED_STACK;
-    if (mode == MT_TRACE_PACKED_SORTED && stats) mode = MT_TRACE_PACKED_STACK; // counting: per-ray order only
-    if (mode == MT_TRACE_PACKED_SORTED) {
+    // On the render's segmented bounce queues (secondary rays, incoherent)
+    // the wave pool wins (1080p x 16 spp guided render: bounce-1 closest
+    // hit 2.80 -> 2.21 ms); on coherent batches its desynchronised lanes
+    // fetch more distinct nodes per load and it loses (camera rays 1.40 ->
+    // 1.90 ms), so plain queues keep one ray per lane.
+    if (mode == MT_TRACE_PACKED) mode = seg.counts ? MT_TRACE_PACKED_POOL : MT_TRACE_PACKED_STACK;
+    if ((mode == MT_TRACE_PACKED_SORTED || mode == MT_TRACE_PACKED_POOL) && stats)
+        mode = MT_TRACE_PACKED_STACK; // counting: per-ray order only
+    if (mode == MT_TRACE_PACKED_POOL) {
+        const int64_t per_group = 256 * (int64_t)POOL_RAYS;
+        const int64_t groups = seg.counts ? (int64_t)seg.nseg * ((seg.seg_cap + per_group - 1) / per_group)
+                                          : (capacity + per_group - 1) / per_group;
+        k_trace_pool<ANY, POOL_RAYS><<<dim3((unsigned)groups), block, 0, s>>>(A);
+    } else if (mode == MT_TRACE_PACKED_SORTED) {
         k_trace_sorted<ANY><<<grid, block, 0, s>>>(A);
     } else if (mode == MT_TRACE_PACKED_REFILL) {
         void* scr = nullptr;
@@ -898,7 +929,7 @@ extern "C" int32_t mt_intersect_primitives(mt_context* ctx, const mt_scene* sc, 
 {
     MT_ARG_CHECK(ctx && sc && rays && isects && d_count, "mt_intersect_primitives: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "mt_intersect_primitives: negative capacity");
-    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_SORTED,
+    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_POOL,
                  "mt_intersect_primitives: unknown mode %d", mode);
     MT_ARG_CHECK((reinterpret_cast<uintptr_t>(rays) & 15) == 0 && (reinterpret_cast<uintptr_t>(isects) & 15) == 0,
                  "mt_intersect_primitives: rays and isects must be 16-byte aligned");
@@ -932,7 +963,7 @@ extern "C" int32_t mt_intersect_occlusion(mt_context* ctx, const mt_scene* sc, c
 {
     MT_ARG_CHECK(ctx && sc && rays && hits && d_count, "mt_intersect_occlusion: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "mt_intersect_occlusion: negative capacity");
-    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_SORTED,
+    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_POOL,
                  "mt_intersect_occlusion: unknown mode %d", mode);
     MT_ARG_CHECK((reinterpret_cast<uintptr_t>(rays) & 15) == 0, "mt_intersect_occlusion: rays must be 16-byte aligned");
     if (capacity == 0) return MT_OK;

@@ -307,6 +307,72 @@ __device__ __forceinline__ bool trace_packed_step(const float4* __restrict__ pn,
     return !parent_walk<STATS>(pn, pt, S.cur, S.bs, S.st);
 }
 
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
+{ // set bits of `mask` below this lane (v_mbcnt_lo/hi)
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Wave-pool traversal of the rays of entries [next, end) (wave-uniform): the
+// wave starts one ray per lane and, after every outer traversal step
+// (trace_packed_step: descend to a leaf, test it, move to the next pending
+// subtree), lanes whose ray has ended take the run's next entries in lane
+// order (ballot + mbcnt on a wave-uniform cursor; no atomics), so lanes stay
+// busy until the run is drained: a wave's tail, where its longest rays run
+// while the other lanes idle, comes once per run instead of once per 64
+// rays.  load(id, r) initialises entry id's ray; done(fin, id, r, S) is
+// called after every step with the whole wave active, fin set on the lanes
+// whose ray ended in that step.  Each ray is traversed in the reference's
+// visiting order; only the lane that computes it changes.
+template <bool ANY, class Load, class Done>
+__device__ __forceinline__ void pool_trace(const float4* __restrict__ pn, const float4* __restrict__ pt, int root,
+                                           int next, int end, Load&& load, Done&& done)
+{
+    bool active = false;
+    int id = 0;
+    RayT r;
+    TravState S;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        if (idle && next < end) {
+            const int avail = end - next, nidle = (int)__popcll(idle);
+            const int n = avail < nidle ? avail : nidle;
+            const int rk = (int)lane_rank(idle);
+            if (!active && rk < n) {
+                id = next + rk;
+                load(id, r);
+                S.init(root);
+                active = true;
+            }
+            next += n;
+        }
+        if (!__any(active)) break; // run drained, every lane done
+        bool fin = false;
+        if (active && trace_packed_step<ANY, false, true>(pn, pt, r, S)) {
+            fin = true;
+            active = false;
+        }
+        done(fin, id, r, S);
+    }
+}
+
+// The run of wave w of work-group b in a segmented queue (nseg segments of
+// seg_cap entries, counts[p] live in segment p) walked chunk-major with
+// runs of `run` entries: group b takes chunk b / nseg of segment b % nseg,
+// 4 runs, one per wave.
+__device__ __forceinline__ void pool_seg_run(const int* __restrict__ counts, int nseg, int seg_cap, int run,
+                                             int& next, int& end)
+{
+    const int p = blockIdx.x % nseg, c = blockIdx.x / nseg;
+    int cnt = counts[p];
+    cnt = cnt < seg_cap ? cnt : seg_cap;
+    const int off = c * (4 * run) + (int)(threadIdx.x >> 6) * run;
+    next = p * seg_cap + off;
+    end = p * seg_cap + (cnt < off + run ? cnt : off + run);
+}
+
+// Rays per lane of a wave's run in the pool kernels.
+constexpr int POOL_RAYS = 4;
+
 template <bool ANY, bool STATS, bool SHORT_STACK>
 __device__ __forceinline__ int trace_packed(const float4* __restrict__ pn,
                                             const float4* __restrict__ pt, int root, RayT& r,

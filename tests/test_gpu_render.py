@@ -152,6 +152,7 @@ def test_fused_and_staged_loops_agree(ctx, c1_small, mesh_lights):
     base = dict(width=100, height=60, spp=3, pos=(0.0, 0.5, -8.0), look=(0.0, -1.0, 0.0), seed=5)
     runs = [W.RenderJob(**base),                                           # fused
             W.RenderJob(**base, mode=_lib.MT_TRACE_PACKED_SORTED),         # fused, direction-sorted lanes
+            W.RenderJob(**base, mode=_lib.MT_TRACE_PACKED_POOL),           # fused, wave-pool lanes
             W.RenderJob(**base, flags=_lib.MT_RENDER_TRAVERSAL_STATS),     # staged, same traversal
             W.RenderJob(**base, mode=_lib.MT_TRACE_PACKED_NO_STACK),       # staged, parent-link pops
             W.RenderJob(**base, mode=_lib.MT_TRACE_REFERENCE)]             # staged, reference arrays

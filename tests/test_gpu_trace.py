@@ -16,7 +16,8 @@ from tests.conftest import c0_rays
 pytestmark = pytest.mark.gpu
 
 MODES = [api._lib.MT_TRACE_PACKED, api._lib.MT_TRACE_REFERENCE, api._lib.MT_TRACE_PACKED_NO_STACK,
-         api._lib.MT_TRACE_PACKED_STACK, api._lib.MT_TRACE_PACKED_REFILL, api._lib.MT_TRACE_PACKED_SORTED]
+         api._lib.MT_TRACE_PACKED_STACK, api._lib.MT_TRACE_PACKED_REFILL, api._lib.MT_TRACE_PACKED_SORTED,
+         api._lib.MT_TRACE_PACKED_POOL]
 
 
 def _scene(ctx, arrs, normals=None):
@@ -273,12 +274,15 @@ def test_records_edge_cases(ctx):
             np.testing.assert_array_equal(hits.cpu().numpy(), [0, 0, 0])
 
 
+@pytest.mark.parametrize("mode", [api._lib.MT_TRACE_PACKED_SORTED, api._lib.MT_TRACE_PACKED_POOL])
 @pytest.mark.parametrize("any_hit", [False, True])
-def test_direction_sorted_lanes_same_results(ctx, c0, any_hit):
+def test_lane_mappings_same_results(ctx, c0, any_hit, mode):
     """MT_TRACE_PACKED_SORTED (each work-group's rays taken in direction-bin
-    order) returns exactly the per-ray results of one ray per lane, on
-    camera rays and on random rays with a live length that ends mid-group."""
-    print(f"[case] sorted lanes any={any_hit}", flush=True)
+    order) and MT_TRACE_PACKED_POOL (each wave refilling its finished lanes
+    from its own run of rays) return exactly the per-ray results of one ray
+    per lane, on camera rays and on random rays with a live length that ends
+    mid-group and mid-run."""
+    print(f"[case] lane mapping mode={mode} any={any_hit}", flush=True)
     points, faces, nodes, bounds, _ = c0
     sc = _scene(ctx, c0)
     o1, d1 = c0_rays(128, 128)
@@ -288,7 +292,7 @@ def test_direction_sorted_lanes_same_results(ctx, c0, any_hit):
         count = ctx.scalar_i32(len(o) - 37)
         fn = api.intersect_occlusion if any_hit else api.intersect_primitives
         a = fn(ctx, sc, do, dd, count=count, mode=api._lib.MT_TRACE_PACKED_STACK)
-        b = fn(ctx, sc, do, dd, count=count, mode=api._lib.MT_TRACE_PACKED_SORTED)
+        b = fn(ctx, sc, do, dd, count=count, mode=mode)
         torch.cuda.synchronize()
         n = len(o) - 37
         for x, y in zip(a, b):
