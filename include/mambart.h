@@ -225,12 +225,12 @@ int32_t mt_trace_any(mt_context* ctx, const mt_scene* scene,
 /* Traversal autotuner (no counterpart in the reference, which has one
  * traversal kernel).  Every packed mode returns the same bits, but their
  * speed depends on the batch: one ray per lane with the short stack wins on
- * coherent path-tracing bounces, persistent waves with per-lane refill on
- * long incoherent batches.  Traces this batch (any = 0: closest hit into
- * prim/t; any != 0: any hit into occluded) with MT_TRACE_PACKED_STACK and
- * MT_TRACE_PACKED_REFILL, `reps` timed runs each after one warm-up, times
- * them with HIP events on the context stream and sets *best_mode to the
- * faster and *best_ms to its mean time.  The outputs hold the batch's
+ * coherent rays, lane refill (persistent waves or wave pools) on incoherent
+ * ones.  Traces this batch (any = 0: closest hit into prim/t; any != 0: any
+ * hit into occluded) with MT_TRACE_PACKED_STACK, MT_TRACE_PACKED_REFILL and
+ * MT_TRACE_PACKED_POOL, `reps` timed runs each after one warm-up, times them
+ * with HIP events on the context stream and sets *best_mode to the fastest
+ * and *best_ms to its mean time.  The outputs hold the batch's
  * result.  Synchronises the context. */
 int32_t mt_trace_autotune(mt_context* ctx, const mt_scene* scene, const float* o_tmin,
                           const float* d_tmax, const int32_t* d_count, int32_t capacity,

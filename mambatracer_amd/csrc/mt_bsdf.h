@@ -232,31 +232,62 @@ __device__ __forceinline__ v3 mt_bsdf_eval_one(const MtBsdf& B, const MtMaterial
 // SampleBsdf, Bsdf.cl:555-606 (with SampleDiffuse :380-394, SampleGlossy
 // :396-432, SampleCosHemisphereW Sampling.cl:129-148).  Returns f (zero when
 // the sample is rejected); wdir/pdf/cos as the reference leaves them.
+// Either pick evaluates the glossy lobe at a (direction, half vector) pair:
+// the diffuse pick at its cosine-sampled direction and their half vector
+// (EvaluateGlossy), the glossy pick at the VNDF half vector and its
+// reflection.  Lanes of one wave draw both picks (u2 per path), so the pair
+// is formed per pick and the lobe's pdf and value computed once after it,
+// on the same values either pick passes; the sums then follow each pick's
+// order (diffuse pick: diffuse term, then glossy; glossy pick: glossy, then
+// EvaluateDiffuse's term).
 __device__ __forceinline__ v3 mt_bsdf_sample_one(const MtBsdf& B, const MtMaterial& m, float u0,
                                                  float u1, float u2, v3& wdir, float& pdf,
                                                  float& cos_out)
 {
-    v3 f = mk3(0, 0, 0);
-    v3 gen = mk3(0, 0, 0);
-    if (u2 < B.diff_prob) {
-        if (B.fix.z >= MT_EPS_COSINE) {
+    const bool dsel = u2 < B.diff_prob;
+    const bool front = B.fix.z >= MT_EPS_COSINE;
+    const float ax = m.glossy_param.y, ay = m.glossy_param.z;
+    v3 gen = mk3(0, 0, 0), wh = mk3(0, 0, 0);
+    if (front) {
+        if (dsel) {
             float term1 = (2.0f * MT_PI) * u0;
             float term2 = sqrtf(1.0f - u1);
             gen = mk3(mt_xm_cosf(term1) * term2, mt_xm_sinf(term1) * term2, sqrtf(u1));
+        } else {
+            wh = ggx_sample_vndf(B.fix, ax, ay, u0, u1);
+            float k = 2.0f * dot3(B.fix, wh);
+            gen = add3(neg3(B.fix), scl3(wh, k)); // reflectFromN, Bsdf.cl:325-328
+        }
+    }
+    // the evaluation functions' own test (mt_eval_diffuse / mt_eval_glossy)
+    const bool eval_ok = !(B.fix.z < MT_EPS_COSINE || gen.z < MT_EPS_COSINE);
+    const bool g_on = dsel ? (B.glossy_prob != 0.0f && eval_ok) : front;
+    float gpdf = 0.0f;
+    v3 gval = mk3(0, 0, 0);
+    if (g_on) {
+        if (dsel) wh = norm3(add3(B.fix, gen)); // halfVector, Bsdf.cl:336-341
+        gpdf = B.glossy_prob * ggx_pdf(gen, wh, ax, ay);
+        gval = mt_glossy_value(m, B.fix, gen, wh, ax, ay);
+    }
+    v3 f = mk3(0, 0, 0);
+    if (dsel) {
+        if (front) {
             pdf += ((gen.z * 1.0f) / MT_PI) * B.diff_prob;
             f = mt_lambert(m);
         }
-        mt_eval_glossy(B, m, gen, pdf, f);
-    } else {
-        if (B.fix.z >= MT_EPS_COSINE) {
-            float ax = m.glossy_param.y, ay = m.glossy_param.z;
-            v3 wh = ggx_sample_vndf(B.fix, ax, ay, u0, u1);
-            float k = 2.0f * dot3(B.fix, wh);
-            gen = add3(neg3(B.fix), scl3(wh, k)); // reflectFromN, Bsdf.cl:325-328
-            pdf += B.glossy_prob * ggx_pdf(gen, wh, ax, ay);
-            f = mt_glossy_value(m, B.fix, gen, wh, ax, ay);
+        if (g_on) {
+            pdf += gpdf;
+            f = add3(f, gval);
         }
-        mt_eval_diffuse(B, m, gen, pdf, f);
+    } else {
+        if (front) {
+            pdf += gpdf;
+            f = gval;
+        }
+        if (B.diff_prob != 0.0f && eval_ok) {
+            pdf += B.diff_prob * fmaxf(0.0f, (gen.z * 1.0f) / MT_PI);
+            f = add3(f, mt_lambert(m));
+        }
     }
     float c = fabsf(gen.z);
     cos_out = c;

@@ -823,7 +823,7 @@ extern "C" int32_t mt_trace_autotune(mt_context* ctx, const mt_scene* scene, con
     MT_ARG_CHECK(ctx && scene && best_mode && best_ms, "mt_trace_autotune: NULL argument");
     MT_ARG_CHECK(reps >= 1 && reps <= 100, "mt_trace_autotune: reps %d not in [1,100]", reps);
     MT_ARG_CHECK(any ? occluded != nullptr : (prim && t), "mt_trace_autotune: NULL outputs");
-    const int32_t modes[2] = {MT_TRACE_PACKED_STACK, MT_TRACE_PACKED_REFILL};
+    const int32_t modes[3] = {MT_TRACE_PACKED_STACK, MT_TRACE_PACKED_REFILL, MT_TRACE_PACKED_POOL};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MT_HIP_CHECK(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
