@@ -17,47 +17,31 @@
 // ---------------------------------------------------------------- CDF search
 // upperBound (LightGrid.cl:212-247, SAT.cl:206-242) by bisection: the number
 // of leading entries of a[b, e) with !(u < a[i]) when a is non-decreasing,
-// and for any a the index the reference's bisection returns.  The next four
-// bisection steps read at most 15 entries whose positions are fixed by the
-// current range (a node's children are the halves of its range), so those
-// entries are loaded together and the four steps then run on the loaded
-// values: a 64-entry range costs two rounds of memory latency instead of
-// six dependent loads, and every comparison is the bisection's own.
-__device__ __forceinline__ int cdf_search4(const float* __restrict__ a, int b, int e, float u)
+// and for any a the index the reference's bisection returns.  Each round
+// loads the range's midpoint and the midpoints of both halves together and
+// takes two bisection steps (the second compares against the half the first
+// step keeps): half the dependent loads of one step per round, for three
+// loads per two steps.  Every comparison is the bisection's own and every
+// load lies inside [b, e).  (Four steps per round, 15 loads, was measured
+// slower: the shading kernels that call this are bound by VALU issue, and
+// the extra selects cost more than the saved latency.)
+__device__ __forceinline__ int cdf_search2(const float* __restrict__ a, int b, int e, float u)
 {
     while (b != e) {
-        int pos[15];
-        float v[15];
-        int lo[15], hi[15];
-        lo[0] = b;
-        hi[0] = e;
-#pragma unroll
-        for (int i = 0; i < 15; i++) { // nodes of the next four levels, breadth first
-            const bool live = lo[i] < hi[i];
-            const int mid = lo[i] + (hi[i] - lo[i]) / 2;
-            pos[i] = live ? mid : -1;
-            if (2 * i + 2 < 15) {
-                lo[2 * i + 1] = lo[i];
-                hi[2 * i + 1] = live ? mid : lo[i];
-                lo[2 * i + 2] = live ? mid + 1 : lo[i];
-                hi[2 * i + 2] = live ? hi[i] : lo[i];
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 15; i++) v[i] = pos[i] >= 0 ? a[pos[i]] : 0.0f;
-        int node = 0;
-#pragma unroll
-        for (int level = 0; level < 4; level++) {
-            if (b == e) break;
-            const int first = (1 << level) - 1; // this level's nodes: [first, 2 * first]
-            float x = v[first];
-#pragma unroll
-            for (int i = first + 1; i <= 2 * first; i++) x = node == i ? v[i] : x;
-            const int mid = b + (e - b) / 2;
-            const bool right = !(u < x);
-            b = right ? mid + 1 : b;
-            e = right ? e : mid;
-            node = right ? 2 * node + 2 : 2 * node + 1;
+        const int mid = b + (e - b) / 2;
+        const int lm = b + (mid - b) / 2;               // midpoint of [b, mid)
+        const int rm = (mid + 1) + (e - (mid + 1)) / 2; // midpoint of [mid + 1, e)
+        const float vm = a[mid];
+        const float vl = b < mid ? a[lm] : 0.0f;
+        const float vr = mid + 1 < e ? a[rm] : 0.0f;
+        const bool right = !(u < vm);
+        b = right ? mid + 1 : b;
+        e = right ? e : mid;
+        if (b != e) {
+            const int m2 = b + (e - b) / 2; // rm or lm
+            const bool r2 = !(u < (right ? vr : vl));
+            b = r2 ? m2 + 1 : b;
+            e = r2 ? e : m2;
         }
     }
     return b;
@@ -170,7 +154,7 @@ __device__ __forceinline__ void lg_sample_tab(const float* __restrict__ func, co
                                               float& pdf)
 {
     const float* T = tab + (size_t)s * LG_TAB;
-    int b = cdf_search4(T, 0, LG_RY, u1);
+    int b = cdf_search2(T, 0, LG_RY, u1);
     oy = b - 1 > 0 ? b - 1 : 0;
     const float m0 = T[oy];
     const float du = (u1 - m0) / (T[oy + 1] - m0);
@@ -178,7 +162,7 @@ __device__ __forceinline__ void lg_sample_tab(const float* __restrict__ func, co
     const float pdf_v = (fi * (float)(LG_RX * LG_RY)) / T[LG_TAB_LAST];
     vv = ((float)oy + du) / (float)LG_RY;
     const float* C = T + LG_TAB_COND + oy * LG_TAB_CSTRIDE;
-    b = cdf_search4(C, 0, LG_RX, u0);
+    b = cdf_search2(C, 0, LG_RX, u0);
     ox = b - 1 > 0 ? b - 1 : 0;
     const float c0 = C[ox];
     const float du2 = (u0 - c0) / (C[ox + 1] - c0);
@@ -495,7 +479,7 @@ __device__ __forceinline__ void r_sample_tile(const EnvView& E, const SatRegion&
     const float* M = T - (R.miny - 1);        // M[y] = marg of row y
     const float* FI = T + nm - (R.miny - 1);  // FI[y]
     const float last = T[nm + nf];
-    int b = cdf_search4(M, R.miny, R.miny + R.sy, u1);
+    int b = cdf_search2(M, R.miny, R.miny + R.sy, u1);
     oy = b - 1 > 0 ? b - 1 : 0;
     const float m0 = M[oy];
     const float du = (u1 - m0) / (M[oy + 1] - m0);
@@ -503,7 +487,7 @@ __device__ __forceinline__ void r_sample_tile(const EnvView& E, const SatRegion&
     const float pdf_v = (fi * (float)(R.sx * R.sy)) / last;
     vv = ((float)oy + du) / (float)R.nv;
     const float* C = T + nm + nf + 1 + (oy - (R.miny - 1)) * cs - (R.minx - 1); // C[x] = cond(x, oy)
-    b = cdf_search4(C, R.minx, R.minx + R.sx, u0);
+    b = cdf_search2(C, R.minx, R.minx + R.sx, u0);
     ox = b - 1 > 0 ? b - 1 : 0;
     const float c0 = C[ox];
     const float du2 = (u0 - c0) / (C[ox + 1] - c0);

@@ -106,31 +106,33 @@ MT_XM_FN float mt_xm_atan_poly(float t)
     return t + (t * z) * p;
 }
 
-/* atan(z) for z in [0, 1] */
+/* atan(z) for z in [0, 1].  Written with selects rather than branches (the
+   same operations on every input, so the same results): lanes of one GPU
+   wave take both sides, and a branch would evaluate the series twice. */
 MT_XM_FN float mt_xm_atan01(float z)
 {
-    if (z > MT_XM_TAN_PI12) return MT_XM_PI6 + mt_xm_atan_poly((z * MT_XM_SQRT3 - 1.0f) / (z + MT_XM_SQRT3));
-    return mt_xm_atan_poly(z);
+    const int red = z > MT_XM_TAN_PI12;
+    const float t = red ? (z * MT_XM_SQRT3 - 1.0f) / (z + MT_XM_SQRT3) : z;
+    const float p = mt_xm_atan_poly(t);
+    return red ? MT_XM_PI6 + p : p;
 }
 
+/* One ratio min/max and one series for every input; the octant and the
+   special cases are selected afterwards (results as the case analysis:
+   NaN -> x + y; both zero -> 0 or pi; equal magnitudes -> pi/4 or 3pi/4;
+   |y| > |x| -> pi/2 -+ atan(|x|/|y|); else atan(|y|/|x|) or pi - it). */
 MT_XM_FN float mt_xm_atan2f(float y, float x)
 {
-    if (x != x || y != y) return x + y; /* NaN */
     const float ax = fabsf(x), ay = fabsf(y);
     const int xneg = signbit(x) != 0, yneg = signbit(y) != 0;
-    float a;
-    if (ay == 0.0f && ax == 0.0f) {
-        a = xneg ? MT_XM_PI_HI : 0.0f;
-    } else if (ax == ay) { /* also both infinite */
-        a = xneg ? MT_XM_3PI4 : MT_XM_PI4;
-    } else if (ay > ax) { /* angle from the y axis */
-        const float t = mt_xm_atan01(ax / ay);
-        a = xneg ? (MT_XM_PIO2_HI + t) + MT_XM_PIO2_LO : (MT_XM_PIO2_HI - t) + MT_XM_PIO2_LO;
-    } else {
-        const float t = mt_xm_atan01(ay / ax);
-        a = xneg ? (MT_XM_PI_HI - t) + MT_XM_PI_LO : t;
-    }
-    return yneg ? -a : a;
+    const int yax = ay > ax; /* angle from the y axis */
+    const float t = mt_xm_atan01((yax ? ax : ay) / (yax ? ay : ax));
+    float a = yax ? (xneg ? (MT_XM_PIO2_HI + t) + MT_XM_PIO2_LO : (MT_XM_PIO2_HI - t) + MT_XM_PIO2_LO)
+                  : (xneg ? (MT_XM_PI_HI - t) + MT_XM_PI_LO : t);
+    a = ax == ay ? (xneg ? MT_XM_3PI4 : MT_XM_PI4) : a; /* also both infinite */
+    a = (ay == 0.0f && ax == 0.0f) ? (xneg ? MT_XM_PI_HI : 0.0f) : a;
+    a = yneg ? -a : a;
+    return (x != x || y != y) ? x + y : a;
 }
 
 MT_XM_FN float mt_xm_acosf(float y) { return mt_xm_atan2f(sqrtf((1.0f - y) * (1.0f + y)), y); }
