@@ -312,16 +312,72 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// One node of the packed traversal per call: an interior node's box pair
+// (descending to the chosen child, or moving to the next pending subtree
+// when both miss) or a leaf's triangle (then moving on).  Returns true when
+// the ray is finished.  Successive calls visit exactly the nodes of
+// trace_packed_step in the same order; a wave that calls this in a loop
+// keeps every lane on a node of its own ray each iteration instead of
+// holding the lanes that reached a leaf until the whole wave has.
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool trace_packed_node(const float4* __restrict__ pn, const float4* __restrict__ pt,
+                                                  RayT& r, TravState& S)
+{
+    if (S.cur >= 0) {
+        const float4* N = pn + 4 * (size_t)S.cur;
+        float4 a = N[0], b = N[1], c = N[2];
+        int4 d = *reinterpret_cast<const int4*>(N + 3);
+        float lt, rt;
+        bool lh = box_hit(r, a.x, a.y, a.z, a.w, b.x, b.y, lt);
+        bool rh = box_hit(r, b.z, b.w, c.x, c.y, c.z, c.w, rt);
+        if (STATS) S.st[0]++;
+        if (lh || rh) {
+            S.bs.push0();
+            int nxt;
+            if (lh && rh) {
+                bool right_first = rt < lt;
+                nxt = right_first ? d.y : d.x;
+                S.ss.push(right_first ? d.x : d.y);
+                S.bs.lo |= 1;
+            } else {
+                nxt = lh ? d.x : d.y;
+            }
+            S.cur = nxt;
+            return false;
+        }
+    } else {
+        if (STATS) S.st[1]++;
+        const float4* T = pt + 3 * (size_t)(~S.cur);
+        float4 t0 = T[0], t1 = T[1], t2 = T[2];
+        float t = tri_hit(r, mk3(t0.x, t0.y, t0.z), mk3(t0.w, t1.x, t1.y), mk3(t1.z, t1.w, t2.x));
+        if ((r.tmin < t) && (t < r.tmax)) {
+            S.hit = 1;
+            if (ANY) return true;
+            r.tmax = t;
+            S.child = __float_as_int(t2.y);
+        }
+    }
+    if (S.bs.empty()) return true;
+    if (S.ss.n > 0) {
+        int z = S.bs.ctz();
+        if (STATS) S.st[2] += z;
+        S.bs.shr(z);
+        S.bs.lo ^= 1;
+        S.cur = S.ss.pop();
+        return false;
+    }
+    return !parent_walk<STATS>(pn, pt, S.cur, S.bs, S.st);
+}
+
 // Wave-pool traversal of the rays of entries [next, end) (wave-uniform): the
-// wave starts one ray per lane and, after every outer traversal step
-// (trace_packed_step: descend to a leaf, test it, move to the next pending
-// subtree), lanes whose ray has ended take the run's next entries in lane
-// order (ballot + mbcnt on a wave-uniform cursor; no atomics), so lanes stay
-// busy until the run is drained: a wave's tail, where its longest rays run
-// while the other lanes idle, comes once per run instead of once per 64
-// rays.  load(id, r) initialises entry id's ray; done(fin, id, r, S) is
-// called after every step with the whole wave active, fin set on the lanes
-// whose ray ended in that step.  Each ray is traversed in the reference's
+// wave starts one ray per lane, every iteration takes each lane one node
+// further on its ray (trace_packed_node), and lanes whose ray has ended take
+// the run's next entries in lane order (ballot + mbcnt on a wave-uniform
+// cursor; no atomics), so lanes stay busy until the run is drained: a
+// wave's tail, where its longest rays run while the other lanes idle, comes
+// once per run instead of once per 64 rays.  load(id, r) initialises entry
+// id's ray; done(fin, id, r, S) is called after every iteration with the
+// whole wave active, fin set on the lanes whose ray ended in it.  Each ray is traversed in the reference's
 // visiting order; only the lane that computes it changes.
 template <bool ANY, class Load, class Done>
 __device__ __forceinline__ void pool_trace(const float4* __restrict__ pn, const float4* __restrict__ pt, int root,
@@ -347,7 +403,7 @@ __device__ __forceinline__ void pool_trace(const float4* __restrict__ pn, const 
         }
         if (!__any(active)) break; // run drained, every lane done
         bool fin = false;
-        if (active && trace_packed_step<ANY, false, true>(pn, pt, r, S)) {
+        if (active && trace_packed_node<ANY, false>(pn, pt, r, S)) {
             fin = true;
             active = false;
         }
