@@ -52,6 +52,20 @@ def traffic_from_profiles(workload: str = "c1"):
         return None
 
 
+def valu_from_profiles(workload: str = "c1"):
+    """VALU instructions per closest-hit launch (rocprofv3 SQ_INSTS_VALU, profiles/) and the
+    issue peak they are compared with, or None."""
+    p = os.path.join(ROOT, "profiles", f"counters_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return float(d["valu_inst_per_launch"]), float(d["valu_peak_inst_per_s"]), d["valu_peak_basis"]
+    except Exception:
+        return None
+
+
 def cpu_baseline(W, scene_arrays, mats, rgba, job, npaths, seconds, threads, grid=None, grid_mode=0):
     import numpy as np
     from oracle import orc_py as orc
@@ -490,7 +504,8 @@ def bench_c1(args) -> int:
         "other_lighting": others,
         "roofline": {
             "kernel": ("closest-hit traversal, packed layout: k_trace_camera (bounce 0, camera rays made in "
-                       "place) and k_trace (bounces 1-7), averaged over the job's launches")
+                       "place) and k_trace_pool (bounces 1-7, wave pools with lane refill), averaged over the "
+                       "job's launches")
                       if job.mode == 0 else "closest-hit traversal (reference layout)",
             "bound": "hbm",
             "achieved": round(achieved, 1) if achieved is not None else None,
@@ -506,11 +521,24 @@ def bench_c1(args) -> int:
             "alg_basis": "reference-layout bytes touched, 40 + 148 I + 116 L + 28 P per ray, no cache credit "
                          "(upper BVH levels are L2/MALL-resident, so this exceeds HBM traffic)",
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+            "valu_issue": valu_issue(valu_from_profiles("c1") if job.mode == 0 else None, avg_launch_s),
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def valu_issue(valu, launch_s):
+    """The closest-hit launches' VALU issue rate (profiled instructions per launch over the
+    live launch time) against the issue peak: the limit these kernels run into."""
+    if valu is None:
+        return None
+    inst, peak, basis = valu
+    return {"inst_per_launch": round(inst), "achieved_inst_per_s": round(inst / launch_s / 1e9, 1),
+            "peak_inst_per_s": round(peak / 1e9, 1), "unit": "G VALU instructions/s",
+            "frac": round(inst / launch_s / peak, 4), "peak_basis": basis,
+            "source": "profiles/counters_c1.md (SQ_INSTS_VALU)"}
 
 
 def bench_c2(args) -> int:
