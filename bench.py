@@ -1047,6 +1047,11 @@ def bench_jobs(args) -> int:
     modes["two_lane"] = serve.summary(two, two_rays)
     log(f"two lanes (large jobs / small jobs shortest first): {modes['two_lane']}")
     mismatched += sum(0 if torch.equal(ref[d.tag], d.image) else 1 for d in two)
+    twob, twob_rays = serve.serve_two_lane(ctx, reqs, keep_images=True, small_order="batched",
+                                           small_iterations=args.iters)
+    modes["two_lane_batched"] = serve.summary(twob, twob_rays)
+    log(f"two lanes (large jobs / small jobs continuously batched): {modes['two_lane_batched']}")
+    mismatched += sum(0 if torch.equal(ref[d.tag], d.image) else 1 for d in twob)
     # Amdahl bound of the multi-job gain on all-at-once arrivals: a job of at
     # least `fill` paths keeps the GPU busy alone, so no mix shortens its own
     # render; batching can at best run every smaller job at the rate of the

@@ -8,9 +8,13 @@ renderThread -> execute -> loop, src/cl/device/CDeviceGI.java:190-204,
 order), keeps stepping the shared queues and timestamps each job when its
 image is complete.  `serve_sequential` runs the same arrivals through plain
 `api.render` calls, first come first served or shortest job first.
-`serve_two_lane` runs plain renders on two streams of the device, one host
-thread each: jobs that fill the GPU by themselves go to one lane, the rest
-to the other, so a small job never waits behind a large one's whole render.
+`serve_two_lane` runs two lanes on two streams of the device, one host
+thread each: jobs that fill the GPU by themselves go to one lane (plain
+renders, first come first served), the rest to the other, so a small job
+never waits behind a large one's whole render; the small-job lane renders
+one job at a time shortest first, or runs its jobs through a multi-job
+engine of its own (continuous batching: every waiting small job is admitted
+at once, whatever the large lane is doing).
 
 A request is a `Request`: arrival time (seconds from the start of the run),
 the resident scene, materials, environment and lights it renders with, and
@@ -132,16 +136,21 @@ def serve_sequential(ctx: api.Context, requests: List[Request], keep_images: boo
 
 
 def serve_two_lane(ctx: api.Context, requests: List[Request], keep_images: bool = False,
-                   big_paths: int = 1 << 22, small_order: str = "sjf"):
+                   big_paths: int = 1 << 22, small_order: str = "sjf", small_queue: int = 1 << 21,
+                   small_arena: int = 16 << 30, small_iterations: int = 1):
     """Two render lanes, each a context on its own stream of ctx's device
     driven by its own host thread: requests of at least `big_paths` paths
     (a 1080p x 16 spp job, a 1024^2 x 64 spp soup) go to the large lane,
-    first come first served; the rest to the small lane, shortest job first
-    (or "fifo").  The lanes' kernels share the GPU, so small jobs are not
-    queued behind a large job's whole render.  Every job is a plain
-    mt_render, so its image is the one it has alone.  Returns (Done records
-    in completion order, rays traced)."""
-    if small_order not in ("fifo", "sjf"):
+    first come first served; the rest to the small lane: one plain render
+    at a time, shortest job first ("sjf") or in arrival order ("fifo"), or
+    ("batched") a multi-job engine of the lane's own (queue capacity
+    `small_queue` paths, arena `small_arena` bytes, smallest job first,
+    `small_iterations` bounce iterations per step) that admits every waiting
+    small job at once.  The lanes' kernels share the GPU, so small jobs are
+    not queued behind a large job's whole render.  Plain renders and the
+    engine give each job the image it has alone.  Returns (Done records in
+    completion order, rays traced)."""
+    if small_order not in ("fifo", "sjf", "batched"):
         raise ValueError(f"unknown order {small_order!r}")
     torch = api._torch()
     torch.cuda.synchronize()
@@ -156,7 +165,49 @@ def serve_two_lane(ctx: api.Context, requests: List[Request], keep_images: bool 
     errors: List[BaseException] = []
     t0 = time.perf_counter()
 
+    def batched_lane(k: int) -> None:
+        torch_ = api._torch()
+        lctx = lanes_ctx[k]
+        b = None
+        try:
+            b = api.Batch(lctx, small_queue, small_arena, 1024, policy=1)
+            b.set_iterations(small_iterations)
+            images, arrivals = {}, {}
+            busy = False
+            while True:
+                with cond:
+                    while not waiting[k] and not closed[0] and not busy:
+                        cond.wait()
+                    if not waiting[k] and closed[0] and not busy:
+                        break
+                    new, waiting[k][:] = waiting[k][:], []
+                for r in new:
+                    img = lctx.empty((r.job.width * r.job.height, 4), torch_.float32)
+                    b.submit(r.tag, r.scene, r.materials, r.env, r.job, img, r.light_faces, r.num_lights)
+                    images[r.tag], arrivals[r.tag] = img, r.arrival
+                    busy = True
+                if busy:
+                    tags, busy = b.step()
+                    t = time.perf_counter() - t0
+                    with cond:
+                        for tag in tags:
+                            img = images.pop(tag)
+                            out.append(Done(tag, arrivals.pop(tag), t, img if keep_images else None))
+            st = b.stats()
+            with cond:
+                rays[0] += int(st.extension_rays) + int(st.shadow_rays)
+        except BaseException as e:  # reported by the caller's thread
+            errors.append(e)
+            with cond:  # nothing more is accepted here: let the feeder finish
+                closed[0] = True
+        finally:
+            if b is not None:
+                b.close()
+
     def lane(k: int) -> None:
+        if k == 0 and small_order == "batched":
+            batched_lane(k)
+            return
         try:
             while True:
                 with cond:

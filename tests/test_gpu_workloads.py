@@ -8,8 +8,9 @@ bit-identical to its own plain render.
 Config 4: Poisson arrivals (40 jobs/s for 2 s) drawn from the config-3 size
 distribution on 31 resident soups, 10% of them 1920x1080x16 spp jobs on the
 config-1 scene, 32 scenes resident in all; served by the continuous-batching
-loop and by the concurrent two-lane loop, each image must equal the same
-request rendered alone."""
+loop and by the concurrent two-lane loop (small-job lane one job at a time,
+and continuously batched), each image must equal the same request rendered
+alone."""
 import numpy as np
 import pytest
 import torch
@@ -76,7 +77,8 @@ def test_c4_served_images_equal_alone(ctx, sky):
         alone[r.tag] = img
     bat, _ = serve.serve_batched(ctx, reqs, queue_capacity=1 << 22, keep_images=True, policy=1)
     two, _ = serve.serve_two_lane(ctx, reqs, keep_images=True)
-    for name, done in (("batched", bat), ("two-lane", two)):
+    twob, _ = serve.serve_two_lane(ctx, reqs, keep_images=True, small_order="batched")
+    for name, done in (("batched", bat), ("two-lane", two), ("two-lane batched", twob)):
         assert sorted(d.tag for d in done) == sorted(alone)
         bad = [d.tag for d in done if not torch.equal(d.image, alone[d.tag])]
         print(f"{name}: images differing {bad}", flush=True)
