@@ -193,6 +193,48 @@ struct PathOut {
     float shadow_lum;             //   this luminance to when unoccluded (-1 none)
 };
 
+// LightHitPass, environment branch (Render.cl:106-123): the radiance a path
+// that escaped the scene gathers (Ladd), and in the guided environment mode
+// its light-grid entry and fixed-point luminance (acc_gidx, acc_q; -1 / 0
+// otherwise).
+__device__ __forceinline__ void shade_escape(const SceneView& V, int bounce, float4 ro, float4 rd, float4 T4,
+                                             v3& Ladd, int& acc_gidx, unsigned long long& acc_q)
+{
+    const v3 d = xyz(rd);
+    const v3 T = xyz(T4);
+    const float last_pdf = T4.w;
+    const bool last_specular = bounce == 0; // InitPathData sets it, SampleBSDFRayDirection clears it
+    Ladd = mk3(0, 0, 0);
+    acc_gidx = -1;
+    acc_q = 0ull;
+    if (V.env.w > 0) {
+        float pdfw;
+        float4 cv;
+        if (V.G.mode) {
+            // getRadianceEnvironmentLight1 (Light.cl:229-268) at the path's
+            // last hit point: InitPathData's (0,0,0) for a camera ray
+            // (Initialize.cl:46), else the ray's origin (SampleBSDFRayDirection
+            // stores isect->p as hitpoint, Render.cl:278-279)
+            const v3 hp = bounce == 0 ? mk3(0.0f, 0.0f, 0.0f) : xyz(ro);
+            float lum;
+            int gidx, cell, tile;
+            cv = env_radiance_adaptive(V.env, mk3(V.G.cx, V.G.cy, V.G.cz), d, hp, pdfw, lum, gidx, cell, tile);
+            if (V.G.mode == 2) pdfw = pdfw * lg_tile_pdf(V.G, cell, tile % LG_RX, tile / LG_RX);
+            acc_gidx = gidx; // accumLightGrid, Light.cl:264
+            acc_q = lg_fixed(lum);
+        } else {
+            int tx;
+            cv = env_radiance(V.env, d, pdfw, tx);
+        }
+        v3 contrib = xyz(cv);
+        if (!(contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f)) {
+            float w = last_specular ? 1.0f : mis2(last_pdf, pdfw);
+            v3 v = mk3(T.x * contrib.x * w, T.y * contrib.y * w, T.z * contrib.z * w);
+            if (rgb_valid(v)) Ladd = v;
+        }
+    }
+}
+
 // One bounce of one path: the ray (ro, rd) of path `pid` at depth `bounce`
 // with throughput T4 (w = last BSDF pdf) hit primitive c at distance tt
 // (c < 0: miss).
@@ -215,33 +257,7 @@ __device__ __forceinline__ void shade_path(const SceneView& V, uint32_t seed, in
     out.shadow_gidx = -1;
     out.shadow_lum = 0.0f;
     if (c < 0) {
-        // LightHitPass, environment branch (Render.cl:106-123)
-        if (V.env.w > 0) {
-            float pdfw;
-            float4 cv;
-            if (V.G.mode) {
-                // getRadianceEnvironmentLight1 (Light.cl:229-268) at the path's
-                // last hit point: InitPathData's (0,0,0) for a camera ray
-                // (Initialize.cl:46), else the ray's origin (SampleBSDFRayDirection
-                // stores isect->p as hitpoint, Render.cl:278-279)
-                const v3 hp = bounce == 0 ? mk3(0.0f, 0.0f, 0.0f) : xyz(ro);
-                float lum;
-                int gidx, cell, tile;
-                cv = env_radiance_adaptive(V.env, mk3(V.G.cx, V.G.cy, V.G.cz), d, hp, pdfw, lum, gidx, cell, tile);
-                if (V.G.mode == 2) pdfw = pdfw * lg_tile_pdf(V.G, cell, tile % LG_RX, tile / LG_RX);
-                out.acc_gidx = gidx; // accumLightGrid, Light.cl:264
-                out.acc_q = lg_fixed(lum);
-            } else {
-                int tx;
-                cv = env_radiance(V.env, d, pdfw, tx);
-            }
-            v3 contrib = xyz(cv);
-            if (!(contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f)) {
-                float w = last_specular ? 1.0f : mis2(last_pdf, pdfw);
-                v3 v = mk3(T.x * contrib.x * w, T.y * contrib.y * w, T.z * contrib.z * w);
-                if (rgb_valid(v)) Ladd = v;
-            }
-        }
+        shade_escape(V, bounce, ro, rd, T4, Ladd, out.acc_gidx, out.acc_q);
     } else {
         // IntersectPrimitives hit shading (NormalBVH.cl:124-220)
         const int* F = V.faces + 10 * c;

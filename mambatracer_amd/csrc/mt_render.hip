@@ -197,6 +197,8 @@ struct ShadeArgs {
     float4* s_d_tmax;
     float4* s_contrib;    // out: unoccluded contribution, w = the ray's slot (int bits)
     int2* s_lg;           // out (guided environment): grid entry, luminance bits of the light sample
+    const int* hits;      // fused loop, bounces >= 1: the queue slots that hit (segmented, k_shade_split)
+    const int* hit_seg;   //   and the segment lengths; k_shade then shades only those slots
     int* shadow_count;    // SHADOW_SEGS segment lengths
     int seg_cap;
     int bounce;
@@ -275,7 +277,13 @@ __global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
 {
     int slot;
     bool live;
-    if (A.in_seg) { // chunk-major over the segmented queue, as seg_entry
+    if (A.hits) { // the slots that hit, dense (chunk-major over their segments)
+        const int p = blockIdx.x % SHADOW_SEGS, off = (blockIdx.x / SHADOW_SEGS) * blockDim.x;
+        const int c = A.hit_seg[p];
+        if (off >= c) return;
+        live = off + (int)threadIdx.x < c;
+        slot = live ? A.hits[p * A.seg_cap + off + threadIdx.x] : 0;
+    } else if (A.in_seg) { // chunk-major over the segmented queue, as seg_entry
         const int p = blockIdx.x % SHADOW_SEGS, off = (blockIdx.x / SHADOW_SEGS) * blockDim.x;
         const int c = A.in_seg[p];
         if (off >= c) return;
@@ -289,6 +297,38 @@ __global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
         live = slot < count;
     }
     shade_slot(A, *A.seed, slot, live);
+}
+
+// Bounces >= 1 of the fused loop, before k_shade: escaped paths gather the
+// environment's radiance here (shade_escape, the shading of a miss) and the
+// slots that hit a triangle are appended, per work-group segment, to a dense
+// list that k_shade then shades.  Most paths escape after the first bounce
+// and the hits are scattered through the queue, so without the split nearly
+// every wave would run the whole hit shading for a few lanes; each path's
+// shading is the same code either way.
+__global__ __launch_bounds__(RT) void k_shade_split(ShadeArgs A, int* __restrict__ hits, int* __restrict__ hit_seg)
+{
+    const int p = blockIdx.x % SHADOW_SEGS, off = (blockIdx.x / SHADOW_SEGS) * blockDim.x;
+    const int c = A.in_seg[p];
+    if (off >= c) return;
+    const int slot = p * A.seg_cap + off + threadIdx.x;
+    const bool live = off + (int)threadIdx.x < c;
+    const int prim = live ? A.prim[slot] : -1;
+    int gidx = -1;
+    unsigned long long q = 0ull;
+    if (live && prim < 0) {
+        const int pid = A.queue[slot];
+        v3 Ladd;
+        shade_escape(A.V, A.bounce, A.o_tmin[slot], A.d_tmax[slot], A.S.T[pid], Ladd, gidx, q);
+        if (Ladd.x != 0.0f || Ladd.y != 0.0f || Ladd.z != 0.0f) {
+            const float4 L = A.S.L[pid];
+            A.S.L[pid] = make_float4(L.x + Ladd.x, L.y + Ladd.y, L.z + Ladd.z, L.w);
+        }
+    }
+    if (A.V.G.mode) lg_wave_accum(gidx >= 0, gidx, q, A.V.G.pend); // the whole wave is active here
+    const bool hit = live && prim >= 0;
+    const int k = p * A.seg_cap + wave_append(hit, hit_seg + p);
+    if (hit) hits[k] = slot;
 }
 
 // Unoccluded shadow-queue entries add their contribution to their path.
@@ -368,7 +408,7 @@ struct RenderPlan {
     char* base;
     size_t bytes;
     size_t oT, oL, oA_o, oA_d, oB_o, oB_d, oQA, oQB, oPrim, oT_, oAlive, oOcc, oSo, oSd, oSc, oCnt, oShc, oQsc,
-        oIlp, oTot, oSeed, oSlg;
+        oIlp, oTot, oSeed, oSlg, oHit, oHsc;
     mt_lightgrid* grid; // guided environment mode: the grid the render samples and accumulates into
     bool fused; // camera rays traced in place, shadow adds fused, next queue appended by the shading stage
     bool pool;  // fused kernels trace wave runs with lane refill (pool_trace)
@@ -447,6 +487,8 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
     R.oCnt = take(4 * 64);
     R.oShc = take(4 * 64 * (size_t)SHADOW_SEGS);
     R.oQsc = take(4 * 64 * (size_t)SHADOW_SEGS);
+    R.oHit = take(4 * R.sq);
+    R.oHsc = take(4 * 64 * (size_t)SHADOW_SEGS);
     R.want_ilp = (desc->flags & MT_RENDER_TRAVERSAL_STATS) != 0;
     R.fused = (desc->mode == MT_TRACE_PACKED || desc->mode == MT_TRACE_PACKED_STACK ||
                desc->mode == MT_TRACE_PACKED_SORTED || desc->mode == MT_TRACE_PACKED_POOL) && !R.want_ilp;
@@ -519,6 +561,8 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
     int* counts = I(R.oCnt);
     int* shc = I(R.oShc);
     int* qsc = I(R.oQsc); // row b: segment lengths of bounce b's queue (fused loop, b >= 1)
+    int* hits = I(R.oHit); // bounce b's hitting slots (fused loop, b >= 1), segment lengths in row b of hsc
+    int* hsc = I(R.oHsc);
     auto chk = [&](void) -> int32_t {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
@@ -548,6 +592,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
     k_zero_i32<<<1, 64, 0, st>>>(counts, 64);
     k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(shc, 64 * SHADOW_SEGS);
     k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(qsc, 64 * SHADOW_SEGS);
+    k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(hsc, 64 * SHADOW_SEGS);
     k_zero_i32<<<mt_blocks(16 * ILP_SLOTS, RT), RT, 0, st>>>(reinterpret_cast<int*>(tot), 16 * ILP_SLOTS);
     // Fused loop (default traversal, no traversal counters): bounce 0 traces
     // camera rays generated in place and shades with slot = pid; shadow rays
@@ -615,6 +660,15 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         A.rr_start = desc->rr_start;
         A.flags = desc->flags;
         A.seed = seed_ptr(R);
+        A.hits = nullptr;
+        A.hit_seg = nullptr;
+        if (A.in_seg) { // escaped paths here, the hits densely in k_shade
+            int* hseg = hsc + (size_t)b * SHADOW_SEGS;
+            k_shade_split<<<mt_blocks((int64_t)R.sq, RT), RT, 0, st>>>(A, hits, hseg);
+            if ((rc = chk())) break;
+            A.hits = hits;
+            A.hit_seg = hseg;
+        }
         k_shade<<<A.in_seg ? mt_blocks((int64_t)R.sq, RT) : gshade, RT, 0, st>>>(A);
         if ((rc = chk())) break;
         if (profile && (rc = mark(ev[4 * b + 2]))) break;
