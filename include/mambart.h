@@ -203,6 +203,9 @@ int32_t mt_scene_destroy(mt_scene* scene);
                                       direction-bin order (same results per ray) */
 #define MT_TRACE_PACKED_POOL 6     /* packed, short stack, each wave refills finished lanes
                                       from its own run of 256 rays (same results per ray) */
+#define MT_TRACE_PACKED_BINNED 7   /* packed: the batch counting-sorted by origin cell and
+                                      direction octant on the device, then traced with
+                                      refill (same results per ray; for incoherent batches) */
 
 /* Closest hit (intersectMesh with bailout = false, NormalBVH.cl:14-92, as
  * run by IntersectPrimitives :94-123; launched at CDeviceGI.java:230 and
@@ -226,9 +229,10 @@ int32_t mt_trace_any(mt_context* ctx, const mt_scene* scene,
  * traversal kernel).  Every packed mode returns the same bits, but their
  * speed depends on the batch: one ray per lane with the short stack wins on
  * coherent rays, lane refill (persistent waves or wave pools) on incoherent
- * ones.  Traces this batch (any = 0: closest hit into prim/t; any != 0: any
- * hit into occluded) with MT_TRACE_PACKED_STACK, MT_TRACE_PACKED_REFILL and
- * MT_TRACE_PACKED_POOL, `reps` timed runs each after one warm-up, times them
+ * ones, ray binning on large incoherent ones.  Traces this batch (any = 0:
+ * closest hit into prim/t; any != 0: any hit into occluded) with
+ * MT_TRACE_PACKED_STACK, _REFILL, _POOL and _BINNED, `reps` timed runs each
+ * after one warm-up, times them
  * with HIP events on the context stream and sets *best_mode to the fastest
  * and *best_ms to its mean time.  The outputs hold the batch's
  * result.  Synchronises the context. */

@@ -27,6 +27,7 @@ MT_TRACE_PACKED_STACK = 3
 MT_TRACE_PACKED_REFILL = 4
 MT_TRACE_PACKED_SORTED = 5
 MT_TRACE_PACKED_POOL = 6
+MT_TRACE_PACKED_BINNED = 7
 MT_BSDF_WORDS = 20
 
 

@@ -46,6 +46,7 @@ struct TraceArgs {
     int* stats;
     unsigned* ctr; // refill kernel: [0] next ray, [1] waves finished; zero between launches
     MtSegQueue seg; // segmented queue (seg.counts set) instead of *d_count
+    const int* out_index; // binned traversal: entry k's results go to out_index[k] (null: to k)
 };
 
 // Whether queue entry `id` holds a ray (plain queue: id < *d_count clamped
@@ -77,6 +78,7 @@ __device__ __forceinline__ int queue_entry(const TraceArgs& A)
 template <bool ANY, bool STATS>
 __device__ __forceinline__ void store_result(const TraceArgs& A, int id, const RayT& r, const TravState& S)
 {
+    if (A.out_index) id = A.out_index[id];
     if (ANY) {
         A.occluded[id] = S.hit;
     } else {
@@ -309,6 +311,99 @@ __global__ __launch_bounds__(256) void k_trace_jobs(const MtSceneRef* __restrict
     A.tout[id] = r.tmax;
 }
 
+// ---------------------------------------------------------------- binned traversal
+// Incoherent batches (random origins and directions) make neighbouring lanes
+// and waves fetch unrelated nodes, and a BVH larger than the last-level
+// cache then costs an HBM line per node visit.  MT_TRACE_PACKED_BINNED first
+// orders the batch by a 12-bit key, the origin's cell in an 8 x 8 x 8 grid
+// over the scene's root box (Morton order) and the direction's octant, with
+// a one-pass counting sort (per-tile histograms in LDS, an exclusive scan of
+// the bin-major tile x bin matrix, a scatter that writes the rays in key
+// order with their original index), then traces the sorted copy with the
+// refill kernel, each result stored at the ray's original index.  Rays that
+// start near each other heading the same way visit much the same nodes while
+// they are in flight together.  Per-ray results are those of any other mode.
+constexpr int BIN_BITS = 12;
+constexpr int BIN_COUNT = 1 << BIN_BITS;
+constexpr int BIN_TILE = 16384; // rays per work-group of the histogram and scatter passes (256 x 64)
+
+__device__ __forceinline__ int bin_axis(float o, float lo, float scale)
+{
+    const float f = (o - lo) * scale; // NaN: cell 0
+    return f >= 0.0f ? (f < 8.0f ? (int)f : 7) : 0;
+}
+
+__device__ __forceinline__ int bin_key(float4 o, float4 d, const float* __restrict__ box, float sx, float sy,
+                                       float sz)
+{
+    const int x = bin_axis(o.x, box[0], sx), y = bin_axis(o.y, box[1], sy), z = bin_axis(o.z, box[2], sz);
+    int m = 0;
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+        m |= (((x >> b) & 1) << (3 * b)) | (((y >> b) & 1) << (3 * b + 1)) | (((z >> b) & 1) << (3 * b + 2));
+    const int oct = (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0);
+    return (m << 3) | oct;
+}
+
+// The root box's per-axis scale 8 / extent (0 for a flat axis).
+__device__ __forceinline__ void bin_scales(const float* __restrict__ box, float& sx, float& sy, float& sz)
+{
+    const float ex = box[4] - box[0], ey = box[5] - box[1], ez = box[6] - box[2];
+    sx = ex > 0.0f ? 8.0f / ex : 0.0f;
+    sy = ey > 0.0f ? 8.0f / ey : 0.0f;
+    sz = ez > 0.0f ? 8.0f / ez : 0.0f;
+}
+
+// Pass 1: key of every live ray, and each tile's key histogram written
+// bin-major (hist[bin * ntiles + tile]) so one exclusive scan gives every
+// (tile, bin) its first output position.
+__global__ __launch_bounds__(256) void k_bin_hist(const float4* __restrict__ o_tmin, const float4* __restrict__ d_tmax,
+                                                  const int* __restrict__ d_count, int capacity,
+                                                  const float* __restrict__ box, uint16_t* __restrict__ keys,
+                                                  int* __restrict__ hist, int ntiles)
+{
+    __shared__ int h[BIN_COUNT];
+    for (int i = threadIdx.x; i < BIN_COUNT; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    int count = *d_count;
+    count = count < capacity ? count : capacity;
+    float sx, sy, sz;
+    bin_scales(box, sx, sy, sz);
+    const int base = blockIdx.x * BIN_TILE;
+    for (int i = base + threadIdx.x; i < base + BIN_TILE && i < count; i += blockDim.x) {
+        const int k = bin_key(o_tmin[i], d_tmax[i], box, sx, sy, sz);
+        keys[i] = (uint16_t)k;
+        atomicAdd(&h[k], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < BIN_COUNT; b += blockDim.x) hist[(size_t)b * ntiles + blockIdx.x] = h[b];
+}
+
+// Pass 2 (after the scan): each ray goes to its (tile, bin) range; rank
+// inside the range by LDS atomics (the order within a range is free: every
+// result is stored by original index).
+__global__ __launch_bounds__(256) void k_bin_scatter(const float4* __restrict__ o_tmin, const float4* __restrict__ d_tmax,
+                                                     const int* __restrict__ d_count, int capacity,
+                                                     const uint16_t* __restrict__ keys, const int* __restrict__ start,
+                                                     int ntiles, float4* __restrict__ so, float4* __restrict__ sd,
+                                                     int* __restrict__ sidx)
+{
+    __shared__ int off[BIN_COUNT];
+    for (int b = threadIdx.x; b < BIN_COUNT; b += blockDim.x) off[b] = start[(size_t)b * ntiles + blockIdx.x];
+    __syncthreads();
+    int count = *d_count;
+    count = count < capacity ? count : capacity;
+    const int base = blockIdx.x * BIN_TILE;
+    for (int i = base + threadIdx.x; i < base + BIN_TILE && i < count; i += blockDim.x) {
+        const int pos = atomicAdd(&off[keys[i]], 1);
+        so[pos] = o_tmin[i];
+        sd[pos] = d_tmax[i];
+        sidx[pos] = i;
+    }
+}
+
+__global__ void k_set_i32(int* p, int v) { *p = v; }
+
 template <bool ANY>
 int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
                      const float* d_tmax, const int32_t* d_count, int32_t capacity,
@@ -317,7 +412,7 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
 {
     MT_ARG_CHECK(ctx && sc && o_tmin && d_tmax && (d_count || seg.counts), "trace: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "trace: negative capacity");
-    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_POOL, "trace: unknown mode %d", mode);
+    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_BINNED, "trace: unknown mode %d", mode);
     if (ANY) MT_ARG_CHECK(occluded != nullptr, "trace_any: occluded is NULL");
     else MT_ARG_CHECK(prim && t, "trace_closest: prim/t NULL");
     if (seg.counts) {
@@ -355,8 +450,43 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
     // fetch more distinct nodes per load and it loses (camera rays 1.40 ->
     // 1.90 ms), so plain queues keep one ray per lane.
     if (mode == MT_TRACE_PACKED) mode = seg.counts ? MT_TRACE_PACKED_POOL : MT_TRACE_PACKED_STACK;
-    if ((mode == MT_TRACE_PACKED_SORTED || mode == MT_TRACE_PACKED_POOL) && stats)
+    if ((mode == MT_TRACE_PACKED_SORTED || mode == MT_TRACE_PACKED_POOL || mode == MT_TRACE_PACKED_BINNED) && stats)
         mode = MT_TRACE_PACKED_STACK; // counting: per-ray order only
+    if (mode == MT_TRACE_PACKED_BINNED && seg.counts) mode = MT_TRACE_PACKED_POOL; // the render's bounce queues
+    if (mode == MT_TRACE_PACKED_BINNED) {
+        // workspace: keys, tile x bin matrix and its scan, the sorted rays and their indices
+        const int64_t ntiles = (capacity + BIN_TILE - 1) / BIN_TILE;
+        const int64_t cells = ntiles * BIN_COUNT;
+        MT_ARG_CHECK(cells < (1ll << 31), "trace: %d rays too many for the binned mode", capacity);
+        auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        const size_t b_keys = up(2 * (size_t)capacity), b_hist = up(4 * (size_t)cells);
+        const size_t b_ray = up(16 * (size_t)capacity), b_idx = up(4 * (size_t)capacity);
+        void* w = nullptr;
+        int32_t rc = mt_ctx_temp(ctx, b_keys + 2 * b_hist + 2 * b_ray + b_idx + 256, &w);
+        if (rc) return rc;
+        char* p = reinterpret_cast<char*>(w);
+        uint16_t* keys = reinterpret_cast<uint16_t*>(p);
+        int* hist = reinterpret_cast<int*>(p + b_keys);
+        int* start = reinterpret_cast<int*>(p + b_keys + b_hist);
+        float4* so = reinterpret_cast<float4*>(p + b_keys + 2 * b_hist);
+        float4* sd = reinterpret_cast<float4*>(p + b_keys + 2 * b_hist + b_ray);
+        int* sidx = reinterpret_cast<int*>(p + b_keys + 2 * b_hist + 2 * b_ray);
+        int* nd = reinterpret_cast<int*>(p + b_keys + 2 * b_hist + 2 * b_ray + b_idx); // [0] cells, [1] total
+        const float* box = sc->desc.bounds; // the root node's box (CNormalBVH: node 0 owns bound 0)
+        k_set_i32<<<1, 1, 0, s>>>(nd, (int)cells);
+        k_bin_hist<<<(unsigned)ntiles, 256, 0, s>>>(A.o_tmin, A.d_tmax, A.d_count, capacity, box, keys, hist,
+                                                   (int)ntiles);
+        MT_LAUNCH_CHECK();
+        rc = mt_scan_excl_i32(ctx, hist, start, nd, (int)cells, nd + 1);
+        if (rc) return rc;
+        k_bin_scatter<<<(unsigned)ntiles, 256, 0, s>>>(A.o_tmin, A.d_tmax, A.d_count, capacity, keys, start,
+                                                      (int)ntiles, so, sd, sidx);
+        MT_LAUNCH_CHECK();
+        A.o_tmin = so;
+        A.d_tmax = sd;
+        A.out_index = sidx;
+        mode = MT_TRACE_PACKED_REFILL;
+    }
     if (mode == MT_TRACE_PACKED_POOL) {
         const int64_t per_group = 256 * (int64_t)POOL_RAYS;
         const int64_t groups = seg.counts ? (int64_t)seg.nseg * ((seg.seg_cap + per_group - 1) / per_group)
@@ -823,7 +953,8 @@ extern "C" int32_t mt_trace_autotune(mt_context* ctx, const mt_scene* scene, con
     MT_ARG_CHECK(ctx && scene && best_mode && best_ms, "mt_trace_autotune: NULL argument");
     MT_ARG_CHECK(reps >= 1 && reps <= 100, "mt_trace_autotune: reps %d not in [1,100]", reps);
     MT_ARG_CHECK(any ? occluded != nullptr : (prim && t), "mt_trace_autotune: NULL outputs");
-    const int32_t modes[3] = {MT_TRACE_PACKED_STACK, MT_TRACE_PACKED_REFILL, MT_TRACE_PACKED_POOL};
+    const int32_t modes[4] = {MT_TRACE_PACKED_STACK, MT_TRACE_PACKED_REFILL, MT_TRACE_PACKED_POOL,
+                              MT_TRACE_PACKED_BINNED};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MT_HIP_CHECK(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -929,7 +1060,7 @@ extern "C" int32_t mt_intersect_primitives(mt_context* ctx, const mt_scene* sc, 
 {
     MT_ARG_CHECK(ctx && sc && rays && isects && d_count, "mt_intersect_primitives: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "mt_intersect_primitives: negative capacity");
-    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_POOL,
+    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_BINNED,
                  "mt_intersect_primitives: unknown mode %d", mode);
     MT_ARG_CHECK((reinterpret_cast<uintptr_t>(rays) & 15) == 0 && (reinterpret_cast<uintptr_t>(isects) & 15) == 0,
                  "mt_intersect_primitives: rays and isects must be 16-byte aligned");
@@ -963,7 +1094,7 @@ extern "C" int32_t mt_intersect_occlusion(mt_context* ctx, const mt_scene* sc, c
 {
     MT_ARG_CHECK(ctx && sc && rays && hits && d_count, "mt_intersect_occlusion: NULL argument");
     MT_ARG_CHECK(capacity >= 0, "mt_intersect_occlusion: negative capacity");
-    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_POOL,
+    MT_ARG_CHECK(mode >= MT_TRACE_PACKED && mode <= MT_TRACE_PACKED_BINNED,
                  "mt_intersect_occlusion: unknown mode %d", mode);
     MT_ARG_CHECK((reinterpret_cast<uintptr_t>(rays) & 15) == 0, "mt_intersect_occlusion: rays must be 16-byte aligned");
     if (capacity == 0) return MT_OK;

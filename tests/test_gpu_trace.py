@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 MODES = [api._lib.MT_TRACE_PACKED, api._lib.MT_TRACE_REFERENCE, api._lib.MT_TRACE_PACKED_NO_STACK,
          api._lib.MT_TRACE_PACKED_STACK, api._lib.MT_TRACE_PACKED_REFILL, api._lib.MT_TRACE_PACKED_SORTED,
-         api._lib.MT_TRACE_PACKED_POOL]
+         api._lib.MT_TRACE_PACKED_POOL, api._lib.MT_TRACE_PACKED_BINNED]
 
 
 def _scene(ctx, arrs, normals=None):
@@ -274,14 +274,16 @@ def test_records_edge_cases(ctx):
             np.testing.assert_array_equal(hits.cpu().numpy(), [0, 0, 0])
 
 
-@pytest.mark.parametrize("mode", [api._lib.MT_TRACE_PACKED_SORTED, api._lib.MT_TRACE_PACKED_POOL])
+@pytest.mark.parametrize("mode", [api._lib.MT_TRACE_PACKED_SORTED, api._lib.MT_TRACE_PACKED_POOL,
+                                  api._lib.MT_TRACE_PACKED_BINNED])
 @pytest.mark.parametrize("any_hit", [False, True])
 def test_lane_mappings_same_results(ctx, c0, any_hit, mode):
     """MT_TRACE_PACKED_SORTED (each work-group's rays taken in direction-bin
-    order) and MT_TRACE_PACKED_POOL (each wave refilling its finished lanes
-    from its own run of rays) return exactly the per-ray results of one ray
-    per lane, on camera rays and on random rays with a live length that ends
-    mid-group and mid-run."""
+    order), MT_TRACE_PACKED_POOL (each wave refilling its finished lanes
+    from its own run of rays) and MT_TRACE_PACKED_BINNED (the batch sorted by
+    origin cell and direction octant first) return exactly the per-ray
+    results of one ray per lane, on camera rays and on random rays with a
+    live length that ends mid-group and mid-run."""
     print(f"[case] lane mapping mode={mode} any={any_hit}", flush=True)
     points, faces, nodes, bounds, _ = c0
     sc = _scene(ctx, c0)
