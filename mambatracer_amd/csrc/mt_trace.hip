@@ -46,7 +46,7 @@ struct TraceArgs {
     int* stats;
     unsigned* ctr; // refill kernel: [0] next ray, [1] waves finished; zero between launches
     MtSegQueue seg; // segmented queue (seg.counts set) instead of *d_count
-    const int* out_index; // binned traversal: entry k's results go to out_index[k] (null: to k)
+    const int* order; // binned traversal: entry k is ray order[k], its results go there (null: ray k)
 };
 
 // Whether queue entry `id` holds a ray (plain queue: id < *d_count clamped
@@ -78,7 +78,7 @@ __device__ __forceinline__ int queue_entry(const TraceArgs& A)
 template <bool ANY, bool STATS>
 __device__ __forceinline__ void store_result(const TraceArgs& A, int id, const RayT& r, const TravState& S)
 {
-    if (A.out_index) id = A.out_index[id];
+    if (A.order) id = A.order[id];
     if (ANY) {
         A.occluded[id] = S.hit;
     } else {
@@ -139,7 +139,8 @@ __global__ __launch_bounds__(256) void k_trace_refill(TraceArgs A)
                     int rk = (int)lane_rank(want);
                     if (rk < take) {
                         id = cnext + rk;
-                        ray_init(r, A.o_tmin[id], A.d_tmax[id]);
+                        const int src = A.order ? A.order[id] : id;
+                        ray_init(r, A.o_tmin[src], A.d_tmax[src]);
                         S.init(A.root);
                         active = true;
                         got = true;
@@ -315,22 +316,23 @@ __global__ __launch_bounds__(256) void k_trace_jobs(const MtSceneRef* __restrict
 // Incoherent batches (random origins and directions) make neighbouring lanes
 // and waves fetch unrelated nodes, and a BVH larger than the last-level
 // cache then costs an HBM line per node visit.  MT_TRACE_PACKED_BINNED first
-// orders the batch by a 12-bit key, the origin's cell in an 8 x 8 x 8 grid
-// over the scene's root box (Morton order) and the direction's octant, with
-// a one-pass counting sort (per-tile histograms in LDS, an exclusive scan of
-// the bin-major tile x bin matrix, a scatter that writes the rays in key
-// order with their original index), then traces the sorted copy with the
-// refill kernel, each result stored at the ray's original index.  Rays that
+// orders the batch by a key made of the origin's cell in a grid over the
+// scene's root box (Morton order) and the direction's octant, with a
+// one-pass counting sort (per-tile histograms in LDS, an exclusive scan of
+// the bin-major tile x bin matrix, a scatter of the ray indices in key
+// order), then traces the rays in that order with the refill kernel, each
+// result stored at the ray's own index.  Rays that
 // start near each other heading the same way visit much the same nodes while
 // they are in flight together.  Per-ray results are those of any other mode.
-constexpr int BIN_BITS = 12;
-constexpr int BIN_COUNT = 1 << BIN_BITS;
+constexpr int BIN_CELL_BITS = 3;                    // cells per axis: 2^BIN_CELL_BITS
+constexpr int BIN_CELLS = 1 << BIN_CELL_BITS;
+constexpr int BIN_COUNT = 1 << (3 * BIN_CELL_BITS + 3); // cells x octants
 constexpr int BIN_TILE = 16384; // rays per work-group of the histogram and scatter passes (256 x 64)
 
 __device__ __forceinline__ int bin_axis(float o, float lo, float scale)
 {
     const float f = (o - lo) * scale; // NaN: cell 0
-    return f >= 0.0f ? (f < 8.0f ? (int)f : 7) : 0;
+    return f >= 0.0f ? (f < (float)BIN_CELLS ? (int)f : BIN_CELLS - 1) : 0;
 }
 
 __device__ __forceinline__ int bin_key(float4 o, float4 d, const float* __restrict__ box, float sx, float sy,
@@ -339,19 +341,19 @@ __device__ __forceinline__ int bin_key(float4 o, float4 d, const float* __restri
     const int x = bin_axis(o.x, box[0], sx), y = bin_axis(o.y, box[1], sy), z = bin_axis(o.z, box[2], sz);
     int m = 0;
 #pragma unroll
-    for (int b = 0; b < 3; b++)
+    for (int b = 0; b < BIN_CELL_BITS; b++)
         m |= (((x >> b) & 1) << (3 * b)) | (((y >> b) & 1) << (3 * b + 1)) | (((z >> b) & 1) << (3 * b + 2));
     const int oct = (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0);
     return (m << 3) | oct;
 }
 
-// The root box's per-axis scale 8 / extent (0 for a flat axis).
+// The root box's per-axis scale cells / extent (0 for a flat axis).
 __device__ __forceinline__ void bin_scales(const float* __restrict__ box, float& sx, float& sy, float& sz)
 {
     const float ex = box[4] - box[0], ey = box[5] - box[1], ez = box[6] - box[2];
-    sx = ex > 0.0f ? 8.0f / ex : 0.0f;
-    sy = ey > 0.0f ? 8.0f / ey : 0.0f;
-    sz = ez > 0.0f ? 8.0f / ez : 0.0f;
+    sx = ex > 0.0f ? (float)BIN_CELLS / ex : 0.0f;
+    sy = ey > 0.0f ? (float)BIN_CELLS / ey : 0.0f;
+    sz = ez > 0.0f ? (float)BIN_CELLS / ez : 0.0f;
 }
 
 // Pass 1: key of every live ray, and each tile's key histogram written
@@ -379,14 +381,14 @@ __global__ __launch_bounds__(256) void k_bin_hist(const float4* __restrict__ o_t
     for (int b = threadIdx.x; b < BIN_COUNT; b += blockDim.x) hist[(size_t)b * ntiles + blockIdx.x] = h[b];
 }
 
-// Pass 2 (after the scan): each ray goes to its (tile, bin) range; rank
-// inside the range by LDS atomics (the order within a range is free: every
-// result is stored by original index).
-__global__ __launch_bounds__(256) void k_bin_scatter(const float4* __restrict__ o_tmin, const float4* __restrict__ d_tmax,
-                                                     const int* __restrict__ d_count, int capacity,
+// Pass 2 (after the scan): each ray's index goes to its (tile, bin) range;
+// rank inside the range by LDS atomics (the order within a range is free:
+// every result is stored by original index).  The rays themselves stay where
+// they are: the traversal reads each through the order, two 16-B loads
+// against the kilobytes of nodes it then fetches.
+__global__ __launch_bounds__(256) void k_bin_scatter(const int* __restrict__ d_count, int capacity,
                                                      const uint16_t* __restrict__ keys, const int* __restrict__ start,
-                                                     int ntiles, float4* __restrict__ so, float4* __restrict__ sd,
-                                                     int* __restrict__ sidx)
+                                                     int ntiles, int* __restrict__ order)
 {
     __shared__ int off[BIN_COUNT];
     for (int b = threadIdx.x; b < BIN_COUNT; b += blockDim.x) off[b] = start[(size_t)b * ntiles + blockIdx.x];
@@ -395,10 +397,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const float4* __restrict__ 
     count = count < capacity ? count : capacity;
     const int base = blockIdx.x * BIN_TILE;
     for (int i = base + threadIdx.x; i < base + BIN_TILE && i < count; i += blockDim.x) {
-        const int pos = atomicAdd(&off[keys[i]], 1);
-        so[pos] = o_tmin[i];
-        sd[pos] = d_tmax[i];
-        sidx[pos] = i;
+        order[atomicAdd(&off[keys[i]], 1)] = i;
     }
 }
 
@@ -454,24 +453,22 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
         mode = MT_TRACE_PACKED_STACK; // counting: per-ray order only
     if (mode == MT_TRACE_PACKED_BINNED && seg.counts) mode = MT_TRACE_PACKED_POOL; // the render's bounce queues
     if (mode == MT_TRACE_PACKED_BINNED) {
-        // workspace: keys, tile x bin matrix and its scan, the sorted rays and their indices
+        // workspace: keys, tile x bin matrix and its scan, the order
         const int64_t ntiles = (capacity + BIN_TILE - 1) / BIN_TILE;
         const int64_t cells = ntiles * BIN_COUNT;
         MT_ARG_CHECK(cells < (1ll << 31), "trace: %d rays too many for the binned mode", capacity);
         auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
         const size_t b_keys = up(2 * (size_t)capacity), b_hist = up(4 * (size_t)cells);
-        const size_t b_ray = up(16 * (size_t)capacity), b_idx = up(4 * (size_t)capacity);
+        const size_t b_idx = up(4 * (size_t)capacity);
         void* w = nullptr;
-        int32_t rc = mt_ctx_temp(ctx, b_keys + 2 * b_hist + 2 * b_ray + b_idx + 256, &w);
+        int32_t rc = mt_ctx_temp(ctx, b_keys + 2 * b_hist + b_idx + 256, &w);
         if (rc) return rc;
         char* p = reinterpret_cast<char*>(w);
         uint16_t* keys = reinterpret_cast<uint16_t*>(p);
         int* hist = reinterpret_cast<int*>(p + b_keys);
         int* start = reinterpret_cast<int*>(p + b_keys + b_hist);
-        float4* so = reinterpret_cast<float4*>(p + b_keys + 2 * b_hist);
-        float4* sd = reinterpret_cast<float4*>(p + b_keys + 2 * b_hist + b_ray);
-        int* sidx = reinterpret_cast<int*>(p + b_keys + 2 * b_hist + 2 * b_ray);
-        int* nd = reinterpret_cast<int*>(p + b_keys + 2 * b_hist + 2 * b_ray + b_idx); // [0] cells, [1] total
+        int* order = reinterpret_cast<int*>(p + b_keys + 2 * b_hist);
+        int* nd = reinterpret_cast<int*>(p + b_keys + 2 * b_hist + b_idx); // [0] cells, [1] total
         const float* box = sc->desc.bounds; // the root node's box (CNormalBVH: node 0 owns bound 0)
         k_set_i32<<<1, 1, 0, s>>>(nd, (int)cells);
         k_bin_hist<<<(unsigned)ntiles, 256, 0, s>>>(A.o_tmin, A.d_tmax, A.d_count, capacity, box, keys, hist,
@@ -479,12 +476,9 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
         MT_LAUNCH_CHECK();
         rc = mt_scan_excl_i32(ctx, hist, start, nd, (int)cells, nd + 1);
         if (rc) return rc;
-        k_bin_scatter<<<(unsigned)ntiles, 256, 0, s>>>(A.o_tmin, A.d_tmax, A.d_count, capacity, keys, start,
-                                                      (int)ntiles, so, sd, sidx);
+        k_bin_scatter<<<(unsigned)ntiles, 256, 0, s>>>(A.d_count, capacity, keys, start, (int)ntiles, order);
         MT_LAUNCH_CHECK();
-        A.o_tmin = so;
-        A.d_tmax = sd;
-        A.out_index = sidx;
+        A.order = order;
         mode = MT_TRACE_PACKED_REFILL;
     }
     if (mode == MT_TRACE_PACKED_POOL) {
