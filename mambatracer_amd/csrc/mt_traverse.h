@@ -369,11 +369,15 @@ __device__ __forceinline__ bool trace_packed_node(const float4* __restrict__ pn,
     return !parent_walk<STATS>(pn, pt, S.cur, S.bs, S.st);
 }
 
+// Idle lanes a pool waits for before starting new rays.
+constexpr int POOL_REFILL_MIN = 16;
+
 // Wave-pool traversal of the rays of entries [next, end) (wave-uniform): the
 // wave starts one ray per lane, every iteration takes each lane one node
-// further on its ray (trace_packed_node), and lanes whose ray has ended take
-// the run's next entries in lane order (ballot + mbcnt on a wave-uniform
-// cursor; no atomics), so lanes stay busy until the run is drained: a
+// further on its ray (trace_packed_node), and once POOL_REFILL_MIN lanes'
+// rays have ended those lanes take the run's next entries in lane order
+// (ballot + mbcnt on a wave-uniform cursor; no atomics), so lanes stay busy
+// until the run is drained: a
 // wave's tail, where its longest rays run while the other lanes idle, comes
 // once per run instead of once per 64 rays.  load(id, r) initialises entry
 // id's ray; done(fin, id, r, S) is called after every iteration with the
@@ -389,7 +393,10 @@ __device__ __forceinline__ void pool_trace(const float4* __restrict__ pn, const 
     TravState S;
     for (;;) {
         const uint64_t idle = __ballot(!active);
-        if (idle && next < end) {
+        // refill once POOL_REFILL_MIN lanes wait (or all do): starting a ray
+        // (two loads, three reciprocals) for one or two lanes at a time would
+        // cost the whole wave that block nearly every iteration
+        if (idle && next < end && (idle == ~0ull || __popcll(idle) >= POOL_REFILL_MIN)) {
             const int avail = end - next, nidle = (int)__popcll(idle);
             const int n = avail < nidle ? avail : nidle;
             const int rk = (int)lane_rank(idle);
