@@ -679,6 +679,15 @@ int32_t mt_batch_set_policy(mt_batch* batch, int32_t policy);
  * iterations between host round trips; a job that finishes early in a step
  * is resolved and reported at the end of the step. */
 int32_t mt_batch_set_iterations(mt_batch* batch, int32_t iterations);
+/* No counterpart in the reference (one job per device, one launch per
+ * kernel): on = 1 (default) records a step's iterations once as a hipGraph
+ * per start parity of the queue ping-pong and replays it every step (one
+ * graph launch instead of about ten kernel launches per iteration; every
+ * size the iterations use lives in device memory, so the recording stays
+ * valid while jobs come and go, until the iterations per step change);
+ * on = 0 launches the kernels one by one.  Images do not depend on it.
+ * Synchronises the context. */
+int32_t mt_batch_set_graphs(mt_batch* batch, int32_t on);
 /* Cancels the job with this tag (CDeviceGI.stop, CDeviceGI.java:438-439,
  * ends the reference's one running job; here any one job of the mix).
  * A job not yet admitted is dropped at once; an admitted job is issued no

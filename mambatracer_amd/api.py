@@ -942,6 +942,11 @@ class Batch:
         """Wavefront iterations per step (mt_batch_set_iterations)."""
         call("mt_batch_set_iterations", self.handle, int(iterations))
 
+    def set_graphs(self, on: bool) -> None:
+        """Replay each step's iterations as a captured hipGraph (default) or
+        launch their kernels one by one (mt_batch_set_graphs)."""
+        call("mt_batch_set_graphs", self.handle, 1 if on else 0)
+
     def close(self) -> None:
         if self.handle:
             _lib.load().mt_batch_destroy(self.handle)

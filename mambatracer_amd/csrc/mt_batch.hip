@@ -95,13 +95,13 @@ __global__ void k_cancel_job(int* npaths, const int* issued, int* cancel, int sl
 
 // done_tot[j] = sum of job j's DONE_SEGS rows, for the active jobs, and
 // fin[j] = 1 once every path the job will ever be issued has ended.
-__global__ __launch_bounds__(BT) void k_done_sum(const int* __restrict__ active, int n_active,
+__global__ __launch_bounds__(BT) void k_done_sum(const int* __restrict__ active, const int* __restrict__ nact,
                                                  const int* __restrict__ done, int max_jobs,
                                                  const int* __restrict__ npaths, int* __restrict__ done_tot,
                                                  int* __restrict__ fin)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_active) return;
+    if (i >= *nact) return;
     const int j = active[i];
     int s = 0;
     for (int r = 0; r < DONE_SEGS; r++) s += done[(size_t)r * max_jobs + j];
@@ -118,13 +118,14 @@ __global__ void k_zero_counts(int* cnt, unsigned long long* tot)
 // One wave.  cnt[0] = live slots after the last compaction; writes
 // cnt[1] = live + new.  gen_start[i] = first new-slot offset of active job
 // i, gen_pid0[i] = its first new path id.
-__global__ __launch_bounds__(64) void k_refill(const int* __restrict__ active, int n_active,
+__global__ __launch_bounds__(64) void k_refill(const int* __restrict__ active, const int* __restrict__ nact,
                                                const int* __restrict__ npaths, int* __restrict__ issued,
                                                int capacity, int* __restrict__ cnt, int* __restrict__ gen_start,
                                                int* __restrict__ gen_pid0, int* __restrict__ gen_job,
                                                int* __restrict__ s_seg, unsigned long long* __restrict__ tot)
 {
     const int lane = threadIdx.x;
+    const int n_active = *nact;
     for (int p = lane; p < JOB_SEGS; p += 64) s_seg[p] = 0; // shadow queue of this iteration
     const int live = cnt[0];
     const int64_t avail = (int64_t)capacity - live;
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(64) void k_refill(const int* __restrict__ active, i
 __global__ __launch_bounds__(BT) void k_generate(const JobRec* __restrict__ jobs, const int* __restrict__ cnt,
                                                  const int* __restrict__ gen_start,
                                                  const int* __restrict__ gen_pid0, const int* __restrict__ gen_job,
-                                                 int n_active, Queue Q)
+                                                 const int* __restrict__ nact, Queue Q)
 {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int live = cnt[0], total = cnt[1];
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(BT) void k_generate(const JobRec* __restrict__ jobs
     const int off = slot - live;
     // last i with gen_start[i] <= off (entries are non-decreasing; a job
     // given no slots shares its start with the next job, which wins)
-    int lo = 0, hi = n_active - 1;
+    int lo = 0, hi = *nact - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         if (gen_start[mid] <= off) lo = mid;
@@ -342,6 +343,16 @@ struct mt_batch {
     bool active_dirty;
     int policy;
     int iters_per_step;       // wavefront iterations between host round trips
+    int* nact;                // device: number of active jobs (read by the iteration kernels)
+    // The iterations of a step as one hipGraph per start parity of the queue
+    // ping-pong: every size they use lives in device memory, so a capture
+    // stays valid until the iterations per step change.
+    bool use_graphs;
+    hipGraph_t graph[2];
+    hipGraphExec_t gexec[2];
+    int g_iters[2];
+    void* gscratch;           // the captured compactions' look-back workspace
+    size_t gscratch_bytes;
     mt_batch_stats st;
 };
 
@@ -439,7 +450,11 @@ int32_t admit(mt_batch* b)
     return MT_OK;
 }
 
-int32_t iterate(mt_batch* b)
+__global__ void k_set_nact(int* nact, int n) { *nact = n; }
+
+// The active jobs' slots in hand-out order, and their count, to the device
+// (stream-ordered; before the iterations that read them).
+int32_t sync_active(mt_batch* b)
 {
     hipStream_t s = b->ctx->stream;
     const int n_active = (int)b->active_jobs.size();
@@ -455,15 +470,25 @@ int32_t iterate(mt_batch* b)
         for (int i = 0; i < n_active; i++) b->h_active[i] = b->active_jobs[order[i]].slot;
         MT_HIP_CHECK(hipMemcpyAsync(b->active, b->h_active, sizeof(int) * (size_t)std::max(n_active, 1),
                                     hipMemcpyHostToDevice, s));
+        k_set_nact<<<1, 1, 0, s>>>(b->nact, n_active);
+        MT_LAUNCH_CHECK();
         b->active_dirty = false;
     }
+    return MT_OK;
+}
+
+// One wavefront iteration: launches only, every size from device memory
+// (capturable).
+int32_t iterate(mt_batch* b)
+{
+    hipStream_t s = b->ctx->stream;
     const int Qc = b->capacity;
     const int grid = mt_blocks(Qc, BT);
     Queue& q = b->q[b->cur];
     Queue& nq = b->q[1 - b->cur];
-    k_refill<<<1, 64, 0, s>>>(b->active, n_active, b->npaths, b->issued, Qc, b->cnt, b->gen_start, b->gen_pid0,
+    k_refill<<<1, 64, 0, s>>>(b->active, b->nact, b->npaths, b->issued, Qc, b->cnt, b->gen_start, b->gen_pid0,
                               b->gen_job, b->s_seg, b->tot);
-    k_generate<<<grid, BT, 0, s>>>(b->jobs, b->cnt, b->gen_start, b->gen_pid0, b->gen_job, n_active, q);
+    k_generate<<<grid, BT, 0, s>>>(b->jobs, b->cnt, b->gen_start, b->gen_pid0, b->gen_job, b->nact, q);
     MT_LAUNCH_CHECK();
     int32_t rc = mt_trace_jobs(b->ctx, b->refs, &q.tag->x, BOUNCE_BITS, q.ro, q.rd, b->cnt + 1, Qc, b->prim,
                                b->t);
@@ -492,11 +517,111 @@ int32_t iterate(mt_batch* b)
     mt_payload P[4] = {{q.tag, nq.tag, 8}, {q.ro, nq.ro, 16}, {q.rd, nq.rd, 16}, {q.T, nq.T, 16}};
     rc = mt_compact(b->ctx, b->alive, 0, b->cnt + 1, Qc, nullptr, b->cnt, P, 4);
     if (rc) return rc;
-    k_done_sum<<<mt_blocks(n_active, BT), BT, 0, s>>>(b->active, n_active, b->done, b->max_jobs, b->npaths,
+    k_done_sum<<<mt_blocks(b->max_jobs, BT), BT, 0, s>>>(b->active, b->nact, b->done, b->max_jobs, b->npaths,
                                                       b->done_tot, b->fin);
     MT_LAUNCH_CHECK();
     b->cur = 1 - b->cur;
-    b->st.iterations++;
+    return MT_OK;
+}
+
+void drop_graphs(mt_batch* b)
+{
+    for (int p = 0; p < 2; p++) {
+        if (b->gexec[p]) (void)hipGraphExecDestroy(b->gexec[p]);
+        if (b->graph[p]) (void)hipGraphDestroy(b->graph[p]);
+        b->gexec[p] = nullptr;
+        b->graph[p] = nullptr;
+        b->g_iters[p] = 0;
+    }
+}
+
+// Records iters_per_step iterations starting at parity p into graph[p].
+// Captured on a stream of its own (the context's may be the null stream)
+// with a look-back workspace of the batch's own (the context's may be
+// regrown and freed later, while the graph keeps addresses).
+int32_t capture_iterations(mt_batch* b, int p)
+{
+    mt_context* ctx = b->ctx;
+    if (b->gexec[p]) (void)hipGraphExecDestroy(b->gexec[p]);
+    if (b->graph[p]) (void)hipGraphDestroy(b->graph[p]);
+    b->gexec[p] = nullptr;
+    b->graph[p] = nullptr;
+    b->g_iters[p] = 0;
+    void* ctx_scratch = ctx->scratch;
+    const size_t ctx_scratch_bytes = ctx->scratch_bytes;
+    const hipStream_t ctx_stream = ctx->stream;
+    int32_t rc = MT_OK;
+    if (!b->gscratch) {
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+        rc = mt_scan_reserve(ctx, b->capacity); // allocated and cleared on the context stream
+        b->gscratch = ctx->scratch;
+        b->gscratch_bytes = ctx->scratch_bytes;
+        ctx->scratch = ctx_scratch;
+        ctx->scratch_bytes = ctx_scratch_bytes;
+        if (rc) return rc;
+    }
+    hipStream_t cs = nullptr;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+        mt_set_error("mt_batch_step: hipStreamCreate failed");
+        return MT_ERR_HIP;
+    }
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        mt_set_error("mt_batch_step: hipStreamBeginCapture failed: %s", hipGetErrorString(e));
+        return MT_ERR_HIP;
+    }
+    ctx->stream = cs;
+    ctx->scratch = b->gscratch;
+    ctx->scratch_bytes = b->gscratch_bytes;
+    for (int it = 0; it < b->iters_per_step && rc == MT_OK; it++) rc = iterate(b);
+    ctx->stream = ctx_stream;
+    ctx->scratch = ctx_scratch;
+    ctx->scratch_bytes = ctx_scratch_bytes;
+    b->cur = p; // nothing ran: the queues are where they were
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (rc == MT_OK && e != hipSuccess) {
+        mt_set_error("mt_batch_step: hipStreamEndCapture failed: %s", hipGetErrorString(e));
+        rc = MT_ERR_HIP;
+    }
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    b->graph[p] = g;
+    e = hipGraphInstantiate(&b->gexec[p], g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        b->gexec[p] = nullptr;
+        mt_set_error("mt_batch_step: hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        return MT_ERR_HIP;
+    }
+    b->g_iters[p] = b->iters_per_step;
+    return MT_OK;
+}
+
+// The step's iterations: one launch of the captured graph, or the kernels.
+int32_t run_iterations(mt_batch* b)
+{
+    int32_t rc = sync_active(b);
+    if (rc) return rc;
+    if (b->use_graphs) {
+        const int p = b->cur;
+        if (!b->gexec[p] || b->g_iters[p] != b->iters_per_step) {
+            rc = capture_iterations(b, p);
+            if (rc) return rc;
+        }
+        MT_HIP_CHECK(hipGraphLaunch(b->gexec[p], b->ctx->stream));
+        b->cur = p ^ (b->iters_per_step & 1);
+    } else {
+        for (int it = 0; it < b->iters_per_step; it++) {
+            rc = iterate(b);
+            if (rc) return rc;
+        }
+    }
+    b->st.iterations += b->iters_per_step;
     return MT_OK;
 }
 
@@ -535,7 +660,7 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     size_t oNp = take(4 * J), oIss = take(4 * J), oDone = take(4 * J * DONE_SEGS), oDtot = take(4 * J);
     size_t oCan = take(4 * J), oFin = take(4 * J);
     size_t oAct = take(4 * J), oGs = take(4 * J), oGp = take(4 * J), oGj = take(4 * J);
-    size_t oCnt = take(16), oTot = take(16), oSeg = take(4 * JOB_SEGS);
+    size_t oCnt = take(16), oTot = take(16), oSeg = take(4 * JOB_SEGS), oNact = take(16);
     b->seg_cap = shadow_seg_cap<JOB_SEGS>(queue_capacity, BT);
     const size_t sq = (size_t)JOB_SEGS * b->seg_cap;
     size_t oQ[2][4];
@@ -581,6 +706,7 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     b->gen_pid0 = reinterpret_cast<int*>(m + oGp);
     b->gen_job = reinterpret_cast<int*>(m + oGj);
     b->cnt = reinterpret_cast<int*>(m + oCnt);
+    b->nact = reinterpret_cast<int*>(m + oNact);
     b->tot = reinterpret_cast<unsigned long long*>(m + oTot);
     for (int k = 0; k < 2; k++) {
         b->q[k].tag = reinterpret_cast<int2*>(m + oQ[k][0]);
@@ -600,6 +726,14 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     b->st = mt_batch_stats{};
     b->policy = MT_BATCH_OLDEST_FIRST;
     b->iters_per_step = 1;
+    b->use_graphs = true;
+    for (int p = 0; p < 2; p++) {
+        b->graph[p] = nullptr;
+        b->gexec[p] = nullptr;
+        b->g_iters[p] = 0;
+    }
+    b->gscratch = nullptr;
+    b->gscratch_bytes = 0;
     for (int k = max_jobs - 1; k >= 0; k--) b->free_slots.push_back(k);
     b->free_blocks[0] = b->arena_bytes;
     k_zero_counts<<<1, 64, 0, ctx->stream>>>(b->cnt, b->tot);
@@ -620,6 +754,8 @@ extern "C" int32_t mt_batch_destroy(mt_batch* b)
     fe(hipHostFree(b->h_active));
     fe(hipHostFree(b->h_done));
     fe(hipHostFree(b->h_tot));
+    drop_graphs(b);
+    if (b->gscratch) fe(hipFree(b->gscratch));
     fe(hipFree(b->arena));
     fe(hipFree(b->mem));
     delete b;
@@ -658,10 +794,8 @@ extern "C" int32_t mt_batch_step(mt_batch* b, int32_t max_done, int64_t* done_ta
     if (!b->active_jobs.empty()) {
         // several iterations back to back: finished jobs are detected on the
         // device (k_done_sum's flags) and read back once per step
-        for (int it = 0; it < b->iters_per_step; it++) {
-            rc = iterate(b);
-            if (rc) return rc;
-        }
+        rc = run_iterations(b);
+        if (rc) return rc;
         MT_HIP_CHECK(hipMemcpyAsync(b->h_done, b->fin, 4 * (size_t)b->max_jobs, hipMemcpyDeviceToHost, s));
         MT_HIP_CHECK(hipMemcpyAsync(b->h_tot, b->tot, 16, hipMemcpyDeviceToHost, s));
         MT_HIP_CHECK(hipStreamSynchronize(s));
@@ -695,6 +829,16 @@ extern "C" int32_t mt_batch_step(mt_batch* b, int32_t max_done, int64_t* done_ta
         }
         if (any) MT_HIP_CHECK(hipStreamSynchronize(s));
         rc = mt_scan_check(b->ctx);
+        if (rc == MT_OK && b->gscratch) { // the captured compactions' workspace
+            mt_context* c = b->ctx;
+            void* sv = c->scratch;
+            const size_t svb = c->scratch_bytes;
+            c->scratch = b->gscratch;
+            c->scratch_bytes = b->gscratch_bytes;
+            rc = mt_scan_check(c);
+            c->scratch = sv;
+            c->scratch_bytes = svb;
+        }
         if (rc) return rc;
     }
     int nd = 0;
@@ -736,6 +880,15 @@ extern "C" int32_t mt_batch_set_iterations(mt_batch* b, int32_t iterations)
     MT_ARG_CHECK(b, "mt_batch_set_iterations: NULL batch");
     MT_ARG_CHECK(iterations >= 1 && iterations <= 1024, "mt_batch_set_iterations: %d not in [1,1024]", iterations);
     b->iters_per_step = iterations;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_batch_set_graphs(mt_batch* b, int32_t on)
+{
+    MT_ARG_CHECK(b, "mt_batch_set_graphs: NULL batch");
+    MT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream)); // a graph being dropped may still run
+    b->use_graphs = on != 0;
+    if (!b->use_graphs) drop_graphs(b);
     return MT_OK;
 }
 

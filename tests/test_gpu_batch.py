@@ -232,13 +232,17 @@ def test_cancel_pending_and_running_jobs(ctx, world):
     np.testing.assert_array_equal(limg.cpu().numpy(), _plain(ctx, world, late_sc, late))
 
 
-@pytest.mark.parametrize("iters", [3, 16])
-def test_iterations_per_step(ctx, world, iters):
-    """Several wavefront iterations per host round trip: same images."""
-    print(f"[case] {iters} iterations per step", flush=True)
+@pytest.mark.parametrize("graphs", [True, False])
+@pytest.mark.parametrize("iters", [1, 3, 16])
+def test_iterations_per_step(ctx, world, iters, graphs):
+    """Several wavefront iterations per host round trip, replayed as a
+    captured hipGraph (one per start parity of the queue ping-pong: an odd
+    count alternates them) or launched kernel by kernel: same images."""
+    print(f"[case] {iters} iterations per step, graphs={graphs}", flush=True)
     scenes, dm, env, (lf, nl) = world
     jobs = list(enumerate(_jobs()))
     b = api.Batch(ctx, queue_capacity=1 << 14, arena_bytes=256 << 20, max_jobs=64)
+    b.set_graphs(graphs)
     b.set_iterations(iters)
     imgs = {}
     for tag, (sc, job) in jobs:
@@ -248,5 +252,30 @@ def test_iterations_per_step(ctx, world, iters):
     order = b.run()
     b.close()
     assert sorted(order) == list(range(len(jobs)))
+    for tag, (sc, job) in jobs:
+        np.testing.assert_array_equal(imgs[tag].cpu().numpy(), _plain(ctx, world, sc, job), err_msg=f"job {tag}")
+
+
+def test_graph_recaptured_when_iterations_change(ctx, world):
+    """Changing the iterations per step between steps, with jobs in flight,
+    records new graphs; every image still equals its plain render."""
+    print("[case] iterations change mid-run (graphs)", flush=True)
+    scenes, dm, env, (lf, nl) = world
+    jobs = list(enumerate(_jobs()))
+    b = api.Batch(ctx, queue_capacity=1 << 13, arena_bytes=256 << 20, max_jobs=64)
+    imgs = {}
+    for tag, (sc, job) in jobs:
+        imgs[tag] = ctx.empty((job.width * job.height, 4), torch.float32)
+        lights = (lf, nl) if sc == 3 else (None, None)
+        b.submit(tag, scenes[sc], dm, env, job, imgs[tag], light_faces=lights[0], num_lights=lights[1])
+    done = []
+    for iters in (2, 5, 1, 4):
+        b.set_iterations(iters)
+        for _ in range(3):
+            tags, busy = b.step()
+            done += tags
+    done += b.run()
+    b.close()
+    assert sorted(done) == list(range(len(jobs)))
     for tag, (sc, job) in jobs:
         np.testing.assert_array_equal(imgs[tag].cpu().numpy(), _plain(ctx, world, sc, job), err_msg=f"job {tag}")
