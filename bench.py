@@ -1029,6 +1029,16 @@ def bench_jobs(args) -> int:
     serve.serve_sequential(ctx, warm)
     serve.serve_batched(ctx, warm, queue_capacity=args.queue)
 
+    # the engine's settings for this workload, picked by the autotuner on a
+    # sample of it (untimed): c3 every 4th job for jobs/s, c4 the first 1.5 s
+    # of arrivals for p99 latency
+    if args.workload == "c3":
+        tsample, tobj = reqs[::4], "jobs_per_s"
+    else:
+        tsample, tobj = [r for r in reqs if r.arrival < 1.5], "p99"
+    tuned, trials = serve.autotune_batch(ctx, tsample, objective=tobj)
+    log(f"autotuned engine settings: {tuned} from {len(trials)} trials")
+
     seq, seq_rays = serve.serve_sequential(ctx, reqs, keep_images=True)
     s_seq = serve.summary(seq, seq_rays)
     log(f"sequential: {s_seq}")
@@ -1050,6 +1060,13 @@ def bench_jobs(args) -> int:
     modes["two_lane"] = serve.summary(two, two_rays)
     log(f"two lanes (large jobs / small jobs shortest first): {modes['two_lane']}")
     mismatched += sum(0 if torch.equal(ref[d.tag], d.image) else 1 for d in two)
+    bt = serve.make_batch(ctx, tuned)
+    batu, _ = serve.serve_batched(ctx, reqs, keep_images=True, batch=bt)
+    st_t = bt.stats()
+    bt.close()
+    modes["batched_autotuned"] = serve.summary(batu, int(st_t.extension_rays) + int(st_t.shadow_rays))
+    log(f"batched, autotuned settings: {modes['batched_autotuned']}")
+    mismatched += sum(0 if torch.equal(ref[d.tag], d.image) else 1 for d in batu)
     twob, twob_rays = serve.serve_two_lane(ctx, reqs, keep_images=True, small_order="batched",
                                            small_iterations=args.iters)
     modes["two_lane_batched"] = serve.summary(twob, twob_rays)
@@ -1099,6 +1116,10 @@ def bench_jobs(args) -> int:
                    "policy": "smallest job first" if args.policy == 1 else "oldest job first",
                    "iterations_per_step": args.iters},
         "amdahl": amdahl,
+        "autotune": {"choice": tuned, "objective": tobj,
+                     "trials": [{"setting": t, tobj: r} for t, r in trials],
+                     "sample": "every 4th request, submitted at once" if tobj == "jobs_per_s"
+                     else "the requests arriving in the first 1.5 s, at their arrival times"},
         **modes,
         "speedup_jobs_per_s": {m: round(modes[m]["jobs_per_s"] / s_seq["jobs_per_s"], 3) for m in modes},
         "images_mismatched": mismatched,

@@ -688,6 +688,12 @@ int32_t mt_batch_set_iterations(mt_batch* batch, int32_t iterations);
  * on = 0 launches the kernels one by one.  Images do not depend on it.
  * Synchronises the context. */
 int32_t mt_batch_set_graphs(mt_batch* batch, int32_t on);
+/* Closest-hit traversal of the engine's shared queue (no counterpart in the
+ * reference): MT_TRACE_PACKED / MT_TRACE_PACKED_STACK (default: one ray per
+ * lane) or MT_TRACE_PACKED_POOL (wave pools with lane refill; each lane
+ * loads its slot's scene).  Images do not depend on it.  The choice the
+ * serving autotuner makes per workload.  Synchronises the context. */
+int32_t mt_batch_set_trace_mode(mt_batch* batch, int32_t mode);
 /* Cancels the job with this tag (CDeviceGI.stop, CDeviceGI.java:438-439,
  * ends the reference's one running job; here any one job of the mix).
  * A job not yet admitted is dropped at once; an admitted job is issued no

@@ -212,6 +212,7 @@ SIGNATURES = {
     "mt_batch_cancel": [P, C.c_int64, P],
     "mt_batch_set_iterations": [P, I32],
     "mt_batch_set_graphs": [P, I32],
+    "mt_batch_set_trace_mode": [P, I32],
     "mt_batch_set_policy": [P, I32],
 }
 MT_BATCH_OLDEST_FIRST = 0

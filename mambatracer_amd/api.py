@@ -947,6 +947,11 @@ class Batch:
         launch their kernels one by one (mt_batch_set_graphs)."""
         call("mt_batch_set_graphs", self.handle, 1 if on else 0)
 
+    def set_trace_mode(self, mode: int) -> None:
+        """Closest-hit traversal of the shared queue: _lib.MT_TRACE_PACKED_STACK
+        (one ray per lane) or _lib.MT_TRACE_PACKED_POOL (mt_batch_set_trace_mode)."""
+        call("mt_batch_set_trace_mode", self.handle, int(mode))
+
     def close(self) -> None:
         if self.handle:
             _lib.load().mt_batch_destroy(self.handle)

@@ -344,6 +344,7 @@ struct mt_batch {
     int policy;
     int iters_per_step;       // wavefront iterations between host round trips
     int* nact;                // device: number of active jobs (read by the iteration kernels)
+    int trace_mode;           // closest hit: MT_TRACE_PACKED_STACK (one ray per lane) or _POOL
     // The iterations of a step as one hipGraph per start parity of the queue
     // ping-pong: every size they use lives in device memory, so a capture
     // stays valid until the iterations per step change.
@@ -491,7 +492,7 @@ int32_t iterate(mt_batch* b)
     k_generate<<<grid, BT, 0, s>>>(b->jobs, b->cnt, b->gen_start, b->gen_pid0, b->gen_job, b->nact, q);
     MT_LAUNCH_CHECK();
     int32_t rc = mt_trace_jobs(b->ctx, b->refs, &q.tag->x, BOUNCE_BITS, q.ro, q.rd, b->cnt + 1, Qc, b->prim,
-                               b->t);
+                               b->t, b->trace_mode);
     if (rc) return rc;
     ShadeJobsArgs A;
     A.jobs = b->jobs;
@@ -727,6 +728,7 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     b->policy = MT_BATCH_OLDEST_FIRST;
     b->iters_per_step = 1;
     b->use_graphs = true;
+    b->trace_mode = MT_TRACE_PACKED_STACK;
     for (int p = 0; p < 2; p++) {
         b->graph[p] = nullptr;
         b->gexec[p] = nullptr;
@@ -880,6 +882,17 @@ extern "C" int32_t mt_batch_set_iterations(mt_batch* b, int32_t iterations)
     MT_ARG_CHECK(b, "mt_batch_set_iterations: NULL batch");
     MT_ARG_CHECK(iterations >= 1 && iterations <= 1024, "mt_batch_set_iterations: %d not in [1,1024]", iterations);
     b->iters_per_step = iterations;
+    return MT_OK;
+}
+
+extern "C" int32_t mt_batch_set_trace_mode(mt_batch* b, int32_t mode)
+{
+    MT_ARG_CHECK(b, "mt_batch_set_trace_mode: NULL batch");
+    MT_ARG_CHECK(mode == MT_TRACE_PACKED || mode == MT_TRACE_PACKED_STACK || mode == MT_TRACE_PACKED_POOL,
+                 "mt_batch_set_trace_mode: mode %d not supported by the batch engine", mode);
+    MT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream)); // recorded graphs hold the old kernel
+    b->trace_mode = mode == MT_TRACE_PACKED ? MT_TRACE_PACKED_STACK : mode;
+    drop_graphs(b);
     return MT_OK;
 }
 

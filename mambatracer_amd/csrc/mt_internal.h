@@ -153,7 +153,7 @@ struct MtSegQueue {
 // refs[tag[2*i] >> shift]; entries i < *d_count (clamped to capacity).
 int32_t mt_trace_jobs(mt_context* ctx, const MtSceneRef* refs, const int32_t* tag, int32_t shift,
                       const float4* o_tmin, const float4* d_tmax, const int32_t* d_count, int32_t capacity,
-                      int32_t* prim, float* t);
+                      int32_t* prim, float* t, int32_t mode);
 // Any hit over a segmented queue for one scene (traversal mode as
 // mt_trace_any; the per-lane refill mode traverses like the short-stack mode
 // here); stats as mt_trace_any, indexed by queue entry.

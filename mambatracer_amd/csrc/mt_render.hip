@@ -86,7 +86,7 @@ __global__ __launch_bounds__(RT) void k_trace_camera(Camera C, int npaths, const
         const Camera Cr = C; // a register copy: a reference into the kernel argument spills it to scratch
         pool_trace<false>(
             pn, pt, root, next, end,
-            [&](int pid, RayT& r) {
+            [&](int pid, RayT& r, const float4*&, const float4*&, int&) {
                 float4 o, d;
                 camera_ray(Cr, pid, seed, o, d);
                 ray_init(r, o, d);
@@ -133,7 +133,8 @@ __global__ __launch_bounds__(RT) void k_trace_shadow(const float4* __restrict__ 
         pool_seg_run(seg_counts, SHADOW_SEGS, seg_cap, 64 * POOL, next, end);
         if (next >= end) return; // wave-uniform
         pool_trace<true>(
-            pn, pt, root, next, end, [&](int k, RayT& r) { ray_init(r, s_o[k], s_d[k]); },
+            pn, pt, root, next, end,
+            [&](int k, RayT& r, const float4*&, const float4*&, int&) { ray_init(r, s_o[k], s_d[k]); },
             [&](bool fin, int k, const RayT&, const TravState& S) {
                 const bool vis = fin && !S.hit;
                 if (vis) {

@@ -218,7 +218,8 @@ __global__ __launch_bounds__(256) void k_trace_pool(TraceArgs A)
     }
     if (next >= end) return;
     pool_trace<ANY>(
-        A.pn, A.pt, A.root, next, end, [&](int id, RayT& r) { ray_init(r, A.o_tmin[id], A.d_tmax[id]); },
+        A.pn, A.pt, A.root, next, end,
+        [&](int id, RayT& r, const float4*&, const float4*&, int&) { ray_init(r, A.o_tmin[id], A.d_tmax[id]); },
         [&](bool fin, int id, const RayT& r, const TravState& S) {
             if (fin) store_result<ANY, false>(A, id, r, S);
         });
@@ -402,6 +403,31 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const int* __restrict__ d_c
 }
 
 __global__ void k_set_i32(int* p, int v) { *p = v; }
+
+// The multi-job queue with wave pools: each wave takes a run of 64 *
+// POOL_RAYS consecutive slots and refills its lanes (pool_trace); a lane
+// loads its slot's scene with its ray.
+__global__ __launch_bounds__(256) void k_trace_jobs_pool(const MtSceneRef* __restrict__ refs,
+                                                         const int* __restrict__ tag, int shift, TraceArgs A)
+{
+    int count = *A.d_count;
+    count = count < A.capacity ? count : A.capacity;
+    const int next = (int)(blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 * POOL_RAYS);
+    const int end = count < next + 64 * POOL_RAYS ? count : next + 64 * POOL_RAYS;
+    if (next >= end) return;
+    pool_trace<false>(
+        nullptr, nullptr, 0, next, end,
+        [&](int id, RayT& r, const float4*& pn, const float4*& pt, int& root) {
+            const MtSceneRef R = refs[tag[2 * id] >> shift];
+            pn = R.pn;
+            pt = R.pt;
+            root = R.root;
+            ray_init(r, A.o_tmin[id], A.d_tmax[id]);
+        },
+        [&](bool fin, int id, const RayT& r, const TravState& S) {
+            if (fin) store_result<false, false>(A, id, r, S);
+        });
+}
 
 template <bool ANY>
 int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
@@ -1014,7 +1040,7 @@ int32_t mt_trace_any_seg(mt_context* ctx, const mt_scene* scene, const float4* o
 
 int32_t mt_trace_jobs(mt_context* ctx, const MtSceneRef* refs, const int32_t* tag, int32_t shift,
                       const float4* o_tmin, const float4* d_tmax, const int32_t* d_count, int32_t capacity,
-                      int32_t* prim, float* t)
+                      int32_t* prim, float* t, int32_t mode)
 {
     if (capacity == 0) return MT_OK;
     TraceArgs A{};
@@ -1024,8 +1050,10 @@ int32_t mt_trace_jobs(mt_context* ctx, const MtSceneRef* refs, const int32_t* ta
     A.capacity = capacity;
     A.prim = prim;
     A.tout = t;
-    dim3 grid(mt_blocks(capacity, 256)), block(256);
-    k_trace_jobs<<<grid, block, 0, ctx->stream>>>(refs, tag, shift, A);
+    if (mode == MT_TRACE_PACKED_POOL)
+        k_trace_jobs_pool<<<mt_blocks(capacity, 256 * POOL_RAYS), 256, 0, ctx->stream>>>(refs, tag, shift, A);
+    else
+        k_trace_jobs<<<mt_blocks(capacity, 256), 256, 0, ctx->stream>>>(refs, tag, shift, A);
     MT_LAUNCH_CHECK();
     return MT_OK;
 }

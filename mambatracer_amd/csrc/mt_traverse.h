@@ -379,9 +379,11 @@ constexpr int POOL_REFILL_MIN = 16;
 // (ballot + mbcnt on a wave-uniform cursor; no atomics), so lanes stay busy
 // until the run is drained: a
 // wave's tail, where its longest rays run while the other lanes idle, comes
-// once per run instead of once per 64 rays.  load(id, r) initialises entry
-// id's ray; done(fin, id, r, S) is called after every iteration with the
-// whole wave active, fin set on the lanes whose ray ended in it.  Each ray is traversed in the reference's
+// once per run instead of once per 64 rays.  load(id, r, pn, pt, root)
+// initialises entry id's ray (and may set the lane's scene: every lane
+// starts on the caller's); done(fin, id, r, S) is called after every
+// iteration with the whole wave active, fin set on the lanes whose ray ended
+// in it.  Each ray is traversed in the reference's
 // visiting order; only the lane that computes it changes.
 template <bool ANY, class Load, class Done>
 __device__ __forceinline__ void pool_trace(const float4* __restrict__ pn, const float4* __restrict__ pt, int root,
@@ -391,6 +393,9 @@ __device__ __forceinline__ void pool_trace(const float4* __restrict__ pn, const 
     int id = 0;
     RayT r;
     TravState S;
+    const float4* lpn = pn; // this lane's scene (load may change it)
+    const float4* lpt = pt;
+    int lroot = root;
     for (;;) {
         const uint64_t idle = __ballot(!active);
         // refill once POOL_REFILL_MIN lanes wait (or all do): starting a ray
@@ -402,15 +407,15 @@ __device__ __forceinline__ void pool_trace(const float4* __restrict__ pn, const 
             const int rk = (int)lane_rank(idle);
             if (!active && rk < n) {
                 id = next + rk;
-                load(id, r);
-                S.init(root);
+                load(id, r, lpn, lpt, lroot);
+                S.init(lroot);
                 active = true;
             }
             next += n;
         }
         if (!__any(active)) break; // run drained, every lane done
         bool fin = false;
-        if (active && trace_packed_node<ANY, false>(pn, pt, r, S)) {
+        if (active && trace_packed_node<ANY, false>(lpn, lpt, r, S)) {
             fin = true;
             active = false;
         }

@@ -252,6 +252,70 @@ def serve_two_lane(ctx: api.Context, requests: List[Request], keep_images: bool 
     return out, rays[0]
 
 
+def make_batch(ctx: api.Context, setting: dict, arena_bytes: int = 32 << 30, max_jobs: int = 1024,
+               policy: int = 1) -> api.Batch:
+    """A multi-job engine with an autotuner setting: queue_capacity,
+    iterations (per step), trace_mode (_lib.MT_TRACE_PACKED_STACK / _POOL),
+    graphs (replay captured iterations)."""
+    b = api.Batch(ctx, int(setting["queue_capacity"]), arena_bytes, max_jobs, policy=policy)
+    b.set_iterations(int(setting["iterations"]))
+    b.set_trace_mode(int(setting["trace_mode"]))
+    b.set_graphs(bool(setting["graphs"]))
+    return b
+
+
+def autotune_batch(ctx: api.Context, requests: List[Request], arena_bytes: int = 32 << 30,
+                   max_jobs: int = 1024, policy: int = 1,
+                   queue_capacities=(1 << 21, 1 << 22, 1 << 23), iterations=(1, 4, 8, 16),
+                   trace_modes=None, graphs=(True, False), objective: str = "jobs_per_s"):
+    """Chooses the multi-job engine's settings for a workload: the queue
+    capacity (paths in flight per iteration), the bounce iterations per host
+    round trip, the closest-hit traversal of the shared queue (one ray per
+    lane, or wave pools) and graph replay.  `requests` is a sample of the
+    workload (its scenes, sizes and sample counts), served through
+    serve_batched per trial: objective "jobs_per_s" submits them at once and
+    the setting with the most jobs/s wins; "p99" keeps their arrival times
+    and the lowest p99 latency wins.  Coordinate search from (largest capacity, 8 iterations, one
+    ray per lane, graphs): each dimension in turn takes its best value with
+    the others held.  Every setting renders the same images.  Returns (best
+    setting, [(setting, jobs/s or p99 ms), ...])."""
+    from . import _lib
+    if objective not in ("jobs_per_s", "p99"):
+        raise ValueError(f"unknown objective {objective!r}")
+    if trace_modes is None:
+        trace_modes = (_lib.MT_TRACE_PACKED_STACK, _lib.MT_TRACE_PACKED_POOL)
+    sample = [Request(0.0 if objective == "jobs_per_s" else r.arrival, r.scene, r.materials, r.env, r.job,
+                      r.light_faces, r.num_lights, tag=k) for k, r in enumerate(requests)]
+    best = {"queue_capacity": max(queue_capacities), "iterations": 8 if 8 in iterations else iterations[0],
+            "trace_mode": trace_modes[0], "graphs": graphs[0]}
+    trials = []
+    scores = {}
+
+    def score(setting):
+        key = tuple(sorted(setting.items()))
+        if key in scores:
+            return scores[key]
+        b = make_batch(ctx, setting, arena_bytes, max_jobs, policy)
+        try:
+            done, _ = serve_batched(ctx, sample, batch=b)
+        finally:
+            b.close()
+        sm = summary(done, 0)
+        rate = sm["jobs_per_s"] if objective == "jobs_per_s" else -sm["latency_ms"]["p99"]
+        scores[key] = rate
+        trials.append((dict(setting), sm["jobs_per_s"] if objective == "jobs_per_s" else sm["latency_ms"]["p99"]))
+        return rate
+
+    score(best)  # warm-up trial (code objects, allocator), not compared
+    scores.clear()
+    trials.clear()
+    for name, values in (("queue_capacity", queue_capacities), ("iterations", iterations),
+                         ("trace_mode", trace_modes), ("graphs", graphs)):
+        rates = {v: score({**best, name: v}) for v in values}
+        best[name] = max(rates, key=lambda v: rates[v])
+    return best, trials
+
+
 def summary(done: List[Done], rays: int) -> dict:
     """Latency percentiles (ms), jobs/s and Mrays/s over the run's span."""
     lat = np.array([d.latency for d in done]) * 1e3

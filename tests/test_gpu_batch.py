@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 import torch
 
-from mambatracer_amd import _lib, api
+from mambatracer_amd import _lib, api, serve
 from mambatracer_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -279,3 +279,45 @@ def test_graph_recaptured_when_iterations_change(ctx, world):
     assert sorted(done) == list(range(len(jobs)))
     for tag, (sc, job) in jobs:
         np.testing.assert_array_equal(imgs[tag].cpu().numpy(), _plain(ctx, world, sc, job), err_msg=f"job {tag}")
+
+
+def test_pool_traversal_in_the_engine(ctx, world):
+    """The shared queue traced with wave pools (each lane loading its slot's
+    scene) gives the images of plain renders."""
+    print("[case] engine with pooled traversal", flush=True)
+    scenes, dm, env, (lf, nl) = world
+    jobs = list(enumerate(_jobs()))
+    b = api.Batch(ctx, queue_capacity=1 << 14, arena_bytes=256 << 20, max_jobs=64)
+    b.set_trace_mode(_lib.MT_TRACE_PACKED_POOL)
+    b.set_iterations(3)
+    imgs = {}
+    for tag, (sc, job) in jobs:
+        imgs[tag] = ctx.empty((job.width * job.height, 4), torch.float32)
+        lights = (lf, nl) if sc == 3 else (None, None)
+        b.submit(tag, scenes[sc], dm, env, job, imgs[tag], light_faces=lights[0], num_lights=lights[1])
+    order = b.run()
+    b.close()
+    assert sorted(order) == list(range(len(jobs)))
+    for tag, (sc, job) in jobs:
+        np.testing.assert_array_equal(imgs[tag].cpu().numpy(), _plain(ctx, world, sc, job), err_msg=f"job {tag}")
+
+
+def test_engine_autotuner(ctx, world):
+    """serve.autotune_batch tries each setting dimension in turn on a sample
+    and returns one of the tried settings; serving with it reproduces the
+    plain images."""
+    print("[case] engine autotuner", flush=True)
+    scenes, dm, env, (lf, nl) = world
+    jobs = [(t, (sc, job)) for t, (sc, job) in enumerate(_jobs()) if sc != 3]
+    reqs = [serve.Request(0.0, scenes[sc], dm, env, job, tag=t) for t, (sc, job) in jobs]
+    best, trials = serve.autotune_batch(ctx, reqs, arena_bytes=256 << 20, max_jobs=64,
+                                        queue_capacities=(1 << 12, 1 << 14), iterations=(1, 4))
+    assert set(best) == {"queue_capacity", "iterations", "trace_mode", "graphs"}
+    assert any(t == best for t, _ in trials) and all(r > 0 for _, r in trials)
+    assert len(trials) >= 5
+    b = serve.make_batch(ctx, best, arena_bytes=256 << 20, max_jobs=64)
+    done, _ = serve.serve_batched(ctx, reqs, keep_images=True, batch=b)
+    b.close()
+    for d in done:
+        sc, job = dict(jobs)[d.tag]
+        np.testing.assert_array_equal(d.image.cpu().numpy(), _plain(ctx, world, sc, job), err_msg=f"job {d.tag}")
