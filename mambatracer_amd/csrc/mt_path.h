@@ -443,6 +443,14 @@ __device__ __forceinline__ int wave_append(bool flag, int* counter)
 // (about one resident producer group per segment); the multi-job engine,
 // whose per-iteration batches are smaller, its own count.
 constexpr int SHADOW_SEGS = 1024;
+// Work-groups per segment in the fused loop's launches that sweep a
+// segmented queue: group b takes segment b % SHADOW_SEGS and walks its
+// chunks b / SHADOW_SEGS, + SEG_SWEEP, ...  The grid (SHADOW_SEGS x
+// SEG_SWEEP) does not scale with the queue's capacity, so a nearly empty
+// late bounce does not dispatch a capacity's worth of empty groups (about
+// 27 us per launch at 130K groups).  A group appends only what it read from
+// its own segment, which keeps every segment within its capacity.
+constexpr int SEG_SWEEP = 8;
 
 template <int NSEG = SHADOW_SEGS>
 inline int shadow_seg_cap(int64_t slots, int group)

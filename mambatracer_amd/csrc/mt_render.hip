@@ -152,28 +152,35 @@ __global__ __launch_bounds__(RT) void k_trace_shadow(const float4* __restrict__ 
             });
         return;
     }
-    const int k = seg_entry(seg_cap);
-    const bool live = seg_live(seg_counts, seg_cap, k);
-    if (__ballot(live) == 0ull) return; // wave-uniform exit
-    bool vis = false;
-    if (live) {
-        RayT r;
-        ray_init(r, s_o[k], s_d[k]);
-        int st[3] = {0, 0, 0};
-        int child = -1;
-        vis = !trace_packed<true, false, true>(pn, pt, root, r, child, st);
-        if (vis) {
-            const float4 c = contrib[k];
-            const int slot = __float_as_int(c.w);
-            const int pid = queue ? queue[slot] : slot;
-            const float4 v = L[pid];
-            L[pid] = make_float4(v.x + c.x, v.y + c.y, v.z + c.z, v.w);
+    // segment sweep (SEG_SWEEP groups per segment): one entry per lane per chunk
+    const int p = blockIdx.x % SHADOW_SEGS;
+    int cnt = seg_counts[p];
+    cnt = cnt < seg_cap ? cnt : seg_cap;
+    const int stride = (int)(gridDim.x / SHADOW_SEGS) * (int)blockDim.x;
+    for (int off = (int)(blockIdx.x / SHADOW_SEGS) * (int)blockDim.x; off < cnt; off += stride) {
+        const int k = p * seg_cap + off + (int)threadIdx.x;
+        const bool live = off + (int)threadIdx.x < cnt;
+        if (__ballot(live) == 0ull) continue; // wave-uniform
+        bool vis = false;
+        if (live) {
+            RayT r;
+            ray_init(r, s_o[k], s_d[k]);
+            int st[3] = {0, 0, 0};
+            int child = -1;
+            vis = !trace_packed<true, false, true>(pn, pt, root, r, child, st);
+            if (vis) {
+                const float4 c = contrib[k];
+                const int slot = __float_as_int(c.w);
+                const int pid = queue ? queue[slot] : slot;
+                const float4 v = L[pid];
+                L[pid] = make_float4(v.x + c.x, v.y + c.y, v.z + c.z, v.w);
+            }
         }
-    }
-    if (pend) { // the whole wave is active here
-        int2 e = make_int2(-1, 0);
-        if (vis) e = s_lg[k];
-        lg_wave_accum(e.x >= 0, e.x, lg_fixed(__int_as_float(e.y)), pend);
+        if (pend) { // the whole wave is active here
+            int2 e = make_int2(-1, 0);
+            if (vis) e = s_lg[k];
+            lg_wave_accum(e.x >= 0, e.x, lg_fixed(__int_as_float(e.y)), pend);
+        }
     }
 }
 
@@ -272,24 +279,30 @@ __device__ __forceinline__ void shade_slot(const ShadeArgs& A, uint32_t seed, in
     }
 }
 
-// One slot per lane over a capacity-sized grid (the segment capacities
-// assume exactly that); work-groups past *d_count leave at once.
+// SWEEP false (bounce 0, the unfused loop): one slot per lane over a
+// capacity-sized grid (the segment capacities assume exactly that;
+// work-groups past *d_count leave at once).  SWEEP true (fused bounces >=
+// 1): a segment sweep over the queue or over the slots k_shade_split found
+// to hit.  Two instantiations, so the sweep's loop does not change the
+// register allocation of the bounce-0 kernel.
+template <bool SWEEP>
 __global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
 {
     int slot;
     bool live;
-    if (A.hits) { // the slots that hit, dense (chunk-major over their segments)
-        const int p = blockIdx.x % SHADOW_SEGS, off = (blockIdx.x / SHADOW_SEGS) * blockDim.x;
-        const int c = A.hit_seg[p];
-        if (off >= c) return;
-        live = off + (int)threadIdx.x < c;
-        slot = live ? A.hits[p * A.seg_cap + off + threadIdx.x] : 0;
-    } else if (A.in_seg) { // chunk-major over the segmented queue, as seg_entry
-        const int p = blockIdx.x % SHADOW_SEGS, off = (blockIdx.x / SHADOW_SEGS) * blockDim.x;
-        const int c = A.in_seg[p];
-        if (off >= c) return;
-        slot = p * A.seg_cap + off + threadIdx.x;
-        live = off + (int)threadIdx.x < c;
+    if (SWEEP) { // segment sweep over the hitting slots, or over the queue
+        const uint32_t seed = *A.seed;
+        const int p = blockIdx.x % SHADOW_SEGS;
+        int cnt = A.hits ? A.hit_seg[p] : A.in_seg[p];
+        cnt = cnt < A.seg_cap ? cnt : A.seg_cap;
+        const int stride = (int)(gridDim.x / SHADOW_SEGS) * (int)blockDim.x;
+        for (int off = (int)(blockIdx.x / SHADOW_SEGS) * (int)blockDim.x; off < cnt; off += stride) {
+            live = off + (int)threadIdx.x < cnt;
+            const int k = p * A.seg_cap + off + (int)threadIdx.x;
+            slot = A.hits ? (live ? A.hits[k] : 0) : k;
+            shade_slot(A, seed, slot, live);
+        }
+        return;
     } else {
         const int base = blockIdx.x * blockDim.x;
         const int count = *A.d_count;
@@ -309,27 +322,30 @@ __global__ __launch_bounds__(RT) void k_shade(ShadeArgs A)
 // shading is the same code either way.
 __global__ __launch_bounds__(RT) void k_shade_split(ShadeArgs A, int* __restrict__ hits, int* __restrict__ hit_seg)
 {
-    const int p = blockIdx.x % SHADOW_SEGS, off = (blockIdx.x / SHADOW_SEGS) * blockDim.x;
-    const int c = A.in_seg[p];
-    if (off >= c) return;
-    const int slot = p * A.seg_cap + off + threadIdx.x;
-    const bool live = off + (int)threadIdx.x < c;
-    const int prim = live ? A.prim[slot] : -1;
-    int gidx = -1;
-    unsigned long long q = 0ull;
-    if (live && prim < 0) {
-        const int pid = A.queue[slot];
-        v3 Ladd;
-        shade_escape(A.V, A.bounce, A.o_tmin[slot], A.d_tmax[slot], A.S.T[pid], Ladd, gidx, q);
-        if (Ladd.x != 0.0f || Ladd.y != 0.0f || Ladd.z != 0.0f) {
-            const float4 L = A.S.L[pid];
-            A.S.L[pid] = make_float4(L.x + Ladd.x, L.y + Ladd.y, L.z + Ladd.z, L.w);
+    const int p = blockIdx.x % SHADOW_SEGS; // segment sweep
+    int cnt = A.in_seg[p];
+    cnt = cnt < A.seg_cap ? cnt : A.seg_cap;
+    const int stride = (int)(gridDim.x / SHADOW_SEGS) * (int)blockDim.x;
+    for (int off = (int)(blockIdx.x / SHADOW_SEGS) * (int)blockDim.x; off < cnt; off += stride) {
+        const int slot = p * A.seg_cap + off + (int)threadIdx.x;
+        const bool live = off + (int)threadIdx.x < cnt;
+        const int prim = live ? A.prim[slot] : -1;
+        int gidx = -1;
+        unsigned long long q = 0ull;
+        if (live && prim < 0) {
+            const int pid = A.queue[slot];
+            v3 Ladd;
+            shade_escape(A.V, A.bounce, A.o_tmin[slot], A.d_tmax[slot], A.S.T[pid], Ladd, gidx, q);
+            if (Ladd.x != 0.0f || Ladd.y != 0.0f || Ladd.z != 0.0f) {
+                const float4 L = A.S.L[pid];
+                A.S.L[pid] = make_float4(L.x + Ladd.x, L.y + Ladd.y, L.z + Ladd.z, L.w);
+            }
         }
+        if (A.V.G.mode) lg_wave_accum(gidx >= 0, gidx, q, A.V.G.pend); // the whole wave is active here
+        const bool hit = live && prim >= 0;
+        const int k = p * A.seg_cap + wave_append(hit, hit_seg + p);
+        if (hit) hits[k] = slot;
     }
-    if (A.V.G.mode) lg_wave_accum(gidx >= 0, gidx, q, A.V.G.pend); // the whole wave is active here
-    const bool hit = live && prim >= 0;
-    const int k = p * A.seg_cap + wave_append(hit, hit_seg + p);
-    if (hit) hits[k] = slot;
 }
 
 // Unoccluded shadow-queue entries add their contribution to their path.
@@ -575,6 +591,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
     const int grid = mt_blocks(np, RT);
     const int gshade = grid;              // one slot per lane; empty work-groups retire at once
     const int gshadow = mt_blocks((int64_t)R.sq, RT);
+    const int gsweep = SHADOW_SEGS * SEG_SWEEP; // the fused loop's segment sweeps
     // wave-pool kernels: runs of 64 * POOL_RAYS entries, 4 per work-group
     const int64_t pool_group = 4 * 64 * (int64_t)POOL_RAYS;
     const int gcam_pool = mt_blocks(np, (int)pool_group);
@@ -665,12 +682,13 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         A.hit_seg = nullptr;
         if (A.in_seg) { // escaped paths here, the hits densely in k_shade
             int* hseg = hsc + (size_t)b * SHADOW_SEGS;
-            k_shade_split<<<mt_blocks((int64_t)R.sq, RT), RT, 0, st>>>(A, hits, hseg);
+            k_shade_split<<<gsweep, RT, 0, st>>>(A, hits, hseg);
             if ((rc = chk())) break;
             A.hits = hits;
             A.hit_seg = hseg;
         }
-        k_shade<<<A.in_seg ? mt_blocks((int64_t)R.sq, RT) : gshade, RT, 0, st>>>(A);
+        if (A.in_seg) k_shade<true><<<gsweep, RT, 0, st>>>(A);
+        else k_shade<false><<<gshade, RT, 0, st>>>(A);
         if ((rc = chk())) break;
         if (profile && (rc = mark(ev[4 * b + 2]))) break;
         if (fused) {
@@ -679,7 +697,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
                     scene->pnodes, scene->ptris, scene->root_ref, seg_counts, seg_cap, F4(R.oSo), F4(R.oSd),
                     F4(R.oSc), first ? nullptr : q[cur], S.L, A.s_lg, R.grid ? R.grid->pend : nullptr);
             else
-                k_trace_shadow<0><<<gshadow, RT, 0, st>>>(
+                k_trace_shadow<0><<<gsweep, RT, 0, st>>>(
                     scene->pnodes, scene->ptris, scene->root_ref, seg_counts, seg_cap, F4(R.oSo), F4(R.oSd),
                     F4(R.oSc), first ? nullptr : q[cur], S.L, A.s_lg, R.grid ? R.grid->pend : nullptr);
             if ((rc = chk())) break;
