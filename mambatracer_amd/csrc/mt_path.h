@@ -450,7 +450,7 @@ constexpr int SHADOW_SEGS = 1024;
 // late bounce does not dispatch a capacity's worth of empty groups (about
 // 27 us per launch at 130K groups).  A group appends only what it read from
 // its own segment, which keeps every segment within its capacity.
-constexpr int SEG_SWEEP = 8;
+constexpr int SEG_SWEEP = 32;
 
 template <int NSEG = SHADOW_SEGS>
 inline int shadow_seg_cap(int64_t slots, int group)
