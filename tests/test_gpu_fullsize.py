@@ -128,15 +128,18 @@ def test_c2_full_batch_sampled_rays_and_whole_batch_properties(ctx):
     o, d = api.gen_random_rays(3, 0, n, bounds[0, :3], bounds[0, 4:7], nthreads=16)
     do, dd = ctx.to_device(o), ctx.to_device(d)
     prim, t, _ = api.intersect_primitives(ctx, sc, do, dd)                               # one ray per lane
-    prim_r, t_r, _ = api.intersect_primitives(ctx, sc, do, dd, mode=_lib.MT_TRACE_PACKED_REFILL)
-    assert torch.equal(prim, prim_r) and torch.equal(t.view(torch.int32), t_r.view(torch.int32))
-    del prim_r, t_r
+    lane_maps = (_lib.MT_TRACE_PACKED_REFILL, _lib.MT_TRACE_PACKED_POOL, _lib.MT_TRACE_PACKED_BINNED)
+    for m in lane_maps:  # every lane mapping gives every ray's result of one ray per lane
+        prim_r, t_r, _ = api.intersect_primitives(ctx, sc, do, dd, mode=m)
+        assert torch.equal(prim, prim_r) and torch.equal(t.view(torch.int32), t_r.view(torch.int32)), m
+        del prim_r, t_r
     light = W.c2_light(bounds)
     so, sd, valid = api.shadow_rays(ctx, do, dd, prim, t, light)
     occ, _ = api.intersect_occlusion(ctx, sc, so, sd)
-    occ_r, _ = api.intersect_occlusion(ctx, sc, so, sd, mode=_lib.MT_TRACE_PACKED_REFILL)
-    assert torch.equal(occ, occ_r)
-    del occ_r
+    for m in lane_maps:
+        occ_r, _ = api.intersect_occlusion(ctx, sc, so, sd, mode=m)
+        assert torch.equal(occ, occ_r), m
+        del occ_r
     # stable compaction of the hit rays: the index list is exactly the
     # sorted positions of prim >= 0 and the payload follows it
     idx, cnt, (co,) = api.compact(ctx, prim, flag_mode=1, payloads=[dd])
