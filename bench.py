@@ -40,14 +40,19 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(workload: str = "c1"):
-    """HBM bytes per closest-hit launch measured by rocprofv3 --pmc (profiles/), or None."""
+def traffic_from_profiles(workload: str = "c1", mode=None):
+    """HBM bytes per closest-hit launch measured by rocprofv3 --pmc (profiles/), for the
+    traversal mode `mode` where the profile holds one figure per mode; or None."""
     p = os.path.join(ROOT, "profiles", f"traffic_trace_closest_{workload}.json")
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
-            return float(json.load(f)["bytes_per_launch"])
+            d = json.load(f)
+        if mode is not None and "modes" in d:
+            m = d["modes"].get(str(int(mode)))
+            return float(m["bytes"]) if m else None
+        return float(d["bytes_per_launch"])
     except Exception:
         return None
 
@@ -61,9 +66,53 @@ def valu_from_profiles(workload: str = "c1"):
     try:
         with open(p) as f:
             d = json.load(f)
-        return float(d["valu_inst_per_launch"]), float(d["valu_peak_inst_per_s"]), d["valu_peak_basis"]
+        return (float(d["valu_inst_per_launch"]), float(d["valu_peak_inst_per_s"]), d["valu_peak_basis"],
+                d.get("cycles_per_inst"))
     except Exception:
         return None
+
+
+# wave64 VALU instructions per lane and loop iteration of each calibration mix
+# (mt_calib.hip; counts read from the kernels' gfx950 assembly)
+VALU_CALIB_MIX = {0: ("v_mul_f32 + v_add_f32", 16), 1: ("v_mul_f32/v_add_f32 + 25% v_rcp_f32", 16),
+                  2: ("v_min_f32/v_max_f32 (+ NaN-quieting v_max)", 24), 3: ("v_add_u32 + v_xor_b32", 16),
+                  4: ("v_cmp_f32 + v_cndmask_b32 + v_add_f32", 24), 5: ("v_fma_f32", 16),
+                  6: ("8 correctly rounded f32 divisions (11 instructions each)", 88),
+                  7: ("8 correctly rounded sqrtf (16 instructions each) + v_add_f32", 136)}
+
+
+def valu_calibration(blocks: int = 8192, iters: int = 4096, reps: int = 5):
+    """Issue rate of wave64 VALU instructions on this GPU, per mix: blocks x
+    256 lanes run `iters` iterations of independent chains (libmt_calib.so);
+    the best of `reps` launches timed with HIP events on the launch stream."""
+    import ctypes
+
+    import torch
+    lib = ctypes.CDLL(os.path.join(ROOT, "mambatracer_amd", "libmt_calib.so"))
+    lib.mt_calib_valu.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    lib.mt_calib_valu.restype = ctypes.c_int
+    out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    res = {"blocks": blocks, "lanes_per_block": 256, "iters": iters, "mixes": {}}
+    waves = blocks * 256 // 64
+    for kind, (what, per_iter) in VALU_CALIB_MIX.items():
+        best = None
+        for _ in range(reps + 1):  # first launch untimed
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            rc = lib.mt_calib_valu(kind, blocks, iters, ctypes.c_void_p(out.data_ptr()),
+                                   ctypes.c_void_p(stream.cuda_stream))
+            e1.record(stream)
+            if rc != 0:
+                raise RuntimeError(f"mt_calib_valu({kind}) returned {rc}")
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1)
+            best = ms if best is None or ms < best else best
+        inst = per_iter * iters * waves
+        res["mixes"][str(kind)] = {"mix": what, "loop_inst": inst, "ms": round(best, 4),
+                                   "G_inst_per_s": round(inst / (best * 1e-3) / 1e9, 1)}
+    res["checksum"] = float(out.double().sum())
+    return res
 
 
 def cpu_baseline(W, scene_arrays, mats, rgba, job, npaths, seconds, threads, grid=None, grid_mode=0):
@@ -168,6 +217,9 @@ def main() -> int:
     ap.add_argument("--gather-calibration", action="store_true",
                     help="gather 1 GiB of 64-byte, 64-byte-aligned records in a random order and exit (PMC "
                          "byte-counter calibration for the traversal's dependent 64-byte node reads)")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="time the vector-ALU issue-rate kernels (libmt_calib.so) and print their rates "
+                         "(the measured peak of the closest-hit kernels' instruction-issue roofline)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by pixels or jobs "
@@ -203,6 +255,9 @@ def main() -> int:
         torch.cuda.synchronize()
         print(f"gathered {n} records of 64 bytes ({n * 64} bytes) in a random order; "
               f"checksum {float(out[:, 0].sum()):.3f}", flush=True)
+        return 0
+    if args.valu_calibration:
+        print(json.dumps(valu_calibration()), flush=True)
         return 0
     if args.workload == "c2":
         return bench_c2(args)
@@ -531,14 +586,16 @@ def bench_c1(args) -> int:
 
 def valu_issue(valu, launch_s):
     """The closest-hit launches' VALU issue rate (profiled instructions per launch over the
-    live launch time) against the issue peak: the limit these kernels run into."""
+    live launch time) against the peak wave64 issue rate of the 2-cycle instruction class, with
+    the measured cycles per instruction beside it."""
     if valu is None:
         return None
-    inst, peak, basis = valu
+    inst, peak, basis, cpi = valu
     return {"inst_per_launch": round(inst), "achieved_inst_per_s": round(inst / launch_s / 1e9, 1),
             "peak_inst_per_s": round(peak / 1e9, 1), "unit": "G VALU instructions/s",
             "frac": round(inst / launch_s / peak, 4), "peak_basis": basis,
-            "source": "profiles/counters_c1.md (SQ_INSTS_VALU)"}
+            "cycles_per_inst": cpi,
+            "source": "profiles/counters_c1.md (SQ_INSTS_VALU, GRBM_GUI_ACTIVE), profiles/valu_calibration.md"}
 
 
 def bench_c2(args) -> int:
@@ -670,7 +727,7 @@ def bench_c2(args) -> int:
     log(f"per step: closest {stage[0]:.2f} ms, shadow setup {stage[1]:.2f} ms, any {stage[2]:.2f} ms, "
         f"compaction {stage[3]:.3f} ms; hits {hits}")
     cbytes = n * (4 + 32) + survivors * (32 + 4)
-    traffic = traffic_from_profiles("c2")
+    traffic = traffic_from_profiles("c2", modes["closest"])
     out = {
         "metric": "Grays/s, c2 stress (closest hit + any hit of 64M incoherent rays, 4M triangles)",
         "value": round(value, 4),

@@ -1,4 +1,6 @@
-"""Builds libmambart.so (HIP, gfx950) in-tree with hipcc.
+"""Builds libmambart.so (HIP, gfx950) in-tree with hipcc, and beside it
+libmt_calib.so, the vector-ALU issue-rate calibration kernels that the
+benchmark's roofline uses (not linked into libmambart).
 
 Every HIP translation unit is compiled with:
   -ffp-contract=off        no FMA contraction: kernels round like the CPU
@@ -20,6 +22,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmambart.so")
+CALIB_LIB = os.path.join(HERE, "libmt_calib.so")
+CALIB_SOURCES = ["mt_calib.hip"]
 ARCH = os.environ.get("MAMBART_ARCH", "gfx950")
 
 SOURCES = [
@@ -69,19 +73,26 @@ def _compile(src: str, verbose: bool) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    """Compiles every source and links libmambart.so; returns its path."""
-    os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
-    tmp = LIB + ".tmp"
+def _link(objs, lib: str, verbose: bool) -> None:
+    tmp = lib + ".tmp"
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
-    os.replace(tmp, LIB)
+        raise RuntimeError(f"link of {os.path.basename(lib)} failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, lib)
+
+
+def build(verbose: bool = False) -> str:
+    """Compiles every source, links libmambart.so and libmt_calib.so; returns
+    libmambart.so's path."""
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = SOURCES + CALIB_SOURCES
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
+    _link(objs[:len(SOURCES)], LIB, verbose)
+    _link(objs[len(SOURCES):], CALIB_LIB, verbose)
     return LIB
 
 
