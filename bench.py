@@ -198,6 +198,9 @@ def main() -> int:
                     help="c1: seconds of Poisson arrivals at 70%% of the measured capacity for p50/p99 latency")
     ap.add_argument("--other-steps", type=int, default=5,
                     help="c1: timed replays of each other lighting path, reported beside the headline")
+    ap.add_argument("--tail", type=int, default=3,
+                    help="c1: bounces from this one on run in one launch, each lane carrying a path to its end "
+                         "(MT_RENDER_TAIL_FROM); 0: three launches per bounce")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "compact", "bvh", "stages"], default="c1",
                     help="c1: one 1080p render per step (default line); c2: 64M random rays through 4M "
@@ -389,19 +392,28 @@ def bench_c1(args) -> int:
     job = W.RenderJob(width=args.width, height=args.height, spp=args.spp, mode=args.mode)
     npaths = job.width * job.height * job.spp
     lflags = lighting_flags(args.lighting)
+    # bounces >= args.tail in one launch (k_tail): this open scene leaves few
+    # paths after the first bounces
+    tflags = _lib.MT_RENDER_TAIL_FROM(args.tail) if 0 < args.tail < job.max_bounces else 0
 
     grid = None
     if lflags:
         def frame(seed):
-            job.seed, job.flags = seed, lflags
+            job.seed, job.flags = seed, lflags | tflags
             api.render(ctx, scene, dmats, env, job, stats=False)
         grid = prepare_grid(api, ctx, env, rgba, job, frame)
         job.seed = W.RenderJob().seed
         log("light grid prepared: 5 frames, each followed by an update")
 
-    # untimed: counting traversal for the algorithmic bytes
+    # untimed: counting traversal for the algorithmic bytes of the
+    # closest-hit launches the roofline times (bounces before the tail; the
+    # counting loop runs per-bounce launches and the same paths)
     job.flags = _lib.MT_RENDER_TRAVERSAL_STATS | lflags
+    nb_all = job.max_bounces
+    if tflags:
+        job.max_bounces = args.tail
     _, st = api.render(ctx, scene, dmats, env, job)
+    job.max_bounces = nb_all
     ilp = [int(x) for x in st.closest_ilp]
     ext_rays = int(st.extension_rays)
     alg_bytes = 40 * ext_rays + 148 * ilp[0] + 116 * ilp[1] + 28 * ilp[2]
@@ -411,7 +423,7 @@ def bench_c1(args) -> int:
     # The job's launch sequence captured once (mt_render_graph: state laid
     # out once, one graph launch per frame); each step replays it with the
     # job's seed and reads its ray counts.
-    job.flags = lflags
+    job.flags = lflags | tflags
     graph = api.RenderGraph(ctx, scene, dmats, env, job)
     for i in range(args.warmup):
         graph.launch(job.seed)
@@ -431,12 +443,14 @@ def bench_c1(args) -> int:
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     img = graph.image.clone()
+    bounce_paths = [int(st.bounce_counts[b]) for b in range(64) if st.bounce_counts[b] > 0]
+    log(f"live paths entering each bounce: {bounce_paths}")
 
     # The same job through mt_render (uncaptured), with every traversal
     # launch bracketed by HIP events on the context stream: the closest-hit
     # kernel's average launch duration for the roofline, and the uncaptured
     # throughput for comparison.
-    job.flags = _lib.MT_RENDER_PROFILE | lflags
+    job.flags = _lib.MT_RENDER_PROFILE | lflags | tflags
     api.render(ctx, scene, dmats, env, job)
     e_rays, closest_ms, launches = 0, 0.0, 0
     torch.cuda.synchronize()
@@ -470,7 +484,7 @@ def bench_c1(args) -> int:
     qps = None
     if args.qps_seconds > 0:
         rate = 0.7 * args.steps / elapsed
-        job.flags = lflags
+        job.flags = lflags | tflags
         qps = latency_at_qps(graph, job.seed, rate, args.qps_seconds)
         log(f"latency at {rate:.1f} jobs/s: {qps}")
 
@@ -482,11 +496,11 @@ def bench_c1(args) -> int:
         of = lighting_flags(other)
         if of and grid is None:
             def frame(seed, of=of):
-                job.seed, job.flags = seed, of
+                job.seed, job.flags = seed, of | tflags
                 api.render(ctx, scene, dmats, env, job, stats=False)
             grid = prepare_grid(api, ctx, env, rgba, job, frame)
             job.seed = W.RenderJob().seed
-        job.flags = of
+        job.flags = of | tflags
         g2 = api.RenderGraph(ctx, scene, dmats, env, job)
         g2.launch(job.seed)
         g2.stats()
@@ -536,13 +550,17 @@ def bench_c1(args) -> int:
             "workload": "c1 wavefront path tracer: 256 UV spheres x 4096 tris + ground (1,048,578 tris), "
                         f"{job.width}x{job.height} x {job.spp} spp, <=8 bounces, RR from bounce 3, "
                         f"{LIGHTING[args.lighting]}, surviving paths compacted every bounce (appended by the "
-                        "shading stage to segmented queues)",
+                        "shading stage to segmented queues)"
+                        + (f"; bounces {args.tail}-{job.max_bounces - 1} in one launch, a path per lane "
+                           "(MT_RENDER_TAIL_FROM)" if tflags else ""),
+            "tail_from_bounce": args.tail if tflags else None,
             "global_batch": npaths,
             "lighting": args.lighting,
             "traversal_layout": "packed" if job.mode == 0 else "reference",
             "parallelism": "single GPU",
         },
         "samples_per_s": round(npaths * args.steps / elapsed, 1),
+        "bounce_paths": bounce_paths,
         "job_latency_ms": qps if qps is not None else {
             "p50": round(lat_sorted[len(lat) // 2], 3),
             "p99": round(lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
@@ -559,8 +577,9 @@ def bench_c1(args) -> int:
         "other_lighting": others,
         "roofline": {
             "kernel": ("closest-hit traversal, packed layout: k_trace_camera (bounce 0, camera rays made in "
-                       "place) and k_trace_pool (bounces 1-7, wave pools with lane refill), averaged over the "
-                       "job's launches")
+                       f"place) and k_trace_pool (bounces 1-{(args.tail if tflags else job.max_bounces) - 1}, "
+                       "wave pools with lane refill), averaged over the job's launches"
+                       + (" (later bounces run inside k_tail)" if tflags else ""))
                       if job.mode == 0 else "closest-hit traversal (reference layout)",
             "bound": "hbm",
             "achieved": round(achieved, 1) if achieved is not None else None,

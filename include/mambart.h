@@ -523,6 +523,20 @@ typedef struct {
  * pdf is used.  Exact for maps whose width is a multiple of 16 and height a
  * multiple of 32 (texel rectangles of equal size). */
 #define MT_RENDER_LIGHTGRID_TILE_PDF 16
+/* MT_RENDER_TAIL_FROM(b), 1 <= b < max_bounces (flags bits 8-13): bounces b
+ * and later run in one launch in which each lane carries a path through all
+ * of its remaining bounces (closest hit, shading, shadow ray) instead of
+ * three launches per bounce.  Same operations per path in the same order,
+ * so images, ray counts and light-grid sums are bit-identical; it pays when
+ * few paths remain at bounce b (open scenes, where most paths escape early:
+ * each late bounce's launches otherwise last as long as their slowest ray),
+ * and costs when many do (the fused wavefront shares work across paths
+ * better).  Applies to the default packed traversal modes without
+ * MT_RENDER_TRAVERSAL_STATS; 0 (the default) or b >= max_bounces: off.  No
+ * counterpart in the reference (its loop, CDeviceGI.java:214-271, launches
+ * every stage of every bounce). */
+#define MT_RENDER_TAIL_SHIFT 8
+#define MT_RENDER_TAIL_FROM(b) ((int32_t)(b) << MT_RENDER_TAIL_SHIFT)
 
 typedef struct {
     int64_t extension_rays; /* closest-hit rays traced (all bounces) */

@@ -97,6 +97,12 @@ MT_RENDER_PROFILE = 2
 MT_RENDER_TRAVERSAL_STATS = 4
 MT_RENDER_LIGHTGRID_ENV = 8
 MT_RENDER_LIGHTGRID_TILE_PDF = 16
+MT_RENDER_TAIL_SHIFT = 8
+
+
+def MT_RENDER_TAIL_FROM(b: int) -> int:
+    """Render flag: bounces b and later in one launch (mambart.h)."""
+    return int(b) << MT_RENDER_TAIL_SHIFT
 
 
 class JobDesc(C.Structure):

@@ -348,6 +348,109 @@ __global__ __launch_bounds__(RT) void k_shade_split(ShadeArgs A, int* __restrict
     }
 }
 
+// Bounces tail..nb-1 of the fused loop in one launch (MT_RENDER_TAIL_FROM):
+// each lane takes a path of bounce `tail`'s segmented queue and carries it
+// through all of its remaining bounces — closest hit, shading (shade_path),
+// shadow ray and its direct-light add — then takes its segment's next path.
+// Every path goes through the same operations in the same order as in the
+// per-bounce kernels (its radiance gains the shading's Ladd, then the
+// unoccluded shadow contribution, bounce after bounce; its random numbers
+// are keyed by (seed, path id, bounce); the light-grid sums are integers),
+// so the image and the grid are bit-identical.  What changes is the time
+// of the sparse late bounces: one launch whose length is the slowest path's
+// chain of dependent loads, instead of three launches per bounce, each as
+// long as that bounce's slowest ray.  Lanes of a wave stay in step (every
+// iteration is trace, shade, shadow for the lanes holding a path), only
+// their bounce numbers differ.  live[b] / shadow[b] count the paths
+// entering bounce b > tail and the shadow rays of bounce b >= tail (the
+// per-bounce kernels' queue lengths), summed in LDS, one atomic per
+// work-group and bounce.
+constexpr int TAIL_SWEEP = 8; // work-groups per segment
+__global__ __launch_bounds__(RT) void k_tail(ShadeArgs A, const float4* __restrict__ pn,
+                                             const float4* __restrict__ pt, int root, int nb,
+                                             int* __restrict__ live_out, int* __restrict__ shadow_out)
+{
+    __shared__ int s_live[64], s_shadow[64];
+    if (threadIdx.x < 64) {
+        s_live[threadIdx.x] = 0;
+        s_shadow[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const uint32_t seed = *A.seed;
+    const int p = blockIdx.x % SHADOW_SEGS;
+    int cnt = A.in_seg[p];
+    cnt = cnt < A.seg_cap ? cnt : A.seg_cap;
+    const int stride = (int)(gridDim.x / SHADOW_SEGS) * (int)blockDim.x;
+    int e = (int)(blockIdx.x / SHADOW_SEGS) * (int)blockDim.x + (int)threadIdx.x; // this lane's next entry
+    bool have = false;
+    int pid = 0, b = 0;
+    float4 ro = make_float4(0, 0, 0, 0), rd = ro, T4 = ro, L = ro;
+    for (;;) {
+        if (!have && e < cnt) { // refill: the next path of this lane's share of the segment
+            const int slot = p * A.seg_cap + e;
+            pid = A.queue[slot];
+            ro = A.o_tmin[slot];
+            rd = A.d_tmax[slot];
+            T4 = A.S.T[pid];
+            L = A.S.L[pid];
+            b = A.bounce;
+            have = true;
+            e += stride;
+        }
+        if (__ballot(have) == 0ull) break; // wave-uniform
+        int prim = -1;
+        float t = 0.0f;
+        if (have) { // closest hit, as k_trace_pool / k_trace_camera find it
+            RayT r;
+            ray_init(r, ro, rd);
+            int st[3] = {0, 0, 0};
+            int child = -1;
+            prim = trace_packed<false, false, true>(pn, pt, root, r, child, st) ? child : -1;
+            t = r.tmax;
+        }
+        PathOut P;
+        P.has_shadow = false;
+        P.alive = false;
+        P.acc_gidx = -1;
+        P.acc_q = 0ull;
+        if (have) {
+            shade_path(A.V, seed, A.rr_start, A.flags, pid, b, ro, rd, T4, prim, t, P);
+            if (P.Ladd.x != 0.0f || P.Ladd.y != 0.0f || P.Ladd.z != 0.0f)
+                L = make_float4(L.x + P.Ladd.x, L.y + P.Ladd.y, L.z + P.Ladd.z, L.w);
+        }
+        if (A.V.G.mode) lg_wave_accum(P.acc_gidx >= 0, P.acc_gidx, P.acc_q, A.V.G.pend); // whole wave active
+        bool vis = false;
+        if (P.has_shadow) { // any hit; an unoccluded sample adds its contribution
+            atomicAdd(&s_shadow[b], 1);
+            RayT r;
+            ray_init(r, P.so, P.sd);
+            int st[3] = {0, 0, 0};
+            int child = -1;
+            vis = !trace_packed<true, false, true>(pn, pt, root, r, child, st);
+            if (vis) L = make_float4(L.x + P.contrib.x, L.y + P.contrib.y, L.z + P.contrib.z, L.w);
+        }
+        if (A.V.G.mode && A.V.G.pend)
+            lg_wave_accum(vis && P.shadow_gidx >= 0, P.shadow_gidx, lg_fixed(P.shadow_lum), A.V.G.pend);
+        if (have) {
+            if (P.alive && b + 1 < nb) { // the path's next bounce, in this lane
+                ro = P.ro;
+                rd = P.rd;
+                T4 = P.T;
+                b++;
+                atomicAdd(&s_live[b], 1);
+            } else { // the path ends: its radiance goes back (throughput is not read again)
+                A.S.L[pid] = L;
+                have = false;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64 && (int)threadIdx.x < nb) {
+        if (s_live[threadIdx.x]) atomicAdd(live_out + threadIdx.x, s_live[threadIdx.x]);
+        if (s_shadow[threadIdx.x]) atomicAdd(shadow_out + threadIdx.x, s_shadow[threadIdx.x]);
+    }
+}
+
 // Unoccluded shadow-queue entries add their contribution to their path.
 // (pend != nullptr: an unoccluded environment sample of the guided mode also
 // adds its luminance to its grid entry, as in k_trace_shadow.)
@@ -425,10 +528,11 @@ struct RenderPlan {
     char* base;
     size_t bytes;
     size_t oT, oL, oA_o, oA_d, oB_o, oB_d, oQA, oQB, oPrim, oT_, oAlive, oOcc, oSo, oSd, oSc, oCnt, oShc, oQsc,
-        oIlp, oTot, oSeed, oSlg, oHit, oHsc;
+        oIlp, oTot, oSeed, oSlg, oHit, oHsc, oTail;
     mt_lightgrid* grid; // guided environment mode: the grid the render samples and accumulates into
     bool fused; // camera rays traced in place, shadow adds fused, next queue appended by the shading stage
     bool pool;  // fused kernels trace wave runs with lane refill (pool_trace)
+    int tail;   // fused loop: bounces tail..nb-1 run in one k_tail launch (0: none)
     bool borrowed; // base is the context's reusable workspace (mt_render), not owned
     hipEvent_t ev[4 * 64];
     int nev;
@@ -506,10 +610,13 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
     R.oQsc = take(4 * 64 * (size_t)SHADOW_SEGS);
     R.oHit = take(4 * R.sq);
     R.oHsc = take(4 * 64 * (size_t)SHADOW_SEGS);
+    R.oTail = take(4 * 2 * 64); // k_tail's live paths per bounce, then its shadow rays per bounce
     R.want_ilp = (desc->flags & MT_RENDER_TRAVERSAL_STATS) != 0;
     R.fused = (desc->mode == MT_TRACE_PACKED || desc->mode == MT_TRACE_PACKED_STACK ||
                desc->mode == MT_TRACE_PACKED_SORTED || desc->mode == MT_TRACE_PACKED_POOL) && !R.want_ilp;
     R.pool = desc->mode == MT_TRACE_PACKED_POOL;
+    R.tail = (desc->flags >> MT_RENDER_TAIL_SHIFT) & 63;
+    if (!R.fused || R.tail >= R.nb) R.tail = 0; // the tail applies to the fused loop's bounces >= 1
     R.profile = (desc->flags & MT_RENDER_PROFILE) != 0;
     R.oIlp = R.want_ilp ? take(12 * (R.sq > n ? R.sq : n)) : 0;
     R.oTot = take(8 * 8 * ILP_SLOTS);
@@ -611,6 +718,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
     k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(shc, 64 * SHADOW_SEGS);
     k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(qsc, 64 * SHADOW_SEGS);
     k_zero_i32<<<mt_blocks(64 * SHADOW_SEGS, RT), RT, 0, st>>>(hsc, 64 * SHADOW_SEGS);
+    k_zero_i32<<<1, 128, 0, st>>>(I(R.oTail), 128);
     k_zero_i32<<<mt_blocks(16 * ILP_SLOTS, RT), RT, 0, st>>>(reinterpret_cast<int*>(tot), 16 * ILP_SLOTS);
     // Fused loop (default traversal, no traversal counters): bounce 0 traces
     // camera rays generated in place and shades with slot = pid; shadow rays
@@ -624,10 +732,49 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         k_camera<<<grid, RT, 0, st>>>(R.C, np, seed_ptr(R), ro[0], rd[0], q[0], S, counts);
         rc = chk();
     }
-    int cur = 0;
+    int cur = 0; // parity of the queue buffers holding the current bounce
+    // the shading stage's arguments for bounce b (queue buffers of parity cur)
+    auto shade_args = [&](int b, bool first) {
+        ShadeArgs A;
+        A.V = R.V;
+        A.S = S;
+        A.C = R.C;
+        A.first = first ? 1 : 0;
+        A.queue = q[cur];
+        A.d_count = counts + b;
+        A.o_tmin = ro[cur];
+        A.d_tmax = rd[cur];
+        A.prim = I(R.oPrim);
+        A.tin = reinterpret_cast<const float*>(base + R.oT_);
+        A.alive = I(R.oAlive);
+        A.in_seg = fused && !first ? qsc + (size_t)b * SHADOW_SEGS : nullptr;
+        A.next_seg = fused ? qsc + (size_t)(b + 1) * SHADOW_SEGS : nullptr;
+        A.q_next = q[1 - cur];
+        A.o_next = ro[1 - cur];
+        A.d_next = rd[1 - cur];
+        A.s_o_tmin = F4(R.oSo);
+        A.s_d_tmax = F4(R.oSd);
+        A.s_contrib = F4(R.oSc);
+        A.s_lg = R.grid ? reinterpret_cast<int2*>(base + R.oSlg) : nullptr;
+        A.shadow_count = shc + (size_t)b * SHADOW_SEGS;
+        A.seg_cap = seg_cap;
+        A.bounce = b;
+        A.rr_start = desc->rr_start;
+        A.flags = desc->flags;
+        A.seed = seed_ptr(R);
+        A.hits = nullptr;
+        A.hit_seg = nullptr;
+        return A;
+    };
     for (int b = 0; b < nb && rc == MT_OK; b++) {
         int* dcount = counts + b;
         const bool first = fused && b == 0;
+        if (R.tail && b == R.tail) { // every remaining bounce in one launch
+            k_tail<<<SHADOW_SEGS * TAIL_SWEEP, RT, 0, st>>>(shade_args(b, false), scene->pnodes, scene->ptris,
+                                                          scene->root_ref, nb, I(R.oTail), I(R.oTail) + 64);
+            rc = chk();
+            break;
+        }
         if (profile && (rc = mark(ev[4 * b]))) break;
         if (first) {
             if (R.pool)
@@ -650,36 +797,8 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         if (rc) break;
         if (profile && (rc = mark(ev[4 * b + 1]))) break;
         if (ilp) k_sum_ilp<<<grid, RT, 0, st>>>(ilp, dcount, nullptr, 0, tot);
-        ShadeArgs A;
-        A.V = R.V;
-        A.S = S;
-        A.C = R.C;
-        A.first = first ? 1 : 0;
-        A.queue = q[cur];
-        A.d_count = dcount;
-        A.o_tmin = ro[cur];
-        A.d_tmax = rd[cur];
-        A.prim = I(R.oPrim);
-        A.tin = reinterpret_cast<const float*>(base + R.oT_);
-        A.alive = I(R.oAlive);
-        A.in_seg = fused && !first ? qsc + (size_t)b * SHADOW_SEGS : nullptr;
-        A.next_seg = fused ? qsc + (size_t)(b + 1) * SHADOW_SEGS : nullptr;
-        A.q_next = q[1 - cur];
-        A.o_next = ro[1 - cur];
-        A.d_next = rd[1 - cur];
-        A.s_o_tmin = F4(R.oSo);
-        A.s_d_tmax = F4(R.oSd);
-        A.s_contrib = F4(R.oSc);
-        A.s_lg = R.grid ? reinterpret_cast<int2*>(base + R.oSlg) : nullptr;
-        int* seg_counts = shc + (size_t)b * SHADOW_SEGS;
-        A.shadow_count = seg_counts;
-        A.seg_cap = seg_cap;
-        A.bounce = b;
-        A.rr_start = desc->rr_start;
-        A.flags = desc->flags;
-        A.seed = seed_ptr(R);
-        A.hits = nullptr;
-        A.hit_seg = nullptr;
+        ShadeArgs A = shade_args(b, first);
+        int* seg_counts = A.shadow_count;
         if (A.in_seg) { // escaped paths here, the hits densely in k_shade
             int* hseg = hsc + (size_t)b * SHADOW_SEGS;
             k_shade_split<<<gsweep, RT, 0, st>>>(A, hits, hseg);
@@ -768,6 +887,17 @@ int32_t read_stats(mt_context* ctx, RenderPlan& R, mt_render_stats* stats)
             hc[b] = (int)c;
         }
     }
+    if (R.tail) { // k_tail's bounces: paths entering bounces > tail, shadow rays of bounces >= tail
+        int tl[128];
+        e = hipMemcpyAsync(tl, R.base + R.oTail, sizeof(tl), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            mt_set_error("mt_render: stats readback failed: %s", hipGetErrorString(e));
+            return MT_ERR_HIP;
+        }
+        for (int b = R.tail + 1; b < nb; b++) hc[b] += tl[b];
+        for (int b = R.tail; b < nb; b++) stats->shadow_rays += tl[64 + b];
+    }
     for (int b = 0; b < 64; b++) stats->bounce_counts[b] = b < nb ? hc[b] : 0;
     for (int b = 0; b < nb; b++) stats->extension_rays += hc[b];
     for (int v : hs) stats->shadow_rays += v;
@@ -781,7 +911,7 @@ int32_t read_stats(mt_context* ctx, RenderPlan& R, mt_render_stats* stats)
     stats->closest_launches = stats->any_launches = 0;
     stats->closest_ms = stats->any_ms = 0.0f;
     if (R.profile) {
-        for (int b = 0; b < nb; b++) {
+        for (int b = 0; b < (R.tail ? R.tail : nb); b++) { // k_tail's bounces have no per-stage events
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, R.ev[4 * b], R.ev[4 * b + 1]) == hipSuccess) {
                 stats->closest_ms += ms;

@@ -120,6 +120,42 @@ def test_c1_full_guided_frames(ctx, tile_pdf):
             assert not bad.any()
 
 
+def test_c1_full_tail_matches_wavefront(ctx):
+    """Config 1 (guided light, 1920x1080 x 16 spp) with bounces 3.. in one
+    k_tail launch (MT_RENDER_TAIL_FROM(3), the benchmark's setting) against
+    the per-bounce wavefront, from the same grid state: image, ray counts,
+    per-bounce path counts and the grid's sums are bit-identical."""
+    print("[case] c1 full tail vs wavefront", flush=True)
+    points, normals, faces, nodes, bounds, depth = W.c1_scene()
+    rgba = W.env_map()
+    lum, _ = orc.env_lum_sat(rgba.reshape(-1, 4), 1024, 512)
+    sc = api.Scene(ctx, points, faces, nodes, bounds, depth, normals=normals)
+    env = api.CEnvironment(ctx)
+    env.setEnvironmentMap(rgba)
+    tile = api.lightgrid_seed_tile(lum, 1024, 512)
+    grid = api.CAdaptiveEnvironment(ctx)
+    job = W.RenderJob(flags=_lib.MT_RENDER_LIGHTGRID_ENV, seed=7)
+    grid.setCameraPosition(job.pos)
+    env.attachLightGrid(grid)
+    dm = ctx.to_device(W.c1_materials())
+    out = []
+    for flags in (_lib.MT_RENDER_LIGHTGRID_ENV, _lib.MT_RENDER_LIGHTGRID_ENV | _lib.MT_RENDER_TAIL_FROM(3)):
+        grid.clearBuffers()
+        grid.setHDRLuminance(tile)
+        job.flags = flags
+        img, st = api.render(ctx, sc, dm, env, job)
+        torch.cuda.synchronize()
+        out.append((img.cpu().numpy(), st, grid.read_arrays()[0].copy()))
+    (img0, st0, acc0), (img1, st1, acc1) = out
+    bad = np.any(img0.view(np.int32) != img1.view(np.int32), axis=1)
+    print(f"rays {st0.extension_rays}+{st0.shadow_rays} vs {st1.extension_rays}+{st1.shadow_rays}; "
+          f"per bounce {list(st0.bounce_counts)[:8]}; pixels differing {int(bad.sum())}", flush=True)
+    assert not bad.any()
+    assert (int(st0.extension_rays), int(st0.shadow_rays)) == (int(st1.extension_rays), int(st1.shadow_rays))
+    assert list(st0.bounce_counts) == list(st1.bounce_counts)
+    assert np.array_equal(acc0.view(np.int32), acc1.view(np.int32)) and acc0.sum() > 0
+
+
 def test_c2_full_batch_sampled_rays_and_whole_batch_properties(ctx):
     print("[case] c2 full batch (64M rays), sampled rays", flush=True)
     points, faces, nodes, bounds, depth = W.c2_scene()

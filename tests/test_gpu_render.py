@@ -153,6 +153,10 @@ def test_fused_and_staged_loops_agree(ctx, c1_small, mesh_lights):
     runs = [W.RenderJob(**base),                                           # fused
             W.RenderJob(**base, mode=_lib.MT_TRACE_PACKED_SORTED),         # fused, direction-sorted lanes
             W.RenderJob(**base, mode=_lib.MT_TRACE_PACKED_POOL),           # fused, wave-pool lanes
+            W.RenderJob(**base, flags=_lib.MT_RENDER_TAIL_FROM(1)),        # fused, bounces 1.. in one launch
+            W.RenderJob(**base, flags=_lib.MT_RENDER_TAIL_FROM(3)),        # fused, bounces 3.. in one launch
+            W.RenderJob(**base, mode=_lib.MT_TRACE_PACKED_POOL,            # wave pools, then the tail
+                        flags=_lib.MT_RENDER_TAIL_FROM(2)),
             W.RenderJob(**base, flags=_lib.MT_RENDER_TRAVERSAL_STATS),     # staged, same traversal
             W.RenderJob(**base, mode=_lib.MT_TRACE_PACKED_NO_STACK),       # staged, parent-link pops
             W.RenderJob(**base, mode=_lib.MT_TRACE_REFERENCE)]             # staged, reference arrays
@@ -177,11 +181,13 @@ def test_tiny_jobs_fused_and_staged_agree(ctx, c1_small, w, h, spp):
     env = api.CEnvironment(ctx)
     env.setEnvironmentMap(W.env_map())
     base = dict(width=w, height=h, spp=spp, pos=(0.0, 0.5, -8.0), look=(0.0, -1.0, 0.0), seed=9)
-    (img_f, st_f), (img_s, st_s) = [api.render(ctx, sc, dm, env, W.RenderJob(**base, **extra))
-                                    for extra in ({}, {"flags": _lib.MT_RENDER_TRAVERSAL_STATS})]
-    np.testing.assert_array_equal(img_f.cpu().numpy().view(np.int32), img_s.cpu().numpy().view(np.int32))
-    assert list(st_f.bounce_counts) == list(st_s.bounce_counts)
-    assert int(st_f.shadow_rays) == int(st_s.shadow_rays)
+    (img_f, st_f), (img_s, st_s), (img_t, st_t) = [
+        api.render(ctx, sc, dm, env, W.RenderJob(**base, **extra))
+        for extra in ({}, {"flags": _lib.MT_RENDER_TRAVERSAL_STATS}, {"flags": _lib.MT_RENDER_TAIL_FROM(1)})]
+    for img, st in ((img_s, st_s), (img_t, st_t)):
+        np.testing.assert_array_equal(img_f.cpu().numpy().view(np.int32), img.cpu().numpy().view(np.int32))
+        assert list(st_f.bounce_counts) == list(st.bounce_counts)
+        assert int(st_f.shadow_rays) == int(st.shadow_rays)
     assert st_f.bounce_counts[0] == w * h * spp == int(st_f.paths)
     assert int(st_f.extension_rays) == sum(st_f.bounce_counts)
     assert np.isfinite(img_f.cpu().numpy()).all()

@@ -56,12 +56,14 @@ def _diff(name, a, b):
     return n
 
 
-@pytest.mark.parametrize("tile_pdf", [False, True])
-def test_guided_frames_match_oracle(ctx, c1_small, tile_pdf):
+@pytest.mark.parametrize("tile_pdf,tail", [(False, 0), (True, 0), (False, 2), (True, 1)])
+def test_guided_frames_match_oracle(ctx, c1_small, tile_pdf, tail):
+    """tail > 0: bounces tail.. run in one k_tail launch (MT_RENDER_TAIL_FROM)."""
     flags = _lib.MT_RENDER_LIGHTGRID_ENV | (_lib.MT_RENDER_LIGHTGRID_TILE_PDF if tile_pdf else 0)
+    flags |= _lib.MT_RENDER_TAIL_FROM(tail)
     job = W.RenderJob(width=96, height=54, spp=4, max_bounces=6, pos=(0.0, 0.5, -8.0), look=(0.0, -1.0, 0.0),
                       flags=flags)
-    print(f"[case] guided frames tile_pdf={tile_pdf}", flush=True)
+    print(f"[case] guided frames tile_pdf={tile_pdf} tail={tail}", flush=True)
     points, normals, faces, nodes, bounds, depth = c1_small
     sc, mats, env, oenv, grid, og = _setup(ctx, c1_small, job)
     dm = ctx.to_device(mats)
@@ -103,11 +105,15 @@ def test_guided_flags_need_a_grid(ctx, c1_small):
         api.render(ctx, sc, dm, env, job)
 
 
-def test_guided_render_graph_replays_match_render(ctx, c1_small):
+@pytest.mark.parametrize("tail", [0, 3])
+def test_guided_render_graph_replays_match_render(ctx, c1_small, tail):
     """A captured guided render replays the same image and grid sums as
-    mt_render from the same grid state (eager work between replays)."""
+    mt_render from the same grid state (eager work between replays); with
+    tail > 0 the capture holds the one-launch tail (MT_RENDER_TAIL_FROM)."""
+    print(f"[case] guided render graph tail={tail}", flush=True)
     job = W.RenderJob(width=64, height=40, spp=2, max_bounces=5, pos=(0.0, 0.5, -8.0), look=(0.0, -1.0, 0.0),
-                      flags=_lib.MT_RENDER_LIGHTGRID_ENV | _lib.MT_RENDER_LIGHTGRID_TILE_PDF)
+                      flags=_lib.MT_RENDER_LIGHTGRID_ENV | _lib.MT_RENDER_LIGHTGRID_TILE_PDF
+                      | _lib.MT_RENDER_TAIL_FROM(tail))
     sc, mats, env, oenv, grid, og = _setup(ctx, c1_small, job)
     dm = ctx.to_device(mats)
     g = api.RenderGraph(ctx, sc, dm, env, job)
