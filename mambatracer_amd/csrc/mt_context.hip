@@ -40,6 +40,7 @@ extern "C" int32_t mt_ctx_create(int32_t device, void* stream, int32_t flags, mt
     c->scratch_bytes = 0;
     c->temp = nullptr;
     c->temp_bytes = 0;
+    c->side = nullptr;
     if (!(flags & MT_CTX_OWN_STREAM)) {
         c->stream = (hipStream_t)stream;
         c->own_stream = false;
@@ -64,9 +65,28 @@ extern "C" int32_t mt_ctx_destroy(mt_context* ctx)
     fe(hipStreamSynchronize(ctx->stream));
     if (ctx->scratch) fe(hipFree(ctx->scratch));
     if (ctx->temp) fe(hipFree(ctx->temp));
+    if (ctx->side) {
+        fe(hipStreamSynchronize(ctx->side));
+        fe(hipStreamDestroy(ctx->side));
+    }
     if (ctx->own_stream) fe(hipStreamDestroy(ctx->stream));
     delete ctx;
     return fe.status("mt_ctx_destroy");
+}
+
+int32_t mt_ctx_side_stream(mt_context* ctx, hipStream_t* out)
+{
+    if (!ctx->side) {
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            ctx->side = nullptr;
+            mt_set_error("side stream: hipStreamCreate failed: %s", hipGetErrorString(e));
+            return MT_ERR_HIP;
+        }
+    }
+    *out = ctx->side;
+    return MT_OK;
 }
 
 extern "C" int32_t mt_ctx_sync(mt_context* ctx)

@@ -63,7 +63,15 @@ struct mt_context {
     // words above, which must only ever hold epoch-tagged values).
     void* temp;
     size_t temp_bytes;
+    // A second stream of the context's own (default priority), created on
+    // first use: work forked to it from `stream` through an event is joined
+    // back to `stream` through another before the call returns, so callers
+    // still see one ordered stream.
+    hipStream_t side;
 };
+
+// The context's side stream (created on first use).
+int32_t mt_ctx_side_stream(mt_context* ctx, hipStream_t* out);
 
 // Packed traversal layout derived from the reference arrays.
 //   node (64 B): float4 a = (L.min.xyz, L.max.x), b = (L.max.yz, R.min.xy),

@@ -536,6 +536,12 @@ struct RenderPlan {
     bool borrowed; // base is the context's reusable workspace (mt_render), not owned
     hipEvent_t ev[4 * 64];
     int nev;
+    // fused loop without per-stage timing: bounce b's any hit runs on the
+    // context's side stream, concurrently with bounce b+1's closest hit
+    // (fork[b] after b's shading, join[b] before b+1's shading)
+    hipEvent_t fork[64], join[64];
+    int nfj;
+    hipStream_t side;
 };
 
 int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envmap* env,
@@ -558,6 +564,8 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
     R.base = nullptr;
     R.borrowed = false;
     R.nev = 0;
+    R.nfj = 0;
+    R.side = nullptr;
     int32_t rc = camera_from_desc(desc, R.C);
     if (rc) return rc;
     R.V.points = reinterpret_cast<const float4*>(scene->desc.points);
@@ -630,17 +638,22 @@ void free_render(RenderPlan& R)
 {
     for (int k = 0; k < R.nev; k++) (void)hipEventDestroy(R.ev[k]);
     R.nev = 0;
+    for (int k = 0; k < R.nfj; k++) {
+        (void)hipEventDestroy(R.fork[k]);
+        (void)hipEventDestroy(R.join[k]);
+    }
+    R.nfj = 0;
     if (R.base && !R.borrowed) (void)hipFree(R.base);
     R.base = nullptr;
 }
 
-// ctx != nullptr: the state lives in the context's grow-only workspace
-// (calls on a context are serialised and mt_render synchronises before it
-// returns), so repeated renders allocate nothing; otherwise it is owned.
-int32_t alloc_render(RenderPlan& R, mt_context* ctx)
+// borrow: the state lives in the context's grow-only workspace (calls on a
+// context are serialised and mt_render synchronises before it returns), so
+// repeated renders allocate nothing; otherwise it is owned.
+int32_t alloc_render(RenderPlan& R, mt_context* ctx, bool borrow)
 {
-    R.borrowed = ctx != nullptr;
-    if (ctx) {
+    R.borrowed = borrow;
+    if (borrow) {
         void* p = nullptr;
         int32_t rc = mt_ctx_temp(ctx, R.bytes, &p);
         if (rc) {
@@ -660,6 +673,27 @@ int32_t alloc_render(RenderPlan& R, mt_context* ctx)
                 mt_set_error("mt_render: hipEventCreate failed");
                 return MT_ERR_HIP;
             }
+    }
+    R.side = nullptr;
+    if (R.fused && !R.profile && R.nb > 1) {
+        int32_t rc = mt_ctx_side_stream(ctx, &R.side);
+        if (rc) {
+            free_render(R);
+            return rc;
+        }
+        for (; R.nfj < R.nb; R.nfj++) {
+            if (hipEventCreateWithFlags(&R.fork[R.nfj], hipEventDisableTiming) != hipSuccess) {
+                free_render(R);
+                mt_set_error("mt_render: hipEventCreate failed");
+                return MT_ERR_HIP;
+            }
+            if (hipEventCreateWithFlags(&R.join[R.nfj], hipEventDisableTiming) != hipSuccess) {
+                (void)hipEventDestroy(R.fork[R.nfj]);
+                free_render(R);
+                mt_set_error("mt_render: hipEventCreate failed");
+                return MT_ERR_HIP;
+            }
+        }
     }
     return MT_OK;
 }
@@ -711,6 +745,22 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         const hipError_t x = hipEventRecord(e, st);
         if (x == hipSuccess) return MT_OK;
         mt_set_error("mt_render: hipEventRecord failed: %s", hipGetErrorString(x));
+        return MT_ERR_HIP;
+    };
+
+    // bounce b's any hit on the side stream while bounce b+1's closest hit
+    // runs here: they read and write disjoint state (the shadow queue and
+    // the radiance / the next queue and the hit records); b+1's shading
+    // waits for it (its radiance adds come after b's, and it refills the
+    // shadow queue)
+    const bool overlap = R.side != nullptr && R.nfj == nb;
+    int pending = -1; // bounce whose any hit runs on the side stream
+    auto join_pending = [&]() -> int32_t {
+        if (pending < 0) return MT_OK;
+        const hipError_t x = hipStreamWaitEvent(st, R.join[pending], 0);
+        pending = -1;
+        if (x == hipSuccess) return MT_OK;
+        mt_set_error("mt_render: hipStreamWaitEvent failed: %s", hipGetErrorString(x));
         return MT_ERR_HIP;
     };
 
@@ -770,6 +820,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         int* dcount = counts + b;
         const bool first = fused && b == 0;
         if (R.tail && b == R.tail) { // every remaining bounce in one launch
+            if ((rc = join_pending())) break;
             k_tail<<<SHADOW_SEGS * TAIL_SWEEP, RT, 0, st>>>(shade_args(b, false), scene->pnodes, scene->ptris,
                                                           scene->root_ref, nb, I(R.oTail), I(R.oTail) + 64);
             rc = chk();
@@ -797,6 +848,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         if (rc) break;
         if (profile && (rc = mark(ev[4 * b + 1]))) break;
         if (ilp) k_sum_ilp<<<grid, RT, 0, st>>>(ilp, dcount, nullptr, 0, tot);
+        if ((rc = join_pending())) break;
         ShadeArgs A = shade_args(b, first);
         int* seg_counts = A.shadow_count;
         if (A.in_seg) { // escaped paths here, the hits densely in k_shade
@@ -811,15 +863,35 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         if ((rc = chk())) break;
         if (profile && (rc = mark(ev[4 * b + 2]))) break;
         if (fused) {
+            hipStream_t ss = st;
+            if (overlap) { // fork: the any hit follows this shading on the side stream
+                hipError_t x = hipEventRecord(R.fork[b], st);
+                if (x == hipSuccess) x = hipStreamWaitEvent(R.side, R.fork[b], 0);
+                if (x != hipSuccess) {
+                    mt_set_error("mt_render: fork to the side stream failed: %s", hipGetErrorString(x));
+                    rc = MT_ERR_HIP;
+                    break;
+                }
+                ss = R.side;
+            }
             if (R.pool)
-                k_trace_shadow<POOL_RAYS><<<gshadow_pool, RT, 0, st>>>(
+                k_trace_shadow<POOL_RAYS><<<gshadow_pool, RT, 0, ss>>>(
                     scene->pnodes, scene->ptris, scene->root_ref, seg_counts, seg_cap, F4(R.oSo), F4(R.oSd),
                     F4(R.oSc), first ? nullptr : q[cur], S.L, A.s_lg, R.grid ? R.grid->pend : nullptr);
             else
-                k_trace_shadow<0><<<gsweep, RT, 0, st>>>(
+                k_trace_shadow<0><<<gsweep, RT, 0, ss>>>(
                     scene->pnodes, scene->ptris, scene->root_ref, seg_counts, seg_cap, F4(R.oSo), F4(R.oSd),
                     F4(R.oSc), first ? nullptr : q[cur], S.L, A.s_lg, R.grid ? R.grid->pend : nullptr);
             if ((rc = chk())) break;
+            if (overlap) {
+                const hipError_t x = hipEventRecord(R.join[b], R.side);
+                if (x != hipSuccess) {
+                    mt_set_error("mt_render: join event failed: %s", hipGetErrorString(x));
+                    rc = MT_ERR_HIP;
+                    break;
+                }
+                pending = b;
+            }
             if (profile && (rc = mark(ev[4 * b + 3]))) break;
         } else {
             // any hit over the segmented shadow queue (segment lengths on the device)
@@ -840,6 +912,10 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
             if (rc) break;
             cur = 1 - cur;
         }
+    }
+    { // the last any hit joins back (also after an error: nothing may still run on the side stream)
+        const int32_t jr = join_pending();
+        if (rc == MT_OK) rc = jr;
     }
     if (rc == MT_OK) {
         k_resolve<<<mt_blocks(R.npix, resolve_pixels_per_group(desc->spp)), RT, 0, st>>>(S.L, desc->width, desc->height, desc->spp,
@@ -936,7 +1012,7 @@ extern "C" int32_t mt_render(mt_context* ctx, const mt_scene* scene, const float
     RenderPlan R;
     int32_t rc = plan_render(scene, materials, env, light_faces, d_num_lights, desc, image, R);
     if (rc) return rc;
-    rc = alloc_render(R, ctx);
+    rc = alloc_render(R, ctx, true);
     if (rc) return rc;
     k_set_seed<<<1, 64, 0, ctx->stream>>>(seed_ptr(R), desc->seed);
     rc = enqueue_render(ctx, R);
@@ -991,8 +1067,9 @@ extern "C" int32_t mt_render_graph_create(mt_context* ctx, const mt_scene* scene
     g->ctx = ctx;
     g->R.base = nullptr;
     g->R.nev = 0;
+    g->R.nfj = 0;
     int32_t rc = plan_render(scene, materials, env, light_faces, d_num_lights, desc, image, g->R);
-    if (rc == MT_OK) rc = alloc_render(g->R, nullptr);
+    if (rc == MT_OK) rc = alloc_render(g->R, ctx, false);
     if (rc) {
         delete g;
         return rc;
