@@ -766,8 +766,8 @@ def bench_c2(args) -> int:
                    "rays": n, "traversal_layout": "packed",
                    "traversal_modes": {k: {3: "short stack", 4: "persistent waves with per-lane refill",
                                            6: "wave pools with lane refill",
-                                           7: "rays counting-sorted by origin cell and direction octant, then "
-                                              "persistent waves with refill"}.get(v, v)
+                                           7: "rays counting-sorted by origin cell (32^3 Morton), then "
+                                              "wave pools over that order"}.get(v, v)
                                        for k, v in modes.items()}},
         "stages_ms": {"closest": round(stage[0], 3), "shadow_setup": round(stage[1], 3),
                       "any": round(stage[2], 3), "compaction": round(stage[3], 4)},
@@ -776,7 +776,7 @@ def bench_c2(args) -> int:
         "compaction_GB_per_s": round(cbytes / stage[3] / 1e6, 1),
         "roofline": {
             "kernel": "closest-hit traversal (packed layout, "
-                      + {4: "k_trace_refill", 6: "k_trace_pool", 7: "k_bin_hist + k_bin_scatter + k_trace_refill"}
+                      + {4: "k_trace_refill", 6: "k_trace_pool", 7: "k_bin_hist + k_bin_scatter + k_trace_pool"}
                       .get(modes["closest"], "k_trace") + ")",
             "bound": "hbm",
             "achieved": round(traffic / (stage[0] / 1e3) / 1e9, 1) if traffic else None,

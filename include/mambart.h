@@ -203,9 +203,10 @@ int32_t mt_scene_destroy(mt_scene* scene);
                                       direction-bin order (same results per ray) */
 #define MT_TRACE_PACKED_POOL 6     /* packed, short stack, each wave refills finished lanes
                                       from its own run of 256 rays (same results per ray) */
-#define MT_TRACE_PACKED_BINNED 7   /* packed: the batch counting-sorted by origin cell and
-                                      direction octant on the device, then traced with
-                                      refill (same results per ray; for incoherent batches) */
+#define MT_TRACE_PACKED_BINNED 7   /* packed: the batch counting-sorted by origin cell
+                                      (32^3 grid, Morton order) on the device, then traced
+                                      in that order by wave pools (same results per ray;
+                                      for incoherent batches) */
 
 /* Closest hit (intersectMesh with bailout = false, NormalBVH.cl:14-92, as
  * run by IntersectPrimitives :94-123; launched at CDeviceGI.java:230 and

@@ -281,7 +281,7 @@ def test_lane_mappings_same_results(ctx, c0, any_hit, mode):
     """MT_TRACE_PACKED_SORTED (each work-group's rays taken in direction-bin
     order), MT_TRACE_PACKED_POOL (each wave refilling its finished lanes
     from its own run of rays) and MT_TRACE_PACKED_BINNED (the batch sorted by
-    origin cell and direction octant first) return exactly the per-ray
+    origin cell first) return exactly the per-ray
     results of one ray per lane, on camera rays and on random rays with a
     live length that ends mid-group and mid-run."""
     print(f"[case] lane mapping mode={mode} any={any_hit}", flush=True)
