@@ -75,8 +75,8 @@ __device__ __forceinline__ int queue_entry(const TraceArgs& A)
     return blockIdx.x * blockDim.x + threadIdx.x;
 }
 
-template <bool ANY, bool STATS>
-__device__ __forceinline__ void store_result(const TraceArgs& A, int id, const RayT& r, const TravState& S)
+template <bool ANY, bool STATS, class State>
+__device__ __forceinline__ void store_result(const TraceArgs& A, int id, const RayT& r, const State& S)
 {
     if (A.order) id = A.order[id];
     if (ANY) {
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
 // consecutive queue entries (segmented queue: work-group b takes chunk
 // b / nseg of segment b % nseg, 256 * POOL entries, one run per wave) and
 // refills its finished lanes from it.
-template <bool ANY, int POOL>
+template <bool ANY, int POOL, int SS = MT_SHORT_STACK>
 __global__ __launch_bounds__(256) void k_trace_pool(TraceArgs A)
 {
     constexpr int RUN = 64 * POOL;
@@ -217,13 +217,13 @@ __global__ __launch_bounds__(256) void k_trace_pool(TraceArgs A)
         end = count < next + RUN ? count : next + RUN;
     }
     if (next >= end) return;
-    pool_trace<ANY>(
+    pool_trace<ANY, SS>(
         A.pn, A.pt, A.root, next, end,
         [&](int id, RayT& r, const float4*&, const float4*&, int&) {
             const int src = A.order ? A.order[id] : id; // binned: entry id is ray order[id]
             ray_init(r, A.o_tmin[src], A.d_tmax[src]);
         },
-        [&](bool fin, int id, const RayT& r, const TravState& S) {
+        [&](bool fin, int id, const RayT& r, const TravStateN<SS>& S) {
             if (fin) store_result<ANY, false>(A, id, r, S);
         });
 }
@@ -525,12 +525,14 @@ int32_t launch_trace(mt_context* ctx, const mt_scene* sc, const float* o_tmin,
     if (mode == MT_TRACE_PACKED_POOL) {
         // sorted (binned) batches take runs of 8 rays per lane (64M random
         // rays, closest / any hit: 49.9 / 34.8 ms against 51.0 / 35.8 with
-        // 4); the render's bounce queues keep 4 (8 cost its job 1%)
+        // 4) and a deeper short stack; the render's bounce queues keep 4
+        // rays and 8 entries (8 rays cost its job 1%, 12 entries 0.7%)
         const int rays = A.order ? BINNED_POOL_RAYS : POOL_RAYS;
         const int64_t per_group = 256 * (int64_t)rays;
         const int64_t groups = seg.counts ? (int64_t)seg.nseg * ((seg.seg_cap + per_group - 1) / per_group)
                                           : (capacity + per_group - 1) / per_group;
-        if (A.order) k_trace_pool<ANY, BINNED_POOL_RAYS><<<dim3((unsigned)groups), block, 0, s>>>(A);
+        if (A.order)
+            k_trace_pool<ANY, BINNED_POOL_RAYS, SHORT_STACK_DEEP><<<dim3((unsigned)groups), block, 0, s>>>(A);
         else k_trace_pool<ANY, POOL_RAYS><<<dim3((unsigned)groups), block, 0, s>>>(A);
     } else if (mode == MT_TRACE_PACKED_SORTED) {
         k_trace_sorted<ANY><<<grid, block, 0, s>>>(A);

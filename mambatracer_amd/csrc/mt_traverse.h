@@ -110,32 +110,34 @@ struct BitStack {
 // dropped, the parent walk finds them exactly as the reference does, reading
 // parent and sibling links from the current node's record (an interior
 // record's last word pair, a leaf triangle's spare words), so no links are
-// carried in registers while descending.  Twelve entries: on a deep tree
-// (4M triangles, midpoint splits, depth 45) they save parent walks (closest
-// hit of 64M random rays 68.6 ms with 8, 66.1 with 12, 65.5 with 16), and
-// on the 1M-triangle render they cost nothing (16 cost 2.5% in registers).
+// carried in registers while descending.  Depth N is a trade: each push
+// and pop moves N registers, and a deeper stack saves parent walks on deep
+// trees.  MT_SHORT_STACK (8) serves the render's 1M-triangle scene; the wave
+// pools over sorted batches of deep trees use SHORT_STACK_DEEP.
 #ifndef MT_SHORT_STACK
-#define MT_SHORT_STACK 12
+#define MT_SHORT_STACK 8
 #endif
-struct ShortStack {
-    int ref[MT_SHORT_STACK];
+template <int N>
+struct ShortStackN {
+    int ref[N];
     int n;
     __device__ __forceinline__ void push(int r)
     {
 #pragma unroll
-        for (int k = MT_SHORT_STACK - 1; k > 0; k--) ref[k] = ref[k - 1];
+        for (int k = N - 1; k > 0; k--) ref[k] = ref[k - 1];
         ref[0] = r;
-        n = n < MT_SHORT_STACK ? n + 1 : MT_SHORT_STACK;
+        n = n < N ? n + 1 : N;
     }
     __device__ __forceinline__ int pop()
     {
         const int r = ref[0];
 #pragma unroll
-        for (int k = 0; k < MT_SHORT_STACK - 1; k++) ref[k] = ref[k + 1];
+        for (int k = 0; k < N - 1; k++) ref[k] = ref[k + 1];
         n--;
         return r;
     }
 };
+using ShortStack = ShortStackN<MT_SHORT_STACK>;
 
 // -------------------------------------------------------------------------
 // Reference layout: NormalBVH.cl:14-92 over nodes[7], bounds[8], faces[10].
@@ -238,11 +240,12 @@ __device__ __forceinline__ bool parent_walk(const float4* __restrict__ pn, const
     return true;
 }
 
-// Resumable traversal state of one ray (packed layout).
-struct TravState {
+// Resumable traversal state of one ray (packed layout), short stack of N.
+template <int N>
+struct TravStateN {
     int cur;
     BitStack bs;
-    ShortStack ss;
+    ShortStackN<N> ss;
     int hit, child;
     int st[3];
     __device__ __forceinline__ void init(int root)
@@ -256,6 +259,7 @@ struct TravState {
         st[0] = st[1] = st[2] = 0;
     }
 };
+using TravState = TravStateN<MT_SHORT_STACK>;
 
 // One outer iteration of the packed traversal: descend to a leaf (or a node
 // whose children both miss), test the leaf, move to the next pending subtree.
@@ -322,9 +326,9 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
 // trace_packed_step in the same order; a wave that calls this in a loop
 // keeps every lane on a node of its own ray each iteration instead of
 // holding the lanes that reached a leaf until the whole wave has.
-template <bool ANY, bool STATS>
+template <bool ANY, bool STATS, int DEPTH>
 __device__ __forceinline__ bool trace_packed_node(const float4* __restrict__ pn, const float4* __restrict__ pt,
-                                                  RayT& r, TravState& S)
+                                                  RayT& r, TravStateN<DEPTH>& S)
 {
     if (S.cur >= 0) {
         const float4* N = pn + 4 * (size_t)S.cur;
@@ -388,14 +392,14 @@ constexpr int POOL_REFILL_MIN = 16;
 // iteration with the whole wave active, fin set on the lanes whose ray ended
 // in it.  Each ray is traversed in the reference's
 // visiting order; only the lane that computes it changes.
-template <bool ANY, class Load, class Done>
+template <bool ANY, int N = MT_SHORT_STACK, class Load, class Done>
 __device__ __forceinline__ void pool_trace(const float4* __restrict__ pn, const float4* __restrict__ pt, int root,
                                            int next, int end, Load&& load, Done&& done)
 {
     bool active = false;
     int id = 0;
     RayT r;
-    TravState S;
+    TravStateN<N> S;
     const float4* lpn = pn; // this lane's scene (load may change it)
     const float4* lpt = pt;
     int lroot = root;
@@ -443,6 +447,10 @@ __device__ __forceinline__ void pool_seg_run(const int* __restrict__ counts, int
 
 // Rays per lane of a wave's run in the pool kernels.
 constexpr int POOL_RAYS = 4;
+// Short-stack depth of the wave pools over sorted (binned) batches: a deep
+// tree's rays keep more postponed subtrees in registers (64M random rays
+// through 4M triangles, depth 45: see mt_trace.hip).
+constexpr int SHORT_STACK_DEEP = 12;
 
 template <bool ANY, bool STATS, bool SHORT_STACK>
 __device__ __forceinline__ int trace_packed(const float4* __restrict__ pn,
