@@ -704,10 +704,16 @@ int32_t mt_batch_set_iterations(mt_batch* batch, int32_t iterations);
  * Synchronises the context. */
 int32_t mt_batch_set_graphs(mt_batch* batch, int32_t on);
 /* Closest-hit traversal of the engine's shared queue (no counterpart in the
- * reference): MT_TRACE_PACKED / MT_TRACE_PACKED_STACK (default: one ray per
- * lane) or MT_TRACE_PACKED_POOL (wave pools with lane refill; each lane
- * loads its slot's scene).  Images do not depend on it.  The choice the
- * serving autotuner makes per workload.  Synchronises the context. */
+ * reference): MT_BATCH_TRACE_SPLIT (MT_TRACE_PACKED, the default: the
+ * queue's front, the paths continuing from earlier bounces, with wave pools;
+ * its back, the camera rays of newly started paths, one ray per lane),
+ * MT_TRACE_PACKED_STACK (one ray per lane) or MT_TRACE_PACKED_POOL (wave
+ * pools with lane refill; each lane loads its slot's scene).  Images do not
+ * depend on it.  The choice the serving autotuner makes per workload
+ * (multi-job workloads of 256 uneven jobs, and of Poisson arrivals: split
+ * 64.0 jobs/s, one ray per lane 60.4, pools 59.8).  Synchronises the
+ * context. */
+#define MT_BATCH_TRACE_SPLIT 8
 int32_t mt_batch_set_trace_mode(mt_batch* batch, int32_t mode);
 /* Cancels the job with this tag (CDeviceGI.stop, CDeviceGI.java:438-439,
  * ends the reference's one running job; here any one job of the mix).

@@ -28,6 +28,7 @@ MT_TRACE_PACKED_REFILL = 4
 MT_TRACE_PACKED_SORTED = 5
 MT_TRACE_PACKED_POOL = 6
 MT_TRACE_PACKED_BINNED = 7
+MT_BATCH_TRACE_SPLIT = 8  # batch engine only: pools for continuing paths, one ray per lane for new ones
 MT_BSDF_WORDS = 20
 
 

@@ -949,7 +949,9 @@ class Batch:
 
     def set_trace_mode(self, mode: int) -> None:
         """Closest-hit traversal of the shared queue: _lib.MT_TRACE_PACKED_STACK
-        (one ray per lane) or _lib.MT_TRACE_PACKED_POOL (mt_batch_set_trace_mode)."""
+        (one ray per lane), _lib.MT_TRACE_PACKED_POOL (wave pools) or
+        _lib.MT_BATCH_TRACE_SPLIT (pools for the continuing paths, one ray per
+        lane for the new camera rays) (mt_batch_set_trace_mode)."""
         call("mt_batch_set_trace_mode", self.handle, int(mode))
 
     def close(self) -> None:

@@ -344,7 +344,7 @@ struct mt_batch {
     int policy;
     int iters_per_step;       // wavefront iterations between host round trips
     int* nact;                // device: number of active jobs (read by the iteration kernels)
-    int trace_mode;           // closest hit: MT_TRACE_PACKED_STACK (one ray per lane) or _POOL
+    int trace_mode;           // closest hit: MT_BATCH_TRACE_SPLIT (default), MT_TRACE_PACKED_STACK or _POOL
     // The iterations of a step as one hipGraph per start parity of the queue
     // ping-pong: every size they use lives in device memory, so a capture
     // stays valid until the iterations per step change.
@@ -492,7 +492,7 @@ int32_t iterate(mt_batch* b)
     k_generate<<<grid, BT, 0, s>>>(b->jobs, b->cnt, b->gen_start, b->gen_pid0, b->gen_job, b->nact, q);
     MT_LAUNCH_CHECK();
     int32_t rc = mt_trace_jobs(b->ctx, b->refs, &q.tag->x, BOUNCE_BITS, q.ro, q.rd, b->cnt + 1, Qc, b->prim,
-                               b->t, b->trace_mode);
+                               b->t, b->trace_mode, b->cnt); // cnt[0]: survivors, cnt[1]: survivors + new
     if (rc) return rc;
     ShadeJobsArgs A;
     A.jobs = b->jobs;
@@ -728,7 +728,7 @@ extern "C" int32_t mt_batch_create(mt_context* ctx, int32_t queue_capacity, uint
     b->policy = MT_BATCH_OLDEST_FIRST;
     b->iters_per_step = 1;
     b->use_graphs = true;
-    b->trace_mode = MT_TRACE_PACKED_STACK;
+    b->trace_mode = MT_BATCH_TRACE_SPLIT; // the serving autotuner's choice on both multi-job workloads
     for (int p = 0; p < 2; p++) {
         b->graph[p] = nullptr;
         b->gexec[p] = nullptr;
@@ -888,10 +888,11 @@ extern "C" int32_t mt_batch_set_iterations(mt_batch* b, int32_t iterations)
 extern "C" int32_t mt_batch_set_trace_mode(mt_batch* b, int32_t mode)
 {
     MT_ARG_CHECK(b, "mt_batch_set_trace_mode: NULL batch");
-    MT_ARG_CHECK(mode == MT_TRACE_PACKED || mode == MT_TRACE_PACKED_STACK || mode == MT_TRACE_PACKED_POOL,
+    MT_ARG_CHECK(mode == MT_TRACE_PACKED || mode == MT_TRACE_PACKED_STACK || mode == MT_TRACE_PACKED_POOL ||
+                     mode == MT_BATCH_TRACE_SPLIT,
                  "mt_batch_set_trace_mode: mode %d not supported by the batch engine", mode);
     MT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream)); // recorded graphs hold the old kernel
-    b->trace_mode = mode == MT_TRACE_PACKED ? MT_TRACE_PACKED_STACK : mode;
+    b->trace_mode = mode == MT_TRACE_PACKED ? MT_BATCH_TRACE_SPLIT : mode;
     drop_graphs(b);
     return MT_OK;
 }

@@ -159,9 +159,11 @@ struct MtSegQueue {
 
 // Closest hit (prim, t) over a queue that mixes jobs: slot i's scene is
 // refs[tag[2*i] >> shift]; entries i < *d_count (clamped to capacity).
+// mode MT_BATCH_TRACE_SPLIT: entries below *d_split (the earlier bounces'
+// survivors) with wave pools, the rest (new camera rays) one ray per lane.
 int32_t mt_trace_jobs(mt_context* ctx, const MtSceneRef* refs, const int32_t* tag, int32_t shift,
                       const float4* o_tmin, const float4* d_tmax, const int32_t* d_count, int32_t capacity,
-                      int32_t* prim, float* t, int32_t mode);
+                      int32_t* prim, float* t, int32_t mode, const int32_t* d_split);
 // Any hit over a segmented queue for one scene (traversal mode as
 // mt_trace_any; the per-lane refill mode traverses like the short-stack mode
 // here); stats as mt_trace_any, indexed by queue entry.

@@ -302,9 +302,10 @@ __global__ __launch_bounds__(256) void k_trace_sorted(TraceArgs A)
 // (the job word of a wavefront slot).  Same per-ray traversal as k_trace
 // LAYOUT 0, so a ray's result does not depend on which queue carries it.
 __global__ __launch_bounds__(256) void k_trace_jobs(const MtSceneRef* __restrict__ refs, const int* __restrict__ tag,
-                                                    int shift, TraceArgs A)
+                                                    int shift, TraceArgs A, const int* __restrict__ first)
 {
-    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    // first != nullptr: the entries from *first on (grid sized for all)
+    const int id = (first ? *first : 0) + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (!entry_live(A, id)) return;
     const MtSceneRef R = refs[tag[2 * id] >> shift];
     RayT r;
@@ -1062,7 +1063,7 @@ int32_t mt_trace_any_seg(mt_context* ctx, const mt_scene* scene, const float4* o
 
 int32_t mt_trace_jobs(mt_context* ctx, const MtSceneRef* refs, const int32_t* tag, int32_t shift,
                       const float4* o_tmin, const float4* d_tmax, const int32_t* d_count, int32_t capacity,
-                      int32_t* prim, float* t, int32_t mode)
+                      int32_t* prim, float* t, int32_t mode, const int32_t* d_split)
 {
     if (capacity == 0) return MT_OK;
     TraceArgs A{};
@@ -1072,10 +1073,20 @@ int32_t mt_trace_jobs(mt_context* ctx, const MtSceneRef* refs, const int32_t* ta
     A.capacity = capacity;
     A.prim = prim;
     A.tout = t;
-    if (mode == MT_TRACE_PACKED_POOL)
+    if (mode == MT_BATCH_TRACE_SPLIT) {
+        MT_ARG_CHECK(d_split, "mt_trace_jobs: the split mode needs the survivors' count");
+        // survivors (incoherent secondary rays) with wave pools ...
+        TraceArgs P = A;
+        P.d_count = d_split;
+        k_trace_jobs_pool<<<mt_blocks(capacity, 256 * POOL_RAYS), 256, 0, ctx->stream>>>(refs, tag, shift, P);
+        MT_LAUNCH_CHECK();
+        // ... and the new camera rays (coherent) one per lane
+        k_trace_jobs<<<mt_blocks(capacity, 256), 256, 0, ctx->stream>>>(refs, tag, shift, A, d_split);
+    } else if (mode == MT_TRACE_PACKED_POOL) {
         k_trace_jobs_pool<<<mt_blocks(capacity, 256 * POOL_RAYS), 256, 0, ctx->stream>>>(refs, tag, shift, A);
-    else
-        k_trace_jobs<<<mt_blocks(capacity, 256), 256, 0, ctx->stream>>>(refs, tag, shift, A);
+    } else {
+        k_trace_jobs<<<mt_blocks(capacity, 256), 256, 0, ctx->stream>>>(refs, tag, shift, A, nullptr);
+    }
     MT_LAUNCH_CHECK();
     return MT_OK;
 }

@@ -255,7 +255,8 @@ def serve_two_lane(ctx: api.Context, requests: List[Request], keep_images: bool 
 def make_batch(ctx: api.Context, setting: dict, arena_bytes: int = 32 << 30, max_jobs: int = 1024,
                policy: int = 1) -> api.Batch:
     """A multi-job engine with an autotuner setting: queue_capacity,
-    iterations (per step), trace_mode (_lib.MT_TRACE_PACKED_STACK / _POOL),
+    iterations (per step), trace_mode (_lib.MT_TRACE_PACKED_STACK / _POOL /
+    _lib.MT_BATCH_TRACE_SPLIT),
     graphs (replay captured iterations)."""
     b = api.Batch(ctx, int(setting["queue_capacity"]), arena_bytes, max_jobs, policy=policy)
     b.set_iterations(int(setting["iterations"]))
@@ -271,19 +272,20 @@ def autotune_batch(ctx: api.Context, requests: List[Request], arena_bytes: int =
     """Chooses the multi-job engine's settings for a workload: the queue
     capacity (paths in flight per iteration), the bounce iterations per host
     round trip, the closest-hit traversal of the shared queue (one ray per
-    lane, or wave pools) and graph replay.  `requests` is a sample of the
+    lane, wave pools, or pools for the continuing paths and one ray per lane
+    for the new ones) and graph replay.  `requests` is a sample of the
     workload (its scenes, sizes and sample counts), served through
     serve_batched per trial: objective "jobs_per_s" submits them at once and
     the setting with the most jobs/s wins; "p99" keeps their arrival times
-    and the lowest p99 latency wins.  Coordinate search from (largest capacity, 8 iterations, one
-    ray per lane, graphs): each dimension in turn takes its best value with
+    and the lowest p99 latency wins.  Coordinate search from (largest capacity, 8 iterations, the
+    split traversal, graphs): each dimension in turn takes its best value with
     the others held.  Every setting renders the same images.  Returns (best
     setting, [(setting, jobs/s or p99 ms), ...])."""
     from . import _lib
     if objective not in ("jobs_per_s", "p99"):
         raise ValueError(f"unknown objective {objective!r}")
     if trace_modes is None:
-        trace_modes = (_lib.MT_TRACE_PACKED_STACK, _lib.MT_TRACE_PACKED_POOL)
+        trace_modes = (_lib.MT_BATCH_TRACE_SPLIT, _lib.MT_TRACE_PACKED_STACK, _lib.MT_TRACE_PACKED_POOL)
     sample = [Request(0.0 if objective == "jobs_per_s" else r.arrival, r.scene, r.materials, r.env, r.job,
                       r.light_faces, r.num_lights, tag=k) for k, r in enumerate(requests)]
     best = {"queue_capacity": max(queue_capacities), "iterations": 8 if 8 in iterations else iterations[0],

@@ -281,14 +281,17 @@ def test_graph_recaptured_when_iterations_change(ctx, world):
         np.testing.assert_array_equal(imgs[tag].cpu().numpy(), _plain(ctx, world, sc, job), err_msg=f"job {tag}")
 
 
-def test_pool_traversal_in_the_engine(ctx, world):
-    """The shared queue traced with wave pools (each lane loading its slot's
-    scene) gives the images of plain renders."""
-    print("[case] engine with pooled traversal", flush=True)
+@pytest.mark.parametrize("mode", [_lib.MT_TRACE_PACKED_STACK, _lib.MT_TRACE_PACKED_POOL, _lib.MT_BATCH_TRACE_SPLIT])
+def test_pool_traversal_in_the_engine(ctx, world, mode):
+    """The shared queue traced one ray per lane, with wave pools (each lane
+    loading its slot's scene), or split (pools for the continuing paths, one
+    ray per lane for the new camera rays; the default) gives the images of
+    plain renders."""
+    print(f"[case] engine with traversal mode {mode}", flush=True)
     scenes, dm, env, (lf, nl) = world
     jobs = list(enumerate(_jobs()))
     b = api.Batch(ctx, queue_capacity=1 << 14, arena_bytes=256 << 20, max_jobs=64)
-    b.set_trace_mode(_lib.MT_TRACE_PACKED_POOL)
+    b.set_trace_mode(mode)
     b.set_iterations(3)
     imgs = {}
     for tag, (sc, job) in jobs:
