@@ -57,15 +57,18 @@ def traffic_from_profiles(workload: str = "c1", mode=None):
         return None
 
 
-def valu_from_profiles(workload: str = "c1"):
+def valu_from_profiles(workload: str = "c1", mode=None):
     """VALU instructions per closest-hit launch (rocprofv3 SQ_INSTS_VALU, profiles/) and the
-    issue peak they are compared with, or None."""
+    issue peak they are compared with, or None (also when the profile was taken for
+    another traversal mode than `mode`)."""
     p = os.path.join(ROOT, "profiles", f"counters_{workload}.json")
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
             d = json.load(f)
+        if mode is not None and "mode" in d and int(d["mode"]) != int(mode):
+            return None
         return (float(d["valu_inst_per_launch"]), float(d["valu_peak_inst_per_s"]), d["valu_peak_basis"],
                 d.get("cycles_per_inst"))
     except Exception:
@@ -603,7 +606,7 @@ def bench_c1(args) -> int:
     return 0
 
 
-def valu_issue(valu, launch_s):
+def valu_issue(valu, launch_s, source: str = "profiles/counters_c1.md (SQ_INSTS_VALU, GRBM_GUI_ACTIVE)"):
     """The closest-hit launches' VALU issue rate (profiled instructions per launch over the
     live launch time) against the peak wave64 issue rate of the 2-cycle instruction class, with
     the measured cycles per instruction beside it."""
@@ -614,7 +617,7 @@ def valu_issue(valu, launch_s):
             "peak_inst_per_s": round(peak / 1e9, 1), "unit": "G VALU instructions/s",
             "frac": round(inst / launch_s / peak, 4), "peak_basis": basis,
             "cycles_per_inst": cpi,
-            "source": "profiles/counters_c1.md (SQ_INSTS_VALU, GRBM_GUI_ACTIVE), profiles/valu_calibration.md"}
+            "source": source + ", profiles/valu_calibration.md"}
 
 
 def bench_c2(args) -> int:
@@ -791,6 +794,8 @@ def bench_c2(args) -> int:
             "algorithmic_bytes_per_launch": int(alg_c),
             "alg_basis": "reference-layout bytes touched, 40 + 148 I + 116 L + 28 P per ray, no cache credit",
             "avg_launch_ms": round(stage[0], 4),
+            "valu_issue": valu_issue(valu_from_profiles("c2", modes["closest"]), stage[0] / 1e3,
+                                     "profiles/traffic_trace_c2.md (SQ_INSTS_VALU, GRBM_GUI_ACTIVE)"),
         },
         "cpu_baseline": cpu,
     }
