@@ -274,6 +274,27 @@ def test_records_edge_cases(ctx):
             np.testing.assert_array_equal(hits.cpu().numpy(), [0, 0, 0])
 
 
+@pytest.mark.parametrize("mode", [api._lib.MT_TRACE_PACKED_POOL, api._lib.MT_TRACE_PACKED_BINNED])
+def test_sorted_and_pooled_modes_at_edge_counts(ctx, c0, mode):
+    """Live counts of 0 and 1 and around the binned sort's 65,536-ray tile
+    (one tile minus one, exactly one, one more, two partial tiles): every
+    live ray's result equals one ray per lane."""
+    print(f"[case] edge counts mode={mode}", flush=True)
+    sc = _scene(ctx, c0)
+    o, d = api.gen_random_rays(91, 0, 70_000, [-1.0] * 3, [1.0] * 3)
+    do, dd = ctx.to_device(o), ctx.to_device(d)
+    for n in (0, 1, 65_535, 65_536, 65_537, 70_000):
+        count = ctx.scalar_i32(n)
+        for fn in (api.intersect_primitives, api.intersect_occlusion):
+            a = fn(ctx, sc, do, dd, count=count, mode=api._lib.MT_TRACE_PACKED_STACK)
+            b = fn(ctx, sc, do, dd, count=count, mode=mode)
+            torch.cuda.synchronize()
+            for x, y in zip(a, b):
+                if x is None:
+                    continue
+                assert np.array_equal(x.cpu().numpy()[:n].view(np.int32), y.cpu().numpy()[:n].view(np.int32)), n
+
+
 @pytest.mark.parametrize("mode", [api._lib.MT_TRACE_PACKED_SORTED, api._lib.MT_TRACE_PACKED_POOL,
                                   api._lib.MT_TRACE_PACKED_BINNED])
 @pytest.mark.parametrize("any_hit", [False, True])
