@@ -155,6 +155,8 @@ def test_fused_and_staged_loops_agree(ctx, c1_small, mesh_lights):
             W.RenderJob(**base, mode=_lib.MT_TRACE_PACKED_POOL),           # fused, wave-pool lanes
             W.RenderJob(**base, flags=_lib.MT_RENDER_TAIL_FROM(1)),        # fused, bounces 1.. in one launch
             W.RenderJob(**base, flags=_lib.MT_RENDER_TAIL_FROM(3)),        # fused, bounces 3.. in one launch
+            W.RenderJob(**base, flags=_lib.MT_RENDER_TAIL_FROM(W.RenderJob().max_bounces - 1)),  # last bounce only
+            W.RenderJob(**base, flags=_lib.MT_RENDER_TAIL_FROM(W.RenderJob().max_bounces)),      # past the end: off
             W.RenderJob(**base, mode=_lib.MT_TRACE_PACKED_POOL,            # wave pools, then the tail
                         flags=_lib.MT_RENDER_TAIL_FROM(2)),
             W.RenderJob(**base, flags=_lib.MT_RENDER_TRAVERSAL_STATS),     # staged, same traversal
