@@ -41,6 +41,7 @@ extern "C" int32_t mt_ctx_create(int32_t device, void* stream, int32_t flags, mt
     c->temp = nullptr;
     c->temp_bytes = 0;
     c->side = nullptr;
+    c->nfj = 0;
     if (!(flags & MT_CTX_OWN_STREAM)) {
         c->stream = (hipStream_t)stream;
         c->own_stream = false;
@@ -69,6 +70,10 @@ extern "C" int32_t mt_ctx_destroy(mt_context* ctx)
         fe(hipStreamSynchronize(ctx->side));
         fe(hipStreamDestroy(ctx->side));
     }
+    for (int k = 0; k < ctx->nfj; k++) {
+        fe(hipEventDestroy(ctx->fork[k]));
+        fe(hipEventDestroy(ctx->join[k]));
+    }
     if (ctx->own_stream) fe(hipStreamDestroy(ctx->stream));
     delete ctx;
     return fe.status("mt_ctx_destroy");
@@ -82,6 +87,17 @@ int32_t mt_ctx_side_stream(mt_context* ctx, hipStream_t* out)
         if (e != hipSuccess) {
             ctx->side = nullptr;
             mt_set_error("side stream: hipStreamCreate failed: %s", hipGetErrorString(e));
+            return MT_ERR_HIP;
+        }
+    }
+    for (; ctx->nfj < 64; ctx->nfj++) {
+        if (hipEventCreateWithFlags(&ctx->fork[ctx->nfj], hipEventDisableTiming) != hipSuccess) {
+            mt_set_error("side stream: hipEventCreate failed");
+            return MT_ERR_HIP;
+        }
+        if (hipEventCreateWithFlags(&ctx->join[ctx->nfj], hipEventDisableTiming) != hipSuccess) {
+            (void)hipEventDestroy(ctx->fork[ctx->nfj]);
+            mt_set_error("side stream: hipEventCreate failed");
             return MT_ERR_HIP;
         }
     }

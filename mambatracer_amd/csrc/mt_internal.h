@@ -68,9 +68,15 @@ struct mt_context {
     // back to `stream` through another before the call returns, so callers
     // still see one ordered stream.
     hipStream_t side;
+    // the fork and join events of the bounce loop (one pair per bounce),
+    // created with the side stream; calls on a context are serialised, so
+    // every render reuses them
+    hipEvent_t fork[64], join[64];
+    int nfj;
 };
 
-// The context's side stream (created on first use).
+// The context's side stream and its per-bounce fork / join events (created
+// on first use).
 int32_t mt_ctx_side_stream(mt_context* ctx, hipStream_t* out);
 
 // Packed traversal layout derived from the reference arrays.

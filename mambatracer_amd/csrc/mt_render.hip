@@ -538,9 +538,9 @@ struct RenderPlan {
     int nev;
     // fused loop without per-stage timing: bounce b's any hit runs on the
     // context's side stream, concurrently with bounce b+1's closest hit
-    // (fork[b] after b's shading, join[b] before b+1's shading)
-    hipEvent_t fork[64], join[64];
-    int nfj;
+    // (the context's fork[b] after b's shading, join[b] before b+1's)
+    hipEvent_t* fork;
+    hipEvent_t* join;
     hipStream_t side;
 };
 
@@ -564,7 +564,7 @@ int32_t plan_render(const mt_scene* scene, const float* materials, const mt_envm
     R.base = nullptr;
     R.borrowed = false;
     R.nev = 0;
-    R.nfj = 0;
+    R.fork = R.join = nullptr;
     R.side = nullptr;
     int32_t rc = camera_from_desc(desc, R.C);
     if (rc) return rc;
@@ -638,11 +638,7 @@ void free_render(RenderPlan& R)
 {
     for (int k = 0; k < R.nev; k++) (void)hipEventDestroy(R.ev[k]);
     R.nev = 0;
-    for (int k = 0; k < R.nfj; k++) {
-        (void)hipEventDestroy(R.fork[k]);
-        (void)hipEventDestroy(R.join[k]);
-    }
-    R.nfj = 0;
+    R.fork = R.join = nullptr; // the context's
     if (R.base && !R.borrowed) (void)hipFree(R.base);
     R.base = nullptr;
 }
@@ -678,22 +674,12 @@ int32_t alloc_render(RenderPlan& R, mt_context* ctx, bool borrow)
     if (R.fused && !R.profile && R.nb > 1) {
         int32_t rc = mt_ctx_side_stream(ctx, &R.side);
         if (rc) {
+            R.side = nullptr;
             free_render(R);
             return rc;
         }
-        for (; R.nfj < R.nb; R.nfj++) {
-            if (hipEventCreateWithFlags(&R.fork[R.nfj], hipEventDisableTiming) != hipSuccess) {
-                free_render(R);
-                mt_set_error("mt_render: hipEventCreate failed");
-                return MT_ERR_HIP;
-            }
-            if (hipEventCreateWithFlags(&R.join[R.nfj], hipEventDisableTiming) != hipSuccess) {
-                (void)hipEventDestroy(R.fork[R.nfj]);
-                free_render(R);
-                mt_set_error("mt_render: hipEventCreate failed");
-                return MT_ERR_HIP;
-            }
-        }
+        R.fork = ctx->fork;
+        R.join = ctx->join;
     }
     return MT_OK;
 }
@@ -732,7 +718,12 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
     const int grid = mt_blocks(np, RT);
     const int gshade = grid;              // one slot per lane; empty work-groups retire at once
     const int gshadow = mt_blocks((int64_t)R.sq, RT);
-    const int gsweep = SHADOW_SEGS * SEG_SWEEP; // the fused loop's segment sweeps
+    // the fused loop's segment sweeps: SEG_SWEEP work-groups per segment, or
+    // fewer when a segment holds fewer chunks of RT slots (small jobs: the
+    // extra groups could never find work, only be dispatched)
+    const int seg_chunks = (int)(((int64_t)seg_cap + RT - 1) / RT);
+    const int gsweep = SHADOW_SEGS * (seg_chunks < SEG_SWEEP ? seg_chunks : SEG_SWEEP);
+    const int gtail = SHADOW_SEGS * (seg_chunks < TAIL_SWEEP ? seg_chunks : TAIL_SWEEP);
     // wave-pool kernels: runs of 64 * POOL_RAYS entries, 4 per work-group
     const int64_t pool_group = 4 * 64 * (int64_t)POOL_RAYS;
     const int gcam_pool = mt_blocks(np, (int)pool_group);
@@ -753,7 +744,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
     // the radiance / the next queue and the hit records); b+1's shading
     // waits for it (its radiance adds come after b's, and it refills the
     // shadow queue)
-    const bool overlap = R.side != nullptr && R.nfj == nb;
+    const bool overlap = R.side != nullptr && R.fork != nullptr;
     int pending = -1; // bounce whose any hit runs on the side stream
     auto join_pending = [&]() -> int32_t {
         if (pending < 0) return MT_OK;
@@ -821,7 +812,7 @@ int32_t enqueue_render(mt_context* ctx, RenderPlan& R)
         const bool first = fused && b == 0;
         if (R.tail && b == R.tail) { // every remaining bounce in one launch
             if ((rc = join_pending())) break;
-            k_tail<<<SHADOW_SEGS * TAIL_SWEEP, RT, 0, st>>>(shade_args(b, false), scene->pnodes, scene->ptris,
+            k_tail<<<gtail, RT, 0, st>>>(shade_args(b, false), scene->pnodes, scene->ptris,
                                                           scene->root_ref, nb, I(R.oTail), I(R.oTail) + 64);
             rc = chk();
             break;
@@ -1067,7 +1058,7 @@ extern "C" int32_t mt_render_graph_create(mt_context* ctx, const mt_scene* scene
     g->ctx = ctx;
     g->R.base = nullptr;
     g->R.nev = 0;
-    g->R.nfj = 0;
+    g->R.fork = g->R.join = nullptr;
     int32_t rc = plan_render(scene, materials, env, light_faces, d_num_lights, desc, image, g->R);
     if (rc == MT_OK) rc = alloc_render(g->R, ctx, false);
     if (rc) {
