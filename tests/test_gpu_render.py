@@ -171,6 +171,47 @@ def test_fused_and_staged_loops_agree(ctx, c1_small, mesh_lights):
         assert list(st.bounce_counts) == list(ref_st.bounce_counts)
 
 
+@pytest.mark.parametrize("mesh_lights", [False, True])
+def test_tail_with_many_live_paths(ctx, mesh_lights):
+    """A dense triangle soup in front of the camera keeps many paths alive
+    into the late bounces (a fifth at bounce 3, against 0.2% in the open
+    config-1 scene), so the one-launch tail carries thousands of paths per
+    segment sweep and refills every lane many times.  Tail from bounce 1, 3
+    and 6 against the per-bounce launches: same image bits, ray counts and
+    per-bounce counts."""
+    print(f"[case] tail on a soup, mesh lights {mesh_lights}", flush=True)
+    points, faces, nodes, bounds, depth = W.soup_scene(200_000, 61)
+    mats = W.c1_materials()
+    lf = nl = None
+    if mesh_lights:
+        emit = W.material_records(np.full((1, 3), 0.5), 0.04, 0.5, 0.5, emission=np.full((1, 3), 1.0),
+                                  emission_power=20.0)
+        mats = np.concatenate([mats, emit])
+        faces[::400, 9] = mats.shape[0] - 1
+    sc = api.Scene(ctx, points, faces, nodes, bounds, depth)
+    dm = ctx.to_device(mats)
+    if mesh_lights:
+        lf, nl = api.mesh_lights(ctx, sc, dm)
+    env = api.CEnvironment(ctx)
+    env.setEnvironmentMap(W.env_map())
+    job = W.soup_job(W.JobSpec(0, 0, 128, 4, 6100 + int(mesh_lights)))
+    out = []
+    for tail in (0, 1, 3, job.max_bounces - 2):
+        job.flags = _lib.MT_RENDER_TAIL_FROM(tail)
+        img, st = api.render(ctx, sc, dm, env, job, light_faces=lf, num_lights=nl)
+        out.append((img.cpu().numpy(), st))
+    ref_img, ref_st = out[0]
+    print(f"paths per bounce {list(ref_st.bounce_counts)}", flush=True)
+    for img, st in out[1:]:
+        np.testing.assert_array_equal(img.view(np.int32), ref_img.view(np.int32))
+        assert int(st.extension_rays) == int(ref_st.extension_rays)
+        assert int(st.shadow_rays) == int(ref_st.shadow_rays)
+        assert list(st.bounce_counts) == list(ref_st.bounce_counts)
+    assert np.isfinite(ref_img).all()
+    paths = 128 * 128 * 4
+    assert ref_st.bounce_counts[1] > paths // 2 and ref_st.bounce_counts[3] > paths // 8
+
+
 @pytest.mark.parametrize("w,h,spp", [(1, 1, 1), (7, 3, 5), (17, 9, 1)])
 def test_tiny_jobs_fused_and_staged_agree(ctx, c1_small, w, h, spp):
     """Jobs far smaller than one work-group or one queue segment: the fused
