@@ -68,7 +68,8 @@ __global__ __launch_bounds__(RT) void k_camera(Camera C, int npaths, const uint3
 // Bounce 0 of the fused loop: path pid = slot traces its camera ray
 // (generated here, never stored) to its closest hit; same per-ray traversal
 // as mt_trace_closest's default mode, so prim/t are the same bits.  POOL 0:
-// one path per lane; POOL > 0: each wave traces the run of 64 * POOL
+// one path per lane, work-groups renumbered in per-XCD runs
+// (xcd_chunked_group); POOL > 0: each wave traces the run of 64 * POOL
 // consecutive paths (4 runs per work-group), refilling finished lanes
 // (pool_trace).
 template <int POOL>
@@ -99,7 +100,8 @@ __global__ __launch_bounds__(RT) void k_trace_camera(Camera C, int npaths, const
             });
         return;
     }
-    const int pid = blockIdx.x * blockDim.x + threadIdx.x;
+    // runs of 64 work-groups (16K paths, a compact patch of pixels) per XCD
+    const int pid = (int)xcd_chunked_group(blockIdx.x, gridDim.x, 64u) * (int)blockDim.x + (int)threadIdx.x;
     if (pid >= npaths) return;
     float4 o, d;
     camera_ray(C, pid, seed, o, d);

@@ -199,18 +199,6 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
     }
 }
 
-// Work-groups go to the 8 XCDs in turn (group b on XCD b mod 8), each XCD
-// with its own L2.  Over a binned order, consecutive entries start in the
-// same cell and walk the same subtrees; numbering the G groups that hold
-// entries so that XCD x takes one contiguous eighth of the order (b ->
-// x * (G / 8) + min(x, G % 8) + b / 8, a bijection of [0, G)) keeps each L2
-// on its own cells' nodes instead of all eight caching the same ones.
-__device__ __forceinline__ unsigned xcd_contiguous_group(unsigned b, unsigned groups)
-{
-    const unsigned per = groups >> 3, rem = groups & 7u, x = b & 7u;
-    return x * per + (x < rem ? x : rem) + (b >> 3);
-}
-
 // Wave-pool traversal (pool_trace): each wave owns a run of 64 * POOL
 // consecutive queue entries (segmented queue: work-group b takes chunk
 // b / nseg of segment b % nseg, 256 * POOL entries, one run per wave; over a
@@ -226,8 +214,10 @@ __global__ __launch_bounds__(256) void k_trace_pool(TraceArgs A)
         int count = *A.d_count;
         count = count < A.capacity ? count : A.capacity;
         count = count > 0 ? count : 0;
-        // groups that hold entries (the grid is sized for the capacity); the
-        // rest leave at once below
+        // over a binned order, consecutive entries start in the same cell and
+        // walk the same subtrees: each XCD takes one contiguous eighth of the
+        // groups that hold entries (the grid is sized for the capacity; the
+        // rest leave at once below)
         const unsigned used = (unsigned)(((int64_t)count + 4 * RUN - 1) / (4 * RUN));
         const unsigned g = A.order && blockIdx.x < used ? xcd_contiguous_group(blockIdx.x, used) : blockIdx.x;
         next = (int)(g * 4 + (threadIdx.x >> 6)) * RUN;

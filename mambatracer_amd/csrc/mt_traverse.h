@@ -376,6 +376,32 @@ __device__ __forceinline__ bool trace_packed_node(const float4* __restrict__ pn,
     return !parent_walk<STATS>(pn, pt, S.cur, S.bs, S.st);
 }
 
+// Work-groups go to the 8 XCDs in turn (group b on XCD b mod 8), each XCD
+// with its own L2.  Where neighbouring work-groups trace rays that walk the
+// same subtrees, renumbering the groups so that each XCD takes neighbouring
+// work keeps each L2 on its own region's nodes instead of all eight caching
+// the same ones.  Both maps are bijections of [0, groups).
+//
+// One contiguous eighth of the groups per XCD: b -> x * (G / 8) +
+// min(x, G % 8) + b / 8 for XCD x = b mod 8.
+__device__ __forceinline__ unsigned xcd_contiguous_group(unsigned b, unsigned groups)
+{
+    const unsigned per = groups >> 3, rem = groups & 7u, x = b & 7u;
+    return x * per + (x < rem ? x : rem) + (b >> 3);
+}
+
+// Runs of `chunk` consecutive groups per XCD, the XCDs taking the runs in
+// turn (XCD x runs x, x + 8, x + 16, ...), so work that varies across the
+// range stays balanced between XCDs; groups past the last whole round of 8
+// runs keep their own number.
+__device__ __forceinline__ unsigned xcd_chunked_group(unsigned b, unsigned groups, unsigned chunk)
+{
+    const unsigned round = 8u * chunk;
+    if (b >= (groups / round) * round) return b;
+    const unsigned x = b & 7u, k = b >> 3;
+    return ((k / chunk) * 8u + x) * chunk + k % chunk;
+}
+
 // Idle lanes a pool waits for before starting new rays.
 constexpr int POOL_REFILL_MIN = 16;
 
