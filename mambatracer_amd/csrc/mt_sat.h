@@ -609,8 +609,10 @@ __host__ __device__ inline unsigned long long lg_fixed(float lum)
 // Call with the whole wave active.
 // Lanes that share an entry with the wave's first pending lane are summed
 // and added with one atomic (camera-coherent misses share a few entries);
-// after four such rounds the rest add their own.  Integer sums: the result
-// does not depend on how the additions are grouped.
+// after four such rounds, or as soon as a round finds at most two lanes on
+// its entry (scattered entries, as for paths escaping after a bounce: the
+// shuffle sum would save one atomic), the rest add their own.  Integer
+// sums: the result does not depend on how the additions are grouped.
 __device__ __forceinline__ void lg_wave_accum(bool add, int gidx, unsigned long long q,
                                               unsigned long long* __restrict__ pend)
 {
@@ -621,6 +623,7 @@ __device__ __forceinline__ void lg_wave_accum(bool add, int gidx, unsigned long 
         const int leader = __builtin_ctzll(m);
         const int g = __shfl(gidx, leader);
         const bool mine = add && gidx == g;
+        if (__popcll(__ballot(mine)) <= 2) break;
         unsigned long long v = mine ? q : 0ull;
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
         if ((int)__lane_id() == leader) atomicAdd(pend + g, v);
