@@ -295,6 +295,28 @@ def test_sorted_and_pooled_modes_at_edge_counts(ctx, c0, mode):
                 assert np.array_equal(x.cpu().numpy()[:n].view(np.int32), y.cpu().numpy()[:n].view(np.int32)), n
 
 
+def test_binned_mode_with_batch_sized_tiles(ctx, c0):
+    """A batch large enough that the binned sort's tiles grow past their
+    65,536-ray minimum (20,000,003 rays: 254 tiles of 78,848, the last one
+    partial) with a live count that ends inside the last tile: every live
+    ray's closest and any hit equal one ray per lane."""
+    print("[case] binned mode, 20M rays", flush=True)
+    sc = _scene(ctx, c0)
+    n = 20_000_003
+    o, d = api.gen_random_rays(93, 0, n, [-1.0] * 3, [1.0] * 3)
+    do, dd = ctx.to_device(o), ctx.to_device(d)
+    del o, d
+    count = ctx.scalar_i32(n - 5)
+    for fn in (api.intersect_primitives, api.intersect_occlusion):
+        a = fn(ctx, sc, do, dd, count=count, mode=api._lib.MT_TRACE_PACKED_STACK)
+        b = fn(ctx, sc, do, dd, count=count, mode=api._lib.MT_TRACE_PACKED_BINNED)
+        torch.cuda.synchronize()
+        for x, y in zip(a, b):
+            if x is None:
+                continue
+            assert torch.equal(x[:n - 5].view(torch.int32), y[:n - 5].view(torch.int32))
+
+
 @pytest.mark.parametrize("mode", [api._lib.MT_TRACE_PACKED_SORTED, api._lib.MT_TRACE_PACKED_POOL,
                                   api._lib.MT_TRACE_PACKED_BINNED])
 @pytest.mark.parametrize("any_hit", [False, True])
