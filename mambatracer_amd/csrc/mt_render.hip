@@ -155,11 +155,12 @@ __global__ __launch_bounds__(RT) void k_trace_shadow(const float4* __restrict__ 
         return;
     }
     // segment sweep (SEG_SWEEP groups per segment): one entry per lane per chunk
-    const int p = blockIdx.x % SHADOW_SEGS;
+    const unsigned v = seg_sweep_group();
+    const int p = (int)(v % SHADOW_SEGS);
     int cnt = seg_counts[p];
     cnt = cnt < seg_cap ? cnt : seg_cap;
     const int stride = (int)(gridDim.x / SHADOW_SEGS) * (int)blockDim.x;
-    for (int off = (int)(blockIdx.x / SHADOW_SEGS) * (int)blockDim.x; off < cnt; off += stride) {
+    for (int off = (int)(v / SHADOW_SEGS) * (int)blockDim.x; off < cnt; off += stride) {
         const int k = p * seg_cap + off + (int)threadIdx.x;
         const bool live = off + (int)threadIdx.x < cnt;
         if (__ballot(live) == 0ull) continue; // wave-uniform

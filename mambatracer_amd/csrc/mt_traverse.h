@@ -402,6 +402,21 @@ __device__ __forceinline__ unsigned xcd_chunked_group(unsigned b, unsigned group
     return ((k / chunk) * 8u + x) * chunk + k % chunk;
 }
 
+// Work-group number v of a sweep over a segmented queue: group v takes
+// segment v mod nseg from chunk v / nseg (the grid a multiple of nseg).  The
+// bounce loop's shading appends work-group b's paths to segment b mod 1024,
+// so consecutive segments hold neighbouring pixel blocks; runs of 64
+// consecutive numbers per XCD keep each XCD's L2 on the nodes of a compact
+// stretch of the image (the traversal sweeps: the config-1 job 9.56 ->
+// 9.49 ms; the shading sweeps, which read per-path state rather than nodes,
+// lost 0.4% with it and keep v = b), and with 1024 segments every group of
+// one segment runs on the same XCD.
+constexpr unsigned SEG_XCD_RUN = 64;
+__device__ __forceinline__ unsigned seg_sweep_group()
+{
+    return xcd_chunked_group(blockIdx.x, gridDim.x, SEG_XCD_RUN);
+}
+
 // Idle lanes a pool waits for before starting new rays.
 constexpr int POOL_REFILL_MIN = 16;
 
@@ -458,12 +473,13 @@ __device__ __forceinline__ void pool_trace(const float4* __restrict__ pn, const 
 
 // The run of wave w of work-group b in a segmented queue (nseg segments of
 // seg_cap entries, counts[p] live in segment p) walked chunk-major with
-// runs of `run` entries: group b takes chunk b / nseg of segment b % nseg,
-// 4 runs, one per wave.
+// runs of `run` entries: group v (seg_sweep_group) takes chunk v / nseg of
+// segment v % nseg, 4 runs, one per wave.
 __device__ __forceinline__ void pool_seg_run(const int* __restrict__ counts, int nseg, int seg_cap, int run,
                                              int& next, int& end)
 {
-    const int p = blockIdx.x % nseg, c = blockIdx.x / nseg;
+    const unsigned v = seg_sweep_group();
+    const int p = (int)(v % (unsigned)nseg), c = (int)(v / (unsigned)nseg);
     int cnt = counts[p];
     cnt = cnt < seg_cap ? cnt : seg_cap;
     const int off = c * (4 * run) + (int)(threadIdx.x >> 6) * run;
