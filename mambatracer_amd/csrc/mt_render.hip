@@ -100,8 +100,8 @@ __global__ __launch_bounds__(RT) void k_trace_camera(Camera C, int npaths, const
             });
         return;
     }
-    // runs of 64 work-groups (16K paths, a compact patch of pixels) per XCD
-    const int pid = (int)xcd_chunked_group(blockIdx.x, gridDim.x, 64u) * (int)blockDim.x + (int)threadIdx.x;
+    // work-groups in per-XCD runs (16K paths, a compact patch of pixels)
+    const int pid = (int)xcd_chunked_group(blockIdx.x, gridDim.x, XCD_RUN) * (int)blockDim.x + (int)threadIdx.x;
     if (pid >= npaths) return;
     float4 o, d;
     camera_ray(C, pid, seed, o, d);

@@ -202,7 +202,8 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
 // Wave-pool traversal (pool_trace): each wave owns a run of 64 * POOL
 // consecutive queue entries (segmented queue: work-group b takes chunk
 // b / nseg of segment b % nseg, 256 * POOL entries, one run per wave; over a
-// binned order, XCD-contiguous runs) and refills its finished lanes from it.
+// binned order, work-groups in per-XCD runs) and refills its finished lanes
+// from it.
 template <bool ANY, int POOL, int SS = MT_SHORT_STACK>
 __global__ __launch_bounds__(256) void k_trace_pool(TraceArgs A)
 {
@@ -215,11 +216,11 @@ __global__ __launch_bounds__(256) void k_trace_pool(TraceArgs A)
         count = count < A.capacity ? count : A.capacity;
         count = count > 0 ? count : 0;
         // over a binned order, consecutive entries start in the same cell and
-        // walk the same subtrees: each XCD takes one contiguous eighth of the
-        // groups that hold entries (the grid is sized for the capacity; the
-        // rest leave at once below)
+        // walk the same subtrees: the groups that hold entries are numbered
+        // in per-XCD runs (the grid is sized for the capacity; the rest leave
+        // at once below)
         const unsigned used = (unsigned)(((int64_t)count + 4 * RUN - 1) / (4 * RUN));
-        const unsigned g = A.order && blockIdx.x < used ? xcd_contiguous_group(blockIdx.x, used) : blockIdx.x;
+        const unsigned g = A.order && blockIdx.x < used ? xcd_chunked_group(blockIdx.x, used, XCD_RUN) : blockIdx.x;
         next = (int)(g * 4 + (threadIdx.x >> 6)) * RUN;
         end = count < next + RUN ? count : next + RUN;
     }

@@ -378,22 +378,14 @@ __device__ __forceinline__ bool trace_packed_node(const float4* __restrict__ pn,
 
 // Work-groups go to the 8 XCDs in turn (group b on XCD b mod 8), each XCD
 // with its own L2.  Where neighbouring work-groups trace rays that walk the
-// same subtrees, renumbering the groups so that each XCD takes neighbouring
-// work keeps each L2 on its own region's nodes instead of all eight caching
-// the same ones.  Both maps are bijections of [0, groups).
+// same subtrees, renumbering the groups so that each XCD takes runs of
+// neighbouring work keeps each L2 on its own region's nodes instead of all
+// eight caching the same ones, while the XCDs taking the runs in turn keeps
+// work that varies across the range balanced between them.
 //
-// One contiguous eighth of the groups per XCD: b -> x * (G / 8) +
-// min(x, G % 8) + b / 8 for XCD x = b mod 8.
-__device__ __forceinline__ unsigned xcd_contiguous_group(unsigned b, unsigned groups)
-{
-    const unsigned per = groups >> 3, rem = groups & 7u, x = b & 7u;
-    return x * per + (x < rem ? x : rem) + (b >> 3);
-}
-
-// Runs of `chunk` consecutive groups per XCD, the XCDs taking the runs in
-// turn (XCD x runs x, x + 8, x + 16, ...), so work that varies across the
-// range stays balanced between XCDs; groups past the last whole round of 8
-// runs keep their own number.
+// Runs of `chunk` consecutive numbers per XCD (XCD x takes runs x, x + 8,
+// x + 16, ...); groups past the last whole round of 8 runs keep their own
+// number.  A bijection of [0, groups).
 __device__ __forceinline__ unsigned xcd_chunked_group(unsigned b, unsigned groups, unsigned chunk)
 {
     const unsigned round = 8u * chunk;
@@ -402,19 +394,27 @@ __device__ __forceinline__ unsigned xcd_chunked_group(unsigned b, unsigned group
     return ((k / chunk) * 8u + x) * chunk + k % chunk;
 }
 
+// Run length of the renumbered kernels (camera rays: 16K paths, a compact
+// patch of pixels; binned order: 128K rays, 64 neighbouring origin cells of
+// a uniform batch).  Measured: config-1 job 9.69 -> 9.63 ms with the camera
+// traversal renumbered (runs of 256 or 1024: the same); 64M random rays,
+// closest / any hit with the sorted order renumbered 48.4 / 34.0 -> 47.6 /
+// 33.1 ms (runs of 16: 47.8 / 33.1; 32: 47.5 / 33.0; 256 or contiguous
+// eighths of the order: 48.2 / 33.4-33.6).
+constexpr unsigned XCD_RUN = 64;
+
 // Work-group number v of a sweep over a segmented queue: group v takes
 // segment v mod nseg from chunk v / nseg (the grid a multiple of nseg).  The
 // bounce loop's shading appends work-group b's paths to segment b mod 1024,
-// so consecutive segments hold neighbouring pixel blocks; runs of 64
+// so consecutive segments hold neighbouring pixel blocks; runs of XCD_RUN
 // consecutive numbers per XCD keep each XCD's L2 on the nodes of a compact
 // stretch of the image (the traversal sweeps: the config-1 job 9.56 ->
 // 9.49 ms; the shading sweeps, which read per-path state rather than nodes,
 // lost 0.4% with it and keep v = b), and with 1024 segments every group of
 // one segment runs on the same XCD.
-constexpr unsigned SEG_XCD_RUN = 64;
 __device__ __forceinline__ unsigned seg_sweep_group()
 {
-    return xcd_chunked_group(blockIdx.x, gridDim.x, SEG_XCD_RUN);
+    return xcd_chunked_group(blockIdx.x, gridDim.x, XCD_RUN);
 }
 
 // Idle lanes a pool waits for before starting new rays.
