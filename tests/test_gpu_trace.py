@@ -295,6 +295,34 @@ def test_sorted_and_pooled_modes_at_edge_counts(ctx, c0, mode):
                 assert np.array_equal(x.cpu().numpy()[:n].view(np.int32), y.cpu().numpy()[:n].view(np.int32)), n
 
 
+def test_binned_mode_on_a_flat_scene_and_non_finite_origins(ctx):
+    """The binned sort's keys on a root box with a zero extent (every
+    triangle in the plane z = 0, so that axis has no cells) and on origins at
+    +-inf or NaN (clamped to the first or last cell): each live ray's closest
+    and any hit equal one ray per lane."""
+    print("[case] binned mode, flat scene", flush=True)
+    points, faces = api.gen_soup(17, 5000, -1.0, 1.0, 0.01, 0.2, first_vertex=0)
+    points[:, 2] = 0.0
+    nodes, bounds, depth = api.CNormalBVH.build(points, faces)
+    sc = api.Scene(ctx, points, faces, nodes, bounds, depth)
+    n = 70_000
+    o, d = api.gen_random_rays(95, 0, n, [-1.2, -1.2, -1.0], [1.2, 1.2, 1.0])
+    o[:50, 0] = np.inf
+    o[50:100, 1] = -np.inf
+    o[100:150, 2] = np.nan
+    o[150:200, 2] = 0.0  # origins in the scene's plane
+    do, dd = ctx.to_device(o), ctx.to_device(d)
+    count = ctx.scalar_i32(n - 3)
+    for fn in (api.intersect_primitives, api.intersect_occlusion):
+        a = fn(ctx, sc, do, dd, count=count, mode=api._lib.MT_TRACE_PACKED_STACK)
+        b = fn(ctx, sc, do, dd, count=count, mode=api._lib.MT_TRACE_PACKED_BINNED)
+        torch.cuda.synchronize()
+        for x, y in zip(a, b):
+            if x is None:
+                continue
+            assert torch.equal(x[:n - 3].view(torch.int32), y[:n - 3].view(torch.int32))
+
+
 def test_binned_mode_with_batch_sized_tiles(ctx, c0):
     """A batch large enough that the binned sort's tiles grow past their
     65,536-ray minimum (20,000,003 rays: 254 tiles of 78,848, the last one
